@@ -1,0 +1,192 @@
+"""Benchmark of the CMM hot path on one MI355X.
+
+Workload (full barotropic vorticity CMM, FP64): icosahedral map mesh level 7
+(163,842 vertices), MWSS velocity bandlimit L = 512, Omega = 2 pi, initial
+vorticity a seeded Gaussian spherical-harmonic field (variance l^-3,
+1 <= l <= 64) held as a scalar PS spline on the L = 1024 MWSS mesh, 100 RK4
+steps of dt = 1/512 with a remap every 20 steps (composition depth 5), then
+the composed map evaluated on the L = 2048 MWSS grid (2049 x 4096 =
+8,392,704 points).
+
+Reported: value = composed-map Mpts/s (8.39 M points through the 5 submaps,
+device resident, location + evaluation, FP64), its roofline against HBM, the
+CMM loop's seconds per steady-state step, and the CPU oracle timed on a
+bounded sample of the same composition on the host cores.  `--steps K
+--warmup W` are the timed / untimed repetitions of the composition.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "cmm-s2_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--L", type=int, default=512)
+    ap.add_argument("--ico", type=int, default=7)
+    ap.add_argument("--Nt", type=int, default=100)
+    ap.add_argument("--n-maps", type=int, default=20)
+    ap.add_argument("--Lf", type=int, default=2048)
+    ap.add_argument("--Lw", type=int, default=1024)
+    ap.add_argument("--lmax", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=262144)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="quick check: L=64 ico-4 Nt=20 Lf=256")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the composition and the "
+              "CMM step shard by points / vertices with no collectives (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if a.small:
+        a.L, a.ico, a.Nt, a.n_maps, a.Lf, a.Lw = 64, 4, 20, 5, 256, 128
+    import numpy as np
+    import torch
+    from cmm_s2.core import mesh_functions as mf, spherical_harmonic_tools as st
+    from cmm_s2.core import dynamical_fields as df
+    from cmm_s2.core.spherical_simulations import BarotropicCMM
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    ico = mf.icosahedral_mesh(a.ico)
+    mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
+    log(f"mesh ico-{a.ico}: {mesh.nv} vertices, {mesh.ntri} triangles, "
+        f"setup {time.perf_counter() - t0:.1f}s, {mesh.stats()}")
+    t0 = time.perf_counter()
+    coeffs = df.random_sh_coefficients(a.seed, a.lmax, a.Lw)
+    wmesh = mf.structure_spherical_triangulation(a.Lw)
+    w0 = st.project_onto_S12(df.lm_to_full(coeffs, a.Lw), a.Lw, wmesh)
+    torch.cuda.synchronize()
+    log(f"initial vorticity spline on L={a.Lw}: {time.perf_counter() - t0:.1f}s")
+
+    T = a.Nt / a.L
+    sim = BarotropicCMM(a.L, a.Nt, T, mesh, w0, 2 * np.pi, a.n_maps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.bootstrap()
+    torch.cuda.synchronize()
+    t_boot = time.perf_counter() - t0
+    step_t, remap_t = [], []
+    while sim.count < sim.Nt:
+        t0 = time.perf_counter()
+        remapped = sim.step()
+        torch.cuda.synchronize()
+        (remap_t if remapped else step_t).append(time.perf_counter() - t0)
+        if sim.count % 20 == 0:
+            log(f"step {sim.count}/{sim.Nt}: {step_t[-1] * 1e3 if step_t else 0:.2f} ms")
+    s_step = statistics.median(step_t[1:] if len(step_t) > 2 else step_t)
+    log(f"bootstrap {t_boot:.3f}s, steady step median {s_step * 1e3:.3f} ms, remap steps "
+        f"{[round(x * 1e3, 3) for x in remap_t]} ms, total loop {sum(step_t) + sum(remap_t):.3f}s")
+
+    # composed map on the Lf grid
+    depth = sim.submaps.Nc
+    grid = mf.structure_spherical_triangulation(a.Lf)
+    q = grid.sample_points()
+    n = q.shape[0]
+    out = torch.empty_like(q)
+    levels = sim.submaps.levels(depth)
+    from cmm_s2.core.spherical_spline import compose_packed
+    for _ in range(a.warmup):
+        compose_packed(mesh, levels, q, "N3", out)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        e0.record()
+        compose_packed(mesh, levels, q, "N3", out)
+        e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    k_ms = statistics.mean(e0.elapsed_time(e1) for e0, e1 in ev)
+    ms_per = wall / a.steps * 1e3
+    mpts = n / (ms_per * 1e-3) / 1e6
+    alg_bytes = (48 + 216 * depth) * n
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    # vorticity on the fine grid (reported separately)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    wf = sim.vorticity_on_grid(a.Lf)
+    torch.cuda.synchronize()
+    t_vort = time.perf_counter() - t0
+    log(f"compose depth {depth} on {n} points: kernel {k_ms:.3f} ms, wall {ms_per:.3f} ms/eval, "
+        f"{mpts:.1f} Mpts/s; vorticity grid {t_vort * 1e3:.2f} ms; |w| max {float(wf.abs().max()):.4f}")
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(mesh, sim, q, depth, a.cpu_sample)
+
+    rec = {
+        "metric": "composed-map Mpts/s + s/step at L=512 ico-7; mixed vs FP64 at 1e-10 err",
+        "value": round(mpts, 3), "unit": "Mpts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms_per, 4), "higher_is_better": True, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"barotropic CMM L={a.L} ico-{a.ico}, {a.Nt} RK4 steps dt=1/{a.L}, remap "
+                               f"every {a.n_maps}; composed map (depth {depth}) on the L={a.Lf} MWSS grid",
+                   "points": n, "depth": depth, "L": a.L, "ico": a.ico, "Lf": a.Lf, "Lw": a.Lw},
+        "s_per_step": round(s_step, 6), "remap_step_s": [round(x, 6) for x in remap_t],
+        "bootstrap_s": round(t_boot, 4), "vorticity_grid_ms": round(t_vort * 1e3, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_tri_compose", "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+def cpu_baseline(mesh, sim, q, depth, nsample):
+    """The oracle's composition (scipy find_simplex + NumPy PS evaluation,
+    single thread) on the first `nsample` grid points with the same submaps."""
+    import numpy as np
+    from oracle import meshes as om, splines as osp
+    t0 = time.perf_counter()
+    opts = None if mesh.ntri <= 20480 else "QJ"
+    oms = om.SphericalTriangulation(mesh.vertices, qhull_options=opts)
+    vals = sim.submaps.vals[:depth].cpu().numpy()
+    levels = [osp.assemble(vals[i], oms.simplices, oms.ps_split) for i in range(depth)]
+    setup = time.perf_counter() - t0
+    qs = q[:nsample].cpu().numpy()
+    t0 = time.perf_counter()
+    ref = osp.compose_eval(oms.query, levels, qs)
+    dt = time.perf_counter() - t0
+    got = sim.submaps(q[:nsample]).cpu().numpy()
+    err = float(np.abs(got - ref).max())
+    log(f"cpu oracle: setup {setup:.1f}s (not timed), {nsample} pts in {dt:.2f}s, max |gpu-cpu| {err:.2e}")
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(nsample / dt / 1e6, 4), "unit": "Mpts/s", "cores": 1, "kind": "port",
+            "sample": f"first {nsample} points (grid order) of the same composition, depth {depth}, "
+                      f"scipy find_simplex + NumPy, single thread",
+            "host_cpus": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": cpu_model,
+            "max_abs_diff_vs_gpu": err}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,59 @@
+"""Device plumbing: torch owns HBM buffers and supplies the HIP stream; all
+arithmetic happens in libcmm_s2_amd.so."""
+import numpy as np
+import torch
+
+from .. import _native as N
+
+F64 = torch.float64
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise N.CmmError("cmm_s2 needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def is_numpy(x):
+    return isinstance(x, np.ndarray)
+
+
+def to_dev(x, dtype=F64):
+    """Contiguous device tensor of dtype (copies host arrays once)."""
+    dev = device()
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=dev, dtype=dtype)
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(x), dtype=np.float64 if dtype == F64 else None)).to(dev, dtype)
+    return t.contiguous()
+
+
+def like(t, ref):
+    """Return t as the caller's array type (numpy in -> numpy out)."""
+    if is_numpy(ref):
+        return t.detach().cpu().numpy()
+    return t
+
+
+def empty(*shape, dtype=F64):
+    return torch.empty(*shape, dtype=dtype, device=device())
+
+
+def zeros(*shape, dtype=F64):
+    return torch.zeros(*shape, dtype=dtype, device=device())
+
+
+def ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def point_strides(t, layout):
+    """(stride between points, stride between components) in elements for a
+    2-D tensor of points in layout 'N3' ((N,3)) or '3N' ((3,N))."""
+    if layout == "N3":
+        return t.stride(0), t.stride(1)
+    return t.stride(1), t.stride(0)

@@ -1,0 +1,161 @@
+"""Barotropic vorticity CMM driver (cmm_s2/core/spherical_simulations.py API).
+
+``euler_simulation_sphere_remapping`` (spherical_simulations.py:144-297)
+keeps its signature and returns the composite submap object.  The loop is
+device resident: per step one fused map update (RK4 integration of vertex and
+stencil feet, stencil evaluation + jet projection, assembly), one fused
+composition of the current map and all submaps on the MWSS sample grid, the
+vorticity sampling, one forward transform, the spectral operators, one
+batched 12-field inverse transform and the velocity assembly -- no host round
+trip.  ``BarotropicCMM`` exposes the same loop step by step (benchmarks).
+"""
+import numpy as np
+import torch
+
+from .. import _native as N
+from . import _backend as B
+from . import evolution_functions as evol
+from .mesh_functions import structure_spherical_triangulation
+from .spherical_harmonic_tools import SHT, nlm, project_stream_lm
+from .spherical_spline import (composite_sphere_diffeomorphism, sphere_diffeomorphism,
+                               spline_interp_velocity, spherical_spline)
+from . import utils
+
+
+def identity_jets(mesh):
+    """spherical_simulations.py:165-169 on the device."""
+    v = mesh.d_vertices.t()
+    jets = B.empty(3, 4, mesh.nv)
+    jets[:, 0, :] = v
+    jets[:, 1:, :] = torch.stack([utils.grad_x(v), utils.grad_y(v), utils.grad_z(v)])
+    return jets
+
+
+class VorticitySampler(object):
+    """w0(X) + 2 rot (cos theta(X) - cos theta_grid) on the device
+    (spherical_simulations.py:268-272)."""
+
+    def __init__(self, vorticity, grid_mesh, rot_rate):
+        self.grid = grid_mesh
+        self.rot = float(rot_rate)
+        if isinstance(vorticity, spherical_spline):
+            if not hasattr(vorticity.mesh, "msimplices"):
+                raise N.CmmError("a spline vorticity must live on an MWSS mesh")
+            self.kind, self.wmesh, self.wpacked = N.VORT_SPLINE, vorticity.mesh._h, vorticity.packed
+            self.params = [0.0] * 4
+            self._keep = vorticity
+        elif hasattr(vorticity, "_cmm_kind") and hasattr(vorticity, "sample_xyz"):
+            self.kind, self.wmesh, self.wpacked = vorticity._cmm_kind, None, None
+            self.params = list(vorticity._cmm_params)
+        else:
+            raise N.CmmError("vorticity must be a spherical_spline on an MWSS mesh or a "
+                             "cmm_s2.core.dynamical_fields vorticity")
+
+    def __call__(self, X, out=None, rot=None, cos_grid=None, ring_len=None):
+        n = X.shape[0]
+        if out is None:
+            out = B.empty(n)
+        cg = self.grid.d_cos_theta if cos_grid is None else cos_grid
+        rl = 2 * self.grid.L if ring_len is None else ring_len
+        N.call("cmm_sample_vorticity", self.kind, self.wmesh, B.ptr(self.wpacked),
+               N.dbl_array(self.params), X.data_ptr(), n, X.stride(0), X.stride(1), cg.data_ptr(),
+               int(rl), float(self.rot if rot is None else rot), out.data_ptr(), B.stream())
+        return out
+
+
+class BarotropicCMM(object):
+    """State of euler_simulation_sphere_remapping (spherical_simulations.py:144-297)."""
+
+    def __init__(self, L, Nt, T, mesh, vorticity, rot_rate=2 * np.pi, n_maps=10):
+        self.L, self.Nt, self.T, self.mesh = L, Nt, T, mesh
+        self.rot = rot_rate
+        self.ns = [i for i in range(1, Nt) if i % n_maps == 0] + [Nt]
+        jet0 = identity_jets(mesh)
+        vals = B.zeros(len(self.ns), 3, 4, mesh.nv)
+        vals[0] = jet0
+        self.map0 = sphere_diffeomorphism(mesh, jet0.clone())
+        self.submaps = composite_sphere_diffeomorphism(mesh, vals, None, self.ns)
+        self.tspan = np.linspace(0, T, Nt, endpoint=False)
+        self.dt = self.tspan[1] - self.tspan[0]
+        self.mesh_u = structure_spherical_triangulation(L)
+        self.U = spline_interp_velocity(self.mesh_u, B.zeros(3, 3, 4, self.mesh_u.nv), None,
+                                        ts=self.tspan[0:3])
+        self.plan = SHT.get(L)
+        self.sampler = VorticitySampler(vorticity, self.mesh_u, rot_rate)
+        self.samples = self.mesh_u.sample_points()
+        n = self.samples.shape[0]
+        self.X = B.empty(n, 3)
+        self.w = B.empty(1, L + 1, 2 * L)
+        self.w_lm = B.empty(nlm(L), 1, 2)
+        self.ops = B.empty(nlm(L), 12, 2)
+        self.fields = B.empty(12, L + 1, 2 * L)
+        self.curr = self.map0
+        self.identity = True
+        self.count = 0
+
+    def _velocity_from(self, X, level, rot=None):
+        self.sampler(X, out=self.w.reshape(-1), rot=rot)
+        self.plan.forward(self.w, out=self.w_lm)
+        project_stream_lm(self.plan, self.w_lm, self.U, level, apply_dinv=True, fields=self.fields,
+                          ops=self.ops)
+
+    def bootstrap(self):
+        """spherical_simulations.py:216-261."""
+        U, dt = self.U, self.dt
+        self._velocity_from(self.samples, 0, rot=0.0)
+        int0 = evol.advect_project_sphere(self.map0, evol.Euler_step_proj, 0, dt, U, identity=True)
+        int0.eval(self.samples, "N3", out=self.X)
+        self._velocity_from(self.X, 1)
+        U.nVs = 2
+        int1 = evol.advect_project_sphere(self.map0, evol.improved_Euler_proj, 0, dt, U, identity=True)
+        int1.eval(self.samples, "N3", out=self.X)
+        self._velocity_from(self.X, 1)
+        int2 = evol.advect_project_sphere(int1, evol.RK4_proj, dt, dt, U)
+        int2.eval(self.samples, "N3", out=self.X)
+        U.nVs = 3
+        self._velocity_from(self.X, 2)
+        self.curr = int2
+        U.stepping = True
+        self.identity = False
+        self.count = 2
+
+    def step(self):
+        """One loop iteration (spherical_simulations.py:263-294).  Returns True
+        when this step ended with a remap."""
+        t = self.tspan[self.count]
+        self.curr = evol.advect_project_sphere(self.curr, evol.RK4_proj, t, self.dt, self.U,
+                                               identity=self.identity)
+        self.submaps.compose(self.samples, "N3", out=self.X, pre=[self.curr.packed])
+        self._velocity_from(self.X, 2)
+        self.U.ts = self.U.ts + self.dt
+        self.count += 1
+        self.identity = False
+        if self.count in self.ns:
+            idx = self.ns.index(self.count)
+            self.submaps.Nc = idx + 1
+            self.submaps.assemble_coefficients(idx, self.curr.vals)
+            self.curr = self.map0
+            self.identity = True
+            return True
+        return False
+
+    def run(self):
+        self.bootstrap()
+        while self.count < self.Nt:
+            self.step()
+        return self.submaps
+
+    def vorticity_on_grid(self, L_out, out=None):
+        """Relative vorticity w0(X) + 2 rot (cos theta(X) - cos theta) of the
+        composed map on the MWSS grid of bandlimit L_out (the upsampling of
+        figures/data_gen_experiments.py:122-144)."""
+        g = structure_spherical_triangulation(L_out)
+        q = g.sample_points()
+        X = self.submaps.compose(q, "N3", pre=[] if self.identity else [self.curr.packed])
+        return self.sampler(X, out=out, cos_grid=g.d_cos_theta, ring_len=2 * L_out).reshape(L_out + 1, 2 * L_out)
+
+
+def euler_simulation_sphere_remapping(L, Nt, T, mesh, vorticity, rot_rate=2 * np.pi, n_maps=10):
+    """spherical_simulations.py:144-297: returns the composite submap object."""
+    sim = BarotropicCMM(L, Nt, T, mesh, vorticity, rot_rate, n_maps)
+    return sim.run()

@@ -1,0 +1,341 @@
+"""Powell-Sabin split-spline interpolants (cmm_s2/core/spherical_spline.py API).
+
+Classes keep the reference's names, constructor arguments and array layouts:
+jets ``vals`` (.., 4, Nv) with the transposed-Jacobian storage of vector
+fields, Bernstein-Bezier ``coeffs`` (.., 19, Ntri).  Device state adds a
+``packed`` evaluation layout (Ntri, 6, ncomp, 6): for every split
+sub-triangle the 6 coefficients the evaluation reads, contiguous (144 bytes
+for a 3-component map), so one query reads one block.  ``coeffs`` is filled
+only when the caller passes a buffer (as the reference's caller-allocated
+arrays) or reads the property.
+"""
+import numpy as np
+import torch
+
+from .. import _native as N
+from . import _backend as B
+from . import utils
+
+# spherical_spline.py:660-666 (kept for callers; the kernels use the same tables)
+Cs = np.array([[[0, 0, 0, 0, 0, 0], [18, 1, 13, 7, 6, 16], [13, 2, 18, 11, 10, 16]],
+               [[18, 14, 0, 17, 5, 4], [0, 0, 0, 0, 0, 0], [14, 18, 2, 17, 10, 9]],
+               [[0, 12, 18, 3, 15, 4], [1, 18, 12, 7, 15, 8], [0, 0, 0, 0, 0, 0]]])
+edges_ps = np.array([[[0, 0, 0], [3, 1, 5], [5, 2, 3]],
+                     [[3, 6, 0], [0, 0, 0], [6, 3, 2]],
+                     [[0, 4, 3], [1, 3, 4], [0, 0, 0]]])
+
+
+def bary_minmax(X):
+    """spherical_spline.py:691-697."""
+    return np.argmin(X, axis=1), np.argmax(X, axis=1)
+
+
+def new_vpoints(v_pts, bmm):
+    """spherical_spline.py:668-689 (host helper)."""
+    v1, v2, v3 = v_pts[:, 0, :], v_pts[:, 1, :], v_pts[:, 2, :]
+    nrm = lambda a: a / np.linalg.norm(a, axis=1)[:, None]
+    EE = np.array([v1, v2, v3, nrm((v1 + v2 + v3) / 3), nrm(v1 / 2 + v2 / 2), nrm(v2 / 2 + v3 / 2),
+                   nrm(v3 / 2 + v1 / 2)])
+    Js = edges_ps[bmm[0], bmm[1]]
+    r = np.arange(len(Js))
+    return np.stack([EE[Js[:, 0], r], EE[Js[:, 1], r], EE[Js[:, 2], r]], 1)
+
+
+def _is_mwss(mesh):
+    return hasattr(mesh, "msimplices") and hasattr(mesh, "grid_points")
+
+
+def _assemble(mesh, vals, ncomp, coeffs, packed):
+    fn = "cmm_mwss_assemble" if _is_mwss(mesh) else "cmm_tri_assemble"
+    N.call(fn, mesh._h, vals.data_ptr(), ncomp, B.ptr(coeffs), B.ptr(packed), B.stream())
+
+
+def _canonical_out(coeffs, shape):
+    """Device buffer for a caller-provided canonical coefficient array."""
+    if coeffs is None:
+        return None, None
+    if isinstance(coeffs, torch.Tensor) and coeffs.is_cuda and coeffs.dtype == torch.float64 \
+            and coeffs.is_contiguous() and tuple(coeffs.shape) == tuple(shape):
+        return coeffs, None
+    return B.empty(*shape), coeffs
+
+
+def _write_back(dev, host):
+    if host is None:
+        return
+    if isinstance(host, np.ndarray):
+        host[...] = dev.detach().cpu().numpy().reshape(host.shape)
+    else:
+        host.copy_(dev.reshape(host.shape))
+
+
+class sphere_diffeomorphism(object):
+    """Spline map of the sphere (spherical_spline.py:155-337).  vals (3,4,Nv);
+    coeffs: optional (3,19,Ntri) buffer written in place; packed: optional
+    device buffer (Ntri,108) to reuse (spline_proj_sphere hands the old map's
+    buffer to the new one, evolution_functions.py:113)."""
+
+    def __init__(self, mesh, vals, coeffs=None, packed=None):
+        self.mesh = mesh
+        self.vals = B.to_dev(vals)
+        self.packed = packed if packed is not None else B.empty(mesh.ntri, 108)
+        self._coeffs = coeffs
+        self.assemble_coefficients(coeffs)
+
+    def assemble_coefficients(self, coeffs=None):
+        dev, host = _canonical_out(coeffs, (3, 19, self.mesh.ntri))
+        _assemble(self.mesh, self.vals, 3, dev, self.packed)
+        if dev is not None:
+            _write_back(dev, host)
+            self._coeffs = coeffs if host is not None else dev
+        return self._coeffs
+
+    @property
+    def coeffs(self):
+        if self._coeffs is None:
+            c = B.empty(3, 19, self.mesh.ntri)
+            _assemble(self.mesh, self.vals, 3, c, None)
+            self._coeffs = c
+        return self._coeffs
+
+    def eval(self, q, layout="N3", out=None, normalize=True):
+        """Device evaluation; returns (N,3) for layout 'N3' or (3,N) for '3N'."""
+        n = q.shape[0] if layout == "N3" else q.shape[1]
+        if out is None:
+            out = B.empty(n, 3) if layout == "N3" else B.empty(3, n)
+        sp, sc = B.point_strides(q, layout)
+        osp, osc = B.point_strides(out, layout)
+        N.call("cmm_tri_eval", self.mesh._h, self.packed.data_ptr(), 3, int(normalize), q.data_ptr(), n,
+               sp, sc, out.data_ptr(), osp, osc, B.stream())
+        return out
+
+    def __call__(self, q_pts):
+        """spherical_spline.py:178-205: q (N,3) -> normalised (3,N)."""
+        q = B.to_dev(q_pts)
+        n = q.shape[0]
+        out = B.empty(3, n)
+        sp, sc = B.point_strides(q, "N3")
+        N.call("cmm_tri_eval", self.mesh._h, self.packed.data_ptr(), 3, 1, q.data_ptr(), n, sp, sc,
+               out.data_ptr(), 1, n, B.stream())
+        return B.like(out, q_pts)
+
+    def stencil_eval(self, q_pts, st_pts):
+        """spherical_spline.py:268-298: locate the vertex feet q (3,Nv), evaluate
+        the four stencil feet st (3,4,Nv) on that split sub-triangle's
+        polynomial (not normalised); st is overwritten and returned."""
+        q = B.to_dev(q_pts)
+        st = B.to_dev(st_pts)
+        nv = self.mesh.nv
+        jets = B.empty(3, 4, nv)
+        ev = B.empty(3, 4, nv)
+        N.call("cmm_tri_stencil_jets", self.mesh._h, self.packed.data_ptr(), q.data_ptr(), q.stride(0),
+               st.data_ptr(), st.stride(0), self.mesh.d_tan.data_ptr(), 1e-5, jets.data_ptr(),
+               ev.data_ptr(), B.stream())
+        if isinstance(st_pts, np.ndarray):
+            st_pts[...] = ev.cpu().numpy()
+            return st_pts
+        if isinstance(st_pts, torch.Tensor) and st_pts.is_cuda and st_pts.dtype == torch.float64:
+            st_pts.copy_(ev)
+            return st_pts
+        return ev
+
+
+class composite_sphere_diffeomorphism(object):
+    """Chain of submaps (spherical_spline.py:340-508).  vals (n,3,4,Nv);
+    coeffs optional (n,3,19,Ntri); ns the remap schedule; Nc filled count."""
+
+    def __init__(self, mesh, vals, coeffs, ns):
+        self.mesh = mesh
+        self.vals = B.to_dev(vals)
+        self.ns = ns
+        self._coeffs = coeffs
+        n = self.vals.shape[0]
+        self.packed = B.empty(n, mesh.ntri, 108)
+        self.Nc = 0
+
+    @property
+    def coeffs(self):
+        if self._coeffs is None:
+            n = self.vals.shape[0]
+            c = B.zeros(n, 3, 19, self.mesh.ntri)
+            for i in range(self.Nc):
+                _assemble(self.mesh, self.vals[i], 3, c[i], None)
+            self._coeffs = c
+        return self._coeffs
+
+    def assemble_coefficients(self, i, vals):
+        """spherical_spline.py:471-508."""
+        self.vals[i] = B.to_dev(vals)
+        dev = None
+        if self._coeffs is not None:
+            dev, host = _canonical_out(self._coeffs[i], (3, 19, self.mesh.ntri))
+        _assemble(self.mesh, self.vals[i], 3, dev, self.packed[i])
+        if dev is not None:
+            _write_back(dev, host)
+
+    def levels(self, N_c=None):
+        """Packed device levels in application order (N_c-1 first)."""
+        if N_c is None:
+            N_c = self.Nc
+        return [self.packed[i] for i in reversed(range(N_c))]
+
+    def compose(self, q, layout="N3", out=None, pre=(), N_c=None, inter=None):
+        """Fused device composition: apply maps ``pre`` (packed buffers), then
+        submaps N_c-1 .. 0."""
+        lv = list(pre) + self.levels(N_c)
+        return compose_packed(self.mesh, lv, q, layout, out, inter)
+
+    def __call__(self, q_pts, N_c=None):
+        """spherical_spline.py:365-398: q (N,3) -> (N,3)."""
+        q = B.to_dev(q_pts)
+        return B.like(self.compose(q, "N3", N_c=N_c), q_pts)
+
+    def eval_all(self, q_pts):
+        """spherical_spline.py:400-437: [q, after level Nc-1, ..., after level 0]."""
+        q = B.to_dev(q_pts)
+        n = q.shape[0]
+        inter = B.empty(max(1, self.Nc), n, 3)
+        self.compose(q, "N3", N_c=self.Nc, inter=inter)
+        outs = [q.clone()] + [inter[i] for i in range(self.Nc)]
+        return [B.like(o, q_pts) for o in outs]
+
+
+def compose_packed(mesh, levels, q, layout="N3", out=None, inter=None):
+    n = q.shape[0] if layout == "N3" else q.shape[1]
+    if out is None:
+        out = B.empty(n, 3) if layout == "N3" else B.empty(3, n)
+    sp, sc = B.point_strides(q, layout)
+    osp, osc = B.point_strides(out, layout)
+    arr = N.ptr_array([p.data_ptr() for p in levels])
+    N.call("cmm_tri_compose", mesh._h, arr, len(levels), q.data_ptr(), n, sp, sc, out.data_ptr(), osp,
+           osc, B.ptr(inter), B.stream())
+    return out
+
+
+def lagrange_weights(nVs, t, dt, ts0):
+    """Time weights of spline_interp_velocity.__call__ (spherical_spline.py:596-612)."""
+    if nVs == 3:
+        tau0, tau1, tau2 = ts0, ts0 + dt, ts0 + 2 * dt
+        return ((t - tau1) * (t - tau2) / (2 * dt ** 2), (t - tau0) * (t - tau2) / (-dt ** 2),
+                (t - tau0) * (t - tau1) / (2 * dt ** 2))
+    if nVs == 2:
+        tau = (t - ts0) / dt
+        return (1 - tau, tau, 0.0)
+    return (1.0, 0.0, 0.0)
+
+
+class spline_interp_velocity(object):
+    """2+1-D velocity spline (spherical_spline.py:545-655) on an MWSS mesh.
+    vals (3 levels, 3, 4, Nu); coeffs optional; ts the time grid.  Time levels
+    live in a 3-slot ring: ``shift()`` rotates slots instead of copying the
+    1.4 GB of level data (spherical_harmonic_tools.py:73-76 copies)."""
+
+    def __init__(self, mesh, vals, coeffs, ts):
+        self.mesh = mesh
+        self._vals = B.to_dev(vals)
+        self._packed = B.zeros(3, mesh.ntri, 108)
+        self._order = [0, 1, 2]
+        self._coeffs_buf = coeffs
+        self.ts = np.array(ts, dtype=np.float64)
+        self.stepping = False
+        self.nVs = 1
+
+    # logical level access
+    def level_vals(self, i):
+        return self._vals[self._order[i]]
+
+    def level_packed(self, i):
+        return self._packed[self._order[i]]
+
+    def shift(self):
+        """Logical levels (0,1,2) <- (1,2,stale); level 2 is rewritten next."""
+        self._order = self._order[1:] + self._order[:1]
+
+    @property
+    def vals(self):
+        return self._vals[self._order]
+
+    @property
+    def coeffs(self):
+        c = B.empty(3, 3, 19, self.mesh.ntri)
+        for i in range(3):
+            _assemble(self.mesh, self.level_vals(i), 3, c[i], None)
+        return c
+
+    def init_coeffs(self, i):
+        """spherical_spline.py:565-568."""
+        dev = None
+        if self._coeffs_buf is not None:
+            dev, host = _canonical_out(self._coeffs_buf[i], (3, 19, self.mesh.ntri))
+        _assemble(self.mesh, self.level_vals(i), 3, dev, self.level_packed(i))
+        if dev is not None:
+            _write_back(dev, host)
+
+    def weights(self, t, dt):
+        return lagrange_weights(self.nVs, t, dt, float(self.ts[0]))
+
+    def levels_used(self):
+        return min(self.nVs, 3) if self.nVs in (1, 2, 3) else 3
+
+    def eval(self, t, dt, q, layout="3N", out=None):
+        n = q.shape[0] if layout == "N3" else q.shape[1]
+        if out is None:
+            out = B.empty(n, 3) if layout == "N3" else B.empty(3, n)
+        nl = self.levels_used()
+        w = self.weights(t, dt)
+        arr = N.ptr_array([self.level_packed(i).data_ptr() for i in range(nl)])
+        warr = N.dbl_array(w[:nl])
+        sp, sc = B.point_strides(q, layout)
+        osp, osc = B.point_strides(out, layout)
+        N.call("cmm_mwss_eval", self.mesh._h, arr, warr, nl, 3, q.data_ptr(), n, sp, sc, out.data_ptr(),
+               osp, osc, B.stream())
+        return out
+
+    def __call__(self, t, dt, q_pts):
+        """spherical_spline.py:570-615: q (3,N) -> (3,N) (not normalised)."""
+        q = B.to_dev(q_pts)
+        return B.like(self.eval(t, dt, q, "3N"), q_pts)
+
+
+class spherical_spline(object):
+    """Scalar PS spline (spherical_spline.py:19-152).  vals (4,Nv): value and
+    Cartesian gradient; works on an MWSS or an unstructured mesh."""
+
+    def __init__(self, mesh, vals, coeffs=None):
+        self.mesh = mesh
+        self.vals = B.to_dev(vals)
+        self.packed = B.empty(mesh.ntri, 36)
+        self._coeffs = coeffs
+        dev, host = _canonical_out(coeffs, (19, mesh.ntri))
+        _assemble(mesh, self.vals, 1, dev, self.packed)
+        if dev is not None:
+            _write_back(dev, host)
+        self.vort = False
+
+    @property
+    def coeffs(self):
+        if self._coeffs is None:
+            c = B.empty(19, self.mesh.ntri)
+            _assemble(self.mesh, self.vals, 1, c, None)
+            self._coeffs = c
+        return self._coeffs
+
+    def eval_xyz(self, q, layout="N3"):
+        n = q.shape[0] if layout == "N3" else q.shape[1]
+        out = B.empty(n)
+        sp, sc = B.point_strides(q, layout)
+        if _is_mwss(self.mesh):
+            arr = N.ptr_array([self.packed.data_ptr()])
+            N.call("cmm_mwss_eval", self.mesh._h, arr, N.dbl_array([1.0]), 1, 1, q.data_ptr(), n, sp, sc,
+                   out.data_ptr(), 1, 1, B.stream())
+        else:
+            N.call("cmm_tri_eval", self.mesh._h, self.packed.data_ptr(), 1, 0, q.data_ptr(), n, sp, sc,
+                   out.data_ptr(), 1, 1, B.stream())
+        return out
+
+    def __call__(self, phi, the):
+        """spherical_spline.py:42-70: 2-D (phi, theta) arrays -> 2-D values."""
+        p, t = B.to_dev(phi), B.to_dev(the)
+        shape = p.shape
+        q = torch.stack(utils.sphere2cart(p.reshape(-1), t.reshape(-1)), 1).contiguous()
+        return B.like(self.eval_xyz(q).reshape(shape), phi)

@@ -1,0 +1,608 @@
+// MWSS structured velocity mesh: closed-form location with numpy semantics,
+// Powell-Sabin assembly / multi-level evaluation, the backward
+// characteristic integrators (fused RK4 over the time-interpolated velocity
+// spline), project_stream's grid post-processing and vorticity sampling.
+//
+// The mesh is implicit: vertex (ring r, column c) = (sin t_r cos p_c,
+// sin t_r sin p_c, cos t_r) from the four small sine / cosine tables, so
+// location and geometry cost no HBM traffic; only the jets (assembly) and
+// the packed coefficients (evaluation, 6 per component per level, one
+// 144-byte block) are read from HBM.
+#include <cmath>
+
+#include "common.h"
+
+using namespace cmm;
+
+struct cmm_mwss_mesh {
+  int L = 0, N = 0, M = 0;  // N = 2L longitudes, M = L+1 colatitudes
+  int64_t Nu = 0, Tu = 0;
+  double dphi = 0, dthe = 0, phi0 = 0, the0 = 0;
+  double* theta = nullptr;   // (M)
+  double* st = nullptr;      // sin theta (M)
+  double* ct = nullptr;      // cos theta (M)
+  double* sp = nullptr;      // sin phi (N)
+  double* cp = nullptr;      // cos phi (N)
+};
+
+extern "C" int cmm_mwss_mesh_create(int L, const double* thetas, const double* phis,
+                                    const double* sin_t, const double* cos_t, const double* sin_p,
+                                    const double* cos_p, cmm_mwss_mesh** out) {
+  if (L < 2 || !thetas || !phis || !sin_t || !cos_t || !sin_p || !cos_p || !out)
+    return fail(CMM_E_ARG, "cmm_mwss_mesh_create: bad arguments");
+  if (L > 8192) return fail(CMM_E_RANGE, "cmm_mwss_mesh_create: L > 8192");
+  cmm_mwss_mesh* m = new cmm_mwss_mesh();
+  m->L = L; m->N = 2 * L; m->M = L + 1;
+  m->Nu = (int64_t)2 * L * (L - 1) + 2;
+  m->Tu = (int64_t)4 * L * (L - 1);
+  m->dphi = fabs(phis[1] - phis[0]);
+  m->dthe = fabs(thetas[1] - thetas[0]);
+  m->phi0 = phis[0];
+  m->the0 = thetas[0];
+  hipError_t e;
+  size_t bm = (size_t)m->M * 8, bn = (size_t)m->N * 8;
+  if ((e = hipMalloc(&m->theta, bm)) != hipSuccess || (e = hipMalloc(&m->st, bm)) != hipSuccess ||
+      (e = hipMalloc(&m->ct, bm)) != hipSuccess || (e = hipMalloc(&m->sp, bn)) != hipSuccess ||
+      (e = hipMalloc(&m->cp, bn)) != hipSuccess) {
+    cmm_mwss_mesh_destroy(m);
+    return hip_check(e, "cmm_mwss_mesh_create: hipMalloc");
+  }
+  if ((e = hipMemcpy(m->theta, thetas, bm, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->st, sin_t, bm, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->ct, cos_t, bm, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->sp, sin_p, bn, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->cp, cos_p, bn, hipMemcpyHostToDevice)) != hipSuccess) {
+    cmm_mwss_mesh_destroy(m);
+    return hip_check(e, "cmm_mwss_mesh_create: hipMemcpy");
+  }
+  *out = m;
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_mesh_destroy(cmm_mwss_mesh* m) {
+  if (!m) return CMM_OK;
+  (void)hipFree(m->theta);
+  (void)hipFree(m->st);
+  (void)hipFree(m->ct);
+  (void)hipFree(m->sp);
+  (void)hipFree(m->cp);
+  delete m;
+  return CMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// location (mesh_functions.py:196-282)
+// ---------------------------------------------------------------------------
+struct MLoc {
+  int64_t tri;
+  int64_t vi[3];
+  d3 v[3];
+  d3 b;
+};
+
+__device__ inline d3 ring_pt(const cmm_mwss_mesh& M, int r, int c) {
+  return d3{M.st[r] * M.cp[c], M.st[r] * M.sp[c], M.ct[r]};
+}
+
+// numpy-faithful decisions: no FMA contraction in the tests that pick cells
+__device__ __noinline__ bool mwss_locate(const cmm_mwss_mesh& M, d3 q, MLoc& o) {
+#pragma clang fp contract(off)
+  const double TWO_PI = 2 * M_PI;
+  double phi = npy_mod(atan2(q.y, q.x) + TWO_PI, TWO_PI);
+  double theta = atan2(sqrt(q.y * q.y + q.x * q.x), q.z);
+  if (!isfinite(phi) || !isfinite(theta)) return false;
+  double phin = npy_mod(phi, TWO_PI);
+  int N = M.N, Mr = M.M;
+  long long plr = (long long)npy_floordiv(phin - M.phi0, M.dphi);
+  int pl = (int)(((plr % N) + N) % N);
+  int tl = (int)npy_floordiv(theta - M.the0, M.dthe);
+  if (tl == Mr - 1) tl = Mr - 2;
+  if (tl < 0 || tl > Mr - 2) return false;
+  double thc = (theta - M.theta[tl]) / M.dthe;
+  int pp1 = (pl + 1) % N;
+  bool in0 = (thc <= 0.03) && (tl != 0);
+  bool in1 = (thc > 0.85) && (tl != Mr - 2);
+  if (in0 || in1) {
+    double sth = sin(theta), cth = cos(theta);
+    d3 qq = d3{sth * cos(phin), sth * sin(phin), cth};
+    if (in0) {
+      d3 n = cross(ring_pt(M, tl, pl), ring_pt(M, tl, pp1));
+      double dd = n.x * qq.x + n.y * qq.y + n.z * qq.z;
+      if (dd > 0) tl -= 1;
+    } else {
+      d3 n = cross(ring_pt(M, tl + 1, pl), ring_pt(M, tl + 1, pp1));
+      double dd = n.x * qq.x + n.y * qq.y + n.z * qq.z;
+      if (-dd > 0) tl += 1;
+    }
+  }
+  d3 a = ring_pt(M, tl, pl), b = ring_pt(M, tl + 1, pp1);
+  d3 n = cross(a, b);
+  double dd = n.x * q.x + n.y * q.y + n.z * q.z;
+  int s = (-dd > 0) ? 1 : 0;
+  const int64_t Nu = M.Nu;
+  auto R = [&](int r, int c) -> int64_t { return 1 + (int64_t)(r - 1) * N + c; };
+  if (tl == 0) {
+    o.tri = pl;
+    o.vi[0] = 0; o.vi[1] = R(1, pl); o.vi[2] = R(1, pp1);
+    o.v[0] = ring_pt(M, 0, 0); o.v[1] = ring_pt(M, 1, pl); o.v[2] = ring_pt(M, 1, pp1);
+  } else if (tl == Mr - 2) {
+    o.tri = M.Tu - N + pl;
+    o.vi[0] = R(Mr - 2, pl); o.vi[1] = Nu - 1; o.vi[2] = R(Mr - 2, pp1);
+    o.v[0] = ring_pt(M, Mr - 2, pl); o.v[1] = ring_pt(M, Mr - 1, 0); o.v[2] = ring_pt(M, Mr - 2, pp1);
+  } else {
+    o.tri = N + 2 * ((int64_t)(tl - 1) * N + pl) + s;
+    if (s == 0) {
+      o.vi[0] = R(tl, pl); o.vi[1] = R(tl + 1, pp1); o.vi[2] = R(tl, pp1);
+      o.v[0] = ring_pt(M, tl, pl); o.v[1] = ring_pt(M, tl + 1, pp1); o.v[2] = ring_pt(M, tl, pp1);
+    } else {
+      o.vi[0] = R(tl, pl); o.vi[1] = R(tl + 1, pl); o.vi[2] = R(tl + 1, pp1);
+      o.v[0] = ring_pt(M, tl, pl); o.v[1] = ring_pt(M, tl + 1, pl); o.v[2] = ring_pt(M, tl + 1, pp1);
+    }
+  }
+  o.b = bary(o.v[0], o.v[1], o.v[2], q);
+  return true;
+}
+
+// vertices of triangle t (closed form of grid_assemble, mesh_functions.py:380-412)
+__device__ inline void mwss_tri_vertices(const cmm_mwss_mesh& M, int64_t t, int64_t vi[3], d3 v[3]) {
+  int N = M.N, Mr = M.M;
+  auto R = [&](int r, int c) -> int64_t { return 1 + (int64_t)(r - 1) * N + c; };
+  if (t < N) {
+    int c = (int)t, c1 = (c + 1) % N;
+    vi[0] = 0; vi[1] = R(1, c); vi[2] = R(1, c1);
+    v[0] = ring_pt(M, 0, 0); v[1] = ring_pt(M, 1, c); v[2] = ring_pt(M, 1, c1);
+  } else if (t >= M.Tu - N) {
+    int c = (int)(t - (M.Tu - N)), c1 = (c + 1) % N;
+    vi[0] = R(Mr - 2, c); vi[1] = M.Nu - 1; vi[2] = R(Mr - 2, c1);
+    v[0] = ring_pt(M, Mr - 2, c); v[1] = ring_pt(M, Mr - 1, 0); v[2] = ring_pt(M, Mr - 2, c1);
+  } else {
+    int64_t u = t - N;
+    int h = (int)(u % 2);
+    int64_t k = u / 2;
+    int r = 1 + (int)(k / N), c = (int)(k % N), c1 = (c + 1) % N;
+    if (h == 0) {
+      vi[0] = R(r, c); vi[1] = R(r + 1, c1); vi[2] = R(r, c1);
+      v[0] = ring_pt(M, r, c); v[1] = ring_pt(M, r + 1, c1); v[2] = ring_pt(M, r, c1);
+    } else {
+      vi[0] = R(r, c); vi[1] = R(r + 1, c); vi[2] = R(r + 1, c1);
+      v[0] = ring_pt(M, r, c); v[1] = ring_pt(M, r + 1, c); v[2] = ring_pt(M, r + 1, c1);
+    }
+  }
+}
+
+// sub-triangle barycentrics from the macro ones, norms from the vertices
+__device__ inline d3 sub_bary_v(int s, d3 b, const d3 v[3]) {
+  d3 s3 = add(add(v[0], v[1]), v[2]);
+  double s4 = norm3(s3);
+  double se;
+  if (s <= 1) se = norm3(add(v[1], v[2]));
+  else if (s <= 3) se = norm3(add(v[2], v[0]));
+  else se = norm3(add(v[0], v[1]));
+  return sub_bary(s, b.x, b.y, b.z, se, se, se, s4);
+}
+
+struct LevelSet {
+  const double* p[3];
+  double w[3];
+  int nlev;
+};
+
+// sum_k w_k * PS_k(q) for ncomp components
+__device__ inline bool mwss_eval_levels(const cmm_mwss_mesh& M, const LevelSet& ls, int ncomp, d3 q,
+                                        double* r) {
+  MLoc o;
+  if (!mwss_locate(M, q, o)) return false;
+  int s = split_index(o.b.x, o.b.y, o.b.z);
+  d3 cb = sub_bary_v(s, o.b, o.v);
+  for (int c = 0; c < ncomp; ++c) r[c] = 0;
+  for (int k = 0; k < ls.nlev; ++k) {
+    const double* p = ls.p[k] + (o.tri * 6 + s) * 6 * ncomp;
+    double w = ls.w[k];
+    for (int c = 0; c < ncomp; ++c) r[c] += w * bb6(cb, p + 6 * c);
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// analytic velocity fields (dynamical_fields.py:18-42) and RH-4
+// ---------------------------------------------------------------------------
+__device__ inline d3 rot_apply(const double R[3][3], d3 x) {
+  return d3{R[0][0] * x.x + R[0][1] * x.y + R[0][2] * x.z, R[1][0] * x.x + R[1][1] * x.y + R[1][2] * x.z,
+            R[2][0] * x.x + R[2][1] * x.y + R[2][2] * x.z};
+}
+
+__device__ inline void matmul3(const double A[3][3], const double B[3][3], double C[3][3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C[i][j] = A[i][0] * B[0][j] + A[i][1] * B[1][j] + A[i][2] * B[2][j];
+}
+
+__device__ inline d3 vel_sbr(d3 X, double T, double alpha) {
+  double c = -2 * M_PI / T;
+  return cross(X, d3{c * sin(alpha), 0.0, c * cos(alpha)});
+}
+
+__device__ inline d3 vel_deform_rot(double t, d3 X, double alpha) {
+  double tau = 2 * M_PI * t;
+  double ct = cos(tau), st = sin(tau), ca = cos(alpha), sa = sin(alpha);
+  double Rz[3][3] = {{ct, -st, 0}, {st, ct, 0}, {0, 0, 1}};
+  double RzT[3][3] = {{ct, st, 0}, {-st, ct, 0}, {0, 0, 1}};
+  double Ry[3][3] = {{ca, 0, -sa}, {0, 1, 0}, {sa, 0, ca}};
+  double RyT[3][3] = {{ca, 0, sa}, {0, 1, 0}, {-sa, 0, ca}};
+  double A[3][3], B[3][3];
+  matmul3(RzT, Ry, A);
+  matmul3(RyT, Rz, B);
+  d3 Xr = rot_apply(A, X);
+  double c1 = 2 * 2 * cos(M_PI * t / 1.0);
+  d3 us = cross(Xr, d3{0.0, c1 * Xr.y, 0.0});
+  d3 usr = rot_apply(B, us);
+  return add(usr, vel_sbr(X, 1.0, 0.0));
+}
+
+__device__ inline d3 vel_rh4(double t, d3 X, double w, double K, double Om) {
+  const double R = 4;
+  double nu = (R * (3 + R) * w - 2 * Om) / ((1 + R) * (2 + R));
+  double c = cos(4 * nu * t), s = sin(4 * nu * t);
+  double x = X.x, y = X.y, z = X.z, x2 = x * x, y2 = y * y;
+  double re4 = x2 * x2 - 6 * x2 * y2 + y2 * y2;
+  double im4 = 4 * x2 * x * y - 4 * x * y2 * y;
+  double A = c * re4 + s * im4;
+  double px = K * z * (c * (4 * x2 * x - 12 * x * y2) + s * (12 * x2 * y - 4 * y2 * y));
+  double py = K * z * (c * (-12 * x2 * y + 4 * y2 * y) + s * (4 * x2 * x - 12 * x * y2));
+  double pz = -w + K * A;
+  return d3{y * pz - z * py, z * px - x * pz, x * py - y * px};
+}
+
+struct VelDev {
+  int kind;
+  cmm_mwss_mesh mesh;
+  LevelSet ls[4];
+  double stage_t[4];
+  double params[8];
+};
+
+__device__ inline d3 vel_eval(const VelDev& V, int stage, d3 x) {
+  switch (V.kind) {
+    case CMM_VEL_SPLINE: {
+      double r[3];
+      if (!mwss_eval_levels(V.mesh, V.ls[stage], 3, x, r)) {
+        double nan = __builtin_nan("");
+        return d3{nan, nan, nan};
+      }
+      return d3{r[0], r[1], r[2]};
+    }
+    case CMM_VEL_SBR: return vel_sbr(x, V.params[0], V.params[1]);
+    case CMM_VEL_DEFORM_ROT: return vel_deform_rot(V.stage_t[stage], x, V.params[1]);
+    default: return vel_rh4(V.stage_t[stage], x, V.params[0], V.params[1], V.params[2]);
+  }
+}
+
+// evolution_functions.div_norm (82-85): x * (1 / |x|)
+__device__ inline d3 proj_norm(d3 x) {
+  double inv = 1 / sqrt(x.x * x.x + x.y * x.y + x.z * x.z);
+  return d3{inv * x.x, inv * x.y, inv * x.z};
+}
+
+__global__ void k_advect(VelDev V, int integ, double dt, const double* __restrict__ x, int64_t m,
+                         double* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  d3 xn = d3{x[i], x[m + i], x[2 * m + i]};
+  d3 out;
+  if (integ == CMM_INT_EVAL) {
+    out = vel_eval(V, 0, xn);
+  } else if (integ == CMM_INT_EULER) {
+    d3 v1 = vel_eval(V, 0, xn);
+    out = proj_norm(sub(xn, scl(dt, v1)));
+  } else if (integ == CMM_INT_IMPROVED_EULER) {
+    d3 v1 = vel_eval(V, 0, xn);
+    d3 s1 = proj_norm(sub(xn, scl(dt, v1)));
+    d3 v2 = vel_eval(V, 1, s1);
+    out = proj_norm(sub(xn, scl(dt / 2, add(v1, v2))));
+  } else if (integ == CMM_INT_RK2) {
+    d3 v1 = vel_eval(V, 0, xn);
+    d3 s1 = proj_norm(sub(xn, scl(dt / 2, v1)));
+    d3 v2 = vel_eval(V, 1, s1);
+    out = proj_norm(sub(xn, scl(dt, v2)));
+  } else {
+    d3 v1 = vel_eval(V, 0, xn);
+    d3 s1 = proj_norm(d3{xn.x - dt * v1.x / 2, xn.y - dt * v1.y / 2, xn.z - dt * v1.z / 2});
+    d3 v2 = vel_eval(V, 1, s1);
+    d3 s2 = proj_norm(sub(xn, scl(dt / 2, v2)));
+    d3 v3 = vel_eval(V, 2, s2);
+    d3 s3 = proj_norm(sub(xn, scl(dt, v3)));
+    d3 v4 = vel_eval(V, 3, s3);
+    d3 acc = d3{v1.x + 2 * v2.x + 2 * v3.x + v4.x, v1.y + 2 * v2.y + 2 * v3.y + v4.y,
+                v1.z + 2 * v2.z + 2 * v3.z + v4.z};
+    out = proj_norm(sub(xn, scl(dt / 6, acc)));
+  }
+  y[i] = out.x;
+  y[m + i] = out.y;
+  y[2 * m + i] = out.z;
+}
+
+__global__ void k_mwss_locate(cmm_mwss_mesh M, const double* __restrict__ q, int64_t n, int64_t sp,
+                              int64_t sc, int32_t* __restrict__ tri, double* __restrict__ bcc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  MLoc o;
+  bool ok = mwss_locate(M, load_pt(q, i, sp, sc), o);
+  tri[i] = ok ? (int32_t)o.tri : -1;
+  if (bcc) {
+    double nan = __builtin_nan("");
+    bcc[3 * i] = ok ? o.b.x : nan;
+    bcc[3 * i + 1] = ok ? o.b.y : nan;
+    bcc[3 * i + 2] = ok ? o.b.z : nan;
+  }
+}
+
+__global__ void k_mwss_eval(cmm_mwss_mesh M, LevelSet ls, int ncomp, const double* __restrict__ q,
+                            int64_t n, int64_t sp, int64_t sc, double* __restrict__ out, int64_t osp,
+                            int64_t osc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r[3];
+  if (!mwss_eval_levels(M, ls, ncomp, load_pt(q, i, sp, sc), r)) {
+    for (int c = 0; c < ncomp; ++c) r[c] = __builtin_nan("");
+  }
+  for (int c = 0; c < ncomp; ++c) out[i * osp + c * osc] = r[c];
+}
+
+__global__ void k_mwss_assemble(cmm_mwss_mesh M, const double* __restrict__ vals, int ncomp,
+                                double* __restrict__ coeffs, double* __restrict__ packed) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M.Tu) return;
+  int64_t vi[3];
+  d3 v[3];
+  mwss_tri_vertices(M, t, vi, v);
+  PSGeom P;
+  ps_geometry(v[0], v[1], v[2], P);
+  const int64_t Nu = M.Nu;
+  for (int c = 0; c < ncomp; ++c) {
+    double f[3];
+    d3 gr[3];
+    for (int i = 0; i < 3; ++i) {
+      if (ncomp == 1) {
+        f[i] = vals[vi[i]];
+        gr[i] = d3{vals[Nu + vi[i]], vals[2 * Nu + vi[i]], vals[3 * Nu + vi[i]]};
+      } else {
+        f[i] = vals[(c * 4) * Nu + vi[i]];
+        gr[i] = d3{vals[(0 * 4 + 1 + c) * Nu + vi[i]], vals[(1 * 4 + 1 + c) * Nu + vi[i]],
+                   vals[(2 * 4 + 1 + c) * Nu + vi[i]]};
+      }
+    }
+    double cf[19];
+    ps_coeffs(P, f, gr, cf);
+    if (coeffs)
+      for (int k = 0; k < 19; ++k) coeffs[((int64_t)c * 19 + k) * M.Tu + t] = cf[k];
+    if (packed)
+      for (int s = 0; s < 6; ++s)
+        for (int j = 0; j < 6; ++j) packed[((t * 6 + s) * ncomp + c) * 6 + j] = cf[cs_table(s, j)];
+  }
+}
+
+// grid index (ring, column) of mesh vertex v: poles take the phi = 0 sample
+__device__ inline void vertex_grid(const cmm_mwss_mesh& M, int64_t v, int& t, int& p) {
+  if (v == 0) { t = 0; p = 0; }
+  else if (v == M.Nu - 1) { t = M.L; p = 0; }
+  else { t = 1 + (int)((v - 1) / M.N); p = (int)((v - 1) % M.N); }
+}
+
+// spherical_harmonic_tools.tan_proj (145-152) of cross(g, s) at s
+__device__ inline d3 rot_grad(d3 g, d3 s) {
+  d3 a = cross(g, s);
+  double scl_ = (a.x * s.x + a.y * s.y + a.z * s.z) / sqrt(s.x * s.x + s.y * s.y + s.z * s.z);
+  return d3{a.x - scl_ * s.x, a.y - scl_ * s.y, a.z - scl_ * s.z};
+}
+
+__global__ void k_velocity_jets(cmm_mwss_mesh M, const double* __restrict__ F, double* __restrict__ vals) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= M.Nu) return;
+  int t, p;
+  vertex_grid(M, v, t, p);
+  const int64_t G = (int64_t)(M.L + 1) * M.N;
+  int64_t gi = (int64_t)t * M.N + p;
+  d3 s = ring_pt(M, t, p);
+  const int64_t Nu = M.Nu;
+  for (int i = 0; i < 3; ++i) {
+    vals[(i * 4) * Nu + v] = F[i * G + gi];
+    d3 g = d3{F[(3 + 3 * i) * G + gi], F[(4 + 3 * i) * G + gi], F[(5 + 3 * i) * G + gi]};
+    d3 r = rot_grad(g, s);
+    vals[(0 * 4 + 1 + i) * Nu + v] = r.x;
+    vals[(1 * 4 + 1 + i) * Nu + v] = r.y;
+    vals[(2 * 4 + 1 + i) * Nu + v] = r.z;
+  }
+}
+
+__global__ void k_scalar_jets(cmm_mwss_mesh M, const double* __restrict__ F, double* __restrict__ vals) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= M.Nu) return;
+  int t, p;
+  vertex_grid(M, v, t, p);
+  const int64_t G = (int64_t)(M.L + 1) * M.N;
+  int64_t gi = (int64_t)t * M.N + p;
+  d3 s = ring_pt(M, t, p);
+  const int64_t Nu = M.Nu;
+  vals[v] = F[gi];
+  d3 r = rot_grad(d3{F[G + gi], F[2 * G + gi], F[3 * G + gi]}, s);
+  vals[Nu + v] = r.x;
+  vals[2 * Nu + v] = r.y;
+  vals[3 * Nu + v] = r.z;
+}
+
+struct VortArgs {
+  int kind;
+  double params[4];
+  double rot;
+  int ring_len;
+};
+
+__device__ inline double vort_analytic(int kind, double phi, double theta, const double* prm) {
+  const double pi = M_PI;
+  switch (kind) {
+    case CMM_VORT_ROSSBY: {
+      double alpha = 1.0 / 30.0, Om = 2 * pi, t = prm[0];
+      double s = sin(theta);
+      return 30 * cos(theta) * cos(4 * (phi + 2 * Om * alpha * t)) * (s * s * s * s);
+    }
+    case CMM_VORT_ZONAL_JET:
+    case CMM_VORT_PERTURBED_JET:
+    case CMM_VORT_MULTI_JET: {
+      auto jet = [&](double ph, double th, bool pert) {
+        double beta2 = 144.0;
+        double thc = pi / 4 + (pert ? 0.01 * cos(12 * ph) : 0.0);
+        double ult = (pi / 2) * exp(-2 * beta2 * (1 - cos(-th + pi / 2 - thc)));
+        return (cos(-th + pi / 2) * (2 * beta2 * (cos(thc) * sin(-th + pi / 2) - sin(thc) * cos(-th + pi / 2))) +
+                sin(pi / 2 - th)) * ult;
+      };
+      if (kind == CMM_VORT_ZONAL_JET) return jet(phi, theta, false);
+      if (kind == CMM_VORT_PERTURBED_JET) return jet(phi, theta, true);
+      return jet(phi, theta, true) + jet(phi, theta - pi / 8, true);
+    }
+    case CMM_VORT_GAUSSIAN: {
+      double x = sin(theta) * cos(phi), y = sin(theta) * sin(phi), z = cos(theta);
+      return 4 * pi * exp(-((x - 1) * (x - 1) + y * y + z * z) / (1.0 / 16));
+    }
+    default: {  // perturbed rossby wave
+      double s = sin(theta);
+      return 30 * cos(theta) * cos(4 * phi) * (s * s * s * s) + 0.1 * sin(theta) * cos(30 * theta);
+    }
+  }
+}
+
+__global__ void k_sample_vorticity(VortArgs A, cmm_mwss_mesh W, const double* __restrict__ wpacked,
+                                   const double* __restrict__ X, int64_t n, int64_t sp, int64_t sc,
+                                   const double* __restrict__ cosg, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  d3 x = load_pt(X, i, sp, sc);
+  // utils.cart2sphere (utils.py:18-23)
+  double phi = npy_mod(atan2(x.y, x.x) + 2 * M_PI, 2 * M_PI);
+  double theta = atan2(sqrt(x.y * x.y + x.x * x.x), x.z);
+  double w0;
+  if (A.kind == CMM_VORT_SPLINE) {
+    // spherical_spline.__call__ (spherical_spline.py:42-70): back to xyz
+    d3 q = d3{sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
+    LevelSet ls;
+    ls.p[0] = wpacked; ls.w[0] = 1.0; ls.nlev = 1;
+    double r[1];
+    if (!mwss_eval_levels(W, ls, 1, q, r)) r[0] = __builtin_nan("");
+    w0 = r[0];
+  } else {
+    w0 = vort_analytic(A.kind, phi, theta, A.params);
+  }
+  int ring = (int)(i / A.ring_len);
+  out[i] = w0 + 2 * A.rot * cos(theta) - 2 * A.rot * cosg[ring];
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int cmm_mwss_locate(const cmm_mwss_mesh* m, const double* q, int64_t n, int64_t sp,
+                               int64_t sc, int32_t* tri, double* bcc, void* stream) {
+  if (!m || n < 0 || (n > 0 && (!q || !tri))) return fail(CMM_E_ARG, "cmm_mwss_locate: bad arguments");
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_mwss_locate, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, q, n,
+                     sp, sc, tri, bcc);
+  CMM_LAUNCH_CHECK("cmm_mwss_locate");
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_assemble(const cmm_mwss_mesh* m, const double* vals, int ncomp,
+                                 double* coeffs, double* packed, void* stream) {
+  if (!m || !vals || (ncomp != 1 && ncomp != 3)) return fail(CMM_E_ARG, "cmm_mwss_assemble: bad arguments");
+  if (!coeffs && !packed) return CMM_OK;
+  hipLaunchKernelGGL(k_mwss_assemble, dim3(grid_for(m->Tu, 128)), dim3(128), 0, as_stream(stream), *m,
+                     vals, ncomp, coeffs, packed);
+  CMM_LAUNCH_CHECK("cmm_mwss_assemble");
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_eval(const cmm_mwss_mesh* m, const double* const* levels, const double* w,
+                             int nlev, int ncomp, const double* q, int64_t n, int64_t sp, int64_t sc,
+                             double* out, int64_t osp, int64_t osc, void* stream) {
+  if (!m || !levels || !w || nlev < 1 || nlev > 3 || (ncomp != 1 && ncomp != 3) || n < 0 ||
+      (n > 0 && (!q || !out)))
+    return fail(CMM_E_ARG, "cmm_mwss_eval: bad arguments");
+  LevelSet ls;
+  ls.nlev = nlev;
+  for (int k = 0; k < nlev; ++k) {
+    if (!levels[k]) return fail(CMM_E_ARG, "cmm_mwss_eval: null level");
+    ls.p[k] = levels[k];
+    ls.w[k] = w[k];
+  }
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_mwss_eval, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, ls, ncomp,
+                     q, n, sp, sc, out, osp, osc);
+  CMM_LAUNCH_CHECK("cmm_mwss_eval");
+  return CMM_OK;
+}
+
+extern "C" int cmm_advect_points(const cmm_velocity* vel, int integ, double dt, const double* x,
+                                 int64_t m, double* y, void* stream) {
+  if (!vel || m < 0 || (m > 0 && (!x || !y)) || integ < 0 || integ > 4)
+    return fail(CMM_E_ARG, "cmm_advect_points: bad arguments");
+  VelDev V;
+  V.kind = vel->kind;
+  for (int s = 0; s < 4; ++s) V.stage_t[s] = vel->stage_t[s];
+  for (int k = 0; k < 8; ++k) V.params[k] = vel->params[k];
+  if (vel->kind == CMM_VEL_SPLINE) {
+    if (!vel->mesh || vel->nlev < 1 || vel->nlev > 3) return fail(CMM_E_ARG, "cmm_advect_points: bad spline velocity");
+    V.mesh = *vel->mesh;
+    for (int s = 0; s < 4; ++s) {
+      V.ls[s].nlev = vel->nlev;
+      for (int k = 0; k < 3; ++k) {
+        V.ls[s].p[k] = k < vel->nlev ? vel->levels[k] : nullptr;
+        V.ls[s].w[k] = vel->stage_w[s][k];
+      }
+      for (int k = 0; k < vel->nlev; ++k)
+        if (!vel->levels[k]) return fail(CMM_E_ARG, "cmm_advect_points: null velocity level");
+    }
+  } else if (vel->kind < 0 || vel->kind > CMM_VEL_RH4) {
+    return fail(CMM_E_ARG, "cmm_advect_points: unknown velocity kind");
+  }
+  if (m == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_advect, dim3(grid_for(m, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, x, m, y);
+  CMM_LAUNCH_CHECK("cmm_advect_points");
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_velocity_jets(const cmm_mwss_mesh* m, const double* fields, double* vals,
+                                      void* stream) {
+  if (!m || !fields || !vals) return fail(CMM_E_ARG, "cmm_mwss_velocity_jets: bad arguments");
+  hipLaunchKernelGGL(k_velocity_jets, dim3(grid_for(m->Nu, 256)), dim3(256), 0, as_stream(stream), *m,
+                     fields, vals);
+  CMM_LAUNCH_CHECK("cmm_mwss_velocity_jets");
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_scalar_jets(const cmm_mwss_mesh* m, const double* fields, double* vals,
+                                    void* stream) {
+  if (!m || !fields || !vals) return fail(CMM_E_ARG, "cmm_mwss_scalar_jets: bad arguments");
+  hipLaunchKernelGGL(k_scalar_jets, dim3(grid_for(m->Nu, 256)), dim3(256), 0, as_stream(stream), *m,
+                     fields, vals);
+  CMM_LAUNCH_CHECK("cmm_mwss_scalar_jets");
+  return CMM_OK;
+}
+
+extern "C" int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const double* wpacked,
+                                    const double* params, const double* X, int64_t n, int64_t sp,
+                                    int64_t sc, const double* cosg, int ring_len, double rot_rate,
+                                    double* out, void* stream) {
+  if (kind < 0 || kind > CMM_VORT_PERTURBED_ROSSBY || n < 0 || (n > 0 && (!X || !out || !cosg)) ||
+      ring_len < 1)
+    return fail(CMM_E_ARG, "cmm_sample_vorticity: bad arguments");
+  if (kind == CMM_VORT_SPLINE && (!wmesh || !wpacked))
+    return fail(CMM_E_ARG, "cmm_sample_vorticity: spline vorticity needs a mesh and coefficients");
+  VortArgs A;
+  A.kind = kind;
+  for (int k = 0; k < 4; ++k) A.params[k] = params ? params[k] : 0.0;
+  A.rot = rot_rate;
+  A.ring_len = ring_len;
+  cmm_mwss_mesh W;
+  if (wmesh) W = *wmesh;
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_sample_vorticity, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), A, W,
+                     wpacked, X, n, sp, sc, cosg, out);
+  CMM_LAUNCH_CHECK("cmm_sample_vorticity");
+  return CMM_OK;
+}

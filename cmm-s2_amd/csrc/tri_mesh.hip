@@ -1,0 +1,539 @@
+// Unstructured spherical triangulations: cube-map bucket location,
+// Powell-Sabin assembly / evaluation, fused composition and the stencil ->
+// jet projection of the map update.
+//
+// Device layout of one mesh (all resident in HBM):
+//   geom   (Ntri, 16) f64, one 128-byte line per triangle:
+//          w0, w1, w2 (dual basis: b_i = q . w_i are the homogeneous
+//          barycentrics of utils.bary_coords), s12 = |v1+v2|, s23, s31,
+//          s4 = |v1+v2+v3|, 3 pad
+//   cells  (6 R R) int4 {count, c0, c1, c2}: candidates of one cube-map cell
+//          inline when count <= 3, else c0 = offset into `over`
+//   over   overflow candidate lists (int32)
+//   verts  (Nv, 3) f64, tris (Ntri, 3) int32
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "common.h"
+
+namespace cmm {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+int hip_check(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return CMM_E_HIP;
+}
+
+constexpr int kMaxDepth = 64;
+
+struct LevelPtrs {
+  const double* p[kMaxDepth];
+};
+
+}  // namespace cmm
+
+using namespace cmm;
+
+struct cmm_tri_mesh {
+  int64_t nv = 0, ntri = 0;
+  int R = 0;
+  int64_t total_cand = 0, max_cand = 0, bytes = 0;
+  double* geom = nullptr;
+  int4* cells = nullptr;
+  int32_t* over = nullptr;
+  double* verts = nullptr;
+  int32_t* tris = nullptr;
+  int device = 0;
+};
+
+extern "C" int cmm_abi_version(void) { return CMM_ABI_VERSION; }
+extern "C" const char* cmm_last_error(void) { return cmm::g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// cube map (host and device agree bit for bit on the cell of a point)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int cube_cell(d3 q, int R, bool* ok) {
+  double ax = fabs(q.x), ay = fabs(q.y), az = fabs(q.z);
+  int k = 0;
+  double m = ax;
+  if (ay > m) { k = 1; m = ay; }
+  if (az > m) { k = 2; m = az; }
+  double c[3] = {q.x, q.y, q.z};
+  int sg = c[k] >= 0 ? 0 : 1;
+  double u = c[(k + 1) % 3] / m, v = c[(k + 2) % 3] / m;
+  *ok = (m > 0) && isfinite(u) && isfinite(v);
+  if (!*ok) return 0;
+  double fu = floor((u + 1.0) * 0.5 * R), fv = floor((v + 1.0) * 0.5 * R);
+  int i = (int)fmin(fmax(fu, 0.0), (double)(R - 1));
+  int j = (int)fmin(fmax(fv, 0.0), (double)(R - 1));
+  return ((2 * k + sg) * R + i) * R + j;
+}
+
+namespace {
+
+struct Tri2 {
+  double u[3], v[3];
+};
+
+// separating-axis overlap test of a 2-D triangle and an axis-aligned box
+bool tri_box_overlap(const Tri2& t, double x0, double x1, double y0, double y1) {
+  for (int e = 0; e < 3; ++e) {
+    int a = e, b = (e + 1) % 3, c = (e + 2) % 3;
+    double nx = -(t.v[b] - t.v[a]), ny = t.u[b] - t.u[a];
+    double sc = nx * (t.u[c] - t.u[a]) + ny * (t.v[c] - t.v[a]);
+    if (sc < 0) { nx = -nx; ny = -ny; }
+    // triangle lies on the side n.(p - a) >= 0; box fully on the other side?
+    double bx = nx >= 0 ? x1 : x0, by = ny >= 0 ? y1 : y0;
+    if (nx * (bx - t.u[a]) + ny * (by - t.v[a]) < 0) return false;
+  }
+  return true;
+}
+
+void raster_piece(const d3 p[3], int32_t tri, int R, std::vector<std::pair<int32_t, int32_t>>& out) {
+  const double eps = 1e-9;
+  for (int k = 0; k < 3; ++k) {
+    for (int sg = 0; sg < 2; ++sg) {
+      double s = sg == 0 ? 1.0 : -1.0;
+      bool inside = true;
+      Tri2 t;
+      for (int i = 0; i < 3; ++i) {
+        double c[3] = {p[i].x, p[i].y, p[i].z};
+        double m = s * c[k];
+        if (m < 1e-3) { inside = false; break; }
+        t.u[i] = c[(k + 1) % 3] / m;
+        t.v[i] = c[(k + 2) % 3] / m;
+      }
+      if (!inside) continue;
+      double umin = std::min({t.u[0], t.u[1], t.u[2]}) - eps;
+      double umax = std::max({t.u[0], t.u[1], t.u[2]}) + eps;
+      double vmin = std::min({t.v[0], t.v[1], t.v[2]}) - eps;
+      double vmax = std::max({t.v[0], t.v[1], t.v[2]}) + eps;
+      if (umax < -1 || umin > 1 || vmax < -1 || vmin > 1) continue;
+      int i0 = std::max(0, (int)std::floor((umin + 1) * 0.5 * R));
+      int i1 = std::min(R - 1, (int)std::floor((umax + 1) * 0.5 * R));
+      int j0 = std::max(0, (int)std::floor((vmin + 1) * 0.5 * R));
+      int j1 = std::min(R - 1, (int)std::floor((vmax + 1) * 0.5 * R));
+      for (int i = i0; i <= i1; ++i) {
+        double x0 = -1 + 2.0 * i / R - eps, x1 = -1 + 2.0 * (i + 1) / R + eps;
+        for (int j = j0; j <= j1; ++j) {
+          double y0 = -1 + 2.0 * j / R - eps, y1 = -1 + 2.0 * (j + 1) / R + eps;
+          if (tri_box_overlap(t, x0, x1, y0, y1))
+            out.push_back({(int32_t)(((2 * k + sg) * R + i) * R + j), tri});
+        }
+      }
+    }
+  }
+}
+
+// split a spherical triangle into geodesic pieces of at most ~20 degrees
+void raster_tri(d3 a, d3 b, d3 c, int32_t tri, int R, std::vector<std::pair<int32_t, int32_t>>& out,
+                int depth = 0) {
+  const double cmax = std::cos(20.0 * M_PI / 180.0);
+  if (depth < 8 && (dot(a, b) < cmax || dot(b, c) < cmax || dot(c, a) < cmax)) {
+    d3 ab = unit(add(a, b)), bc = unit(add(b, c)), ca = unit(add(c, a));
+    raster_tri(a, ab, ca, tri, R, out, depth + 1);
+    raster_tri(ab, b, bc, tri, R, out, depth + 1);
+    raster_tri(ca, bc, c, tri, R, out, depth + 1);
+    raster_tri(ab, bc, ca, tri, R, out, depth + 1);
+    return;
+  }
+  d3 p[3] = {a, b, c};
+  raster_piece(p, tri, R, out);
+}
+
+}  // namespace
+
+extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int64_t* simplices,
+                                   int64_t ntri, int cells_per_face, cmm_tri_mesh** out) {
+  if (!vertices || !simplices || !out || nv < 3 || ntri < 1)
+    return fail(CMM_E_ARG, "cmm_tri_mesh_create: bad arguments");
+  if (ntri > (int64_t)1 << 30 || nv > (int64_t)1 << 30)
+    return fail(CMM_E_RANGE, "cmm_tri_mesh_create: mesh too large");
+  for (int64_t t = 0; t < 3 * ntri; ++t)
+    if (simplices[t] < 0 || simplices[t] >= nv)
+      return fail(CMM_E_ARG, "cmm_tri_mesh_create: simplex index out of range");
+  int R = cells_per_face;
+  if (R <= 0) R = std::max(4, (int)std::ceil(std::sqrt((double)ntri * 0.75)));
+  if ((int64_t)6 * R * R > ((int64_t)1 << 31) - 1) return fail(CMM_E_RANGE, "cmm_tri_mesh_create: too many cells");
+
+  // geometry records
+  std::vector<double> geom((size_t)ntri * 16, 0.0);
+  auto V = [&](int64_t i) { return d3{vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]}; };
+  std::vector<int32_t> tris((size_t)ntri * 3);
+  std::vector<std::pair<int32_t, int32_t>> pairs;
+  pairs.reserve((size_t)ntri * 8);
+  for (int64_t t = 0; t < ntri; ++t) {
+    d3 v1 = V(simplices[3 * t]), v2 = V(simplices[3 * t + 1]), v3 = V(simplices[3 * t + 2]);
+    for (int i = 0; i < 3; ++i) tris[3 * t + i] = (int32_t)simplices[3 * t + i];
+    double D = det3(v1, v2, v3);
+    if (!(D > 0)) return fail(CMM_E_ARG, "cmm_tri_mesh_create: triangle not counter-clockwise");
+    d3 w0 = cross(v2, v3), w1 = cross(v3, v1), w2 = cross(v1, v2);
+    double* g = &geom[16 * t];
+    g[0] = w0.x / D; g[1] = w0.y / D; g[2] = w0.z / D;
+    g[3] = w1.x / D; g[4] = w1.y / D; g[5] = w1.z / D;
+    g[6] = w2.x / D; g[7] = w2.y / D; g[8] = w2.z / D;
+    g[9] = norm3(add(v1, v2));
+    g[10] = norm3(add(v2, v3));
+    g[11] = norm3(add(v3, v1));
+    g[12] = norm3(add(add(v1, v2), v3));
+    raster_tri(v1, v2, v3, (int32_t)t, R, pairs);
+  }
+  std::sort(pairs.begin(), pairs.end());
+  pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+  int64_t ncell = (int64_t)6 * R * R;
+  std::vector<int4> cells((size_t)ncell, int4{0, -1, -1, -1});
+  std::vector<int32_t> over;
+  int64_t maxc = 0;
+  for (size_t a = 0; a < pairs.size();) {
+    size_t b = a;
+    while (b < pairs.size() && pairs[b].first == pairs[a].first) ++b;
+    int64_t n = (int64_t)(b - a);
+    maxc = std::max(maxc, n);
+    int4& c = cells[pairs[a].first];
+    c.x = (int)n;
+    if (n <= 3) {
+      int ids[3] = {-1, -1, -1};
+      for (int64_t i = 0; i < n; ++i) ids[i] = pairs[a + i].second;
+      c.y = ids[0]; c.z = ids[1]; c.w = ids[2];
+    } else {
+      c.y = (int)over.size();
+      for (size_t i = a; i < b; ++i) over.push_back(pairs[i].second);
+    }
+    a = b;
+  }
+  for (int64_t c = 0; c < ncell; ++c)
+    if (cells[c].x == 0) return fail(CMM_E_ARG, "cmm_tri_mesh_create: triangulation does not cover the sphere");
+  if (over.empty()) over.push_back(0);
+
+  cmm_tri_mesh* m = new cmm_tri_mesh();
+  m->nv = nv; m->ntri = ntri; m->R = R;
+  m->total_cand = (int64_t)pairs.size();
+  m->max_cand = maxc;
+  (void)hipGetDevice(&m->device);
+  size_t bg = geom.size() * 8, bc = cells.size() * sizeof(int4), bo = over.size() * 4,
+         bv = (size_t)nv * 24, bt = tris.size() * 4;
+  m->bytes = (int64_t)(bg + bc + bo + bv + bt);
+  hipError_t e;
+  if ((e = hipMalloc(&m->geom, bg)) != hipSuccess || (e = hipMalloc(&m->cells, bc)) != hipSuccess ||
+      (e = hipMalloc(&m->over, bo)) != hipSuccess || (e = hipMalloc(&m->verts, bv)) != hipSuccess ||
+      (e = hipMalloc(&m->tris, bt)) != hipSuccess) {
+    cmm_tri_mesh_destroy(m);
+    return hip_check(e, "cmm_tri_mesh_create: hipMalloc");
+  }
+  if ((e = hipMemcpy(m->geom, geom.data(), bg, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->cells, cells.data(), bc, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->over, over.data(), bo, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->verts, vertices, bv, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->tris, tris.data(), bt, hipMemcpyHostToDevice)) != hipSuccess) {
+    cmm_tri_mesh_destroy(m);
+    return hip_check(e, "cmm_tri_mesh_create: hipMemcpy");
+  }
+  *out = m;
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_mesh_destroy(cmm_tri_mesh* m) {
+  if (!m) return CMM_OK;
+  (void)hipFree(m->geom);
+  (void)hipFree(m->cells);
+  (void)hipFree(m->over);
+  (void)hipFree(m->verts);
+  (void)hipFree(m->tris);
+  delete m;
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_mesh_stats(const cmm_tri_mesh* m, int64_t* out) {
+  if (!m || !out) return fail(CMM_E_ARG, "cmm_tri_mesh_stats: bad arguments");
+  out[0] = m->nv; out[1] = m->ntri; out[2] = m->R;
+  out[3] = m->total_cand; out[4] = m->max_cand; out[5] = m->bytes;
+  return CMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device location
+// ---------------------------------------------------------------------------
+struct Located {
+  int tri;
+  double b0, b1, b2;
+  double s12, s23, s31, s4;
+  d3 w0, w1, w2;
+};
+
+__device__ inline void load_geom(const double* __restrict__ geom, int t, Located& L) {
+  const double2* g = reinterpret_cast<const double2*>(geom + (int64_t)t * 16);
+  double2 a = g[0], b = g[1], c = g[2], d = g[3], e = g[4], f = g[5], h = g[6];
+  L.w0 = d3{a.x, a.y, b.x};
+  L.w1 = d3{b.y, c.x, c.y};
+  L.w2 = d3{d.x, d.y, e.x};
+  L.s12 = e.y; L.s23 = f.x; L.s31 = f.y; L.s4 = h.x;
+}
+
+// containing triangle: first candidate with all barycentrics >= 0, else the
+// candidate whose smallest barycentric is largest
+__device__ inline bool tri_locate(const cmm_tri_mesh& M, const double* __restrict__ geom,
+                                  const int4* __restrict__ cells, const int32_t* __restrict__ over,
+                                  d3 q, Located& L) {
+  bool ok;
+  int cell = cube_cell(q, M.R, &ok);
+  if (!ok) { L.tri = -1; return false; }
+  int4 c = cells[cell];
+  int n = c.x;
+  int best = -1;
+  double bestm = -INFINITY;
+  Located cur;
+  for (int i = 0; i < n; ++i) {
+    int t = (n <= 3) ? (i == 0 ? c.y : (i == 1 ? c.z : c.w)) : over[c.y + i];
+    load_geom(geom, t, cur);
+    double b0 = dot(q, cur.w0), b1 = dot(q, cur.w1), b2 = dot(q, cur.w2);
+    double mn = fmin(b0, fmin(b1, b2));
+    if (mn > bestm) {
+      bestm = mn;
+      best = t;
+      L = cur;
+      L.b0 = b0; L.b1 = b1; L.b2 = b2;
+    }
+    if (mn >= 0) break;
+  }
+  L.tri = best;
+  return best >= 0;
+}
+
+// evaluation of one packed level at a located point (raw polynomial)
+__device__ inline d3 eval_level3(const double* __restrict__ packed, const Located& L, d3 bq,
+                                 int s) {
+  d3 c = sub_bary(s, bq.x, bq.y, bq.z, L.s12, L.s23, L.s31, L.s4);
+  const double2* p = reinterpret_cast<const double2*>(packed + ((int64_t)L.tri * 6 + s) * 18);
+  double buf[18];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    double2 v = p[i];
+    buf[2 * i] = v.x;
+    buf[2 * i + 1] = v.y;
+  }
+  return d3{bb6(c, buf), bb6(c, buf + 6), bb6(c, buf + 12)};
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void k_tri_locate(cmm_tri_mesh M, const double* __restrict__ q, int64_t n, int64_t sp,
+                             int64_t sc, int32_t* __restrict__ tri, double* __restrict__ bcc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  d3 x = load_pt(q, i, sp, sc);
+  Located L;
+  bool ok = tri_locate(M, M.geom, M.cells, M.over, x, L);
+  tri[i] = ok ? L.tri : -1;
+  if (bcc) {
+    double nan = __builtin_nan("");
+    bcc[3 * i] = ok ? L.b0 : nan;
+    bcc[3 * i + 1] = ok ? L.b1 : nan;
+    bcc[3 * i + 2] = ok ? L.b2 : nan;
+  }
+}
+
+__global__ void k_tri_eval(cmm_tri_mesh M, const double* __restrict__ packed, int ncomp, int normalize,
+                           const double* __restrict__ q, int64_t n, int64_t sp, int64_t sc,
+                           double* __restrict__ out, int64_t osp, int64_t osc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  d3 x = load_pt(q, i, sp, sc);
+  Located L;
+  double nan = __builtin_nan("");
+  if (!tri_locate(M, M.geom, M.cells, M.over, x, L)) {
+    for (int c = 0; c < ncomp; ++c) out[i * osp + c * osc] = nan;
+    return;
+  }
+  int s = split_index(L.b0, L.b1, L.b2);
+  d3 cb = sub_bary(s, L.b0, L.b1, L.b2, L.s12, L.s23, L.s31, L.s4);
+  const double* p = packed + ((int64_t)L.tri * 6 + s) * 6 * ncomp;
+  double r[3];
+  for (int c = 0; c < ncomp; ++c) r[c] = bb6(cb, p + 6 * c);
+  if (normalize && ncomp == 3) {
+    double nn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    for (int c = 0; c < 3; ++c) r[c] = (1 / nn) * r[c];
+  }
+  for (int c = 0; c < ncomp; ++c) out[i * osp + c * osc] = r[c];
+}
+
+__global__ void k_tri_compose(cmm_tri_mesh M, LevelPtrs lv, int depth, const double* __restrict__ q,
+                              int64_t n, int64_t sp, int64_t sc, double* __restrict__ out, int64_t osp,
+                              int64_t osc, double* __restrict__ inter) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  d3 x = load_pt(q, i, sp, sc);
+  for (int l = 0; l < depth; ++l) {
+    Located L;
+    if (!tri_locate(M, M.geom, M.cells, M.over, x, L)) {
+      double nan = __builtin_nan("");
+      x = d3{nan, nan, nan};
+    } else {
+      int s = split_index(L.b0, L.b1, L.b2);
+      d3 r = eval_level3(lv.p[l], L, d3{L.b0, L.b1, L.b2}, s);
+      double nn = sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+      double inv = 1 / nn;
+      x = d3{inv * r.x, inv * r.y, inv * r.z};
+    }
+    if (inter) store_pt(inter + (int64_t)l * n * 3, i, 3, 1, x);
+  }
+  store_pt(out, i, osp, osc, x);
+}
+
+__global__ void k_tri_assemble(cmm_tri_mesh M, const double* __restrict__ vals, int ncomp,
+                               double* __restrict__ coeffs, double* __restrict__ packed) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M.ntri) return;
+  int vi[3] = {M.tris[3 * t], M.tris[3 * t + 1], M.tris[3 * t + 2]};
+  d3 v[3];
+  for (int i = 0; i < 3; ++i)
+    v[i] = d3{M.verts[3 * (int64_t)vi[i]], M.verts[3 * (int64_t)vi[i] + 1], M.verts[3 * (int64_t)vi[i] + 2]};
+  PSGeom P;
+  ps_geometry(v[0], v[1], v[2], P);
+  const int64_t Nv = M.nv;
+  for (int c = 0; c < ncomp; ++c) {
+    double f[3];
+    d3 gr[3];
+    for (int i = 0; i < 3; ++i) {
+      if (ncomp == 1) {
+        f[i] = vals[vi[i]];
+        gr[i] = d3{vals[Nv + vi[i]], vals[2 * Nv + vi[i]], vals[3 * Nv + vi[i]]};
+      } else {
+        f[i] = vals[(c * 4) * Nv + vi[i]];
+        gr[i] = d3{vals[(0 * 4 + 1 + c) * Nv + vi[i]], vals[(1 * 4 + 1 + c) * Nv + vi[i]],
+                   vals[(2 * 4 + 1 + c) * Nv + vi[i]]};
+      }
+    }
+    double cf[19];
+    ps_coeffs(P, f, gr, cf);
+    if (coeffs)
+      for (int k = 0; k < 19; ++k) coeffs[((int64_t)c * 19 + k) * M.ntri + t] = cf[k];
+    if (packed)
+      for (int s = 0; s < 6; ++s)
+        for (int j = 0; j < 6; ++j) packed[((t * 6 + s) * ncomp + c) * 6 + j] = cf[cs_table(s, j)];
+  }
+}
+
+__global__ void k_tri_stencil_jets(cmm_tri_mesh M, const double* __restrict__ cur,
+                                   const double* __restrict__ yv, int64_t yrow,
+                                   const double* __restrict__ sf, int64_t srow,
+                                   const double* __restrict__ tanv, double eps,
+                                   double* __restrict__ jets, double* __restrict__ st) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t Nv = M.nv;
+  if (v >= Nv) return;
+  d3 S[4];
+  for (int k = 0; k < 4; ++k) {
+    int64_t col = k * Nv + v;
+    S[k] = d3{sf[col], sf[srow + col], sf[2 * srow + col]};
+  }
+  if (cur) {
+    d3 y = d3{yv[v], yv[yrow + v], yv[2 * yrow + v]};
+    Located L;
+    double nan = __builtin_nan("");
+    if (!tri_locate(M, M.geom, M.cells, M.over, y, L)) {
+      for (int k = 0; k < 4; ++k) S[k] = d3{nan, nan, nan};
+    } else {
+      int s = split_index(L.b0, L.b1, L.b2);
+      for (int k = 0; k < 4; ++k) {
+        d3 bq = d3{dot(S[k], L.w0), dot(S[k], L.w1), dot(S[k], L.w2)};
+        S[k] = eval_level3(cur, L, bq, s);
+      }
+    }
+  }
+  if (st)
+    for (int k = 0; k < 4; ++k) {
+      st[(0 * 4 + k) * Nv + v] = S[k].x;
+      st[(1 * 4 + k) * Nv + v] = S[k].y;
+      st[(2 * 4 + k) * Nv + v] = S[k].z;
+    }
+  // spline_proj_sphere (evolution_functions.py:97-110)
+  d3 m = d3{(S[0].x + S[1].x + S[2].x + S[3].x) / 4, (S[0].y + S[1].y + S[2].y + S[3].y) / 4,
+            (S[0].z + S[1].z + S[2].z + S[3].z) / 4};
+  m = unit(m);
+  double e4 = 4 * eps;
+  d3 d1 = d3{(S[1].x - S[0].x + S[3].x - S[2].x) / e4, (S[1].y - S[0].y + S[3].y - S[2].y) / e4,
+             (S[1].z - S[0].z + S[3].z - S[2].z) / e4};
+  d3 d2 = d3{(S[2].x - S[0].x + S[3].x - S[1].x) / e4, (S[2].y - S[0].y + S[3].y - S[1].y) / e4,
+             (S[2].z - S[0].z + S[3].z - S[1].z) / e4};
+  d3 a1 = d3{tanv[3 * v], tanv[3 * v + 1], tanv[3 * v + 2]};
+  d3 a2 = d3{tanv[3 * (Nv + v)], tanv[3 * (Nv + v) + 1], tanv[3 * (Nv + v) + 2]};
+  jets[(0 * 4 + 0) * Nv + v] = m.x;
+  jets[(1 * 4 + 0) * Nv + v] = m.y;
+  jets[(2 * 4 + 0) * Nv + v] = m.z;
+  double dc1[3] = {d1.x, d1.y, d1.z}, dc2[3] = {d2.x, d2.y, d2.z};
+  double A1[3] = {a1.x, a1.y, a1.z}, A2[3] = {a2.x, a2.y, a2.z};
+  for (int c = 0; c < 3; ++c)
+    for (int d = 0; d < 3; ++d) jets[(d * 4 + 1 + c) * Nv + v] = dc1[c] * A1[d] + dc2[c] * A2[d];
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int cmm_tri_locate(const cmm_tri_mesh* m, const double* q, int64_t n, int64_t sp,
+                              int64_t sc, int32_t* tri, double* bcc, void* stream) {
+  if (!m || (!q && n > 0) || (!tri && n > 0) || n < 0) return fail(CMM_E_ARG, "cmm_tri_locate: bad arguments");
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_tri_locate, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, q, n,
+                     sp, sc, tri, bcc);
+  CMM_LAUNCH_CHECK("cmm_tri_locate");
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_assemble(const cmm_tri_mesh* m, const double* vals, int ncomp, double* coeffs,
+                                double* packed, void* stream) {
+  if (!m || !vals || (ncomp != 1 && ncomp != 3)) return fail(CMM_E_ARG, "cmm_tri_assemble: bad arguments");
+  if (!coeffs && !packed) return CMM_OK;
+  hipLaunchKernelGGL(k_tri_assemble, dim3(grid_for(m->ntri, 128)), dim3(128), 0, as_stream(stream), *m,
+                     vals, ncomp, coeffs, packed);
+  CMM_LAUNCH_CHECK("cmm_tri_assemble");
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_eval(const cmm_tri_mesh* m, const double* packed, int ncomp, int normalize,
+                            const double* q, int64_t n, int64_t sp, int64_t sc, double* out,
+                            int64_t osp, int64_t osc, void* stream) {
+  if (!m || !packed || (ncomp != 1 && ncomp != 3) || n < 0 || (n > 0 && (!q || !out)))
+    return fail(CMM_E_ARG, "cmm_tri_eval: bad arguments");
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_tri_eval, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, packed,
+                     ncomp, normalize, q, n, sp, sc, out, osp, osc);
+  CMM_LAUNCH_CHECK("cmm_tri_eval");
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_compose(const cmm_tri_mesh* m, const double* const* levels, int depth,
+                               const double* q, int64_t n, int64_t sp, int64_t sc, double* out,
+                               int64_t osp, int64_t osc, double* inter, void* stream) {
+  if (!m || depth < 0 || n < 0 || (n > 0 && (!q || !out)) || (depth > 0 && !levels))
+    return fail(CMM_E_ARG, "cmm_tri_compose: bad arguments");
+  if (depth > kMaxDepth) return fail(CMM_E_RANGE, "cmm_tri_compose: depth > 64");
+  LevelPtrs lv;
+  for (int i = 0; i < depth; ++i) {
+    if (!levels[i]) return fail(CMM_E_ARG, "cmm_tri_compose: null level");
+    lv.p[i] = levels[i];
+  }
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_tri_compose, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, lv,
+                     depth, q, n, sp, sc, out, osp, osc, inter);
+  CMM_LAUNCH_CHECK("cmm_tri_compose");
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_stencil_jets(const cmm_tri_mesh* m, const double* cur, const double* yv,
+                                    int64_t yrow, const double* sf, int64_t srow, const double* tanv,
+                                    double eps, double* jets, double* st, void* stream) {
+  if (!m || !sf || !tanv || !jets || !(eps > 0) || (cur && !yv))
+    return fail(CMM_E_ARG, "cmm_tri_stencil_jets: bad arguments");
+  hipLaunchKernelGGL(k_tri_stencil_jets, dim3(grid_for(m->nv, 128)), dim3(128), 0, as_stream(stream), *m,
+                     cur, yv, yrow, sf, srow, tanv, eps, jets, st);
+  CMM_LAUNCH_CHECK("cmm_tri_stencil_jets");
+  return CMM_OK;
+}

@@ -1,0 +1,226 @@
+/*
+ * cmm_s2.h -- C ABI of the MI355X-native Characteristic Mapping Method engine.
+ *
+ * Plain pointers and sizes only.  Every `const double* q`, `double* out` ...
+ * below is a DEVICE pointer (HBM) unless the comment says "host".  Every
+ * function is asynchronous on the caller's HIP stream (`void* stream`, a
+ * hipStream_t; NULL = the default stream) and returns 0 on success or a
+ * negative CMM_E* code; cmm_last_error() then holds a message (thread local).
+ * Mesh / plan handles own their device tables on the device that was current
+ * when they were created.
+ *
+ * Array layouts follow the reference (seth-tl/CMM-S2, cmm_s2/core):
+ *   jets      vals[c][k][v]  (ncomp, 4, Nv): k = 0 value, k = 1+d derivative
+ *             along Cartesian axis d, stored transposed for vector fields
+ *             (vals[d][1+c][v] = d f_c / d x_d), spherical_spline.py:300-337
+ *   coeffs    (ncomp, 19, Ntri) Bernstein-Bezier coefficients, same file
+ *   packed    (Ntri, 6, ncomp, 6) evaluation layout: for each split
+ *             sub-triangle (argmin, argmax of the barycentrics) the 6
+ *             coefficients of the Cs table (spherical_spline.py:660-662)
+ *   points    strided: element (i, c) at q[i*stride_pt + c*stride_comp], so
+ *             both (N,3) and (3,N) arrays are accepted without copies
+ *   spectra   [lm][f] complex128 (interleaved re, im), lm = packed (l, m>=0)
+ *             index m*L - m*(m-1)/2 + (l - m), f = field index fastest
+ *   grids     (nf, L+1, 2L) real, MWSS rings theta_t = 2 t pi / (2L)
+ */
+#ifndef CMM_S2_H
+#define CMM_S2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMM_OK 0
+#define CMM_E_ARG (-1)      /* bad argument / shape */
+#define CMM_E_HIP (-2)      /* HIP runtime error (launch, alloc, copy) */
+#define CMM_E_RANGE (-3)    /* size outside the supported range */
+
+#define CMM_ABI_VERSION 1
+
+int cmm_abi_version(void);
+const char* cmm_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Unstructured spherical triangulations (icosahedral meshes).              */
+/* Replaces spherical_triangulation.__init__ / query                        */
+/* (cmm_s2/core/mesh_functions.py:29-88): location is a cube-map bucket     */
+/* search over exact barycentric tests instead of Qhull's find_simplex.     */
+/* ------------------------------------------------------------------------ */
+typedef struct cmm_tri_mesh cmm_tri_mesh;
+
+/* vertices: host (nv,3); simplices: host (ntri,3) int64, counter-clockwise
+ * (det(v0,v1,v2) > 0).  cells_per_face <= 0 picks a size from ntri. */
+int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int64_t* simplices,
+                        int64_t ntri, int cells_per_face, cmm_tri_mesh** out);
+int cmm_tri_mesh_destroy(cmm_tri_mesh* mesh);
+/* host out[6] = {nv, ntri, cells_per_face, total candidates, max candidates
+ * per cell, bytes of device tables} */
+int cmm_tri_mesh_stats(const cmm_tri_mesh* mesh, int64_t* out);
+
+/* mesh_functions.py:68-88 -- containing triangle (row of simplices) and
+ * homogeneous barycentrics bcc (n,3) (bcc may be NULL).  Never misses: a
+ * query outside every candidate by round-off gets the most-inside one. */
+int cmm_tri_locate(const cmm_tri_mesh* mesh, const double* q, int64_t n,
+                   int64_t q_stride_pt, int64_t q_stride_comp, int32_t* tri,
+                   double* bcc, void* stream);
+
+/* sphere_diffeomorphism / spherical_spline .assemble_coefficients
+ * (spherical_spline.py:115-152, 300-337, 471-508): jets (ncomp,4,Nv) ->
+ * coeffs (ncomp,19,Ntri) and/or packed (Ntri,6,ncomp,6); either output may
+ * be NULL.  ncomp in {1, 3}. */
+int cmm_tri_assemble(const cmm_tri_mesh* mesh, const double* vals, int ncomp,
+                     double* coeffs, double* packed, void* stream);
+
+/* sphere_diffeomorphism.__call__ (spherical_spline.py:178-205) when
+ * normalize != 0, the raw PS polynomial otherwise: out(i, c) for c < ncomp. */
+int cmm_tri_eval(const cmm_tri_mesh* mesh, const double* packed, int ncomp, int normalize,
+                 const double* q, int64_t n, int64_t q_stride_pt, int64_t q_stride_comp,
+                 double* out, int64_t out_stride_pt, int64_t out_stride_comp, void* stream);
+
+/* composite_sphere_diffeomorphism.__call__ / eval_all
+ * (spherical_spline.py:365-437) fused over the chain: levels is a HOST array
+ * of `depth` device pointers to packed map coefficients, applied in array
+ * order (levels[0] first), each followed by normalisation.  inter (optional,
+ * may be NULL): (depth, n, 3) positions after every level. */
+int cmm_tri_compose(const cmm_tri_mesh* mesh, const double* const* levels, int depth,
+                    const double* q, int64_t n, int64_t q_stride_pt, int64_t q_stride_comp,
+                    double* out, int64_t out_stride_pt, int64_t out_stride_comp,
+                    double* inter, void* stream);
+
+/* advect (stencil part) + spline_proj_sphere (evolution_functions.py:90-133,
+ * spherical_spline.py:268-298) fused.  yv: (3, Nv) integrated vertex feet,
+ * element (c, v) at yv[c*yv_row + v]; st: (3, 4*Nv) stencil feet, element
+ * (c, k*Nv + v) at st[c*st_row + k*Nv + v].  cur_packed is the current map:
+ * the foot yv_v is located in it and the four stencil feet are evaluated on
+ * that split sub-triangle's polynomial (un-normalised extrapolation).  With
+ * cur_packed == NULL (identity map) the stencil feet are the stencil values
+ * and yv is not read.  tan_vects: (2, Nv, 3) = [a1_n, a2_n].  Writes jets
+ * (3,4,Nv) and, if st_evals != NULL, the stencil values (3,4,Nv) before
+ * projection. */
+int cmm_tri_stencil_jets(const cmm_tri_mesh* mesh, const double* cur_packed, const double* yv,
+                         int64_t yv_row, const double* st, int64_t st_row, const double* tan_vects,
+                         double eps, double* jets, double* st_evals, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* MWSS structured velocity mesh (mesh_functions.py:169-318, 352-412).       */
+/* ------------------------------------------------------------------------ */
+typedef struct cmm_mwss_mesh cmm_mwss_mesh;
+
+/* Host tables: thetas (L+1), phis (2L) and their sines / cosines exactly as
+ * the caller's numpy computes them (the location tests then agree bit for
+ * bit with numpy away from ties). */
+int cmm_mwss_mesh_create(int L, const double* thetas, const double* phis,
+                         const double* sin_t, const double* cos_t,
+                         const double* sin_p, const double* cos_p, cmm_mwss_mesh** out);
+int cmm_mwss_mesh_destroy(cmm_mwss_mesh* mesh);
+
+/* structure_spherical_triangulation.query (mesh_functions.py:196-282):
+ * numpy floor-division / remainder semantics and the 0.03 / 0.85 band
+ * corrections.  tri (n,) int32, bcc (n,3) or NULL. */
+int cmm_mwss_locate(const cmm_mwss_mesh* mesh, const double* q, int64_t n,
+                    int64_t q_stride_pt, int64_t q_stride_comp, int32_t* tri,
+                    double* bcc, void* stream);
+
+/* assemble_coefficients on the MWSS mesh; jets (ncomp,4,Nu). */
+int cmm_mwss_assemble(const cmm_mwss_mesh* mesh, const double* vals, int ncomp,
+                      double* coeffs, double* packed, void* stream);
+
+/* spline_interp_velocity.__call__ (spherical_spline.py:570-615) and the
+ * scalar spherical_spline (42-70): out(i,c) = sum_k w[k] * P_k(q_i)_c over
+ * nlev packed levels (host arrays `levels`, `w`). */
+int cmm_mwss_eval(const cmm_mwss_mesh* mesh, const double* const* levels, const double* w,
+                  int nlev, int ncomp, const double* q, int64_t n, int64_t q_stride_pt,
+                  int64_t q_stride_comp, double* out, int64_t out_stride_pt,
+                  int64_t out_stride_comp, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Backward characteristic integrators (evolution_functions.py:19-80).       */
+/* ------------------------------------------------------------------------ */
+#define CMM_VEL_SPLINE 0     /* spline_interp_velocity on an MWSS mesh */
+#define CMM_VEL_SBR 1        /* dynamical_fields.u_sbr: params[0]=T, [1]=alpha */
+#define CMM_VEL_DEFORM_ROT 2 /* dynamical_fields.u_deform_rot: params[1]=alpha */
+#define CMM_VEL_RH4 3        /* Rossby-Haurwitz-4: params[0]=w, [1]=K, [2]=Omega */
+
+#define CMM_INT_EULER 0      /* Euler_step_proj */
+#define CMM_INT_IMPROVED_EULER 1 /* improved_Euler_proj */
+#define CMM_INT_RK4 2        /* RK4_proj */
+#define CMM_INT_RK2 3        /* RK2_proj */
+#define CMM_INT_EVAL 4       /* no step: y = V(stage_t[0], x) (the velocity itself) */
+
+typedef struct cmm_velocity {
+  int kind;
+  int nlev;                       /* spline: time levels read (1..3) */
+  const cmm_mwss_mesh* mesh;      /* spline */
+  const double* levels[3];        /* spline: packed coefficients per level */
+  double stage_w[4][3];           /* spline: time weights for each stage */
+  double stage_t[4];              /* analytic: time of each stage */
+  double params[8];
+} cmm_velocity;
+
+/* x, y: (3, m) component-major; y may alias x. */
+int cmm_advect_points(const cmm_velocity* vel, int integrator, double dt, const double* x,
+                      int64_t m, double* y, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* MWSS spherical-harmonic transforms (replace pyssht forward / inverse,     */
+/* spherical_harmonic_tools.py:16,84,91,123,133) and spectral operators.     */
+/* ------------------------------------------------------------------------ */
+typedef struct cmm_sht_plan cmm_sht_plan;
+
+/* cos_t, sin_t: host (L+1) cos / sin of the MWSS colatitudes. */
+int cmm_sht_plan_create(int L, const double* cos_t, const double* sin_t, cmm_sht_plan** out);
+int cmm_sht_plan_destroy(cmm_sht_plan* plan);
+/* host copies of the plan's Gauss-Legendre nodes / weights (L each) */
+int cmm_sht_plan_gl(const cmm_sht_plan* plan, double* x, double* w);
+/* bytes of device workspace needed by forward / inverse for nf fields */
+int cmm_sht_workspace_bytes(const cmm_sht_plan* plan, int nf, int64_t* bytes);
+
+/* synthesis: flm [lm][nf] complex -> f (nf, L+1, 2L) */
+int cmm_sht_inverse(const cmm_sht_plan* plan, const double* flm, int nf, double* f,
+                    void* work, void* stream);
+/* analysis: f (nf, L+1, 2L) -> flm [lm][nf] complex */
+int cmm_sht_forward(const cmm_sht_plan* plan, const double* f, int nf, double* flm,
+                    void* work, void* stream);
+
+/* inv_Laplacian scaling + angular_momentum twice (spherical_harmonic_tools.py
+ * :14-19, 49-62, 79-91): from omega_lm [lm] (one field) writes out [lm][12]:
+ * fields 0..2 = (r x grad) psi, field 3+3i+k = (r x grad)_k of field i, with
+ * psi = D_inv omega when apply_dinv != 0 (psi = omega_lm otherwise). */
+int cmm_sht_stream_operators(const cmm_sht_plan* plan, const double* omega_lm,
+                             int apply_dinv, double* out, void* stream);
+/* angular_momentum once: in [lm] -> out [lm][4] = (f, (r x grad) f) */
+int cmm_sht_scalar_operators(const cmm_sht_plan* plan, const double* f_lm, double* out,
+                             void* stream);
+
+/* project_stream grid post-processing (spherical_harmonic_tools.py:85-101):
+ * fields (12, L+1, 2L) -> velocity jets vals (3,4,Nu) (J^T layout). */
+int cmm_mwss_velocity_jets(const cmm_mwss_mesh* mesh, const double* fields, double* vals,
+                           void* stream);
+/* project_onto_S12 post-processing (spherical_harmonic_tools.py:121-138):
+ * fields (4, L+1, 2L) = (f, r x grad f) -> scalar jets vals (4, Nu). */
+int cmm_mwss_scalar_jets(const cmm_mwss_mesh* mesh, const double* fields, double* vals,
+                         void* stream);
+
+/* Vorticity sampling (spherical_simulations.py:268-273):
+ * out_t,p = w0(X) + 2 rot (cos theta(X) - cos theta_t) at the MWSS grid of
+ * bandlimit L (theta_t from cos_t, device (L+1)); X (n,3) with n = (L+1)*2L.
+ * w0 is a scalar PS spline on `wmesh` (packed) when kind = 0, else an
+ * analytic field CMM_VORT_*.  */
+#define CMM_VORT_SPLINE 0
+#define CMM_VORT_ROSSBY 1        /* dynamical_fields.rossby_wave, params[0]=t */
+#define CMM_VORT_ZONAL_JET 2
+#define CMM_VORT_GAUSSIAN 3
+#define CMM_VORT_PERTURBED_JET 4
+#define CMM_VORT_MULTI_JET 5
+#define CMM_VORT_PERTURBED_ROSSBY 6
+int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const double* wpacked,
+                         const double* params, const double* X, int64_t n, int64_t x_stride_pt,
+                         int64_t x_stride_comp, const double* cos_grid_theta, int ring_len,
+                         double rot_rate, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,182 @@
+"""GPU parity: MWSS structured-mesh location and velocity splines, the MWSS
+spherical-harmonic transforms, spectral operators and project_stream."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import random_unit, rel_err
+
+pytestmark = pytest.mark.gpu
+
+T_POS = 1e-11
+
+
+def _grid_and_ring_points(L, seed):
+    from oracle import meshes as om
+    th, ph = om.mwss_thetas(L), om.mwss_phis(L)
+    rng = np.random.default_rng(seed)
+    # points on and next to rings / meridians exercise the 0.03 / 0.85 bands
+    t = np.concatenate([th[rng.integers(0, L + 1, 3000)] + rng.uniform(-0.1, 1.1, 3000) * (np.pi / L),
+                        th[rng.integers(0, L + 1, 2000)]])
+    p = np.concatenate([ph[rng.integers(0, 2 * L, 3000)] + rng.uniform(0, 1, 3000) * (np.pi / L),
+                        ph[rng.integers(0, 2 * L, 2000)]])
+    t = np.clip(t, 0, np.pi)
+    q = np.stack([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)], 1)
+    return np.concatenate([q, random_unit(20000, seed + 1)])
+
+
+@pytest.mark.parametrize("L", [8, 16, 24, 64])
+def test_mwss_locate(L):
+    from cmm_s2.core import mesh_functions as mf
+    from oracle import meshes as om
+    pm = mf.structure_spherical_triangulation(L)
+    o = om.StructuredTriangulation(L)
+    assert np.array_equal(pm.simplices, o.simplices)
+    assert np.array_equal(pm.msimplices, o.msimplices)
+    assert np.array_equal(pm.vertices, o.vertices)
+    q = _grid_and_ring_points(L, L)
+    bo, to, vo = o.query(q)
+    t, b = pm.locate(torch.from_numpy(q).cuda(), "N3")
+    t = t.cpu().numpy().astype(np.int64)
+    b = b.cpu().numpy()
+    same = t == to
+    # a point whose oracle barycentrics are all clear of zero has a unique
+    # triangle: identical index required.  Ties (points on an edge or a grid
+    # vertex, e.g. the exact grid points above) may pick any triangle that
+    # contains the point.
+    clear = bo.min(axis=1) > 1e-12
+    assert same[clear].all()
+    bad = ~same
+    if bad.any():
+        vs = pm.vertices[pm.simplices[t[bad]]]
+        bg = np.linalg.solve(np.transpose(vs, (0, 2, 1)), q[bad][:, :, None])[:, :, 0]
+        assert (bg.min(axis=1) > -1e-12).all()
+    assert np.abs(b[same] - bo[same]).max() < 1e-13
+
+
+def test_velocity_spline_eval():
+    from cmm_s2.core import mesh_functions as mf
+    from cmm_s2.core.spherical_spline import spline_interp_velocity
+    from oracle import meshes as om, splines as osp
+    L = 16
+    pm = mf.structure_spherical_triangulation(L)
+    o = om.StructuredTriangulation(L)
+    rng = np.random.default_rng(0)
+    vals = rng.standard_normal((3, 3, 4, pm.nv))
+    U = spline_interp_velocity(pm, torch.from_numpy(vals).cuda(), None, ts=[0.0, 0.1, 0.2])
+    Uo = osp.VelocitySpline(o, vals.copy(), np.zeros((3, 3, 19, pm.ntri)), ts=np.array([0.0, 0.1, 0.2]))
+    for i in range(3):
+        U.init_coeffs(i)
+        Uo.init_coeffs(i)
+    assert rel_err(U.coeffs.cpu().numpy(), Uo.coeffs) < T_POS
+    q = random_unit(20000, 1).T.copy()
+    for nVs, t in [(1, 0.05), (2, 0.17), (3, 0.3), (3, 0.25)]:
+        U.nVs = Uo.nVs = nVs
+        got = U(t, 0.1, torch.from_numpy(q).cuda()).cpu().numpy()
+        ref = Uo(t, 0.1, q)
+        assert rel_err(got, ref) < T_POS
+
+
+@pytest.mark.parametrize("L", [16, 32, 48])
+def test_sht_inverse_forward(L):
+    from cmm_s2.core.spherical_harmonic_tools import SHT, nlm
+    from oracle import sht as osh
+    plan = SHT.get(L)
+    x, w = plan.gl_nodes()
+    po = osh.plan(L)
+    assert np.abs(x - po.xgl).max() < 1e-15 and rel_err(w, po.wgl) < 1e-14
+    rng = np.random.default_rng(L)
+    nf = 3
+    a = np.zeros((nf, L, L), complex)
+    for f in range(nf):
+        for l in range(L):
+            a[f, l, :l + 1] = rng.standard_normal(l + 1) + 1j * rng.standard_normal(l + 1)
+        a[f, :, 0] = a[f, :, 0].real
+    packed = np.zeros((nlm(L), nf, 2))
+    for m in range(L):
+        for l in range(m, L):
+            i = m * L - m * (m - 1) // 2 + (l - m)
+            packed[i, :, 0] = a[:, l, m].real
+            packed[i, :, 1] = a[:, l, m].imag
+    g = plan.inverse(torch.from_numpy(packed).cuda()).cpu().numpy()
+    ref = osh.inverse_lm(a, L)
+    assert rel_err(g, ref) < 1e-12
+    back = plan.forward(torch.from_numpy(ref).cuda()).cpu().numpy()
+    assert rel_err(back, packed) < 1e-12
+    # forward on non-band-limited data follows the same (documented) algorithm
+    f = rng.standard_normal((2, L + 1, 2 * L))
+    f[:, 0, :] = f[:, 0, :1]
+    f[:, -1, :] = f[:, -1, :1]
+    fo = osh.forward_lm(f, L)
+    fg = plan.forward(torch.from_numpy(f).cuda()).cpu().numpy()
+    for fi in range(2):
+        ref_p = np.zeros((nlm(L), 2))
+        for m in range(L):
+            for l in range(m, L):
+                i = m * L - m * (m - 1) // 2 + (l - m)
+                ref_p[i] = [fo[fi, l, m].real, fo[fi, l, m].imag]
+        assert rel_err(fg[:, fi], ref_p) < 1e-12
+
+
+def test_project_stream_matches_oracle():
+    from cmm_s2.core import mesh_functions as mf
+    from cmm_s2.core.spherical_spline import spline_interp_velocity
+    from cmm_s2.core import spherical_harmonic_tools as st
+    from oracle import meshes as om, splines as osp, sht as osh, fields as ofl
+    L = 32
+    Phi, The = np.meshgrid(om.mwss_phis(L), om.mwss_thetas(L))
+    w0 = ofl.rossby_wave(Phi, The) + 0.3 * ofl.gaussian_vortex(Phi, The)
+    D_inv, Lp, Lm, Lz = osh.operator_tables(L)
+    psi_o = osh.inv_Laplacian(w0, L, D_inv)
+    o = om.StructuredTriangulation(L)
+    Uo = osp.VelocitySpline(o, np.zeros((3, 3, 4, o.vertices.shape[0])),
+                            np.zeros((3, 3, 19, len(o.simplices))), ts=np.zeros(3))
+    osh.project_stream(psi_o, L, Lp, Lm, Lz, Uo, 0, 0 * psi_o)
+    pm = mf.structure_spherical_triangulation(L)
+    U = spline_interp_velocity(pm, torch.zeros(3, 3, 4, pm.nv, dtype=torch.float64, device="cuda"),
+                               None, ts=[0, 0, 0])
+    plan = st.SHT.get(L)
+    w_lm = plan.forward(torch.from_numpy(w0).cuda().reshape(1, L + 1, 2 * L))
+    st.project_stream_lm(plan, w_lm, U, 0, apply_dinv=True)
+    gv = U.level_vals(0).cpu().numpy()
+    assert rel_err(gv, Uo.vals[0]) < 1e-11
+    # reference-API path from (L, 2L+1) coefficients
+    psi_g = st.inv_Laplacian(torch.from_numpy(w0).cuda(), L, D_inv).cpu().numpy()
+    assert rel_err(psi_g, psi_o) < 1e-12
+    U2 = spline_interp_velocity(pm, torch.zeros(3, 3, 4, pm.nv, dtype=torch.float64, device="cuda"),
+                                None, ts=[0, 0, 0])
+    st.project_stream(psi_o, L, Lp, Lm, Lz, U2, 0, None)
+    assert rel_err(U2.level_vals(0).cpu().numpy(), Uo.vals[0]) < 1e-11
+
+
+def test_project_onto_S12_and_sampling():
+    from cmm_s2.core import mesh_functions as mf
+    from cmm_s2.core import spherical_harmonic_tools as st, dynamical_fields as df
+    from cmm_s2.core.spherical_simulations import VorticitySampler
+    from oracle import meshes as om, sht as osh, fields as ofl
+    L = 24
+    a = ofl.random_sh_coefficients(2, 8, L)
+    assert np.array_equal(a, df.random_sh_coefficients(2, 8, L))
+    full = osh.lm_to_full(a, L)
+    o = om.StructuredTriangulation(L)
+    so = osh.project_onto_S12(full, L, o)
+    pm = mf.structure_spherical_triangulation(L)
+    sg = st.project_onto_S12(full, L, pm)
+    assert rel_err(sg.vals.cpu().numpy(), so.vals) < 1e-11
+    # sampling w0(X) + 2 rot (cos th(X) - cos th) at arbitrary points
+    Lg = 12
+    g = mf.structure_spherical_triangulation(Lg)
+    X = random_unit((Lg + 1) * 2 * Lg, 3)
+    samp = VorticitySampler(sg, g, 2 * np.pi)
+    got = samp(torch.from_numpy(X).cuda()).cpu().numpy().reshape(Lg + 1, 2 * Lg)
+    ph, th = (np.arctan2(X[:, 1], X[:, 0]) + 2 * np.pi) % (2 * np.pi), np.arctan2(np.hypot(X[:, 0], X[:, 1]), X[:, 2])
+    ref = so(ph.reshape(Lg + 1, 2 * Lg), th.reshape(Lg + 1, 2 * Lg))
+    ref = ref + 2 * 2 * np.pi * np.cos(th.reshape(Lg + 1, -1)) - 2 * 2 * np.pi * np.cos(om.mwss_thetas(Lg))[:, None]
+    assert rel_err(got, ref) < 1e-11
+    # analytic vorticity kernels
+    for name in ["rossby_wave", "zonal_jet", "gaussian_vortex", "perturbed_zonal_jet", "multi_jet",
+                 "perturbed_rossby_wave"]:
+        samp = VorticitySampler(getattr(df, name), g, 0.0)
+        got = samp(torch.from_numpy(X).cuda()).cpu().numpy()
+        ref = getattr(ofl, name)(ph, th)
+        assert rel_err(got, ref) < 1e-12, name
