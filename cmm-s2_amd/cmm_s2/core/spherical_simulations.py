@@ -145,11 +145,11 @@ class BarotropicCMM(object):
             self.step()
         return self.submaps
 
-    def vorticity_on_grid(self, L_out, out=None):
+    def vorticity_on_grid(self, L_out, out=None, grid=None):
         """Relative vorticity w0(X) + 2 rot (cos theta(X) - cos theta) of the
         composed map on the MWSS grid of bandlimit L_out (the upsampling of
         figures/data_gen_experiments.py:122-144)."""
-        g = structure_spherical_triangulation(L_out)
+        g = grid if grid is not None else structure_spherical_triangulation(L_out)
         q = g.sample_points()
         X = self.submaps.compose(q, "N3", pre=[] if self.identity else [self.curr.packed])
         return self.sampler(X, out=out, cos_grid=g.d_cos_theta, ring_len=2 * L_out).reshape(L_out + 1, 2 * L_out)

@@ -62,6 +62,12 @@ __host__ __device__ inline d3 bary(d3 v1, d3 v2, d3 v3, d3 q) {
   return d3{det3(q, v2, v3) / den, det3(v1, q, v3) / den, det3(v1, v2, q) / den};
 }
 
+// same with one reciprocal (hot paths; differs from bary by rounding)
+__host__ __device__ inline d3 bary_fast(d3 v1, d3 v2, d3 v3, d3 q) {
+  double r = 1.0 / det3(v1, v2, v3);
+  return d3{det3(q, v2, v3) * r, det3(v1, q, v3) * r, det3(v1, v2, q) * r};
+}
+
 // utils.div_norm on one point: x / |x|
 __host__ __device__ inline d3 unit(d3 a) {
   double n = sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
@@ -129,7 +135,19 @@ struct PSGeom {
   d3 H[9];        // h12,h21,h23,h32,h13,h31,h41,h42,h43
   double G[10][3];// g12,g21,g23,g32,g13,g31,g14,g24,g34,mid
   double E[3][3]; // bary of e1,e2,e3
+  double Gi[9];   // 1/G of the divisor entry of coefficients c3..c11 (the
+                  // reference also forms 1/g first and multiplies)
 };
+
+// (G row, column) of the divisor and H index of coefficients c3..c11
+__host__ __device__ inline int ps_div_row(int k) {
+  const int r[9] = {0, 6, 4, 2, 7, 1, 5, 8, 3};
+  return r[k];
+}
+__host__ __device__ inline int ps_div_col(int k) {
+  const int c[9] = {1, 1, 2, 1, 1, 2, 1, 1, 2};
+  return c[k];
+}
 
 __host__ __device__ inline void ps_geometry(d3 v1, d3 v2, d3 v3, PSGeom& o) {
   // the reference normalises (v1+v2+v3)/3 and v/2 + w/2
@@ -161,6 +179,65 @@ __host__ __device__ inline void ps_geometry(d3 v1, d3 v2, d3 v3, PSGeom& o) {
   g = bary(v1, v2, v3, e1); o.E[0][0] = g.x; o.E[0][1] = g.y; o.E[0][2] = g.z;
   g = bary(v1, v2, v3, e2); o.E[1][0] = g.x; o.E[1][1] = g.y; o.E[1][2] = g.z;
   g = bary(v1, v2, v3, e3); o.E[2][0] = g.x; o.E[2][1] = g.y; o.E[2][2] = g.z;
+  for (int k = 0; k < 9; ++k) o.Gi[k] = 1 / o.G[ps_div_row(k)][ps_div_col(k)];
+}
+
+// PSGeom as 66 doubles (H 27, G 30, E 9) at p[i * stride]
+constexpr int kPSGeomLen = 75;
+__host__ __device__ inline void ps_pack(const PSGeom& P, double* p, int64_t stride) {
+  int k = 0;
+  for (int i = 0; i < 9; ++i) { p[(k++) * stride] = P.H[i].x; p[(k++) * stride] = P.H[i].y; p[(k++) * stride] = P.H[i].z; }
+  for (int i = 0; i < 10; ++i) for (int j = 0; j < 3; ++j) p[(k++) * stride] = P.G[i][j];
+  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) p[(k++) * stride] = P.E[i][j];
+  for (int i = 0; i < 9; ++i) p[(k++) * stride] = P.Gi[i];
+}
+__host__ __device__ inline void ps_unpack(const double* __restrict__ p, int64_t stride, PSGeom& P) {
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    double x = p[(k++) * stride], y = p[(k++) * stride], z = p[(k++) * stride];
+    P.H[i] = d3{x, y, z};
+  }
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P.G[i][j] = p[(k++) * stride];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P.E[i][j] = p[(k++) * stride];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) P.Gi[i] = p[(k++) * stride];
+}
+
+// Write the packed evaluation layout (Ntri, 6, NC, 6) of a block of up to
+// 256 triangles through LDS in rounds of 64 triangles so that the global
+// stores are contiguous.  cf[c][k] are this thread's coefficients.
+template <int NC>
+__device__ inline void store_packed_block(const double (&cf)[NC][19], int64_t t0, int64_t ntri,
+                                          double* __restrict__ packed, double* lds) {
+  constexpr int PER = 6 * NC * 6;
+  const int tid = threadIdx.x;
+  const int64_t t = t0 + tid;
+  for (int r = 0; r < 4; ++r) {
+    if ((tid >> 6) == r && t < ntri) {
+      double* dst = lds + (tid & 63) * PER;
+#pragma unroll
+      for (int s = 0; s < 6; ++s)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int j = 0; j < 6; ++j) dst[(s * NC + c) * 6 + j] = cf[c][cs_table(s, j)];
+    }
+    __syncthreads();
+    const int64_t tr = t0 + 64 * r;
+    if (tr < ntri) {
+      const int nvalid = (int)((ntri - tr) < 64 ? (ntri - tr) : 64);
+      double* out = packed + tr * PER;
+      for (int e = tid; e < nvalid * PER; e += blockDim.x) out[e] = lds[e];
+    }
+    __syncthreads();
+  }
 }
 
 __host__ __device__ inline void ps_coeffs(const PSGeom& P, const double f[3], const d3 gr[3],
@@ -168,18 +245,13 @@ __host__ __device__ inline void ps_coeffs(const PSGeom& P, const double f[3], co
   c[0] = f[0];
   c[1] = f[1];
   c[2] = f[2];
-  auto vert = [&](int Gi, int col, int Hi, int i) {
-    return (1 / P.G[Gi][col]) * (dot(P.H[Hi], gr[i]) / 2 - P.G[Gi][0] * f[i]);
-  };
-  c[3] = vert(0, 1, 0, 0);
-  c[4] = vert(6, 1, 6, 0);
-  c[5] = vert(4, 2, 4, 0);
-  c[6] = vert(2, 1, 2, 1);
-  c[7] = vert(7, 1, 7, 1);
-  c[8] = vert(1, 2, 1, 1);
-  c[9] = vert(5, 1, 5, 2);
-  c[10] = vert(8, 1, 8, 2);
-  c[11] = vert(3, 2, 3, 2);
+  // c_{3+k} = (1/g)(h . grad f(v_i) / 2 - g_0 f(v_i)) with g = G[row], h = H[row]
+  // and i = k / 3 (spherical_spline.py:314-324)
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int r = ps_div_row(k), i = k / 3;
+    c[3 + k] = P.Gi[k] * (dot(P.H[r], gr[i]) / 2 - P.G[r][0] * f[i]);
+  }
   c[12] = P.E[0][0] * c[3] + P.E[0][1] * c[8];
   c[13] = P.E[1][1] * c[6] + P.E[1][2] * c[11];
   c[14] = P.E[2][2] * c[9] + P.E[2][0] * c[5];

@@ -9,6 +9,7 @@
 // the packed coefficients (evaluation, 6 per component per level, one
 // 144-byte block) are read from HBM.
 #include <cmath>
+#include <vector>
 
 #include "common.h"
 
@@ -23,6 +24,7 @@ struct cmm_mwss_mesh {
   double* ct = nullptr;      // cos theta (M)
   double* sp = nullptr;      // sin phi (N)
   double* cp = nullptr;      // cos phi (N)
+  double* rowgeom = nullptr; // (L, 2, kPSGeomLen) Powell-Sabin geometry of column 0 per (row, half)
 };
 
 extern "C" int cmm_mwss_mesh_create(int L, const double* thetas, const double* phis,
@@ -55,6 +57,29 @@ extern "C" int cmm_mwss_mesh_create(int L, const double* thetas, const double* p
     cmm_mwss_mesh_destroy(m);
     return hip_check(e, "cmm_mwss_mesh_create: hipMemcpy");
   }
+  // Powell-Sabin geometry of the column-0 triangle of every (row, half): the
+  // mesh is invariant under rotation about z by phi_c, so the assembly
+  // rotates the tangent vectors H by phi_c and reuses G and E as they are
+  {
+    auto V = [&](int r, int c) { return d3{sin_t[r] * cos_p[c], sin_t[r] * sin_p[c], cos_t[r]}; };
+    std::vector<double> tab((size_t)L * 2 * kPSGeomLen);
+    for (int r = 0; r < L; ++r)
+      for (int h = 0; h < 2; ++h) {
+        d3 a, b, c;
+        if (r == 0) { a = V(0, 0); b = V(1, 0); c = V(1, 1); }
+        else if (r == L - 1) { a = V(L - 1, 0); b = V(L, 0); c = V(L - 1, 1); }
+        else if (h == 0) { a = V(r, 0); b = V(r + 1, 1); c = V(r, 1); }
+        else { a = V(r, 0); b = V(r + 1, 0); c = V(r + 1, 1); }
+        PSGeom P;
+        ps_geometry(a, b, c, P);
+        ps_pack(P, &tab[((size_t)r * 2 + h) * kPSGeomLen], 1);
+      }
+    if ((e = hipMalloc(&m->rowgeom, tab.size() * 8)) != hipSuccess ||
+        (e = hipMemcpy(m->rowgeom, tab.data(), tab.size() * 8, hipMemcpyHostToDevice)) != hipSuccess) {
+      cmm_mwss_mesh_destroy(m);
+      return hip_check(e, "cmm_mwss_mesh_create: geometry table");
+    }
+  }
   *out = m;
   return CMM_OK;
 }
@@ -66,6 +91,7 @@ extern "C" int cmm_mwss_mesh_destroy(cmm_mwss_mesh* m) {
   (void)hipFree(m->ct);
   (void)hipFree(m->sp);
   (void)hipFree(m->cp);
+  (void)hipFree(m->rowgeom);
   delete m;
   return CMM_OK;
 }
@@ -85,7 +111,7 @@ __device__ inline d3 ring_pt(const cmm_mwss_mesh& M, int r, int c) {
 }
 
 // numpy-faithful decisions: no FMA contraction in the tests that pick cells
-__device__ __noinline__ bool mwss_locate(const cmm_mwss_mesh& M, d3 q, MLoc& o) {
+__device__ __forceinline__ bool mwss_locate(const cmm_mwss_mesh& M, d3 q, MLoc& o) {
 #pragma clang fp contract(off)
   const double TWO_PI = 2 * M_PI;
   double phi = npy_mod(atan2(q.y, q.x) + TWO_PI, TWO_PI);
@@ -103,8 +129,11 @@ __device__ __noinline__ bool mwss_locate(const cmm_mwss_mesh& M, d3 q, MLoc& o) 
   bool in0 = (thc <= 0.03) && (tl != 0);
   bool in1 = (thc > 0.85) && (tl != Mr - 2);
   if (in0 || in1) {
-    double sth = sin(theta), cth = cos(theta);
-    d3 qq = d3{sth * cos(phin), sth * sin(phin), cth};
+    // the reference re-derives the point from its angles here
+    // (sphere2cart(phi, theta), mesh_functions.py:240, 249); that differs
+    // from q by rounding only, which can change the side of a great circle
+    // only for points on it (a location tie)
+    const d3 qq = q;
     if (in0) {
       d3 n = cross(ring_pt(M, tl, pl), ring_pt(M, tl, pp1));
       double dd = n.x * qq.x + n.y * qq.y + n.z * qq.z;
@@ -139,7 +168,7 @@ __device__ __noinline__ bool mwss_locate(const cmm_mwss_mesh& M, d3 q, MLoc& o) 
       o.v[0] = ring_pt(M, tl, pl); o.v[1] = ring_pt(M, tl + 1, pl); o.v[2] = ring_pt(M, tl + 1, pp1);
     }
   }
-  o.b = bary(o.v[0], o.v[1], o.v[2], q);
+  o.b = bary_fast(o.v[0], o.v[1], o.v[2], q);
   return true;
 }
 
@@ -188,17 +217,26 @@ struct LevelSet {
 };
 
 // sum_k w_k * PS_k(q) for ncomp components
-__device__ inline bool mwss_eval_levels(const cmm_mwss_mesh& M, const LevelSet& ls, int ncomp, d3 q,
+__device__ __forceinline__ bool mwss_eval_levels(const cmm_mwss_mesh& M, const LevelSet& ls, int ncomp, d3 q,
                                         double* r) {
   MLoc o;
   if (!mwss_locate(M, q, o)) return false;
   int s = split_index(o.b.x, o.b.y, o.b.z);
   d3 cb = sub_bary_v(s, o.b, o.v);
-  for (int c = 0; c < ncomp; ++c) r[c] = 0;
-  for (int k = 0; k < ls.nlev; ++k) {
-    const double* p = ls.p[k] + (o.tri * 6 + s) * 6 * ncomp;
-    double w = ls.w[k];
-    for (int c = 0; c < ncomp; ++c) r[c] += w * bb6(cb, p + 6 * c);
+  // compile-time trip counts with guards: runtime-indexed arrays would be
+  // placed in scratch memory
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    if (c < ncomp) r[c] = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (k < ls.nlev) {
+      const double* p = ls.p[k] + (o.tri * 6 + s) * 6 * ncomp;
+      const double w = ls.w[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (c < ncomp) r[c] += w * bb6(cb, p + 6 * c);
+    }
   }
   return true;
 }
@@ -260,7 +298,7 @@ struct VelDev {
   double params[8];
 };
 
-__device__ inline d3 vel_eval(const VelDev& V, int stage, d3 x) {
+__device__ __forceinline__ d3 vel_eval(const VelDev& V, int stage, d3 x) {
   switch (V.kind) {
     case CMM_VEL_SPLINE: {
       double r[3];
@@ -282,7 +320,7 @@ __device__ inline d3 proj_norm(d3 x) {
   return d3{inv * x.x, inv * x.y, inv * x.z};
 }
 
-__global__ void k_advect(VelDev V, int integ, double dt, const double* __restrict__ x, int64_t m,
+__global__ __launch_bounds__(128) void k_advect(VelDev V, int integ, double dt, const double* __restrict__ x, int64_t m,
                          double* __restrict__ y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
@@ -320,7 +358,7 @@ __global__ void k_advect(VelDev V, int integ, double dt, const double* __restric
   y[2 * m + i] = out.z;
 }
 
-__global__ void k_mwss_locate(cmm_mwss_mesh M, const double* __restrict__ q, int64_t n, int64_t sp,
+__global__ __launch_bounds__(256) void k_mwss_locate(cmm_mwss_mesh M, const double* __restrict__ q, int64_t n, int64_t sp,
                               int64_t sc, int32_t* __restrict__ tri, double* __restrict__ bcc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -335,7 +373,7 @@ __global__ void k_mwss_locate(cmm_mwss_mesh M, const double* __restrict__ q, int
   }
 }
 
-__global__ void k_mwss_eval(cmm_mwss_mesh M, LevelSet ls, int ncomp, const double* __restrict__ q,
+__global__ __launch_bounds__(256) void k_mwss_eval(cmm_mwss_mesh M, LevelSet ls, int ncomp, const double* __restrict__ q,
                             int64_t n, int64_t sp, int64_t sc, double* __restrict__ out, int64_t osp,
                             int64_t osc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -347,37 +385,58 @@ __global__ void k_mwss_eval(cmm_mwss_mesh M, LevelSet ls, int ncomp, const doubl
   for (int c = 0; c < ncomp; ++c) out[i * osp + c * osc] = r[c];
 }
 
-__global__ void k_mwss_assemble(cmm_mwss_mesh M, const double* __restrict__ vals, int ncomp,
-                                double* __restrict__ coeffs, double* __restrict__ packed) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= M.Tu) return;
-  int64_t vi[3];
-  d3 v[3];
-  mwss_tri_vertices(M, t, vi, v);
-  PSGeom P;
-  ps_geometry(v[0], v[1], v[2], P);
-  const int64_t Nu = M.Nu;
-  for (int c = 0; c < ncomp; ++c) {
-    double f[3];
-    d3 gr[3];
-    for (int i = 0; i < 3; ++i) {
-      if (ncomp == 1) {
-        f[i] = vals[vi[i]];
-        gr[i] = d3{vals[Nu + vi[i]], vals[2 * Nu + vi[i]], vals[3 * Nu + vi[i]]};
-      } else {
-        f[i] = vals[(c * 4) * Nu + vi[i]];
-        gr[i] = d3{vals[(0 * 4 + 1 + c) * Nu + vi[i]], vals[(1 * 4 + 1 + c) * Nu + vi[i]],
-                   vals[(2 * 4 + 1 + c) * Nu + vi[i]]};
-      }
+template <int NC>
+__global__ __launch_bounds__(256) void k_mwss_assemble(cmm_mwss_mesh M, const double* __restrict__ vals,
+                                                       double* __restrict__ coeffs, double* __restrict__ packed) {
+  __shared__ double lds[64 * 6 * NC * 6];
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t t = t0 + threadIdx.x;
+  double cf[NC][19];
+  if (t < M.Tu) {
+    const int N = M.N, Mr = M.M;
+    int row, col, half;
+    if (t < N) { row = 0; col = (int)t; half = 0; }
+    else if (t >= M.Tu - N) { row = Mr - 2; col = (int)(t - (M.Tu - N)); half = 0; }
+    else {
+      int64_t u = t - N;
+      half = (int)(u % 2);
+      int64_t k = u / 2;
+      row = 1 + (int)(k / N);
+      col = (int)(k % N);
     }
-    double cf[19];
-    ps_coeffs(P, f, gr, cf);
-    if (coeffs)
-      for (int k = 0; k < 19; ++k) coeffs[((int64_t)c * 19 + k) * M.Tu + t] = cf[k];
-    if (packed)
-      for (int s = 0; s < 6; ++s)
-        for (int j = 0; j < 6; ++j) packed[((t * 6 + s) * ncomp + c) * 6 + j] = cf[cs_table(s, j)];
+    int64_t vi[3];
+    d3 v[3];
+    mwss_tri_vertices(M, t, vi, v);
+    PSGeom P;
+    ps_unpack(M.rowgeom + ((int64_t)row * 2 + half) * kPSGeomLen, 1, P);
+    const double c = M.cp[col], sn = M.sp[col];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      d3 h = P.H[i];
+      P.H[i] = d3{c * h.x - sn * h.y, sn * h.x + c * h.y, h.z};
+    }
+    const int64_t Nu = M.Nu;
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) {
+      double f[3];
+      d3 gr[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        if (NC == 1) {
+          f[i] = vals[vi[i]];
+          gr[i] = d3{vals[Nu + vi[i]], vals[2 * Nu + vi[i]], vals[3 * Nu + vi[i]]};
+        } else {
+          f[i] = vals[(cc * 4) * Nu + vi[i]];
+          gr[i] = d3{vals[(0 * 4 + 1 + cc) * Nu + vi[i]], vals[(1 * 4 + 1 + cc) * Nu + vi[i]],
+                     vals[(2 * 4 + 1 + cc) * Nu + vi[i]]};
+        }
+      }
+      ps_coeffs(P, f, gr, cf[cc]);
+      if (coeffs)
+        for (int k = 0; k < 19; ++k) coeffs[((int64_t)cc * 19 + k) * M.Tu + t] = cf[cc][k];
+    }
   }
+  if (packed) store_packed_block<NC>(cf, t0, M.Tu, packed, lds);
 }
 
 // grid index (ring, column) of mesh vertex v: poles take the phi = 0 sample
@@ -394,7 +453,7 @@ __device__ inline d3 rot_grad(d3 g, d3 s) {
   return d3{a.x - scl_ * s.x, a.y - scl_ * s.y, a.z - scl_ * s.z};
 }
 
-__global__ void k_velocity_jets(cmm_mwss_mesh M, const double* __restrict__ F, double* __restrict__ vals) {
+__global__ __launch_bounds__(256) void k_velocity_jets(cmm_mwss_mesh M, const double* __restrict__ F, double* __restrict__ vals) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= M.Nu) return;
   int t, p;
@@ -413,7 +472,7 @@ __global__ void k_velocity_jets(cmm_mwss_mesh M, const double* __restrict__ F, d
   }
 }
 
-__global__ void k_scalar_jets(cmm_mwss_mesh M, const double* __restrict__ F, double* __restrict__ vals) {
+__global__ __launch_bounds__(256) void k_scalar_jets(cmm_mwss_mesh M, const double* __restrict__ F, double* __restrict__ vals) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= M.Nu) return;
   int t, p;
@@ -469,7 +528,7 @@ __device__ inline double vort_analytic(int kind, double phi, double theta, const
   }
 }
 
-__global__ void k_sample_vorticity(VortArgs A, cmm_mwss_mesh W, const double* __restrict__ wpacked,
+__global__ __launch_bounds__(256) void k_sample_vorticity(VortArgs A, cmm_mwss_mesh W, const double* __restrict__ wpacked,
                                    const double* __restrict__ X, int64_t n, int64_t sp, int64_t sc,
                                    const double* __restrict__ cosg, double* __restrict__ out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -511,8 +570,12 @@ extern "C" int cmm_mwss_assemble(const cmm_mwss_mesh* m, const double* vals, int
                                  double* coeffs, double* packed, void* stream) {
   if (!m || !vals || (ncomp != 1 && ncomp != 3)) return fail(CMM_E_ARG, "cmm_mwss_assemble: bad arguments");
   if (!coeffs && !packed) return CMM_OK;
-  hipLaunchKernelGGL(k_mwss_assemble, dim3(grid_for(m->Tu, 128)), dim3(128), 0, as_stream(stream), *m,
-                     vals, ncomp, coeffs, packed);
+  if (ncomp == 3)
+    hipLaunchKernelGGL(k_mwss_assemble<3>, dim3(grid_for(m->Tu, 256)), dim3(256), 0, as_stream(stream), *m,
+                       vals, coeffs, packed);
+  else
+    hipLaunchKernelGGL(k_mwss_assemble<1>, dim3(grid_for(m->Tu, 256)), dim3(256), 0, as_stream(stream), *m,
+                       vals, coeffs, packed);
   CMM_LAUNCH_CHECK("cmm_mwss_assemble");
   return CMM_OK;
 }

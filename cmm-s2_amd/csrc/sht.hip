@@ -25,7 +25,7 @@ using namespace cmm;
 
 namespace {
 constexpr int kScaleExp = 600;
-constexpr int kMaxFFT = 4096;
+constexpr int kMaxFFT = 2048;
 }
 
 struct cmm_sht_plan {
@@ -43,6 +43,7 @@ struct cmm_sht_plan {
   double* K = nullptr;       // (2, L, L+1) resampling matrices (even, odd)
   double2* tw = nullptr;     // exp(-2 pi i n / N), n < N
   double* dinv = nullptr;    // D_inv[l] (L)
+  double2* rec = nullptr;    // recurrence (a_lm, b_lm) per packed lm
   std::vector<double> hx_g, hw_g;
 };
 
@@ -118,14 +119,14 @@ extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
   if (!p) return CMM_OK;
   (void)hipFree(p->x_t); (void)hipFree(p->dP_t); (void)hipFree(p->dS_t);
   (void)hipFree(p->x_g); (void)hipFree(p->w_g); (void)hipFree(p->dP_g); (void)hipFree(p->dS_g);
-  (void)hipFree(p->K); (void)hipFree(p->tw); (void)hipFree(p->dinv);
+  (void)hipFree(p->K); (void)hipFree(p->tw); (void)hipFree(p->dinv); (void)hipFree(p->rec);
   delete p;
   return CMM_OK;
 }
 
 extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin_t, cmm_sht_plan** out) {
   if (L < 2 || !cos_t || !sin_t || !out) return fail(CMM_E_ARG, "cmm_sht_plan_create: bad arguments");
-  if (2 * L > kMaxFFT) return fail(CMM_E_RANGE, "cmm_sht_plan_create: L > 2048");
+  if (2 * L > kMaxFFT) return fail(CMM_E_RANGE, "cmm_sht_plan_create: L > 1024");
   cmm_sht_plan* p = new cmm_sht_plan();
   p->L = L;
   p->N = 2 * L;
@@ -166,6 +167,16 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
     double a = -2.0 * M_PI * k / p->N;
     tw[k] = double2{std::cos(a), std::sin(a)};
   }
+  std::vector<double2> rec((size_t)L * (L + 1) / 2, double2{0.0, 0.0});
+  for (int m = 0; m < L; ++m) {
+    size_t lm0 = (size_t)m * L - (size_t)m * (m - 1) / 2;
+    for (int l = m + 1; l < L; ++l) {
+      double a = std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+      double lm1 = l - 1.0;
+      double b = std::sqrt((lm1 * lm1 - (double)m * m) / (4.0 * lm1 * lm1 - 1.0));
+      rec[lm0 + (l - m)] = double2{a, b};
+    }
+  }
   std::vector<double> dinv(L);
   for (int l = 0; l < L; ++l) dinv[l] = l == 0 ? 0.0 : -1.0 / (double)((int64_t)l * (l + 1));
   int rc;
@@ -173,7 +184,8 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
       (rc = upload(&p->dS_t, St.data(), St.size())) || (rc = upload(&p->x_g, xg.data(), xg.size())) ||
       (rc = upload(&p->w_g, wg.data(), wg.size())) || (rc = upload(&p->dP_g, Pg.data(), Pg.size())) ||
       (rc = upload(&p->dS_g, Sg.data(), Sg.size())) || (rc = upload(&p->K, K.data(), K.size())) ||
-      (rc = upload(&p->tw, tw.data(), tw.size())) || (rc = upload(&p->dinv, dinv.data(), dinv.size()))) {
+      (rc = upload(&p->tw, tw.data(), tw.size())) || (rc = upload(&p->dinv, dinv.data(), dinv.size())) ||
+      (rc = upload(&p->rec, rec.data(), rec.size()))) {
     cmm_sht_plan_destroy(p);
     return rc;
   }
@@ -277,7 +289,7 @@ __device__ double2* fft_lds(const FFTPlanDev& P, const double2* __restrict__ tw,
 }
 
 // real rings -> half spectrum F[f][t][m] (m < L), scaled by pi / L
-__global__ void k_fft_r2c(FFTPlanDev P, const double2* __restrict__ tw, const double* __restrict__ f,
+__global__ __launch_bounds__(256) void k_fft_r2c(FFTPlanDev P, const double2* __restrict__ tw, const double* __restrict__ f,
                           int L, double2* __restrict__ F) {
   extern __shared__ double2 lds[];
   double2* b0 = lds;
@@ -303,7 +315,7 @@ __global__ void k_fft_r2c(FFTPlanDev P, const double2* __restrict__ tw, const do
 }
 
 // half spectrum F[f][t][m] -> real rings (irfft * 2L)
-__global__ void k_fft_c2r(FFTPlanDev P, const double2* __restrict__ tw, const double2* __restrict__ F,
+__global__ __launch_bounds__(256) void k_fft_c2r(FFTPlanDev P, const double2* __restrict__ tw, const double2* __restrict__ F,
                           int L, double* __restrict__ f) {
   extern __shared__ double2 lds[];
   double2* b0 = lds;
@@ -349,87 +361,95 @@ __global__ void k_fft_c2r(FFTPlanDev P, const double2* __restrict__ tw, const do
 }
 
 // ---------------------------------------------------------------------------
-// Legendre recurrence helpers
+// Legendre recurrence: Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m}),
+// a_lm = sqrt((4l^2-1)/(l^2-m^2)), b_lm = sqrt(((l-1)^2-m^2)/(4(l-1)^2-1)),
+// tabulated per packed (l, m) so the per-lane work is two FMAs and the
+// coefficients arrive by uniform (scalar) loads.  At l = m+1, b = 0.
 // ---------------------------------------------------------------------------
-struct LegState {
-  double p0, p1;
-  int s;
-};
-
-__device__ inline double leg_a(int l, int m) {
-  return sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+__device__ inline void leg_step(double a, double b, double x, double& pp, double& pc, int& s) {
+  const double inv_big = 0x1p-600, hi = 0x1p300;
+  double pn = a * (x * pc - b * pp);
+  pp = pc;
+  pc = pn;
+  if (s < 0 && fabs(pc) > hi) {
+    pc *= inv_big;
+    pp *= inv_big;
+    s += 1;
+  }
 }
-__device__ inline double leg_b(int l, int m) {
-  double lm1 = l - 1.0;
-  return sqrt((lm1 * lm1 - (double)m * m) / (4.0 * lm1 * lm1 - 1.0));
-}
 
 // ---------------------------------------------------------------------------
-// inverse Legendre: F[f][t][m] for all rings from flm [lm][nf]
+// inverse Legendre: F[f][t][m] for all rings from flm [lm][nf].
+// One block per pair of orders (m, L-1-m): every block runs L+1 recurrence
+// steps, which balances the triangular (l >= m) work over the CUs.  Threads
+// own the northern rings t <= L/2; even / odd l+m partial sums give ring t
+// and its mirror L-t.  The coefficients of a chunk of l are staged through
+// LDS once per block and read back as broadcasts.
 // ---------------------------------------------------------------------------
-constexpr int kInvFields = 12;
+constexpr int kInvChunk = 32;
 
-__global__ void k_leg_inverse(int L, int nf, int f0, int nfb, const double* __restrict__ x_t,
-                              const double* __restrict__ dP, const int* __restrict__ dS,
-                              const double2* __restrict__ flm, double2* __restrict__ F) {
+template <int NFB>
+__global__ __launch_bounds__(576) void k_leg_inverse(int L, int nf, int f0, const double* __restrict__ x_t,
+                                                     const double* __restrict__ dP, const int* __restrict__ dS,
+                                                     const double2* __restrict__ rec,
+                                                     const double2* __restrict__ flm, double2* __restrict__ F) {
+  __shared__ double2 cs[kInvChunk][NFB];
+  __shared__ double2 rs[kInvChunk];
   const int nh = L / 2 + 1;  // northern rings incl. equator (t <= L/2)
-  const int chunks = (nh + 63) / 64;
-  const int m = blockIdx.x / chunks;
-  const int t = (blockIdx.x % chunks) * 64 + threadIdx.x;
+  const int t = threadIdx.x;
   const bool active = t < nh;
   const int tt = active ? t : 0;
   const double x = x_t[tt];
-  double2 E[kInvFields], O[kInvFields];
-  for (int k = 0; k < kInvFields; ++k) { E[k] = double2{0, 0}; O[k] = double2{0, 0}; }
-  const double inv_big = ldexp(1.0, -kScaleExp), hi = ldexp(1.0, 300);
-  double p0 = dP[(int64_t)m * (L + 1) + tt];
-  int s = dS[(int64_t)m * (L + 1) + tt];
-  double p1 = 0.0;
-  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
-  for (int l = m; l < L; ++l) {
-    double pv;
-    if (l == m) {
-      pv = p0;
-    } else if (l == m + 1) {
-      p1 = sqrt(2.0 * m + 3.0) * x * p0;
-      pv = p1;
-    } else {
-      double p2 = leg_a(l, m) * (x * p1 - leg_b(l, m) * p0);
-      if (s < 0 && fabs(p2) > hi) {
-        p2 *= inv_big;
-        p1 *= inv_big;
-        s += 1;
+  const int nm = ((int)blockIdx.x == L - 1 - (int)blockIdx.x) ? 1 : 2;
+  for (int q = 0; q < nm; ++q) {
+    const int m = q == 0 ? (int)blockIdx.x : L - 1 - (int)blockIdx.x;
+    double2 E[NFB], O[NFB];
+#pragma unroll
+    for (int k = 0; k < NFB; ++k) { E[k] = double2{0, 0}; O[k] = double2{0, 0}; }
+    double pp = 0.0, pc = dP[(int64_t)m * (L + 1) + tt];
+    int s = dS[(int64_t)m * (L + 1) + tt];
+    const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+    for (int lc = m; lc < L; lc += kInvChunk) {
+      const int n = min(kInvChunk, L - lc);
+      __syncthreads();
+      for (int e = threadIdx.x; e < n * NFB; e += blockDim.x) {
+        const int i = e / NFB, k = e % NFB;
+        cs[i][k] = flm[(lm0 + (lc - m) + i) * nf + f0 + k];
       }
-      p0 = p1;
-      p1 = p2;
-      pv = p2;
+      for (int i = threadIdx.x; i < n; i += blockDim.x) rs[i] = rec[lm0 + (lc - m) + i];
+      __syncthreads();
+      for (int i = 0; i < n; ++i) {
+        const int l = lc + i;
+        if (l > m) {
+          double2 ab = rs[i];
+          leg_step(ab.x, ab.y, x, pp, pc, s);
+        }
+        const double pv = (s == 0) ? pc : 0.0;
+        if (((l - m) & 1) == 0) {
+#pragma unroll
+          for (int k = 0; k < NFB; ++k) { E[k].x += pv * cs[i][k].x; E[k].y += pv * cs[i][k].y; }
+        } else {
+#pragma unroll
+          for (int k = 0; k < NFB; ++k) { O[k].x += pv * cs[i][k].x; O[k].y += pv * cs[i][k].y; }
+        }
+      }
     }
-    if (s != 0) continue;
-    const double2* a = flm + (lm0 + (l - m)) * nf + f0;
-    if (((l + m) & 1) == 0) {
+    if (active) {
 #pragma unroll
-      for (int k = 0; k < kInvFields; ++k)
-        if (k < nfb) { E[k].x += pv * a[k].x; E[k].y += pv * a[k].y; }
-    } else {
-#pragma unroll
-      for (int k = 0; k < kInvFields; ++k)
-        if (k < nfb) { O[k].x += pv * a[k].x; O[k].y += pv * a[k].y; }
+      for (int k = 0; k < NFB; ++k) {
+        const int fi = f0 + k;
+        F[((int64_t)fi * (L + 1) + t) * L + m] = double2{E[k].x + O[k].x, E[k].y + O[k].y};
+        if (L - t != t)
+          F[((int64_t)fi * (L + 1) + (L - t)) * L + m] = double2{E[k].x - O[k].x, E[k].y - O[k].y};
+      }
     }
-  }
-  if (!active) return;
-#pragma unroll
-  for (int k = 0; k < kInvFields; ++k) {
-    if (k >= nfb) break;
-    int fi = f0 + k;
-    F[((int64_t)fi * (L + 1) + t) * L + m] = double2{E[k].x + O[k].x, E[k].y + O[k].y};
-    if (L - t != t) F[((int64_t)fi * (L + 1) + (L - t)) * L + m] = double2{E[k].x - O[k].x, E[k].y - O[k].y};
   }
 }
 
 // ---------------------------------------------------------------------------
 // forward: resampling GEMM G[f][j][m] = sum_t K_{m%2}[j][t] F[f][t][m]
 // ---------------------------------------------------------------------------
-__global__ void k_resample(int L, const double* __restrict__ K, const double2* __restrict__ F,
+__global__ __launch_bounds__(256) void k_resample(int L, const double* __restrict__ K, const double2* __restrict__ F,
                            double2* __restrict__ G) {
   // block: 16 nodes j x 64 m-columns; grid: (ceil(L/64), ceil(L/16), nf)
   __shared__ double Ks[2][16][17];
@@ -470,79 +490,84 @@ __global__ void k_resample(int L, const double* __restrict__ K, const double2* _
 }
 
 // ---------------------------------------------------------------------------
-// forward Legendre analysis: flm[lm][f] = sum_j w_j Pbar_lm(x_j) G[f][j][m]
-// block per (m, field); threads over the northern GL nodes
+// forward Legendre analysis: flm[lm][f] = sum_j w_j Pbar_lm(x_j) G[f][j][m].
+// One block per pair of orders (m, L-1-m) and field; threads own the
+// northern Gauss-Legendre nodes.  For a chunk of kAnaChunk degrees every
+// thread writes its 2*kAnaChunk products to LDS, then groups of 8 lanes sum
+// columns of that tile and finish with 3 butterfly steps.
 // ---------------------------------------------------------------------------
 constexpr int kAnaChunk = 8;
 
-__global__ void k_leg_forward(int L, int nf, const double* __restrict__ xg, const double* __restrict__ wg,
-                              const double* __restrict__ dP, const int* __restrict__ dS,
-                              const double2* __restrict__ G, double2* __restrict__ flm) {
-  extern __shared__ double red[];  // [kAnaChunk*2][nthreads]
-  const int m = blockIdx.x, fi = blockIdx.y;
+__global__ __launch_bounds__(512) void k_leg_forward(int L, int nf, const double* __restrict__ xg,
+                                                     const double* __restrict__ wg, const double* __restrict__ dP,
+                                                     const int* __restrict__ dS, const double2* __restrict__ rec,
+                                                     const double2* __restrict__ G, double2* __restrict__ flm) {
+  extern __shared__ double tile[];  // [2*kAnaChunk][T]
+  __shared__ double2 rs[kAnaChunk];
+  const int fi = blockIdx.y;
   const int nh = (L + 1) / 2;  // northern nodes (incl. the middle one when L is odd)
-  const int T = blockDim.x;    // power of two >= nh
+  const int T = blockDim.x;    // multiple of 64 >= nh
   const int j = threadIdx.x;
   const bool active = j < nh;
   const int jj = active ? j : 0;
   const int jm = L - 1 - jj;
   const double x = xg[jj];
-  double2 S = {0, 0}, D = {0, 0};
-  if (active) {
-    double2 a = G[((int64_t)fi * L + jj) * L + m];
-    double w = wg[jj];
-    if (jm != jj) {
-      double2 b = G[((int64_t)fi * L + jm) * L + m];
-      S = double2{w * (a.x + b.x), w * (a.y + b.y)};
-      D = double2{w * (a.x - b.x), w * (a.y - b.y)};
-    } else {
-      S = double2{w * a.x, w * a.y};
-      D = S;
+  const int nm = ((int)blockIdx.x == L - 1 - (int)blockIdx.x) ? 1 : 2;
+  // reduction role: output o = j / 8 (of 2*kAnaChunk), part = j % 8
+  const int o = j >> 3, part = j & 7;
+  for (int q = 0; q < nm; ++q) {
+    const int m = q == 0 ? (int)blockIdx.x : L - 1 - (int)blockIdx.x;
+    double2 S = {0, 0}, D = {0, 0};
+    if (active) {
+      double2 a = G[((int64_t)fi * L + jj) * L + m];
+      double w = wg[jj];
+      if (jm != jj) {
+        double2 b = G[((int64_t)fi * L + jm) * L + m];
+        S = double2{w * (a.x + b.x), w * (a.y + b.y)};
+        D = double2{w * (a.x - b.x), w * (a.y - b.y)};
+      } else {
+        S = double2{w * a.x, w * a.y};
+        D = S;
+      }
     }
-  }
-  const double inv_big = ldexp(1.0, -kScaleExp), hi = ldexp(1.0, 300);
-  double p0 = dP[(int64_t)m * L + jj];
-  int s = dS[(int64_t)m * L + jj];
-  double p1 = 0.0;
-  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
-  for (int lc = m; lc < L; lc += kAnaChunk) {
-    for (int q = 0; q < kAnaChunk; ++q) {
-      int l = lc + q;
-      double pv = 0.0;
-      if (l < L) {
-        if (l == m) {
-          pv = p0;
-        } else if (l == m + 1) {
-          p1 = sqrt(2.0 * m + 3.0) * x * p0;
-          pv = p1;
-        } else {
-          double p2 = leg_a(l, m) * (x * p1 - leg_b(l, m) * p0);
-          if (s < 0 && fabs(p2) > hi) {
-            p2 *= inv_big;
-            p1 *= inv_big;
-            s += 1;
+    double pp = 0.0, pc = dP[(int64_t)m * L + jj];
+    int s = dS[(int64_t)m * L + jj];
+    const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+    for (int lc = m; lc < L; lc += kAnaChunk) {
+      if (j < kAnaChunk && lc + j < L) rs[j] = rec[lm0 + (lc - m) + j];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < kAnaChunk; ++i) {
+        const int l = lc + i;
+        double vr = 0.0, vi = 0.0;
+        if (l < L) {
+          if (l > m) {
+            double2 ab = rs[i];
+            leg_step(ab.x, ab.y, x, pp, pc, s);
           }
-          p0 = p1;
-          p1 = p2;
-          pv = p2;
+          const double pv = (s == 0 && active) ? pc : 0.0;
+          const double2 src = (((l - m) & 1) == 0) ? S : D;
+          vr = pv * src.x;
+          vi = pv * src.y;
+        }
+        tile[(2 * i) * T + j] = vr;
+        tile[(2 * i + 1) * T + j] = vi;
+      }
+      __syncthreads();
+      if (o < 2 * kAnaChunk) {
+        double acc = 0.0;
+        for (int c = part; c < T; c += 8) acc += tile[o * T + c];
+        acc += __shfl_xor(acc, 1, 8);
+        acc += __shfl_xor(acc, 2, 8);
+        acc += __shfl_xor(acc, 4, 8);
+        const int l = lc + (o >> 1);
+        if (part == 0 && l < L) {
+          double* dst = reinterpret_cast<double*>(flm + (lm0 + (l - m)) * nf + fi);
+          dst[o & 1] = acc;
         }
       }
-      if (s != 0 || !active) pv = 0.0;
-      double2 src = (((l + m) & 1) == 0) ? S : D;
-      red[(2 * q) * T + j] = pv * src.x;
-      red[(2 * q + 1) * T + j] = pv * src.y;
-    }
-    __syncthreads();
-    for (int h = T / 2; h > 0; h >>= 1) {
-      if (j < h)
-        for (int q = 0; q < 2 * kAnaChunk; ++q) red[q * T + j] += red[q * T + j + h];
       __syncthreads();
     }
-    if (j < kAnaChunk) {
-      int l = lc + j;
-      if (l < L) flm[(lm0 + (l - m)) * nf + fi] = double2{red[(2 * j) * T], red[(2 * j + 1) * T]};
-    }
-    __syncthreads();
   }
 }
 
@@ -593,7 +618,7 @@ __device__ inline void angmom(int l, int m, F get, Cplx out[3]) {
   out[2] = Cplx{0.0 * az.re - 1.0 * az.im, 0.0 * az.im + 1.0 * az.re};
 }
 
-__global__ void k_stream_ops(int L, const double2* __restrict__ w, const double* __restrict__ dinv,
+__global__ __launch_bounds__(128) void k_stream_ops(int L, const double2* __restrict__ w, const double* __restrict__ dinv,
                              int apply_dinv, double2* __restrict__ out) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t total = (int64_t)L * (L + 1) / 2;
@@ -629,7 +654,7 @@ __global__ void k_stream_ops(int L, const double2* __restrict__ w, const double*
   }
 }
 
-__global__ void k_scalar_ops(int L, const double2* __restrict__ w, double2* __restrict__ out) {
+__global__ __launch_bounds__(128) void k_scalar_ops(int L, const double2* __restrict__ w, double2* __restrict__ out) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t total = (int64_t)L * (L + 1) / 2;
   if (idx >= total) return;
@@ -667,11 +692,25 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   const int L = p->L;
   double2* F = reinterpret_cast<double2*>(work);
   hipStream_t st = as_stream(stream);
-  const int nh = L / 2 + 1, chunks = (nh + 63) / 64;
-  for (int f0 = 0; f0 < nf; f0 += kInvFields) {
-    int nfb = std::min(kInvFields, nf - f0);
-    hipLaunchKernelGGL(k_leg_inverse, dim3(L * chunks), dim3(64), 0, st, L, nf, f0, nfb, p->x_t, p->dP_t,
-                       p->dS_t, reinterpret_cast<const double2*>(flm), F);
+  const int nh = L / 2 + 1;
+  const int T = ((nh + 63) / 64) * 64;
+  if (T > 576) return fail(CMM_E_RANGE, "cmm_sht_inverse: L > 1024");
+  const dim3 grid((L + 1) / 2), block(T);
+  for (int f0 = 0; f0 < nf;) {
+    int left = nf - f0;
+    if (left >= 6) {
+      hipLaunchKernelGGL(k_leg_inverse<6>, grid, block, 0, st, L, nf, f0, p->x_t, p->dP_t, p->dS_t, p->rec,
+                         reinterpret_cast<const double2*>(flm), F);
+      f0 += 6;
+    } else if (left >= 4) {
+      hipLaunchKernelGGL(k_leg_inverse<4>, grid, block, 0, st, L, nf, f0, p->x_t, p->dP_t, p->dS_t, p->rec,
+                         reinterpret_cast<const double2*>(flm), F);
+      f0 += 4;
+    } else {
+      hipLaunchKernelGGL(k_leg_inverse<1>, grid, block, 0, st, L, nf, f0, p->x_t, p->dP_t, p->dS_t, p->rec,
+                         reinterpret_cast<const double2*>(flm), F);
+      f0 += 1;
+    }
     CMM_LAUNCH_CHECK("cmm_sht_inverse: legendre");
   }
   int pairs = (L + 2) / 2;
@@ -695,11 +734,12 @@ extern "C" int cmm_sht_forward(const cmm_sht_plan* p, const double* f, int nf, d
   hipLaunchKernelGGL(k_resample, dim3((L + 63) / 64, (L + 15) / 16, nf), dim3(256), 0, st, L, p->K, F, G);
   CMM_LAUNCH_CHECK("cmm_sht_forward: resample");
   int nh = (L + 1) / 2;
-  int T = 64;
-  while (T < nh) T *= 2;
+  int T = ((nh + 63) / 64) * 64;
+  if (T < 128) T = 128;  // 2*kAnaChunk outputs x 8 lanes
+  if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_forward: L > 1024");
   size_t lds2 = (size_t)2 * kAnaChunk * T * sizeof(double);
-  hipLaunchKernelGGL(k_leg_forward, dim3(L, nf), dim3(T), lds2, st, L, nf, p->x_g, p->w_g, p->dP_g,
-                     p->dS_g, G, reinterpret_cast<double2*>(flm));
+  hipLaunchKernelGGL(k_leg_forward, dim3((L + 1) / 2, nf), dim3(T), lds2, st, L, nf, p->x_g, p->w_g, p->dP_g,
+                     p->dS_g, p->rec, G, reinterpret_cast<double2*>(flm));
   CMM_LAUNCH_CHECK("cmm_sht_forward: legendre");
   return CMM_OK;
 }

@@ -50,6 +50,7 @@ struct cmm_tri_mesh {
   int32_t* over = nullptr;
   double* verts = nullptr;
   int32_t* tris = nullptr;
+  double* psgeom = nullptr;  // (kPSGeomLen, Ntri) Powell-Sabin geometry, structure of arrays
   int device = 0;
 };
 
@@ -169,6 +170,7 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
   std::vector<int32_t> tris((size_t)ntri * 3);
   std::vector<std::pair<int32_t, int32_t>> pairs;
   pairs.reserve((size_t)ntri * 8);
+  std::vector<double> psg((size_t)ntri * kPSGeomLen);
   for (int64_t t = 0; t < ntri; ++t) {
     d3 v1 = V(simplices[3 * t]), v2 = V(simplices[3 * t + 1]), v3 = V(simplices[3 * t + 2]);
     for (int i = 0; i < 3; ++i) tris[3 * t + i] = (int32_t)simplices[3 * t + i];
@@ -184,6 +186,9 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
     g[11] = norm3(add(v3, v1));
     g[12] = norm3(add(add(v1, v2), v3));
     raster_tri(v1, v2, v3, (int32_t)t, R, pairs);
+    PSGeom P;
+    ps_geometry(v1, v2, v3, P);
+    ps_pack(P, &psg[t], ntri);
   }
   std::sort(pairs.begin(), pairs.end());
   pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
@@ -218,12 +223,12 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
   m->max_cand = maxc;
   (void)hipGetDevice(&m->device);
   size_t bg = geom.size() * 8, bc = cells.size() * sizeof(int4), bo = over.size() * 4,
-         bv = (size_t)nv * 24, bt = tris.size() * 4;
-  m->bytes = (int64_t)(bg + bc + bo + bv + bt);
+         bv = (size_t)nv * 24, bt = tris.size() * 4, bp = psg.size() * 8;
+  m->bytes = (int64_t)(bg + bc + bo + bv + bt + bp);
   hipError_t e;
   if ((e = hipMalloc(&m->geom, bg)) != hipSuccess || (e = hipMalloc(&m->cells, bc)) != hipSuccess ||
       (e = hipMalloc(&m->over, bo)) != hipSuccess || (e = hipMalloc(&m->verts, bv)) != hipSuccess ||
-      (e = hipMalloc(&m->tris, bt)) != hipSuccess) {
+      (e = hipMalloc(&m->tris, bt)) != hipSuccess || (e = hipMalloc(&m->psgeom, bp)) != hipSuccess) {
     cmm_tri_mesh_destroy(m);
     return hip_check(e, "cmm_tri_mesh_create: hipMalloc");
   }
@@ -231,7 +236,8 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
       (e = hipMemcpy(m->cells, cells.data(), bc, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(m->over, over.data(), bo, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(m->verts, vertices, bv, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(m->tris, tris.data(), bt, hipMemcpyHostToDevice)) != hipSuccess) {
+      (e = hipMemcpy(m->tris, tris.data(), bt, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->psgeom, psg.data(), bp, hipMemcpyHostToDevice)) != hipSuccess) {
     cmm_tri_mesh_destroy(m);
     return hip_check(e, "cmm_tri_mesh_create: hipMemcpy");
   }
@@ -246,6 +252,7 @@ extern "C" int cmm_tri_mesh_destroy(cmm_tri_mesh* m) {
   (void)hipFree(m->over);
   (void)hipFree(m->verts);
   (void)hipFree(m->tris);
+  (void)hipFree(m->psgeom);
   delete m;
   return CMM_OK;
 }
@@ -324,7 +331,7 @@ __device__ inline d3 eval_level3(const double* __restrict__ packed, const Locate
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-__global__ void k_tri_locate(cmm_tri_mesh M, const double* __restrict__ q, int64_t n, int64_t sp,
+__global__ __launch_bounds__(256) void k_tri_locate(cmm_tri_mesh M, const double* __restrict__ q, int64_t n, int64_t sp,
                              int64_t sc, int32_t* __restrict__ tri, double* __restrict__ bcc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -340,7 +347,7 @@ __global__ void k_tri_locate(cmm_tri_mesh M, const double* __restrict__ q, int64
   }
 }
 
-__global__ void k_tri_eval(cmm_tri_mesh M, const double* __restrict__ packed, int ncomp, int normalize,
+__global__ __launch_bounds__(256) void k_tri_eval(cmm_tri_mesh M, const double* __restrict__ packed, int ncomp, int normalize,
                            const double* __restrict__ q, int64_t n, int64_t sp, int64_t sc,
                            double* __restrict__ out, int64_t osp, int64_t osc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,7 +371,7 @@ __global__ void k_tri_eval(cmm_tri_mesh M, const double* __restrict__ packed, in
   for (int c = 0; c < ncomp; ++c) out[i * osp + c * osc] = r[c];
 }
 
-__global__ void k_tri_compose(cmm_tri_mesh M, LevelPtrs lv, int depth, const double* __restrict__ q,
+__global__ __launch_bounds__(256) void k_tri_compose(cmm_tri_mesh M, LevelPtrs lv, int depth, const double* __restrict__ q,
                               int64_t n, int64_t sp, int64_t sc, double* __restrict__ out, int64_t osp,
                               int64_t osc, double* __restrict__ inter) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -387,41 +394,42 @@ __global__ void k_tri_compose(cmm_tri_mesh M, LevelPtrs lv, int depth, const dou
   store_pt(out, i, osp, osc, x);
 }
 
-__global__ void k_tri_assemble(cmm_tri_mesh M, const double* __restrict__ vals, int ncomp,
-                               double* __restrict__ coeffs, double* __restrict__ packed) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= M.ntri) return;
-  int vi[3] = {M.tris[3 * t], M.tris[3 * t + 1], M.tris[3 * t + 2]};
-  d3 v[3];
-  for (int i = 0; i < 3; ++i)
-    v[i] = d3{M.verts[3 * (int64_t)vi[i]], M.verts[3 * (int64_t)vi[i] + 1], M.verts[3 * (int64_t)vi[i] + 2]};
-  PSGeom P;
-  ps_geometry(v[0], v[1], v[2], P);
-  const int64_t Nv = M.nv;
-  for (int c = 0; c < ncomp; ++c) {
-    double f[3];
-    d3 gr[3];
-    for (int i = 0; i < 3; ++i) {
-      if (ncomp == 1) {
-        f[i] = vals[vi[i]];
-        gr[i] = d3{vals[Nv + vi[i]], vals[2 * Nv + vi[i]], vals[3 * Nv + vi[i]]};
-      } else {
-        f[i] = vals[(c * 4) * Nv + vi[i]];
-        gr[i] = d3{vals[(0 * 4 + 1 + c) * Nv + vi[i]], vals[(1 * 4 + 1 + c) * Nv + vi[i]],
-                   vals[(2 * 4 + 1 + c) * Nv + vi[i]]};
+template <int NC>
+__global__ __launch_bounds__(256) void k_tri_assemble(cmm_tri_mesh M, const double* __restrict__ vals,
+                                                      double* __restrict__ coeffs, double* __restrict__ packed) {
+  __shared__ double lds[64 * 6 * NC * 6];
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t t = t0 + threadIdx.x;
+  double cf[NC][19];
+  if (t < M.ntri) {
+    int vi[3] = {M.tris[3 * t], M.tris[3 * t + 1], M.tris[3 * t + 2]};
+    PSGeom P;
+    ps_unpack(M.psgeom + t, M.ntri, P);
+    const int64_t Nv = M.nv;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      double f[3];
+      d3 gr[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        if (NC == 1) {
+          f[i] = vals[vi[i]];
+          gr[i] = d3{vals[Nv + vi[i]], vals[2 * Nv + vi[i]], vals[3 * Nv + vi[i]]};
+        } else {
+          f[i] = vals[(c * 4) * Nv + vi[i]];
+          gr[i] = d3{vals[(0 * 4 + 1 + c) * Nv + vi[i]], vals[(1 * 4 + 1 + c) * Nv + vi[i]],
+                     vals[(2 * 4 + 1 + c) * Nv + vi[i]]};
+        }
       }
+      ps_coeffs(P, f, gr, cf[c]);
+      if (coeffs)
+        for (int k = 0; k < 19; ++k) coeffs[((int64_t)c * 19 + k) * M.ntri + t] = cf[c][k];
     }
-    double cf[19];
-    ps_coeffs(P, f, gr, cf);
-    if (coeffs)
-      for (int k = 0; k < 19; ++k) coeffs[((int64_t)c * 19 + k) * M.ntri + t] = cf[k];
-    if (packed)
-      for (int s = 0; s < 6; ++s)
-        for (int j = 0; j < 6; ++j) packed[((t * 6 + s) * ncomp + c) * 6 + j] = cf[cs_table(s, j)];
   }
+  if (packed) store_packed_block<NC>(cf, t0, M.ntri, packed, lds);
 }
 
-__global__ void k_tri_stencil_jets(cmm_tri_mesh M, const double* __restrict__ cur,
+__global__ __launch_bounds__(128) void k_tri_stencil_jets(cmm_tri_mesh M, const double* __restrict__ cur,
                                    const double* __restrict__ yv, int64_t yrow,
                                    const double* __restrict__ sf, int64_t srow,
                                    const double* __restrict__ tanv, double eps,
@@ -491,8 +499,12 @@ extern "C" int cmm_tri_assemble(const cmm_tri_mesh* m, const double* vals, int n
                                 double* packed, void* stream) {
   if (!m || !vals || (ncomp != 1 && ncomp != 3)) return fail(CMM_E_ARG, "cmm_tri_assemble: bad arguments");
   if (!coeffs && !packed) return CMM_OK;
-  hipLaunchKernelGGL(k_tri_assemble, dim3(grid_for(m->ntri, 128)), dim3(128), 0, as_stream(stream), *m,
-                     vals, ncomp, coeffs, packed);
+  if (ncomp == 3)
+    hipLaunchKernelGGL(k_tri_assemble<3>, dim3(grid_for(m->ntri, 256)), dim3(256), 0, as_stream(stream), *m,
+                       vals, coeffs, packed);
+  else
+    hipLaunchKernelGGL(k_tri_assemble<1>, dim3(grid_for(m->ntri, 256)), dim3(256), 0, as_stream(stream), *m,
+                       vals, coeffs, packed);
   CMM_LAUNCH_CHECK("cmm_tri_assemble");
   return CMM_OK;
 }

@@ -51,7 +51,9 @@ def test_mwss_locate(L):
         vs = pm.vertices[pm.simplices[t[bad]]]
         bg = np.linalg.solve(np.transpose(vs, (0, 2, 1)), q[bad][:, :, None])[:, :, 0]
         assert (bg.min(axis=1) > -1e-12).all()
-    assert np.abs(b[same] - bo[same]).max() < 1e-13
+    # Cramer barycentrics: det / det on triangles of area ~ (pi/L)^2 lose
+    # log10(1/area) digits; FMA vs numpy rounding differs at that level
+    assert np.abs(b[same] - bo[same]).max() < 1e-12
 
 
 def test_velocity_spline_eval():
