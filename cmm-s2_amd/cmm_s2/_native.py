@@ -49,6 +49,8 @@ SIGNATURES = {
     "cmm_mwss_assemble": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
     "cmm_mwss_eval": (C.c_int, [vp, P_vp, P_dbl, C.c_int, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp]),
     "cmm_advect_points": (C.c_int, [C.POINTER(Velocity), C.c_int, dbl, vp, i64, vp, vp]),
+    "cmm_advect_stencil": (C.c_int, [C.POINTER(Velocity), C.c_int, dbl, vp, i64, vp, i64, i64, vp, i64, vp, i64,
+                                     vp]),
     "cmm_sht_plan_create": (C.c_int, [C.c_int, P_dbl, P_dbl, P_vp]),
     "cmm_sht_plan_destroy": (C.c_int, [vp]),
     "cmm_sht_plan_gl": (C.c_int, [vp, P_dbl, P_dbl]),

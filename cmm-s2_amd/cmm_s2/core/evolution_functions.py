@@ -112,9 +112,11 @@ def _feet(interp, integrator, t, dt, V, identity, w):
     mesh = interp.mesh
     if integrator in _KIND:
         kind = _KIND[integrator]
-        if not identity:
-            integrate(kind, t, dt, mesh.d_verts3, V, w.yv)
-        integrate(kind, t, dt, mesh.d_spts, V, w.ys)
+        desc = velocity_descriptor(V, _STAGES[kind](t, dt), dt)
+        xv, xs = mesh.d_verts3, mesh.d_spts
+        N.call("cmm_advect_stencil", desc, kind, float(dt), None if identity else xv.data_ptr(), xv.stride(0),
+               xs.data_ptr(), xs.stride(0), mesh.nv, w.yv.data_ptr(), w.yv.stride(0), w.ys.data_ptr(),
+               w.ys.stride(0), B.stream())
     else:  # a caller-provided integrator callable
         if not identity:
             w.yv.copy_(B.to_dev(integrator(t, dt, mesh.d_verts3, V)))

@@ -320,42 +320,68 @@ __device__ inline d3 proj_norm(d3 x) {
   return d3{inv * x.x, inv * x.y, inv * x.z};
 }
 
-__global__ __launch_bounds__(128) void k_advect(VelDev V, int integ, double dt, const double* __restrict__ x, int64_t m,
-                         double* __restrict__ y) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  d3 xn = d3{x[i], x[m + i], x[2 * m + i]};
-  d3 out;
-  if (integ == CMM_INT_EVAL) {
-    out = vel_eval(V, 0, xn);
-  } else if (integ == CMM_INT_EULER) {
+// one backward step of integrator `integ` for the point xn
+__device__ __forceinline__ d3 advect_point(const VelDev& V, int integ, double dt, d3 xn) {
+  if (integ == CMM_INT_EVAL) return vel_eval(V, 0, xn);
+  if (integ == CMM_INT_EULER) {
     d3 v1 = vel_eval(V, 0, xn);
-    out = proj_norm(sub(xn, scl(dt, v1)));
-  } else if (integ == CMM_INT_IMPROVED_EULER) {
+    return proj_norm(sub(xn, scl(dt, v1)));
+  }
+  if (integ == CMM_INT_IMPROVED_EULER) {
     d3 v1 = vel_eval(V, 0, xn);
     d3 s1 = proj_norm(sub(xn, scl(dt, v1)));
     d3 v2 = vel_eval(V, 1, s1);
-    out = proj_norm(sub(xn, scl(dt / 2, add(v1, v2))));
-  } else if (integ == CMM_INT_RK2) {
+    return proj_norm(sub(xn, scl(dt / 2, add(v1, v2))));
+  }
+  if (integ == CMM_INT_RK2) {
     d3 v1 = vel_eval(V, 0, xn);
     d3 s1 = proj_norm(sub(xn, scl(dt / 2, v1)));
     d3 v2 = vel_eval(V, 1, s1);
-    out = proj_norm(sub(xn, scl(dt, v2)));
-  } else {
-    d3 v1 = vel_eval(V, 0, xn);
-    d3 s1 = proj_norm(d3{xn.x - dt * v1.x / 2, xn.y - dt * v1.y / 2, xn.z - dt * v1.z / 2});
-    d3 v2 = vel_eval(V, 1, s1);
-    d3 s2 = proj_norm(sub(xn, scl(dt / 2, v2)));
-    d3 v3 = vel_eval(V, 2, s2);
-    d3 s3 = proj_norm(sub(xn, scl(dt, v3)));
-    d3 v4 = vel_eval(V, 3, s3);
-    d3 acc = d3{v1.x + 2 * v2.x + 2 * v3.x + v4.x, v1.y + 2 * v2.y + 2 * v3.y + v4.y,
-                v1.z + 2 * v2.z + 2 * v3.z + v4.z};
-    out = proj_norm(sub(xn, scl(dt / 6, acc)));
+    return proj_norm(sub(xn, scl(dt, v2)));
   }
+  d3 v1 = vel_eval(V, 0, xn);
+  d3 s1 = proj_norm(d3{xn.x - dt * v1.x / 2, xn.y - dt * v1.y / 2, xn.z - dt * v1.z / 2});
+  d3 v2 = vel_eval(V, 1, s1);
+  d3 s2 = proj_norm(sub(xn, scl(dt / 2, v2)));
+  d3 v3 = vel_eval(V, 2, s2);
+  d3 s3 = proj_norm(sub(xn, scl(dt, v3)));
+  d3 v4 = vel_eval(V, 3, s3);
+  d3 acc = d3{v1.x + 2 * v2.x + 2 * v3.x + v4.x, v1.y + 2 * v2.y + 2 * v3.y + v4.y,
+              v1.z + 2 * v2.z + 2 * v3.z + v4.z};
+  return proj_norm(sub(xn, scl(dt / 6, acc)));
+}
+
+__global__ __launch_bounds__(128) void k_advect(VelDev V, int integ, double dt, const double* __restrict__ x, int64_t m,
+                                                double* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const d3 out = advect_point(V, integ, dt, d3{x[i], x[m + i], x[2 * m + i]});
   y[i] = out.x;
   y[m + i] = out.y;
   y[2 * m + i] = out.z;
+}
+
+// vertex feet and stencil feet of one map in one launch: point i is
+// (vertex v = i / P, slot k = i % P), slot 0 the vertex itself when P = 5,
+// the other slots its four stencil points, so the feet of one vertex sit in
+// neighbouring lanes and their velocity gathers share cache lines
+template <int P>
+__global__ __launch_bounds__(128) void k_advect_stencil(VelDev V, int integ, double dt, const double* __restrict__ xv,
+                                                        int64_t xv_row, const double* __restrict__ xs, int64_t xs_row,
+                                                        int64_t nv, double* __restrict__ yv, int64_t yv_row,
+                                                        double* __restrict__ ys, int64_t ys_row) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * nv) return;
+  const int64_t v = i / P;
+  const int k = (int)(i - v * P) - (P - 4);  // stencil index, -1 = vertex
+  const double* src = k < 0 ? xv + v : xs + (int64_t)k * nv + v;
+  const int64_t row = k < 0 ? xv_row : xs_row;
+  const d3 out = advect_point(V, integ, dt, d3{src[0], src[row], src[2 * row]});
+  double* dst = k < 0 ? yv + v : ys + (int64_t)k * nv + v;
+  const int64_t orow = k < 0 ? yv_row : ys_row;
+  dst[0] = out.x;
+  dst[orow] = out.y;
+  dst[2 * orow] = out.z;
 }
 
 __global__ __launch_bounds__(256) void k_mwss_locate(cmm_mwss_mesh M, const double* __restrict__ q, int64_t n, int64_t sp,
@@ -600,16 +626,12 @@ extern "C" int cmm_mwss_eval(const cmm_mwss_mesh* m, const double* const* levels
   return CMM_OK;
 }
 
-extern "C" int cmm_advect_points(const cmm_velocity* vel, int integ, double dt, const double* x,
-                                 int64_t m, double* y, void* stream) {
-  if (!vel || m < 0 || (m > 0 && (!x || !y)) || integ < 0 || integ > 4)
-    return fail(CMM_E_ARG, "cmm_advect_points: bad arguments");
-  VelDev V;
+static int velocity_device(const cmm_velocity* vel, const char* who, VelDev& V) {
   V.kind = vel->kind;
   for (int s = 0; s < 4; ++s) V.stage_t[s] = vel->stage_t[s];
   for (int k = 0; k < 8; ++k) V.params[k] = vel->params[k];
   if (vel->kind == CMM_VEL_SPLINE) {
-    if (!vel->mesh || vel->nlev < 1 || vel->nlev > 3) return fail(CMM_E_ARG, "cmm_advect_points: bad spline velocity");
+    if (!vel->mesh || vel->nlev < 1 || vel->nlev > 3) return fail(CMM_E_ARG, std::string(who) + ": bad spline velocity");
     V.mesh = *vel->mesh;
     for (int s = 0; s < 4; ++s) {
       V.ls[s].nlev = vel->nlev;
@@ -618,14 +640,44 @@ extern "C" int cmm_advect_points(const cmm_velocity* vel, int integ, double dt, 
         V.ls[s].w[k] = vel->stage_w[s][k];
       }
       for (int k = 0; k < vel->nlev; ++k)
-        if (!vel->levels[k]) return fail(CMM_E_ARG, "cmm_advect_points: null velocity level");
+        if (!vel->levels[k]) return fail(CMM_E_ARG, std::string(who) + ": null velocity level");
     }
   } else if (vel->kind < 0 || vel->kind > CMM_VEL_RH4) {
-    return fail(CMM_E_ARG, "cmm_advect_points: unknown velocity kind");
+    return fail(CMM_E_ARG, std::string(who) + ": unknown velocity kind");
   }
+  return CMM_OK;
+}
+
+extern "C" int cmm_advect_points(const cmm_velocity* vel, int integ, double dt, const double* x,
+                                 int64_t m, double* y, void* stream) {
+  if (!vel || m < 0 || (m > 0 && (!x || !y)) || integ < 0 || integ > 4)
+    return fail(CMM_E_ARG, "cmm_advect_points: bad arguments");
+  VelDev V;
+  int rc = velocity_device(vel, "cmm_advect_points", V);
+  if (rc) return rc;
   if (m == 0) return CMM_OK;
   hipLaunchKernelGGL(k_advect, dim3(grid_for(m, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, x, m, y);
   CMM_LAUNCH_CHECK("cmm_advect_points");
+  return CMM_OK;
+}
+
+extern "C" int cmm_advect_stencil(const cmm_velocity* vel, int integ, double dt, const double* verts,
+                                  int64_t verts_row, const double* st, int64_t st_row, int64_t nv, double* yv,
+                                  int64_t yv_row, double* yst, int64_t yst_row, void* stream) {
+  if (!vel || nv < 0 || (nv > 0 && (!st || !yst)) || (verts && !yv) || integ < 0 || integ > 4 ||
+      st_row < 4 * nv || yst_row < 4 * nv || (verts && (verts_row < nv || yv_row < nv)))
+    return fail(CMM_E_ARG, "cmm_advect_stencil: bad arguments");
+  VelDev V;
+  int rc = velocity_device(vel, "cmm_advect_stencil", V);
+  if (rc) return rc;
+  if (nv == 0) return CMM_OK;
+  if (verts)
+    hipLaunchKernelGGL(k_advect_stencil<5>, dim3(grid_for(5 * nv, 128)), dim3(128), 0, as_stream(stream), V, integ,
+                       dt, verts, verts_row, st, st_row, nv, yv, yv_row, yst, yst_row);
+  else
+    hipLaunchKernelGGL(k_advect_stencil<4>, dim3(grid_for(4 * nv, 128)), dim3(128), 0, as_stream(stream), V, integ,
+                       dt, verts, verts_row, st, st_row, nv, yv, yv_row, yst, yst_row);
+  CMM_LAUNCH_CHECK("cmm_advect_stencil");
   return CMM_OK;
 }
 

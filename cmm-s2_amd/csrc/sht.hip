@@ -26,6 +26,7 @@ using namespace cmm;
 namespace {
 constexpr int kScaleExp = 600;
 constexpr int kMaxFFT = 2048;
+constexpr int kFwdSegHost = 128;  // degrees per forward work unit (kFwdSeg)
 }
 
 struct cmm_sht_plan {
@@ -40,12 +41,20 @@ struct cmm_sht_plan {
   double* w_g = nullptr;     // GL weights (L)
   double* dP_g = nullptr;    // (L, L) [m][j]
   int* dS_g = nullptr;
-  double* K = nullptr;       // (2, L, L+1) resampling matrices (even, odd)
+  double* K = nullptr;       // (2, tpad, L) resampling matrices (even, odd), transposed, zero rows t > L
+  int tpad = 0;
   double2* tw = nullptr;     // exp(-2 pi i n / N), n < N
   double* dinv = nullptr;    // D_inv[l] (L)
-  double2* rec = nullptr;    // recurrence (a_lm, b_lm) per packed lm
+  double2* rec = nullptr;    // recurrence (a_lm, a_lm b_lm) per packed lm
+  int nunits = 0;            // forward work units (m, first degree of a segment)
+  int2* units = nullptr;
+  int* unit0 = nullptr;      // first unit of each m (L)
+  double2* st_p = nullptr;   // recurrence state at each unit start (nunits, L/2 nodes)
+  int* st_s = nullptr;
   std::vector<double> hx_g, hw_g;
 };
+
+static int build_restart(cmm_sht_plan* p, const std::vector<int2>& units, const std::vector<int>& unit0);
 
 // ---------------------------------------------------------------------------
 // host setup
@@ -120,6 +129,7 @@ extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
   (void)hipFree(p->x_t); (void)hipFree(p->dP_t); (void)hipFree(p->dS_t);
   (void)hipFree(p->x_g); (void)hipFree(p->w_g); (void)hipFree(p->dP_g); (void)hipFree(p->dS_g);
   (void)hipFree(p->K); (void)hipFree(p->tw); (void)hipFree(p->dinv); (void)hipFree(p->rec);
+  (void)hipFree(p->units); (void)hipFree(p->unit0); (void)hipFree(p->st_p); (void)hipFree(p->st_s);
   delete p;
   return CMM_OK;
 }
@@ -152,15 +162,19 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
   std::vector<int> St, Sg;
   legendre_diag(L, st, Pt, St);
   legendre_diag(L, sg, Pg, Sg);
-  // resampling matrices: K_par[j][t] = D(th_j - th_t) + (-1)^par D(th_j + th_t), 0 < t < L
-  std::vector<double> K((size_t)2 * L * (L + 1));
+  // resampling matrices K_par[j][t] = D(th_j - th_t) + (-1)^par D(th_j + th_t)
+  // (second term for 0 < t < L only), stored transposed K[par][t][j] with
+  // the rows t > L up to a multiple of 4 set to zero
+  const int tpad = (L + 1 + 3) / 4 * 4;
+  p->tpad = tpad;
+  std::vector<double> K((size_t)2 * L * tpad, 0.0);
   for (int j = 0; j < L; ++j)
     for (int t = 0; t <= L; ++t) {
       double tht = 2.0 * t * M_PI / (2.0 * L);
       double d1 = dirichlet(thg[j] - tht, L);
       double d2 = (t > 0 && t < L) ? dirichlet(thg[j] + tht, L) : 0.0;
-      K[(size_t)j * (L + 1) + t] = d1 + d2;
-      K[(size_t)L * (L + 1) + (size_t)j * (L + 1) + t] = d1 - d2;
+      K[(size_t)t * L + j] = d1 + d2;
+      K[(size_t)L * tpad + (size_t)t * L + j] = d1 - d2;
     }
   std::vector<double2> tw(p->N);
   for (int k = 0; k < p->N; ++k) {
@@ -174,7 +188,7 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
       double a = std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
       double lm1 = l - 1.0;
       double b = std::sqrt((lm1 * lm1 - (double)m * m) / (4.0 * lm1 * lm1 - 1.0));
-      rec[lm0 + (l - m)] = double2{a, b};
+      rec[lm0 + (l - m)] = double2{a, a * b};
     }
   }
   std::vector<double> dinv(L);
@@ -186,6 +200,16 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
       (rc = upload(&p->dS_g, Sg.data(), Sg.size())) || (rc = upload(&p->K, K.data(), K.size())) ||
       (rc = upload(&p->tw, tw.data(), tw.size())) || (rc = upload(&p->dinv, dinv.data(), dinv.size())) ||
       (rc = upload(&p->rec, rec.data(), rec.size()))) {
+    cmm_sht_plan_destroy(p);
+    return rc;
+  }
+  std::vector<int2> units;
+  std::vector<int> unit0(L);
+  for (int m = 0; m < L; ++m) {
+    unit0[m] = (int)units.size();
+    for (int ls = m; ls < L; ls += kFwdSegHost) units.push_back(int2{m, ls});
+  }
+  if ((rc = build_restart(p, units, unit0))) {
     cmm_sht_plan_destroy(p);
     return rc;
   }
@@ -361,213 +385,398 @@ __global__ __launch_bounds__(256) void k_fft_c2r(FFTPlanDev P, const double2* __
 }
 
 // ---------------------------------------------------------------------------
-// Legendre recurrence: Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m}),
-// a_lm = sqrt((4l^2-1)/(l^2-m^2)), b_lm = sqrt(((l-1)^2-m^2)/(4(l-1)^2-1)),
-// tabulated per packed (l, m) so the per-lane work is two FMAs and the
-// coefficients arrive by uniform (scalar) loads.  At l = m+1, b = 0.
+// Legendre recurrence
+//   Pbar_lm = a_lm (x Pbar_{l-1,m} - b_lm Pbar_{l-2,m}),
+//   a_lm = sqrt((4l^2-1)/(l^2-m^2)), b_lm = sqrt(((l-1)^2-m^2)/(4(l-1)^2-1)),
+// evaluated as (a x) P_{l-1} - (a b) P_{l-2} from the tabulated pairs
+// (a_lm, a_lm b_lm): one FMA on the dependency chain per degree.  Values are
+// carried as mantissa * 2^(600 s); the scale check (s < 0 and |P| > 2^300 ->
+// multiply by 2^-600, s += 1) runs once per chunk of degrees, and values
+// still at s < 0 (below ~2^-280) count as zero.
 // ---------------------------------------------------------------------------
-__device__ inline void leg_step(double a, double b, double x, double& pp, double& pc, int& s) {
-  const double inv_big = 0x1p-600, hi = 0x1p300;
-  double pn = a * (x * pc - b * pp);
+__device__ inline void leg_step(double a, double ab, double x, double& pp, double& pc) {
+  const double pn = fma(a * x, pc, -ab * pp);
   pp = pc;
   pc = pn;
-  if (s < 0 && fabs(pc) > hi) {
-    pc *= inv_big;
-    pp *= inv_big;
-    s += 1;
-  }
+}
+__device__ inline void leg_rescale(double& pp, double& pc, int& s) {
+  const bool c = (s < 0) & (fabs(pc) > 0x1p300);
+  const double f = c ? 0x1p-600 : 1.0;
+  pc *= f;
+  pp *= f;
+  s += c ? 1 : 0;
+}
+
+// value of lane `src` (a compile-time constant) of every lane's v, as a
+// wave-uniform (scalar register) double
+__device__ inline double readlane_d(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // ---------------------------------------------------------------------------
-// inverse Legendre: F[f][t][m] for all rings from flm [lm][nf].
-// One block per pair of orders (m, L-1-m): every block runs L+1 recurrence
-// steps, which balances the triangular (l >= m) work over the CUs.  Threads
-// own the northern rings t <= L/2; even / odd l+m partial sums give ring t
-// and its mirror L-t.  The coefficients of a chunk of l are staged through
-// LDS once per block and read back as broadcasts.
+// inverse Legendre on FP64 matrix cores: F[f][t][m] for all rings from
+// flm [lm][nf].  One wave per (order m, block of 64 northern rings), blocks
+// ordered by ascending m so the longest recurrences start first.  Rings
+// t >= 1 only for m > 0 (Pbar_lm vanishes at the poles; those two rings are
+// written as zeros).  Per chunk of 8 degrees every lane advances the
+// recurrence of its ring and stores Pbar into an LDS tile [degree][ring];
+// the contraction over degrees is then v_mfma_f64_16x16x4f64 with
+//   A (16 rings x 4 degrees of one parity of l - m)  read from the tile,
+//   B (4 degrees x 16 columns = 8 fields x re/im)     read from flm,
+// one accumulator per (ring tile, parity, column tile).  The coefficients
+// reach each lane as its own element (no broadcast), loaded one chunk
+// ahead; the recurrence pairs of the order are staged in LDS at block
+// start; the recurrence of chunk c+1 is issued under the matrix
+// instructions of chunk c.  Even / odd l - m sums give ring t and its
+// mirror L - t.
+// Operand layouts of the f64 16x16x4 instruction (measured on gfx950):
+//   A: lane l holds A[l % 16][l / 16]    B: lane l holds B[l / 16][l % 16]
+//   D: lane l, element r holds D[4 r + l / 16][l % 16]
 // ---------------------------------------------------------------------------
-constexpr int kInvChunk = 32;
+typedef double dx4 __attribute__((ext_vector_type(4)));
+constexpr int kInvRings = 64;  // rings per wave
+constexpr int kInvDeg = 8;     // degrees per chunk (4 of each parity)
 
-template <int NFB>
-__global__ __launch_bounds__(576) void k_leg_inverse(int L, int nf, int f0, const double* __restrict__ x_t,
-                                                     const double* __restrict__ dP, const int* __restrict__ dS,
-                                                     const double2* __restrict__ rec,
-                                                     const double2* __restrict__ flm, double2* __restrict__ F) {
-  __shared__ double2 cs[kInvChunk][NFB];
-  __shared__ double2 rs[kInvChunk];
-  const int nh = L / 2 + 1;  // northern rings incl. equator (t <= L/2)
-  const int t = threadIdx.x;
-  const bool active = t < nh;
-  const int tt = active ? t : 0;
-  const double x = x_t[tt];
-  const int nm = ((int)blockIdx.x == L - 1 - (int)blockIdx.x) ? 1 : 2;
-  for (int q = 0; q < nm; ++q) {
-    const int m = q == 0 ? (int)blockIdx.x : L - 1 - (int)blockIdx.x;
-    double2 E[NFB], O[NFB];
-#pragma unroll
-    for (int k = 0; k < NFB; ++k) { E[k] = double2{0, 0}; O[k] = double2{0, 0}; }
-    double pp = 0.0, pc = dP[(int64_t)m * (L + 1) + tt];
-    int s = dS[(int64_t)m * (L + 1) + tt];
-    const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
-    for (int lc = m; lc < L; lc += kInvChunk) {
-      const int n = min(kInvChunk, L - lc);
-      __syncthreads();
-      for (int e = threadIdx.x; e < n * NFB; e += blockDim.x) {
-        const int i = e / NFB, k = e % NFB;
-        cs[i][k] = flm[(lm0 + (lc - m) + i) * nf + f0 + k];
-      }
-      for (int i = threadIdx.x; i < n; i += blockDim.x) rs[i] = rec[lm0 + (lc - m) + i];
-      __syncthreads();
-      for (int i = 0; i < n; ++i) {
-        const int l = lc + i;
-        if (l > m) {
-          double2 ab = rs[i];
-          leg_step(ab.x, ab.y, x, pp, pc, s);
-        }
-        const double pv = (s == 0) ? pc : 0.0;
-        if (((l - m) & 1) == 0) {
-#pragma unroll
-          for (int k = 0; k < NFB; ++k) { E[k].x += pv * cs[i][k].x; E[k].y += pv * cs[i][k].y; }
-        } else {
-#pragma unroll
-          for (int k = 0; k < NFB; ++k) { O[k].x += pv * cs[i][k].x; O[k].y += pv * cs[i][k].y; }
-        }
-      }
-    }
-    if (active) {
-#pragma unroll
-      for (int k = 0; k < NFB; ++k) {
-        const int fi = f0 + k;
-        F[((int64_t)fi * (L + 1) + t) * L + m] = double2{E[k].x + O[k].x, E[k].y + O[k].y};
-        if (L - t != t)
-          F[((int64_t)fi * (L + 1) + (L - t)) * L + m] = double2{E[k].x - O[k].x, E[k].y - O[k].y};
-      }
-    }
+template <int NT>
+__global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, int nfp, int nrb,
+                                                       const double* __restrict__ x_t, const double* __restrict__ dP,
+                                                       const int* __restrict__ dS, const double2* __restrict__ rec,
+                                                       const double* __restrict__ flm, double* __restrict__ F) {
+  __shared__ double Pt[kInvDeg][kInvRings + 2];
+  extern __shared__ double2 rsh[];  // recurrence pairs of this order, degrees m .. L-1 (+ padding)
+  const int m = (int)blockIdx.x / nrb, rb = (int)blockIdx.x % nrb;
+  const int lane = threadIdx.x;
+  const int nh = L / 2 + 1;
+  const int tfirst = m == 0 ? 0 : 1;
+  const int nr = nh - tfirst;
+  if (rb * kInvRings >= nr) return;
+  if (m > 0 && rb == 0 && lane < nfp) {
+    const double2 z = {0.0, 0.0};
+    double2* F2 = reinterpret_cast<double2*>(F);
+    F2[((int64_t)(f0 + lane) * (L + 1) + 0) * L + m] = z;
+    F2[((int64_t)(f0 + lane) * (L + 1) + L) * L + m] = z;
   }
+  const int r = rb * kInvRings + lane;
+  const bool valid = r < nr;
+  const int t = tfirst + (valid ? r : 0);
+  const double x = x_t[t];
+  double pp = 0.0, pc = dP[(int64_t)m * (L + 1) + t];
+  int s = dS[(int64_t)m * (L + 1) + t];
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  const int bk = lane >> 4, bj = lane & 15;
+
+  // recurrence pairs of every degree of this order, staged once
+  const int nch = (L - m + kInvDeg - 1) / kInvDeg;          // chunks of degrees
+  const int nrs = (nch + (nch & 1) + 1) * kInvDeg;           // padded to whole chunk pairs
+  for (int i = lane; i < nrs; i += 64) rsh[i] = rec[lm0 + min(i, L - 1 - m)];
+  __syncthreads();
+  // B operands of one chunk from clamped (always valid) addresses.  No
+  // masking: the tile rows of degrees >= L are zero, and columns of fields
+  // >= nfp are never stored.
+  auto load = [&](int l0, double (&b)[2][NT]) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int l = min(l0 + 2 * bk + p, L - 1);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int col = 16 * nt + bj, f = min(col >> 1, nfp - 1);
+        b[p][nt] = flm[((lm0 + (l - m)) * nf + f0 + f) * 2 + (col & 1)];
+      }
+    }
+  };
+  // recurrence of the chunk starting at degree l0 into the tile
+  auto recur = [&](int l0, bool first) {
+    leg_rescale(pp, pc, s);
+    const bool live = (s == 0) & valid;
+#pragma unroll
+    for (int i = 0; i < kInvDeg; ++i) {
+      const double2 ab = rsh[l0 - m + i];
+      if (!(first && i == 0)) leg_step(ab.x, ab.y, x, pp, pc);
+      Pt[i][lane] = (live & (l0 + i < L)) ? pc : 0.0;
+    }
+  };
+  dx4 acc[4][2][NT];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[mt][p][nt] = dx4{0.0, 0.0, 0.0, 0.0};
+  double a[4][2];
+  auto readA = [&]() {
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) a[mt][p] = Pt[2 * bk + p][16 * mt + bj];
+  };
+  auto mma = [&](const double (&b)[2][NT]) {
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          acc[mt][p][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt][p], b[p][nt], acc[mt][p][nt], 0, 0, 0);
+  };
+  // Two chunks per iteration with no exit in between (an odd chunk count is
+  // padded with a chunk whose tile rows and coefficients are zero), so the
+  // operand loads stay one chunk ahead of their use.
+  double bA[2][NT], bB[2][NT];
+  load(m, bA);
+  recur(m, true);
+  __syncthreads();
+  load(m + kInvDeg, bB);
+  for (int c = 0; c < nch; c += 2) {
+    const int l0 = m + c * kInvDeg;
+    readA();
+    mma(bA);
+    recur(l0 + kInvDeg, false);
+    __syncthreads();
+    load(l0 + 2 * kInvDeg, bA);
+    readA();
+    mma(bB);
+    recur(l0 + 2 * kInvDeg, false);
+    __syncthreads();
+    load(l0 + 3 * kInvDeg, bB);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int col = 16 * nt + bj, f = col >> 1;
+      if (f >= nfp) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int rr = rb * kInvRings + 16 * mt + 4 * e + bk;
+        if (rr >= nr) continue;
+        const int tr = tfirst + rr;
+        const double ev = acc[mt][0][nt][e], od = acc[mt][1][nt][e];
+        F[(((int64_t)(f0 + f) * (L + 1) + tr) * L + m) * 2 + (col & 1)] = ev + od;
+        if (L - tr != tr) F[(((int64_t)(f0 + f) * (L + 1) + (L - tr)) * L + m) * 2 + (col & 1)] = ev - od;
+      }
+    }
 }
 
 // ---------------------------------------------------------------------------
-// forward: resampling GEMM G[f][j][m] = sum_t K_{m%2}[j][t] F[f][t][m]
+// forward: resampling GEMM on FP64 matrix cores,
+//   G[f][j][m] = sum_t K_{m%2}[j][t] F[f][t][m],  j < L, t <= L.
+// One wave per (16 nodes j) x (16 orders m of one parity, re/im = 32 real
+// columns) x field; K is stored transposed (Kt[p][t][j]) so the A operand
+// is a coalesced row read.  Degrees of freedom t advance 4 per MFMA; the
+// operands of 8 steps are loaded one block ahead.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_resample(int L, const double* __restrict__ K, const double2* __restrict__ F,
-                           double2* __restrict__ G) {
-  // block: 16 nodes j x 64 m-columns; grid: (ceil(L/64), ceil(L/16), nf)
-  __shared__ double Ks[2][16][17];
-  __shared__ double2 Fs[16][64];
+constexpr int kResSteps = 8;
+
+__global__ __launch_bounds__(64) void k_resample(int L, int nmg, int tpad, const double* __restrict__ Kt,
+                                                 const double* __restrict__ F, double* __restrict__ G) {
+  const int lane = threadIdx.x;
+  const int bk = lane >> 4, bj = lane & 15;
   const int fi = blockIdx.z;
-  const int j0 = blockIdx.y * 16, m0 = blockIdx.x * 64;
-  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;  // 64 x 4 threads
-  double2 acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  const int m = m0 + tx;
-  const int par = m & 1;
-  for (int t0 = 0; t0 <= L; t0 += 16) {
-    for (int e = threadIdx.x; e < 2 * 16 * 16; e += blockDim.x) {
-      int p = e / 256, jj = (e / 16) % 16, tt = e % 16;
-      int j = j0 + jj, t = t0 + tt;
-      Ks[p][jj][tt] = (j < L && t <= L) ? K[(int64_t)p * L * (L + 1) + (int64_t)j * (L + 1) + t] : 0.0;
+  const int j0 = blockIdx.y * 16;
+  const int p = (int)blockIdx.x / nmg, qb = (int)blockIdx.x % nmg;
+  const double* K = Kt + (int64_t)p * tpad * L;
+  const double* Ff = F + (int64_t)fi * (L + 1) * L * 2;
+  // B column of this lane in N-tile nt: order m = p + 2 (16 qb + 8 nt + bj / 2), part bj & 1
+  int mcol[2];
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt) mcol[nt] = p + 2 * (16 * qb + 8 * nt + (bj >> 1));
+  const int ja = j0 + bj;  // A row of this lane
+  const int jac = min(ja, L - 1);
+  int mcc[2];
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt) mcc[nt] = min(mcol[nt], L - 1);
+  // unconditional loads from clamped addresses: rows t > L of K are zero
+  // (padding to tpad rows), rows j >= L and columns m >= L are never stored
+  auto load = [&](int s0, double (&a)[kResSteps], double (&b)[kResSteps][2]) {
+#pragma unroll
+    for (int u = 0; u < kResSteps; ++u) {
+      const int tw = 4 * (s0 + u) + bk;
+      a[u] = K[(int64_t)min(tw, tpad - 1) * L + jac];
+      const int t = min(tw, L);
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) b[u][nt] = Ff[((int64_t)t * L + mcc[nt]) * 2 + (bj & 1)];
     }
-    for (int e = threadIdx.x; e < 16 * 64; e += blockDim.x) {
-      int tt = e / 64, mm = e % 64;
-      int t = t0 + tt, mc = m0 + mm;
-      Fs[tt][mm] = (t <= L && mc < L) ? F[((int64_t)fi * (L + 1) + t) * L + mc] : double2{0, 0};
-    }
-    __syncthreads();
-    for (int tt = 0; tt < 16; ++tt) {
-      double2 fv = Fs[tt][tx];
-      for (int r = 0; r < 4; ++r) {
-        double kv = Ks[par][ty * 4 + r][tt];
-        acc[r].x += kv * fv.x;
-        acc[r].y += kv * fv.y;
-      }
-    }
-    __syncthreads();
+  };
+  dx4 acc[2] = {dx4{0.0, 0.0, 0.0, 0.0}, dx4{0.0, 0.0, 0.0, 0.0}};
+  const int nsteps = tpad / 4;
+  auto mma = [&](const double (&a)[kResSteps], const double (&b)[kResSteps][2]) {
+#pragma unroll
+    for (int u = 0; u < kResSteps; ++u)
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][nt], acc[nt], 0, 0, 0);
+  };
+  double aA[kResSteps], bA[kResSteps][2], aB[kResSteps], bB[kResSteps][2];
+  load(0, aA, bA);
+  for (int s0 = 0;;) {
+    load(s0 + kResSteps, aB, bB);
+    mma(aA, bA);
+    s0 += kResSteps;
+    if (s0 >= nsteps) break;
+    load(s0 + kResSteps, aA, bA);
+    mma(aB, bB);
+    s0 += kResSteps;
+    if (s0 >= nsteps) break;
   }
-  if (m >= L) return;
-  for (int r = 0; r < 4; ++r) {
-    int j = j0 + ty * 4 + r;
-    if (j < L) G[((int64_t)fi * L + j) * L + m] = acc[r];
+  double* Gf = G + (int64_t)fi * L * L * 2;
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt) {
+    if (mcol[nt] >= L) continue;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int j = j0 + 4 * e + bk;
+      if (j < L) Gf[((int64_t)j * L + mcol[nt]) * 2 + (bj & 1)] = acc[nt][e];
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // forward Legendre analysis: flm[lm][f] = sum_j w_j Pbar_lm(x_j) G[f][j][m].
-// One block per pair of orders (m, L-1-m) and field; threads own the
-// northern Gauss-Legendre nodes.  For a chunk of kAnaChunk degrees every
-// thread writes its 2*kAnaChunk products to LDS, then groups of 8 lanes sum
-// columns of that tile and finish with 3 butterfly steps.
+// Work unit = (order m, segment of at most kFwdSeg degrees); the recurrence
+// state at every segment start is tabulated once per plan (k_leg_restart,
+// same chunked schedule), so no unit is longer than one segment and the
+// outputs of different units are disjoint.  One block per (unit, field);
+// each thread owns kFwdNpt northern Gauss-Legendre nodes (folded with their
+// mirrors into S / D for even / odd l - m) and adds their products before
+// they reach LDS.  Per chunk of C degrees: products -> LDS tile
+// [degree][thread]; every wave sums its 64 columns of each row (RQ = 64 / C
+// lanes per row, interleaved columns so 8 neighbouring lanes hit 8 bank
+// groups); the first C threads add the per-wave sums.  The recurrence
+// pairs of the next chunk are staged through LDS during the current one.
+// Fixed summation order: deterministic.
 // ---------------------------------------------------------------------------
-constexpr int kAnaChunk = 8;
+constexpr int kFwdSeg = 128;
+constexpr int kFwdChunk = 16;
+constexpr int kFwdNpt = 2;
 
-__global__ __launch_bounds__(512) void k_leg_forward(int L, int nf, const double* __restrict__ xg,
-                                                     const double* __restrict__ wg, const double* __restrict__ dP,
-                                                     const int* __restrict__ dS, const double2* __restrict__ rec,
-                                                     const double2* __restrict__ G, double2* __restrict__ flm) {
-  extern __shared__ double tile[];  // [2*kAnaChunk][T]
-  __shared__ double2 rs[kAnaChunk];
+// state (P_{l-2}, P_{l-1}, s) before the chunk at every segment start
+// l = m + kFwdSeg k, for every northern node: one thread per (m, node)
+__global__ __launch_bounds__(256) void k_leg_restart(int L, int nh, const double* __restrict__ xg,
+                                                     const double* __restrict__ dP, const int* __restrict__ dS,
+                                                     const double2* __restrict__ rec, const int* __restrict__ unit0,
+                                                     double2* __restrict__ st_p, int* __restrict__ st_s) {
+  const int m = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nh) return;
+  const double x = xg[j];
+  double pp = 0.0, pc = dP[(int64_t)m * L + j];
+  int s = dS[(int64_t)m * L + j];
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  int u = unit0[m];
+  for (int lc = m; lc < L; lc += kFwdChunk) {
+    if ((lc - m) % kFwdSeg == 0) {
+      st_p[(int64_t)u * nh + j] = double2{pp, pc};
+      st_s[(int64_t)u * nh + j] = s;
+      ++u;
+    }
+    leg_rescale(pp, pc, s);
+    for (int i = 0; i < kFwdChunk; ++i) {
+      const int l = lc + i;
+      if (l > m && l < L) {
+        const double2 ab = rec[lm0 + (l - m)];
+        leg_step(ab.x, ab.y, x, pp, pc);
+      }
+    }
+  }
+}
+
+template <int C>
+__global__ __launch_bounds__(512) void k_leg_forward(int L, int nf, const int2* __restrict__ units,
+                                                     const double* __restrict__ xg, const double* __restrict__ wg,
+                                                     const double2* __restrict__ st_p, const int* __restrict__ st_s,
+                                                     const double2* __restrict__ rec, const double2* __restrict__ G,
+                                                     double2* __restrict__ flm) {
+  constexpr int RQ = 64 / C;
+  extern __shared__ double2 tile[];  // [C][T + 4]
+  __shared__ double2 part[2][8][C];
+  __shared__ double2 rs[kFwdSeg];  // recurrence pairs of the segment
   const int fi = blockIdx.y;
+  const int u = blockIdx.x;
+  const int m = units[u].x, ls = units[u].y;
+  const int le = min(L, ls + kFwdSeg);
   const int nh = (L + 1) / 2;  // northern nodes (incl. the middle one when L is odd)
-  const int T = blockDim.x;    // multiple of 64 >= nh
-  const int j = threadIdx.x;
-  const bool active = j < nh;
-  const int jj = active ? j : 0;
-  const int jm = L - 1 - jj;
-  const double x = xg[jj];
-  const int nm = ((int)blockIdx.x == L - 1 - (int)blockIdx.x) ? 1 : 2;
-  // reduction role: output o = j / 8 (of 2*kAnaChunk), part = j % 8
-  const int o = j >> 3, part = j & 7;
-  for (int q = 0; q < nm; ++q) {
-    const int m = q == 0 ? (int)blockIdx.x : L - 1 - (int)blockIdx.x;
-    double2 S = {0, 0}, D = {0, 0};
-    if (active) {
-      double2 a = G[((int64_t)fi * L + jj) * L + m];
-      double w = wg[jj];
-      if (jm != jj) {
-        double2 b = G[((int64_t)fi * L + jm) * L + m];
-        S = double2{w * (a.x + b.x), w * (a.y + b.y)};
-        D = double2{w * (a.x - b.x), w * (a.y - b.y)};
-      } else {
-        S = double2{w * a.x, w * a.y};
-        D = S;
-      }
-    }
-    double pp = 0.0, pc = dP[(int64_t)m * L + jj];
-    int s = dS[(int64_t)m * L + jj];
-    const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
-    for (int lc = m; lc < L; lc += kAnaChunk) {
-      if (j < kAnaChunk && lc + j < L) rs[j] = rec[lm0 + (lc - m) + j];
-      __syncthreads();
+  const int T = blockDim.x, TP = T + 4, nw = T >> 6;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  double x[kFwdNpt], pp[kFwdNpt], pc[kFwdNpt];
+  int s[kFwdNpt];
+  double2 S[kFwdNpt], D[kFwdNpt];
+  bool act[kFwdNpt];
 #pragma unroll
-      for (int i = 0; i < kAnaChunk; ++i) {
-        const int l = lc + i;
-        double vr = 0.0, vi = 0.0;
-        if (l < L) {
-          if (l > m) {
-            double2 ab = rs[i];
-            leg_step(ab.x, ab.y, x, pp, pc, s);
-          }
-          const double pv = (s == 0 && active) ? pc : 0.0;
-          const double2 src = (((l - m) & 1) == 0) ? S : D;
-          vr = pv * src.x;
-          vi = pv * src.y;
-        }
-        tile[(2 * i) * T + j] = vr;
-        tile[(2 * i + 1) * T + j] = vi;
-      }
-      __syncthreads();
-      if (o < 2 * kAnaChunk) {
-        double acc = 0.0;
-        for (int c = part; c < T; c += 8) acc += tile[o * T + c];
-        acc += __shfl_xor(acc, 1, 8);
-        acc += __shfl_xor(acc, 2, 8);
-        acc += __shfl_xor(acc, 4, 8);
-        const int l = lc + (o >> 1);
-        if (part == 0 && l < L) {
-          double* dst = reinterpret_cast<double*>(flm + (lm0 + (l - m)) * nf + fi);
-          dst[o & 1] = acc;
-        }
-      }
-      __syncthreads();
+  for (int q = 0; q < kFwdNpt; ++q) {
+    const int jw = tid + q * T;
+    act[q] = jw < nh;
+    const int j = act[q] ? jw : 0;
+    const int jm = L - 1 - j;
+    x[q] = xg[j];
+    const double2 ps = st_p[(int64_t)u * nh + j];
+    pp[q] = ps.x;
+    pc[q] = ps.y;
+    s[q] = st_s[(int64_t)u * nh + j];
+    const double2 ga = G[((int64_t)fi * L + j) * L + m];
+    const double wj = act[q] ? wg[j] : 0.0;
+    if (jm != j) {
+      const double2 gb = G[((int64_t)fi * L + jm) * L + m];
+      S[q] = double2{wj * (ga.x + gb.x), wj * (ga.y + gb.y)};
+      D[q] = double2{wj * (ga.x - gb.x), wj * (ga.y - gb.y)};
+    } else {
+      S[q] = double2{wj * ga.x, wj * ga.y};
+      D[q] = S[q];
     }
+  }
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  const int ri = lane / RQ, rq = lane % RQ;
+  for (int i = tid; i < kFwdSeg; i += T) rs[i] = rec[lm0 + (min(ls + i, L - 1) - m)];
+  __syncthreads();
+  int buf = 0;
+  for (int lc = ls; lc < le; lc += C) {
+    bool live[kFwdNpt];
+#pragma unroll
+    for (int q = 0; q < kFwdNpt; ++q) {
+      leg_rescale(pp[q], pc[q], s[q]);
+      live[q] = (s[q] == 0) & act[q];
+    }
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      const int l = lc + i;
+      const double2 ab = rs[lc - ls + i];
+      double2 v = {0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < kFwdNpt; ++q) {
+        if (l > m) leg_step(ab.x, ab.y, x[q], pp[q], pc[q]);
+        const double pv = live[q] ? pc[q] : 0.0;
+        const double2 src = ((l - m) & 1) ? D[q] : S[q];
+        v.x = fma(pv, src.x, v.x);
+        v.y = fma(pv, src.y, v.y);
+      }
+      tile[i * TP + tid] = v;
+    }
+    __syncthreads();
+    double2 acc = {0.0, 0.0};
+    const double2* row = tile + ri * TP + 64 * w + rq;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const double2 v = row[RQ * c];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+#pragma unroll
+    for (int o = 1; o < RQ; o <<= 1) {
+      acc.x += __shfl_xor(acc.x, o, RQ);
+      acc.y += __shfl_xor(acc.y, o, RQ);
+    }
+    if (rq == 0) part[buf][w][ri] = acc;
+    __syncthreads();
+    if (tid < C && lc + tid < le) {
+      double2 sum = part[buf][0][tid];
+      for (int ww = 1; ww < nw; ++ww) {
+        sum.x += part[buf][ww][tid].x;
+        sum.y += part[buf][ww][tid].y;
+      }
+      flm[(lm0 + (lc + tid - m)) * nf + fi] = sum;
+    }
+    buf ^= 1;
   }
 }
 
@@ -675,6 +884,27 @@ __global__ __launch_bounds__(128) void k_scalar_ops(int L, const double2* __rest
   for (int i = 0; i < 3; ++i) out[idx * 4 + 1 + i] = double2{u[i].re, u[i].im};
 }
 
+static_assert(kFwdSeg == kFwdSegHost, "segment length");
+
+static int build_restart(cmm_sht_plan* p, const std::vector<int2>& units, const std::vector<int>& unit0) {
+  const int L = p->L, nh = (L + 1) / 2;
+  int rc;
+  if ((rc = upload(&p->units, units.data(), units.size())) || (rc = upload(&p->unit0, unit0.data(), unit0.size())))
+    return rc;
+  p->nunits = (int)units.size();
+  hipError_t e = hipMalloc(&p->st_p, sizeof(double2) * (size_t)p->nunits * nh);
+  if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: hipMalloc");
+  e = hipMalloc(&p->st_s, sizeof(int) * (size_t)p->nunits * nh);
+  if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: hipMalloc");
+  hipLaunchKernelGGL(k_leg_restart, dim3((nh + 255) / 256, L), dim3(256), 0, 0, L, nh, p->x_g, p->dP_g, p->dS_g,
+                     p->rec, p->unit0, p->st_p, p->st_s);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: restart kernel");
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: restart kernel");
+  return CMM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -693,25 +923,20 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   double2* F = reinterpret_cast<double2*>(work);
   hipStream_t st = as_stream(stream);
   const int nh = L / 2 + 1;
-  const int T = ((nh + 63) / 64) * 64;
-  if (T > 576) return fail(CMM_E_RANGE, "cmm_sht_inverse: L > 1024");
-  const dim3 grid((L + 1) / 2), block(T);
+  const int nrb = (nh + kInvRings - 1) / kInvRings;
+  const dim3 grid((unsigned)(L * nrb)), block(64);
+  // staged recurrence pairs: at most ceil(L / 8) + 2 chunks of 8 degrees
+  const size_t lds_rec = (size_t)((L + kInvDeg - 1) / kInvDeg + 2) * kInvDeg * sizeof(double2);
   for (int f0 = 0; f0 < nf;) {
-    int left = nf - f0;
-    if (left >= 6) {
-      hipLaunchKernelGGL(k_leg_inverse<6>, grid, block, 0, st, L, nf, f0, p->x_t, p->dP_t, p->dS_t, p->rec,
-                         reinterpret_cast<const double2*>(flm), F);
-      f0 += 6;
-    } else if (left >= 4) {
-      hipLaunchKernelGGL(k_leg_inverse<4>, grid, block, 0, st, L, nf, f0, p->x_t, p->dP_t, p->dS_t, p->rec,
-                         reinterpret_cast<const double2*>(flm), F);
-      f0 += 4;
-    } else {
-      hipLaunchKernelGGL(k_leg_inverse<1>, grid, block, 0, st, L, nf, f0, p->x_t, p->dP_t, p->dS_t, p->rec,
-                         reinterpret_cast<const double2*>(flm), F);
-      f0 += 1;
-    }
+    const int nfp = std::min(16, nf - f0);
+    if (nfp > 8)
+      hipLaunchKernelGGL(k_leg_inverse<2>, grid, block, lds_rec, st, L, nf, f0, nfp, nrb, p->x_t, p->dP_t, p->dS_t,
+                         p->rec, flm, reinterpret_cast<double*>(F));
+    else
+      hipLaunchKernelGGL(k_leg_inverse<1>, grid, block, lds_rec, st, L, nf, f0, nfp, nrb, p->x_t, p->dP_t, p->dS_t,
+                         p->rec, flm, reinterpret_cast<double*>(F));
     CMM_LAUNCH_CHECK("cmm_sht_inverse: legendre");
+    f0 += nfp;
   }
   int pairs = (L + 2) / 2;
   size_t lds = (size_t)2 * p->N * sizeof(double2);
@@ -731,15 +956,16 @@ extern "C" int cmm_sht_forward(const cmm_sht_plan* p, const double* f, int nf, d
   size_t lds = (size_t)2 * p->N * sizeof(double2);
   hipLaunchKernelGGL(k_fft_r2c, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, f, L, F);
   CMM_LAUNCH_CHECK("cmm_sht_forward: fft");
-  hipLaunchKernelGGL(k_resample, dim3((L + 63) / 64, (L + 15) / 16, nf), dim3(256), 0, st, L, p->K, F, G);
+  const int nmg = ((L + 1) / 2 + 15) / 16;  // groups of 16 orders of one parity (at most ceil(L / 2) orders)
+  hipLaunchKernelGGL(k_resample, dim3(2 * nmg, (L + 15) / 16, nf), dim3(64), 0, st, L, nmg, p->tpad, p->K,
+                     reinterpret_cast<const double*>(F), reinterpret_cast<double*>(G));
   CMM_LAUNCH_CHECK("cmm_sht_forward: resample");
-  int nh = (L + 1) / 2;
-  int T = ((nh + 63) / 64) * 64;
-  if (T < 128) T = 128;  // 2*kAnaChunk outputs x 8 lanes
+  const int nh = (L + 1) / 2;
+  const int T = std::max(64, ((nh + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
   if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_forward: L > 1024");
-  size_t lds2 = (size_t)2 * kAnaChunk * T * sizeof(double);
-  hipLaunchKernelGGL(k_leg_forward, dim3((L + 1) / 2, nf), dim3(T), lds2, st, L, nf, p->x_g, p->w_g, p->dP_g,
-                     p->dS_g, p->rec, G, reinterpret_cast<double2*>(flm));
+  size_t lds2 = (size_t)kFwdChunk * (T + 4) * sizeof(double2);
+  hipLaunchKernelGGL(k_leg_forward<kFwdChunk>, dim3(p->nunits, nf), dim3(T), lds2, st, L, nf, p->units, p->x_g,
+                     p->w_g, p->st_p, p->st_s, p->rec, G, reinterpret_cast<double2*>(flm));
   CMM_LAUNCH_CHECK("cmm_sht_forward: legendre");
   return CMM_OK;
 }

@@ -163,6 +163,17 @@ typedef struct cmm_velocity {
 int cmm_advect_points(const cmm_velocity* vel, int integrator, double dt, const double* x,
                       int64_t m, double* y, void* stream);
 
+/* The integration part of advect (evolution_functions.py:117-133) for one
+ * map in one launch: vertex feet verts (3, nv) (row stride verts_row) and
+ * stencil feet st (3, 4 nv) (element (c, k nv + v) at st[c st_row + k nv + v])
+ * -> yv, yst in the same layouts.  The foot of a vertex and its four stencil
+ * points run in neighbouring lanes (they are 1e-5 apart, so their velocity
+ * gathers share cache lines).  verts == NULL (identity map): stencil feet
+ * only.  Outputs must not alias the inputs. */
+int cmm_advect_stencil(const cmm_velocity* vel, int integrator, double dt, const double* verts,
+                       int64_t verts_row, const double* st, int64_t st_row, int64_t nv, double* yv,
+                       int64_t yv_row, double* yst, int64_t yst_row, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* MWSS spherical-harmonic transforms (replace pyssht forward / inverse,     */
 /* spherical_harmonic_tools.py:16,84,91,123,133) and spectral operators.     */

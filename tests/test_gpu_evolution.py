@@ -50,6 +50,35 @@ def test_analytic_velocities(vel):
     assert rel_err(got, oev.RK4_proj(0.3, 0.01, x, Vo)) < 1e-12
 
 
+def test_stencil_advection_matches_pointwise():
+    """cmm_advect_stencil (vertex + stencil feet interleaved in one launch)
+    is the same arithmetic per point as cmm_advect_points: bit-identical."""
+    from cmm_s2 import _native as N
+    from cmm_s2.core import mesh_functions as mf, evolution_functions as ev
+    from cmm_s2.core import _backend as B
+    U, _ = _velocity_pair(16, 4)
+    U.nVs = 3
+    ico = mf.icosahedral_mesh(2)
+    mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
+    xv, xs = mesh.d_verts3, mesh.d_spts
+    for kind in (N.INT_RK4, N.INT_EULER):
+        ref_v = ev.integrate(kind, 0.1, 0.05, xv, U)
+        ref_s = ev.integrate(kind, 0.1, 0.05, xs, U)
+        for with_verts in (True, False):
+            yv = torch.full_like(xv, -7.0)
+            ys = torch.empty_like(xs)
+            desc = ev.velocity_descriptor(U, ev._STAGES[kind](0.1, 0.05), 0.05)
+            N.call("cmm_advect_stencil", desc, kind, 0.05, xv.data_ptr() if with_verts else None, xv.stride(0),
+                   xs.data_ptr(), xs.stride(0), mesh.nv, yv.data_ptr(), yv.stride(0), ys.data_ptr(), ys.stride(0),
+                   B.stream())
+            torch.cuda.synchronize()
+            assert torch.equal(ys, ref_s)
+            if with_verts:
+                assert torch.equal(yv, ref_v)
+            else:
+                assert bool((yv == -7.0).all())
+
+
 def test_fused_map_update_matches_oracle():
     """advect_project_sphere with identity=True then False, analytic and
     spline velocities, against evolution_functions.py restated."""
