@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--Lw", type=int, default=1024)
     ap.add_argument("--lmax", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=0, help="cube-map cells per face edge of the location grid (0: auto)")
     ap.add_argument("--cpu-sample", type=int, default=2097152,
                     help="points of the CPU-baseline sample (every k-th grid point)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -73,7 +74,7 @@ def main():
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
     ico = mf.icosahedral_mesh(a.ico)
-    mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
+    mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices, cells_per_face=a.cells)
     log(f"mesh ico-{a.ico}: {mesh.nv} vertices, {mesh.ntri} triangles, "
         f"setup {time.perf_counter() - t0:.1f}s, {mesh.stats()}")
     t0 = time.perf_counter()

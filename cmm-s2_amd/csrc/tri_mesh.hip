@@ -161,7 +161,9 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
     if (simplices[t] < 0 || simplices[t] >= nv)
       return fail(CMM_E_ARG, "cmm_tri_mesh_create: simplex index out of range");
   int R = cells_per_face;
-  if (R <= 0) R = std::max(4, (int)std::ceil(std::sqrt((double)ntri * 0.75)));
+  // default: about 9 cells per triangle (fewer cells straddle an edge, so
+  // fewer candidate tests per query; 94 MB of cells at 327,680 triangles)
+  if (R <= 0) R = std::max(4, (int)std::ceil(std::sqrt((double)ntri * 3.0)));
   if ((int64_t)6 * R * R > ((int64_t)1 << 31) - 1) return fail(CMM_E_RANGE, "cmm_tri_mesh_create: too many cells");
 
   // geometry records
@@ -181,10 +183,10 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
     g[0] = w0.x / D; g[1] = w0.y / D; g[2] = w0.z / D;
     g[3] = w1.x / D; g[4] = w1.y / D; g[5] = w1.z / D;
     g[6] = w2.x / D; g[7] = w2.y / D; g[8] = w2.z / D;
-    g[9] = norm3(add(v1, v2));
-    g[10] = norm3(add(v2, v3));
-    g[11] = norm3(add(v3, v1));
-    g[12] = norm3(add(add(v1, v2), v3));
+    g[10] = norm3(add(v1, v2));
+    g[11] = norm3(add(v2, v3));
+    g[12] = norm3(add(v3, v1));
+    g[13] = norm3(add(add(v1, v2), v3));
     raster_tri(v1, v2, v3, (int32_t)t, R, pairs);
     PSGeom P;
     ps_geometry(v1, v2, v3, P);
@@ -215,6 +217,55 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
   }
   for (int64_t c = 0; c < ncell; ++c)
     if (cells[c].x == 0) return fail(CMM_E_ARG, "cmm_tri_mesh_create: triangulation does not cover the sphere");
+  // Order the candidates of every cell by how much of the cell they cover
+  // (containment of 5 sample points: centre and the four quarter points), so
+  // the first test usually succeeds for every lane of a wave.  Only the
+  // order changes, which matters only for points on a shared edge (ties).
+  {
+    const double su[5] = {0.5, 0.25, 0.75, 0.25, 0.75}, sv[5] = {0.5, 0.25, 0.25, 0.75, 0.75};
+    std::vector<int32_t> ids;
+    std::vector<std::pair<int, int32_t>> score;
+    for (int64_t cidx = 0; cidx < ncell; ++cidx) {
+      int4& c = cells[cidx];
+      if (c.x < 2) continue;
+      const int n = c.x;
+      ids.clear();
+      if (n <= 3) {
+        const int inl[3] = {c.y, c.z, c.w};
+        for (int i = 0; i < n; ++i) ids.push_back(inl[i]);
+      } else {
+        for (int i = 0; i < n; ++i) ids.push_back(over[c.y + i]);
+      }
+      const int face = (int)(cidx / ((int64_t)R * R)), k = face / 2;
+      const double sgn = (face % 2) ? -1.0 : 1.0;
+      const int ci = (int)((cidx / R) % R), cj = (int)(cidx % R);
+      score.clear();
+      for (int32_t t : ids) {
+        const double* g = &geom[16 * (size_t)t];
+        int in = 0;
+        for (int sidx = 0; sidx < 5; ++sidx) {
+          double pt[3];
+          pt[k] = sgn;
+          pt[(k + 1) % 3] = -1.0 + 2.0 * (ci + su[sidx]) / R;
+          pt[(k + 2) % 3] = -1.0 + 2.0 * (cj + sv[sidx]) / R;
+          double b0 = g[0] * pt[0] + g[1] * pt[1] + g[2] * pt[2];
+          double b1 = g[3] * pt[0] + g[4] * pt[1] + g[5] * pt[2];
+          double b2 = g[6] * pt[0] + g[7] * pt[1] + g[8] * pt[2];
+          in += (b0 >= 0 && b1 >= 0 && b2 >= 0) ? 1 : 0;
+        }
+        score.push_back({-in, t});
+      }
+      std::stable_sort(score.begin(), score.end(),
+                       [](const std::pair<int, int32_t>& a, const std::pair<int, int32_t>& b) { return a.first < b.first; });
+      if (n <= 3) {
+        int32_t o[3] = {-1, -1, -1};
+        for (int i = 0; i < n; ++i) o[i] = score[i].second;
+        c.y = o[0]; c.z = o[1]; c.w = o[2];
+      } else {
+        for (int i = 0; i < n; ++i) over[c.y + i] = score[i].second;
+      }
+    }
+  }
   if (over.empty()) over.push_back(0);
 
   cmm_tri_mesh* m = new cmm_tri_mesh();
@@ -274,13 +325,24 @@ struct Located {
   d3 w0, w1, w2;
 };
 
-__device__ inline void load_geom(const double* __restrict__ geom, int t, Located& L) {
-  const double2* g = reinterpret_cast<const double2*>(geom + (int64_t)t * 16);
-  double2 a = g[0], b = g[1], c = g[2], d = g[3], e = g[4], f = g[5], h = g[6];
+// geometry line of a triangle (16 doubles): w0 w1 w2 (dual basis, doubles
+// 0..8) and s12 s23 s31 s4 (doubles 10..13).  A candidate test reads the
+// first 72 bytes (5 loads); the winner's norms are read once (2 loads).
+__device__ inline void load_geom_test(const double* __restrict__ geom, int t, Located& L) {
+  const double* g = geom + (int64_t)t * 16;
+  const double2* g2 = reinterpret_cast<const double2*>(g);
+  double2 a = g2[0], b = g2[1], c = g2[2], d = g2[3];
   L.w0 = d3{a.x, a.y, b.x};
   L.w1 = d3{b.y, c.x, c.y};
-  L.w2 = d3{d.x, d.y, e.x};
-  L.s12 = e.y; L.s23 = f.x; L.s31 = f.y; L.s4 = h.x;
+  L.w2 = d3{d.x, d.y, g[8]};
+}
+__device__ inline void load_geom_rest(const double* __restrict__ geom, Located& L) {
+  const double2* g2 = reinterpret_cast<const double2*>(geom + (int64_t)L.tri * 16);
+  const double2 e = g2[5], f = g2[6];
+  L.s12 = e.x;
+  L.s23 = e.y;
+  L.s31 = f.x;
+  L.s4 = f.y;
 }
 
 // containing triangle: first candidate with all barycentrics >= 0, else the
@@ -298,7 +360,7 @@ __device__ inline bool tri_locate(const cmm_tri_mesh& M, const double* __restric
   Located cur;
   for (int i = 0; i < n; ++i) {
     int t = (n <= 3) ? (i == 0 ? c.y : (i == 1 ? c.z : c.w)) : over[c.y + i];
-    load_geom(geom, t, cur);
+    load_geom_test(geom, t, cur);
     double b0 = dot(q, cur.w0), b1 = dot(q, cur.w1), b2 = dot(q, cur.w2);
     double mn = fmin(b0, fmin(b1, b2));
     if (mn > bestm) {
@@ -310,7 +372,9 @@ __device__ inline bool tri_locate(const cmm_tri_mesh& M, const double* __restric
     if (mn >= 0) break;
   }
   L.tri = best;
-  return best >= 0;
+  if (best < 0) return false;
+  load_geom_rest(geom, L);
+  return true;
 }
 
 // evaluation of one packed level at a located point (raw polynomial)
