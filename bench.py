@@ -27,6 +27,11 @@ for _p in (ROOT, os.path.join(ROOT, "cmm-s2_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of k_tri_compose for the default workload (ico-7 map mesh,
+# 992 cells per cube-face edge, depth 5, L=2048 grid) from two rocprofv3 counter
+# passes, FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a 256 MiB device copy):
+# profiles/compose_L2048_depth5_ico7_pmc.txt
+MEASURED_TRAFFIC = {(7, 2048, 5, 992): 2.616e9}
 
 
 def log(*a):
@@ -152,6 +157,8 @@ def main():
     log(f"compose depth {depth} on {n} points: kernel {k_ms:.3f} ms, wall {ms_per:.3f} ms/eval, "
         f"{mpts:.1f} Mpts/s; vorticity grid {t_vort * 1e3:.2f} ms; |w| max {float(wf.abs().max()):.4f}")
 
+    cells = int(mesh.stats()["cells_per_face"])
+    traffic = MEASURED_TRAFFIC.get((a.ico, a.Lf, depth, cells))
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(mesh, sim, q, depth, a.cpu_sample)
@@ -168,9 +175,12 @@ def main():
         "remap_step_s": [round(x, 6) for x in remap_t],
         "bootstrap_s": round(t_boot, 4), "vorticity_grid_ms": round(t_vort * 1e3, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_tri_compose", "kernel_ms": round(k_ms, 4),
-                     "algorithmic_bytes": alg_bytes},
+                     "algorithmic_bytes": alg_bytes,
+                     "note": "algorithmic gather bytes (48 + 216 x depth per point) over the event-timed kernel; "
+                             "they exceed the DRAM bytes ('traffic', rocprofv3 counters) because neighbouring "
+                             "points share triangles in L2 / MALL; the kernel is bound by its load-address rate"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

@@ -139,11 +139,38 @@ class BarotropicCMM(object):
             return True
         return False
 
-    def run(self):
-        self.bootstrap()
+    def run(self, checkpoint=None, every=None):
+        """Bootstrap (unless resumed) and step to Nt.  With ``checkpoint`` (a
+        path) and ``every`` = k, the state is saved after every k-th step and
+        at the end (replaces the pickles of spherical_simulations.py:434-438,
+        453-454)."""
+        if self.count == 0:
+            self.bootstrap()
         while self.count < self.Nt:
             self.step()
+            if checkpoint is not None and every and self.count % every == 0:
+                self.save(checkpoint)
+        if checkpoint is not None:
+            self.save(checkpoint)
         return self.submaps
+
+    # ---- checkpoint / resume -------------------------------------------
+    def save(self, path):
+        """Write the loop state to a safetensors file: submap jets, current
+        map jets, the three velocity time levels (logical order), counters
+        and times, and the initial vorticity (spline jets, or the analytic
+        field's kind and parameters).  Coefficients are not stored: they are
+        re-assembled on resume, which reproduces them bit for bit."""
+        from . import checkpoint as ck
+        ck.save_state(self, path)
+
+    @classmethod
+    def resume(cls, path, mesh, vorticity=None):
+        """Rebuild a BarotropicCMM from ``save`` on the icosahedral ``mesh``
+        (the map mesh is an input, as in the reference).  ``vorticity`` may be
+        omitted when the file holds it."""
+        from . import checkpoint as ck
+        return ck.load_state(cls, path, mesh, vorticity)
 
     def vorticity_on_grid(self, L_out, out=None, grid=None):
         """Relative vorticity w0(X) + 2 rot (cos theta(X) - cos theta) of the

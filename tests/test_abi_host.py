@@ -122,3 +122,14 @@ def test_random_field_generator_matches_oracle():
     full = df.lm_to_full(ofl.random_sh_coefficients(4, 8, 16), 16)
     from oracle import sht as osh
     assert np.array_equal(full, osh.lm_to_full(ofl.random_sh_coefficients(4, 8, 16), 16))
+
+
+def test_checkpoint_rejects_foreign_files(tmp_path):
+    import torch
+    from safetensors.torch import save_file
+    from cmm_s2 import _native as N
+    from cmm_s2.core import checkpoint as ck
+    p = str(tmp_path / "x.safetensors")
+    save_file({"a": torch.zeros(2)}, p, metadata={"cmm": '{"format": "something-else"}'})
+    with pytest.raises(N.CmmError):
+        ck.read_state(p)
