@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=2097152,
                     help="points of the CPU-baseline sample (every k-th grid point)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-Legendre comparison run")
     ap.add_argument("--small", action="store_true", help="quick check: L=64 ico-4 Nt=20 Lf=256")
     ap.add_argument("--save-maps", default=None, help="write the submap jets (torch.save) after the run")
     ap.add_argument("--compose-only", default=None,
@@ -157,6 +158,9 @@ def main():
     log(f"compose depth {depth} on {n} points: kernel {k_ms:.3f} ms, wall {ms_per:.3f} ms/eval, "
         f"{mpts:.1f} Mpts/s; vorticity grid {t_vort * 1e3:.2f} ms; |w| max {float(wf.abs().max()):.4f}")
 
+    mixed = None
+    if not a.compose_only and not a.no_mixed:
+        mixed = mixed_run(a, mesh, w0, wf, grid, s_step)
     cells = int(mesh.stats()["cells_per_face"])
     traffic = MEASURED_TRAFFIC.get((a.ico, a.Lf, depth, cells))
     cpu = None
@@ -182,8 +186,36 @@ def main():
                              "they exceed the DRAM bytes ('traffic', rocprofv3 counters) because neighbouring "
                              "points share triangles in L2 / MALL; the kernel is bound by its load-address rate"},
         "cpu_baseline": cpu,
+        "mixed": mixed,
     }
     print(json.dumps(rec), flush=True)
+
+
+def mixed_run(a, mesh, w0, wf64, grid, s64):
+    """The same CMM run with the Legendre synthesis on FP32 matrix cores
+    (FP64 recurrence, FP64-compensated partial sums): seconds per steady step
+    and the relative max-norm vorticity difference on the fine grid."""
+    import numpy as np
+    import torch
+    from cmm_s2.core.spherical_simulations import BarotropicCMM
+    sim = BarotropicCMM(a.L, a.Nt, a.Nt / a.L, mesh, w0, 2 * np.pi, a.n_maps, legendre="fp32")
+    sim.bootstrap()
+    torch.cuda.synchronize()
+    st = []
+    while sim.count < sim.Nt:
+        t0 = time.perf_counter()
+        remapped = sim.step()
+        torch.cuda.synchronize()
+        if not remapped:
+            st.append(time.perf_counter() - t0)
+    s32 = statistics.median(st[1:] if len(st) > 2 else st)
+    w32 = sim.vorticity_on_grid(a.Lf, grid=grid)
+    err = float((w32 - wf64).abs().max() / wf64.abs().max())
+    log(f"fp32 Legendre: steady step {s32 * 1e3:.3f} ms (fp64 {s64 * 1e3:.3f} ms), vorticity rel err {err:.3e}")
+    return {"legendre": "fp32 matrix cores, fp64 recurrence, fp64-compensated 32-degree partial sums",
+            "s_per_step": round(s32, 6), "fp64_s_per_step": round(s64, 6),
+            "step_speedup_vs_fp64": round(s64 / s32, 4), "vorticity_max_rel_err": err,
+            "target_rel_err": 1e-10, "meets_target": bool(err <= 1e-10 and s32 < s64)}
 
 
 def cpu_baseline(mesh, sim, q, depth, nsample):

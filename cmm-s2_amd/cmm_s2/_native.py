@@ -21,6 +21,7 @@ P_i64 = C.POINTER(C.c_int64)
 
 VEL_SPLINE, VEL_SBR, VEL_DEFORM_ROT, VEL_RH4 = 0, 1, 2, 3
 INT_EULER, INT_IMPROVED_EULER, INT_RK4, INT_RK2, INT_EVAL = 0, 1, 2, 3, 4
+LEGENDRE_FP64, LEGENDRE_FP32 = 0, 1
 VORT_SPLINE, VORT_ROSSBY, VORT_ZONAL_JET, VORT_GAUSSIAN = 0, 1, 2, 3
 VORT_PERTURBED_JET, VORT_MULTI_JET, VORT_PERTURBED_ROSSBY = 4, 5, 6
 
@@ -54,6 +55,7 @@ SIGNATURES = {
     "cmm_sht_plan_create": (C.c_int, [C.c_int, P_dbl, P_dbl, P_vp]),
     "cmm_sht_plan_destroy": (C.c_int, [vp]),
     "cmm_sht_plan_gl": (C.c_int, [vp, P_dbl, P_dbl]),
+    "cmm_sht_set_legendre": (C.c_int, [vp, C.c_int]),
     "cmm_sht_workspace_bytes": (C.c_int, [vp, C.c_int, P_i64]),
     "cmm_sht_inverse": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
     "cmm_sht_forward": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),

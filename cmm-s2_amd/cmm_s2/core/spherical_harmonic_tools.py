@@ -33,29 +33,39 @@ def _lm_tables(L):
     return np.array(ls), np.array(ms)
 
 
+# arithmetic of the Legendre synthesis: "fp64" (reference precision) or
+# "fp32" (FP32 matrix cores, FP64 recurrence, FP64-compensated partial sums)
+_LEGENDRE = {"fp64": N.LEGENDRE_FP64, "fp32": N.LEGENDRE_FP32}
+
+
 class SHT(object):
     """MWSS transform plan for bandlimit L (0 <= l < L)."""
 
     _cache = {}
 
-    def __init__(self, L):
+    def __init__(self, L, legendre="fp64"):
         self.L = L
+        self.legendre = legendre
         th = G.mwss_thetas(L)
         self.cos_t, self.sin_t = np.cos(th), np.sin(th)
         h = N.vp()
         import ctypes
         N.call("cmm_sht_plan_create", int(L), N.dptr(self.cos_t), N.dptr(self.sin_t), ctypes.byref(h))
         self._h = h.value
+        if legendre not in _LEGENDRE:
+            raise N.CmmError(f"legendre must be one of {sorted(_LEGENDRE)}")
+        N.call("cmm_sht_set_legendre", self._h, _LEGENDRE[legendre])
         self._work = {}
         ls, ms = _lm_tables(L)
         self.l_of, self.m_of = ls, ms
 
     @classmethod
-    def get(cls, L):
-        p = cls._cache.get((L, torch.cuda.current_device()))
+    def get(cls, L, legendre="fp64"):
+        key = (L, torch.cuda.current_device(), legendre)
+        p = cls._cache.get(key)
         if p is None:
-            p = cls(L)
-            cls._cache[(L, torch.cuda.current_device())] = p
+            p = cls(L, legendre)
+            cls._cache[key] = p
         return p
 
     def __del__(self):

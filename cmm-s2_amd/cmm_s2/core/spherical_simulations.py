@@ -66,8 +66,9 @@ class VorticitySampler(object):
 class BarotropicCMM(object):
     """State of euler_simulation_sphere_remapping (spherical_simulations.py:144-297)."""
 
-    def __init__(self, L, Nt, T, mesh, vorticity, rot_rate=2 * np.pi, n_maps=10):
+    def __init__(self, L, Nt, T, mesh, vorticity, rot_rate=2 * np.pi, n_maps=10, legendre="fp64"):
         self.L, self.Nt, self.T, self.mesh = L, Nt, T, mesh
+        self.legendre = legendre
         self.rot = rot_rate
         self.ns = [i for i in range(1, Nt) if i % n_maps == 0] + [Nt]
         jet0 = identity_jets(mesh)
@@ -80,7 +81,7 @@ class BarotropicCMM(object):
         self.mesh_u = structure_spherical_triangulation(L)
         self.U = spline_interp_velocity(self.mesh_u, B.zeros(3, 3, 4, self.mesh_u.nv), None,
                                         ts=self.tspan[0:3])
-        self.plan = SHT.get(L)
+        self.plan = SHT.get(L, legendre)
         self.sampler = VorticitySampler(vorticity, self.mesh_u, rot_rate)
         self.samples = self.mesh_u.sample_points()
         n = self.samples.shape[0]

@@ -46,6 +46,7 @@ struct cmm_sht_plan {
   double2* tw = nullptr;     // exp(-2 pi i n / N), n < N
   double* dinv = nullptr;    // D_inv[l] (L)
   double2* rec = nullptr;    // recurrence (a_lm, a_lm b_lm) per packed lm
+  int legendre = 0;          // CMM_LEGENDRE_FP64 / CMM_LEGENDRE_FP32 for the synthesis
   int nunits = 0;            // forward work units (m, first degree of a segment)
   int2* units = nullptr;
   int* unit0 = nullptr;      // first unit of each m (L)
@@ -214,6 +215,13 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
     return rc;
   }
   *out = p;
+  return CMM_OK;
+}
+
+extern "C" int cmm_sht_set_legendre(cmm_sht_plan* p, int mode) {
+  if (!p || (mode != CMM_LEGENDRE_FP64 && mode != CMM_LEGENDRE_FP32))
+    return fail(CMM_E_ARG, "cmm_sht_set_legendre: bad arguments");
+  p->legendre = mode;
   return CMM_OK;
 }
 
@@ -440,7 +448,17 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 constexpr int kInvRings = 64;  // rings per wave
 constexpr int kInvDeg = 8;     // degrees per chunk (4 of each parity)
 
-template <int NT>
+// MIXED: FP32 matrix cores (v_mfma_f32_16x16x4f32, twice the FP64 rate) on
+// FP32 copies of Pbar and of the coefficients; the recurrence stays FP64.
+// The FP32 accumulators cover kMixedFlush chunks (32 degrees of each parity)
+// and are then added into FP64 accumulators, so the FP32 rounding of the
+// accumulation is bounded by 32-term partial sums (compensated FP64
+// correction).  The f32 instruction's D layout differs from the f64 one:
+// lane l, element r holds D[4 (l / 16) + r][l % 16].
+typedef float fx4 __attribute__((ext_vector_type(4)));
+constexpr int kMixedFlush = 8;
+
+template <int NT, bool MIXED>
 __global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, int nfp, int nrb,
                                                        const double* __restrict__ x_t, const double* __restrict__ dP,
                                                        const int* __restrict__ dS, const double2* __restrict__ rec,
@@ -499,12 +517,16 @@ __global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, in
     }
   };
   dx4 acc[4][2][NT];
+  fx4 acc32[4][2][NT];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) acc[mt][p][nt] = dx4{0.0, 0.0, 0.0, 0.0};
+      for (int nt = 0; nt < NT; ++nt) {
+        acc[mt][p][nt] = dx4{0.0, 0.0, 0.0, 0.0};
+        acc32[mt][p][nt] = fx4{0.0f, 0.0f, 0.0f, 0.0f};
+      }
   double a[4][2];
   auto readA = [&]() {
 #pragma unroll
@@ -512,14 +534,35 @@ __global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, in
 #pragma unroll
       for (int p = 0; p < 2; ++p) a[mt][p] = Pt[2 * bk + p][16 * mt + bj];
   };
+  int nmix = 0;
   auto mma = [&](const double (&b)[2][NT]) {
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-          acc[mt][p][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt][p], b[p][nt], acc[mt][p][nt], 0, 0, 0);
+        for (int nt = 0; nt < NT; ++nt) {
+          if constexpr (MIXED)
+            acc32[mt][p][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a[mt][p], (float)b[p][nt],
+                                                                    acc32[mt][p][nt], 0, 0, 0);
+          else
+            acc[mt][p][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt][p], b[p][nt], acc[mt][p][nt], 0, 0, 0);
+        }
+    if constexpr (MIXED) {
+      if (++nmix == kMixedFlush) {
+        nmix = 0;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+          for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+              for (int e = 0; e < 4; ++e) acc[mt][p][nt][e] += (double)acc32[mt][p][nt][e];
+              acc32[mt][p][nt] = fx4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+      }
+    }
   };
   // Two chunks per iteration with no exit in between (an odd chunk count is
   // padded with a chunk whose tile rows and coefficients are zero), so the
@@ -542,6 +585,16 @@ __global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, in
     __syncthreads();
     load(l0 + 3 * kInvDeg, bB);
   }
+  if constexpr (MIXED) {
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc[mt][p][nt][e] += (double)acc32[mt][p][nt][e];
+  }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
@@ -550,7 +603,7 @@ __global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, in
       if (f >= nfp) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int rr = rb * kInvRings + 16 * mt + 4 * e + bk;
+        const int rr = rb * kInvRings + 16 * mt + (MIXED ? 4 * bk + e : 4 * e + bk);
         if (rr >= nr) continue;
         const int tr = tfirst + rr;
         const double ev = acc[mt][0][nt][e], od = acc[mt][1][nt][e];
@@ -929,12 +982,10 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   const size_t lds_rec = (size_t)((L + kInvDeg - 1) / kInvDeg + 2) * kInvDeg * sizeof(double2);
   for (int f0 = 0; f0 < nf;) {
     const int nfp = std::min(16, nf - f0);
-    if (nfp > 8)
-      hipLaunchKernelGGL(k_leg_inverse<2>, grid, block, lds_rec, st, L, nf, f0, nfp, nrb, p->x_t, p->dP_t, p->dS_t,
-                         p->rec, flm, reinterpret_cast<double*>(F));
-    else
-      hipLaunchKernelGGL(k_leg_inverse<1>, grid, block, lds_rec, st, L, nf, f0, nfp, nrb, p->x_t, p->dP_t, p->dS_t,
-                         p->rec, flm, reinterpret_cast<double*>(F));
+    auto kern = p->legendre == CMM_LEGENDRE_FP32 ? (nfp > 8 ? k_leg_inverse<2, true> : k_leg_inverse<1, true>)
+                                                 : (nfp > 8 ? k_leg_inverse<2, false> : k_leg_inverse<1, false>);
+    hipLaunchKernelGGL(kern, grid, block, lds_rec, st, L, nf, f0, nfp, nrb, p->x_t, p->dP_t, p->dS_t, p->rec, flm,
+                       reinterpret_cast<double*>(F));
     CMM_LAUNCH_CHECK("cmm_sht_inverse: legendre");
     f0 += nfp;
   }

@@ -183,6 +183,14 @@ typedef struct cmm_sht_plan cmm_sht_plan;
 /* cos_t, sin_t: host (L+1) cos / sin of the MWSS colatitudes. */
 int cmm_sht_plan_create(int L, const double* cos_t, const double* sin_t, cmm_sht_plan** out);
 int cmm_sht_plan_destroy(cmm_sht_plan* plan);
+/* Arithmetic of the Legendre synthesis of cmm_sht_inverse (the analysis and
+ * every other kernel stay FP64).  No counterpart in the reference, which is
+ * FP64 only: FP32 runs the contraction on FP32 matrix cores with FP32
+ * operands, the recurrence in FP64 and FP32 partial sums of 32 degrees per
+ * parity added into FP64 accumulators; relative error ~1e-7 of the field. */
+#define CMM_LEGENDRE_FP64 0
+#define CMM_LEGENDRE_FP32 1
+int cmm_sht_set_legendre(cmm_sht_plan* plan, int mode);
 /* host copies of the plan's Gauss-Legendre nodes / weights (L each) */
 int cmm_sht_plan_gl(const cmm_sht_plan* plan, double* x, double* w);
 /* bytes of device workspace needed by forward / inverse for nf fields */
