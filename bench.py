@@ -158,6 +158,7 @@ def main():
     log(f"compose depth {depth} on {n} points: kernel {k_ms:.3f} ms, wall {ms_per:.3f} ms/eval, "
         f"{mpts:.1f} Mpts/s; vorticity grid {t_vort * 1e3:.2f} ms; |w| max {float(wf.abs().max()):.4f}")
 
+    area = area_distortion(sim, q)
     mixed = None
     if not a.compose_only and not a.no_mixed:
         mixed = mixed_run(a, mesh, w0, wf, grid, s_step)
@@ -187,8 +188,28 @@ def main():
                              "points share triangles in L2 / MALL; the kernel is bound by its load-address rate"},
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "area_distortion": area,
     }
     print(json.dumps(rec), flush=True)
+
+
+def area_distortion(sim, q):
+    """Area factor |J t1 x J t2| of the composed map on the fine grid (J its
+    differential on the tangent planes, t1, t2 an orthonormal tangent basis):
+    exactly 1 for the incompressible flow, so its deviation measures the
+    discretisation error of the maps at full size."""
+    import torch
+    X, J = sim.submaps.jacobian(q)
+    a = torch.cross(q, torch.tensor([0.3, -0.5, 0.8], dtype=q.dtype, device=q.device).expand_as(q), dim=1)
+    t1 = a / a.norm(dim=1, keepdim=True)
+    t2 = torch.cross(q, t1, dim=1)
+    u = torch.einsum("cdn,nd->nc", J, t1)
+    v = torch.einsum("cdn,nd->nc", J, t2)
+    f = torch.cross(u, v, dim=1).norm(dim=1)
+    dev = (f - 1).abs()
+    out = {"points": int(q.shape[0]), "max_abs_dev": float(dev.max()), "rms_dev": float((dev ** 2).mean().sqrt())}
+    log(f"area factor of the composed map: max |f-1| {out['max_abs_dev']:.3e}, rms {out['rms_dev']:.3e}")
+    return out
 
 
 def mixed_run(a, mesh, w0, wf64, grid, s64):

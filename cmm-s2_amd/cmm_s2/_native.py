@@ -43,6 +43,9 @@ SIGNATURES = {
     "cmm_tri_assemble": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
     "cmm_tri_eval": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp]),
     "cmm_tri_compose": (C.c_int, [vp, P_vp, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp, vp]),
+    "cmm_tri_eval_grad": (C.c_int, [vp, vp, C.c_int, vp, i64, i64, i64, vp, vp]),
+    "cmm_tri_compose_grad": (C.c_int, [vp, P_vp, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp, vp]),
+    "cmm_mwss_eval_grad": (C.c_int, [vp, vp, vp, i64, i64, i64, vp, vp]),
     "cmm_tri_stencil_jets": (C.c_int, [vp, vp, vp, i64, vp, i64, vp, dbl, vp, vp, vp]),
     "cmm_mwss_mesh_create": (C.c_int, [C.c_int, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_vp]),
     "cmm_mwss_mesh_destroy": (C.c_int, [vp]),

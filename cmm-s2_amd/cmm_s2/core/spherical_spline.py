@@ -119,6 +119,19 @@ class sphere_diffeomorphism(object):
                out.data_ptr(), 1, n, B.stream())
         return B.like(out, q_pts)
 
+    def eval_grad(self, q_pts, eval_pts=None):
+        """spherical_spline.py:208-265: differential of the map at q (N,3) ->
+        (3,3,N), [c][d] = d phi_c / d x_d on the tangent plane at q (the map
+        value is recomputed, ``eval_pts`` is accepted for the reference's
+        signature and ignored)."""
+        q = B.to_dev(q_pts)
+        n = q.shape[0]
+        out = B.empty(3, 3, n)
+        sp, sc = B.point_strides(q, "N3")
+        N.call("cmm_tri_eval_grad", self.mesh._h, self.packed.data_ptr(), 3, q.data_ptr(), n, sp, sc,
+               out.data_ptr(), B.stream())
+        return B.like(out, q_pts)
+
     def stencil_eval(self, q_pts, st_pts):
         """spherical_spline.py:268-298: locate the vertex feet q (3,Nv), evaluate
         the four stencil feet st (3,4,Nv) on that split sub-triangle's
@@ -189,6 +202,22 @@ class composite_sphere_diffeomorphism(object):
         """spherical_spline.py:365-398: q (N,3) -> (N,3)."""
         q = B.to_dev(q_pts)
         return B.like(self.compose(q, "N3", N_c=N_c), q_pts)
+
+    def jacobian(self, q_pts, N_c=None, pre=()):
+        """Composed map and its differential through the whole chain:
+        q (N,3) -> positions (N,3) and (3,3,N) [c][d] (tangent planes at q and
+        at the image).  Its determinant on the tangent planes is the area
+        distortion (1 for incompressible flows)."""
+        q = B.to_dev(q_pts)
+        n = q.shape[0]
+        lv = list(pre) + self.levels(N_c)
+        out = B.empty(n, 3)
+        jac = B.empty(3, 3, n)
+        sp, sc = B.point_strides(q, "N3")
+        arr = N.ptr_array([p.data_ptr() for p in lv])
+        N.call("cmm_tri_compose_grad", self.mesh._h, arr, len(lv), q.data_ptr(), n, sp, sc, out.data_ptr(), 3, 1,
+               jac.data_ptr(), B.stream())
+        return B.like(out, q_pts), B.like(jac, q_pts)
 
     def eval_all(self, q_pts):
         """spherical_spline.py:400-437: [q, after level Nc-1, ..., after level 0]."""
@@ -332,6 +361,20 @@ class spherical_spline(object):
             N.call("cmm_tri_eval", self.mesh._h, self.packed.data_ptr(), 1, 0, q.data_ptr(), n, sp, sc,
                    out.data_ptr(), 1, 1, B.stream())
         return out
+
+    def gradient(self, q_pts):
+        """spherical_spline.py:73-111: tangential gradient at q (N,3) -> (3,N)."""
+        q = B.to_dev(q_pts)
+        n = q.shape[0]
+        out = B.empty(3, n)
+        sp, sc = B.point_strides(q, "N3")
+        if _is_mwss(self.mesh):
+            N.call("cmm_mwss_eval_grad", self.mesh._h, self.packed.data_ptr(), q.data_ptr(), n, sp, sc,
+                   out.data_ptr(), B.stream())
+        else:
+            N.call("cmm_tri_eval_grad", self.mesh._h, self.packed.data_ptr(), 1, q.data_ptr(), n, sp, sc,
+                   out.data_ptr(), B.stream())
+        return B.like(out, q_pts)
 
     def __call__(self, phi, the):
         """spherical_spline.py:42-70: 2-D (phi, theta) arrays -> 2-D values."""

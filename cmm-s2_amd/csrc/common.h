@@ -127,6 +127,35 @@ __host__ __device__ inline double bb6(d3 c, const double* p) {
          2 * (c.z * c.y) * p[4] + 2 * (c.z * c.x) * p[5];
 }
 
+// gradient of bb6 with respect to the three barycentrics (grad_HBB,
+// spherical_spline.py:700-711)
+__host__ __device__ inline d3 bb6_db(d3 c, const double* p) {
+  return d3{2 * (p[0] * c.x + p[3] * c.y + p[5] * c.z), 2 * (p[1] * c.y + p[3] * c.x + p[4] * c.z),
+            2 * (p[2] * c.z + p[4] * c.y + p[5] * c.x)};
+}
+
+// Jacobian of x -> p(x) / |p(x)| restricted to the tangent plane at q:
+// J = (I / r - X X^T / r^3) G (I - q q^T), G[c] the ambient gradient of
+// component c, X the raw value, r = |X| (the normalisation of
+// spherical_spline.py:250-263, with the input-side projection that makes it
+// the differential T_q S^2 -> T_X S^2).  Row-major J[c][d].
+__host__ __device__ inline void normalized_jacobian(d3 X, const d3 G[3], d3 q, double J[9]) {
+  const double r2 = X.x * X.x + X.y * X.y + X.z * X.z, r = sqrt(r2), ir = 1 / r, ir3 = ir / r2;
+  const double Xv[3] = {X.x, X.y, X.z}, qv[3] = {q.x, q.y, q.z};
+  double A[9];
+  for (int d = 0; d < 3; ++d) {
+    const double gd[3] = {d == 0 ? G[0].x : (d == 1 ? G[0].y : G[0].z),
+                          d == 0 ? G[1].x : (d == 1 ? G[1].y : G[1].z),
+                          d == 0 ? G[2].x : (d == 1 ? G[2].y : G[2].z)};
+    const double xg = Xv[0] * gd[0] + Xv[1] * gd[1] + Xv[2] * gd[2];
+    for (int c = 0; c < 3; ++c) A[3 * c + d] = gd[c] * ir - Xv[c] * xg * ir3;
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double aq = A[3 * c] * qv[0] + A[3 * c + 1] * qv[1] + A[3 * c + 2] * qv[2];
+    for (int d = 0; d < 3; ++d) J[3 * c + d] = A[3 * c + d] - aq * qv[d];
+  }
+}
+
 // 19 Bernstein-Bezier coefficients of one component on one macro triangle
 // (spherical_spline.py:300-337) from the vertex values f[i] and vertex
 // gradients gr[i] (i = 0..2) with the Powell-Sabin geometry recomputed from

@@ -411,6 +411,37 @@ __global__ __launch_bounds__(256) void k_mwss_eval(cmm_mwss_mesh M, LevelSet ls,
   for (int c = 0; c < ncomp; ++c) out[i * osp + c * osc] = r[c];
 }
 
+// tangential gradient of a scalar spline on the MWSS mesh
+// (spherical_spline.py:73-111): out (3, n), NaN where not located
+__global__ __launch_bounds__(256) void k_mwss_eval_grad(cmm_mwss_mesh M, const double* __restrict__ packed,
+                                                        const double* __restrict__ q, int64_t n, int64_t sp,
+                                                        int64_t sc, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const d3 x = load_pt(q, i, sp, sc);
+  MLoc o;
+  if (!mwss_locate(M, x, o)) {
+    for (int k = 0; k < 3; ++k) out[(int64_t)k * n + i] = __builtin_nan("");
+    return;
+  }
+  const int s = split_index(o.b.x, o.b.y, o.b.z);
+  const d3 cb = sub_bary_v(s, o.b, o.v);
+  // macro dual basis w_i = (v_j x v_k) / det: grad of the barycentric b_i
+  const double D = det3(o.v[0], o.v[1], o.v[2]);
+  const d3 w0 = scl(1 / D, cross(o.v[1], o.v[2])), w1 = scl(1 / D, cross(o.v[2], o.v[0])),
+           w2 = scl(1 / D, cross(o.v[0], o.v[1]));
+  const d3 gx = sub_bary_v(s, d3{w0.x, w1.x, w2.x}, o.v);
+  const d3 gy = sub_bary_v(s, d3{w0.y, w1.y, w2.y}, o.v);
+  const d3 gz = sub_bary_v(s, d3{w0.z, w1.z, w2.z}, o.v);
+  const double* p = packed + (o.tri * 6 + s) * 6;
+  const d3 db = bb6_db(cb, p);
+  const d3 g = d3{dot(db, gx), dot(db, gy), dot(db, gz)};
+  const double gq = dot(g, x);
+  out[i] = g.x - gq * x.x;
+  out[n + i] = g.y - gq * x.y;
+  out[2 * n + i] = g.z - gq * x.z;
+}
+
 template <int NC>
 __global__ __launch_bounds__(256) void k_mwss_assemble(cmm_mwss_mesh M, const double* __restrict__ vals,
                                                        double* __restrict__ coeffs, double* __restrict__ packed) {
@@ -678,6 +709,16 @@ extern "C" int cmm_advect_stencil(const cmm_velocity* vel, int integ, double dt,
     hipLaunchKernelGGL(k_advect_stencil<4>, dim3(grid_for(4 * nv, 128)), dim3(128), 0, as_stream(stream), V, integ,
                        dt, verts, verts_row, st, st_row, nv, yv, yv_row, yst, yst_row);
   CMM_LAUNCH_CHECK("cmm_advect_stencil");
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_eval_grad(const cmm_mwss_mesh* m, const double* packed, const double* q, int64_t n,
+                                  int64_t sp, int64_t sc, double* out, void* stream) {
+  if (!m || !packed || n < 0 || (n > 0 && (!q || !out))) return fail(CMM_E_ARG, "cmm_mwss_eval_grad: bad arguments");
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_mwss_eval_grad, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, packed, q, n,
+                     sp, sc, out);
+  CMM_LAUNCH_CHECK("cmm_mwss_eval_grad");
   return CMM_OK;
 }
 

@@ -392,6 +392,24 @@ __device__ inline d3 eval_level3(const double* __restrict__ packed, const Locate
   return d3{bb6(c, buf), bb6(c, buf + 6), bb6(c, buf + 12)};
 }
 
+// value and ambient gradients of the raw polynomial components at a located
+// point: grad p_c = sum_i dp_c/db_i grad b_i, where the sub-triangle
+// barycentrics are linear in the macro ones (sub_bary), so their gradients
+// are sub_bary applied to the columns of the macro dual basis
+__device__ inline void grad_level(const double* __restrict__ packed, int ncomp, const Located& L, int s,
+                                  double* val, d3* g) {
+  const d3 cb = sub_bary(s, L.b0, L.b1, L.b2, L.s12, L.s23, L.s31, L.s4);
+  const d3 gx = sub_bary(s, L.w0.x, L.w1.x, L.w2.x, L.s12, L.s23, L.s31, L.s4);
+  const d3 gy = sub_bary(s, L.w0.y, L.w1.y, L.w2.y, L.s12, L.s23, L.s31, L.s4);
+  const d3 gz = sub_bary(s, L.w0.z, L.w1.z, L.w2.z, L.s12, L.s23, L.s31, L.s4);
+  const double* p = packed + ((int64_t)L.tri * 6 + s) * 6 * ncomp;
+  for (int c = 0; c < ncomp; ++c) {
+    val[c] = bb6(cb, p + 6 * c);
+    const d3 db = bb6_db(cb, p + 6 * c);
+    g[c] = d3{dot(db, gx), dot(db, gy), dot(db, gz)};
+  }
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
@@ -456,6 +474,76 @@ __global__ __launch_bounds__(256) void k_tri_compose(cmm_tri_mesh M, LevelPtrs l
     if (inter) store_pt(inter + (int64_t)l * n * 3, i, 3, 1, x);
   }
   store_pt(out, i, osp, osc, x);
+}
+
+// map Jacobian (ncomp 3, out (3,3,n) [c][d]) or tangential gradient of a
+// scalar spline (ncomp 1, out (3,n)); NaN where the point is not located
+__global__ __launch_bounds__(256) void k_tri_eval_grad(cmm_tri_mesh M, const double* __restrict__ packed, int ncomp,
+                                                       const double* __restrict__ q, int64_t n, int64_t sp, int64_t sc,
+                                                       double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const d3 x = load_pt(q, i, sp, sc);
+  Located L;
+  const int nout = ncomp == 3 ? 9 : 3;
+  if (!tri_locate(M, M.geom, M.cells, M.over, x, L)) {
+    for (int k = 0; k < nout; ++k) out[(int64_t)k * n + i] = __builtin_nan("");
+    return;
+  }
+  const int s = split_index(L.b0, L.b1, L.b2);
+  double v[3];
+  d3 g[3];
+  grad_level(packed, ncomp, L, s, v, g);
+  if (ncomp == 3) {
+    double J[9];
+    normalized_jacobian(d3{v[0], v[1], v[2]}, g, x, J);
+    for (int k = 0; k < 9; ++k) out[(int64_t)k * n + i] = J[k];
+  } else {
+    const double gq = dot(g[0], x);
+    out[i] = g[0].x - gq * x.x;
+    out[n + i] = g[0].y - gq * x.y;
+    out[2 * n + i] = g[0].z - gq * x.z;
+  }
+}
+
+// composed map and its Jacobian: J <- J_level(x) J through the chain
+// (levels in application order), out (n,3) positions and jac (3,3,n)
+__global__ __launch_bounds__(256) void k_tri_compose_grad(cmm_tri_mesh M, LevelPtrs lv, int depth,
+                                                          const double* __restrict__ q, int64_t n, int64_t sp,
+                                                          int64_t sc, double* __restrict__ out, int64_t osp,
+                                                          int64_t osc, double* __restrict__ jac) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  d3 x = load_pt(q, i, sp, sc);
+  double J[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  // project the identity on the input tangent plane
+  {
+    const double qv[3] = {x.x, x.y, x.z};
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) J[3 * c + d] -= qv[c] * qv[d];
+  }
+  for (int l = 0; l < depth; ++l) {
+    Located L;
+    if (!tri_locate(M, M.geom, M.cells, M.over, x, L)) {
+      const double nan = __builtin_nan("");
+      x = d3{nan, nan, nan};
+      for (int k = 0; k < 9; ++k) J[k] = nan;
+      break;
+    }
+    const int s = split_index(L.b0, L.b1, L.b2);
+    double v[3];
+    d3 g[3];
+    grad_level(lv.p[l], 3, L, s, v, g);
+    double Jl[9], Jn[9];
+    normalized_jacobian(d3{v[0], v[1], v[2]}, g, x, Jl);
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) Jn[3 * c + d] = Jl[3 * c] * J[d] + Jl[3 * c + 1] * J[3 + d] + Jl[3 * c + 2] * J[6 + d];
+    for (int k = 0; k < 9; ++k) J[k] = Jn[k];
+    const double inv = 1 / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    x = d3{inv * v[0], inv * v[1], inv * v[2]};
+  }
+  store_pt(out, i, osp, osc, x);
+  for (int k = 0; k < 9; ++k) jac[(int64_t)k * n + i] = J[k];
 }
 
 template <int NC>
@@ -600,6 +688,35 @@ extern "C" int cmm_tri_compose(const cmm_tri_mesh* m, const double* const* level
   hipLaunchKernelGGL(k_tri_compose, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, lv,
                      depth, q, n, sp, sc, out, osp, osc, inter);
   CMM_LAUNCH_CHECK("cmm_tri_compose");
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_eval_grad(const cmm_tri_mesh* m, const double* packed, int ncomp, const double* q,
+                                 int64_t n, int64_t sp, int64_t sc, double* out, void* stream) {
+  if (!m || !packed || (ncomp != 1 && ncomp != 3) || n < 0 || (n > 0 && (!q || !out)))
+    return fail(CMM_E_ARG, "cmm_tri_eval_grad: bad arguments");
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_tri_eval_grad, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, packed, ncomp,
+                     q, n, sp, sc, out);
+  CMM_LAUNCH_CHECK("cmm_tri_eval_grad");
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_compose_grad(const cmm_tri_mesh* m, const double* const* levels, int depth,
+                                    const double* q, int64_t n, int64_t sp, int64_t sc, double* out,
+                                    int64_t osp, int64_t osc, double* jac, void* stream) {
+  if (!m || depth < 0 || depth > kMaxDepth || (depth > 0 && !levels) || n < 0 ||
+      (n > 0 && (!q || !out || !jac)))
+    return fail(CMM_E_ARG, "cmm_tri_compose_grad: bad arguments");
+  LevelPtrs lv;
+  for (int l = 0; l < depth; ++l) {
+    if (!levels[l]) return fail(CMM_E_ARG, "cmm_tri_compose_grad: null level");
+    lv.p[l] = levels[l];
+  }
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_tri_compose_grad, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, lv, depth, q,
+                     n, sp, sc, out, osp, osc, jac);
+  CMM_LAUNCH_CHECK("cmm_tri_compose_grad");
   return CMM_OK;
 }
 

@@ -89,6 +89,25 @@ int cmm_tri_compose(const cmm_tri_mesh* mesh, const double* const* levels, int d
                     double* out, int64_t out_stride_pt, int64_t out_stride_comp,
                     double* inter, void* stream);
 
+/* Differential of an evaluated spline.  ncomp 3: Jacobian of the normalised
+ * map x -> p(x)/|p(x)| restricted to the tangent plane at x, out (3,3,n)
+ * row-major [c][d] at out[(3c + d) n + i] (sphere_diffeomorphism.eval_grad,
+ * spherical_spline.py:208-265 with grad_HBB 700-711; the reference function
+ * reads attributes it never sets, so its math is restated, plus the input-side
+ * projection).  ncomp 1: tangential gradient of a scalar spline, out (3,n)
+ * (spherical_spline.gradient 73-111 with d_Proj 714-720). */
+int cmm_tri_eval_grad(const cmm_tri_mesh* mesh, const double* packed, int ncomp, const double* q,
+                      int64_t n, int64_t q_stride_pt, int64_t q_stride_comp, double* out, void* stream);
+
+/* composite map and its Jacobian (chain rule through the levels, applied in
+ * array order): out(i, c) positions, jac (3,3,n) as cmm_tri_eval_grad.  No
+ * counterpart in the reference; used for the area-preservation diagnostic
+ * det(Dphi) = 1 of incompressible flows. */
+int cmm_tri_compose_grad(const cmm_tri_mesh* mesh, const double* const* levels, int depth,
+                         const double* q, int64_t n, int64_t q_stride_pt, int64_t q_stride_comp,
+                         double* out, int64_t out_stride_pt, int64_t out_stride_comp, double* jac,
+                         void* stream);
+
 /* advect (stencil part) + spline_proj_sphere (evolution_functions.py:90-133,
  * spherical_spline.py:268-298) fused.  yv: (3, Nv) integrated vertex feet,
  * element (c, v) at yv[c*yv_row + v]; st: (3, 4*Nv) stencil feet, element
@@ -126,6 +145,11 @@ int cmm_mwss_locate(const cmm_mwss_mesh* mesh, const double* q, int64_t n,
 /* assemble_coefficients on the MWSS mesh; jets (ncomp,4,Nu). */
 int cmm_mwss_assemble(const cmm_mwss_mesh* mesh, const double* vals, int ncomp,
                       double* coeffs, double* packed, void* stream);
+
+/* spherical_spline.gradient (spherical_spline.py:73-111) for a scalar spline
+ * (packed, ncomp 1) on the MWSS mesh: tangential gradient, out (3, n). */
+int cmm_mwss_eval_grad(const cmm_mwss_mesh* mesh, const double* packed, const double* q, int64_t n,
+                       int64_t q_stride_pt, int64_t q_stride_comp, double* out, void* stream);
 
 /* spline_interp_velocity.__call__ (spherical_spline.py:570-615) and the
  * scalar spherical_spline (42-70): out(i,c) = sum_k w[k] * P_k(q_i)_c over

@@ -235,3 +235,44 @@ def grad_eval(query, coeffs, q_pts):
     outy = (-X[:, 0] * X[:, 1]) * A[0] + (1 - X[:, 1] ** 2) * A[1] + (-X[:, 1] * X[:, 2]) * A[2]
     outz = (-X[:, 0] * X[:, 2]) * A[0] + (-X[:, 1] * X[:, 2]) * A[1] + (1 - X[:, 2] ** 2) * A[2]
     return [outx, outy, outz]
+
+
+def _ambient_grads(query, coeffs, q_pts):
+    """Raw values (C,N) and ambient gradients (C,3,N) of the located BB
+    polynomials of coeffs (C,19,T): grad p = sum_i dp/db_i grad b_i with the
+    barycentric gradients bary_coords(.., e_x / e_y / e_z) of
+    spherical_spline.py:228-240 and grad_HBB (700-711)."""
+    bcc, trangs, v_pts = query(q_pts)
+    bb = bary_minmax(bcc)
+    nCs = Cs[bb[0], bb[1]]
+    v_pts_n = new_vpoints(v_pts, bb)
+    n = len(q_pts)
+    unos = np.ones([n, 3])
+    V0, V1, V2 = v_pts_n[:, 0, :], v_pts_n[:, 1, :], v_pts_n[:, 2, :]
+    bcc_n = g.bary_coords(V0, V1, V2, q_pts)
+    Bs = [g.bary_coords(V0, V1, V2, unos * e) for e in np.eye(3)]
+    r = np.arange(n)
+    vals, grads = [], []
+    for cf in coeffs:
+        cfs = cf[:, trangs]
+        c = lambda j: cfs[nCs[:, j], r]
+        vals.append(_bb(bcc_n, cfs[None], nCs)[0])
+        dp1 = 2 * c(0) * bcc_n[:, 0] + 2 * c(3) * bcc_n[:, 1] + 2 * c(5) * bcc_n[:, 2]
+        dp2 = 2 * c(1) * bcc_n[:, 1] + 2 * c(3) * bcc_n[:, 0] + 2 * c(4) * bcc_n[:, 2]
+        dp3 = 2 * c(2) * bcc_n[:, 2] + 2 * c(4) * bcc_n[:, 1] + 2 * c(5) * bcc_n[:, 0]
+        grads.append(np.array([b[:, 0] * dp1 + b[:, 1] * dp2 + b[:, 2] * dp3 for b in Bs]))
+    return np.array(vals), np.array(grads)
+
+
+def map_grad(query, coeffs, q_pts):
+    """Differential of the normalised map (sphere_diffeomorphism.eval_grad,
+    spherical_spline.py:208-265): J[c][d] = grad(X_c / |X|)_d restricted to
+    the tangent plane at q, i.e. (I/|X| - X X^T/|X|^3) G (I - q q^T).
+    coeffs (3,19,T), q_pts (N,3) -> (3,3,N)."""
+    X, G = _ambient_grads(query, coeffs, q_pts)
+    r = np.sqrt((X ** 2).sum(0))
+    xg = np.einsum("kn,kdn->dn", X, G)
+    A = G / r - X[:, None, :] * xg[None, :, :] / r ** 3
+    q = q_pts.T
+    Aq = np.einsum("cdn,dn->cn", A, q)
+    return A - Aq[:, None, :] * q[None, :, :]

@@ -180,3 +180,65 @@ def _rot_jets(R, V):
     jets[:, 0] = R @ V.T
     jets[:, 1:] = np.transpose(np.einsum("ij,njk->nik", R, P), (2, 1, 0))
     return jets
+
+
+def _tangent_fd(f, q, h=1e-6):
+    """Central differences of f along the two tangent directions at q (N,3):
+    returns (J t1, J t2) estimates and the directions."""
+    a = np.cross(q, np.array([0.3, -0.5, 0.8]))
+    t1 = a / np.linalg.norm(a, axis=1)[:, None]
+    t2 = np.cross(q, t1)
+    out = []
+    for t in (t1, t2):
+        qp = q + h * t
+        qm = q - h * t
+        qp /= np.linalg.norm(qp, axis=1)[:, None]
+        qm /= np.linalg.norm(qm, axis=1)[:, None]
+        out.append((f(qp) - f(qm)) / (2 * h))
+    return out, (t1, t2)
+
+
+def test_map_jacobian_against_finite_differences_and_quadratics():
+    m = om.SphericalTriangulation(om.icosahedral_points(3))
+    jets = perturbed_identity_jets(m.vertices, 7, amp=0.05)
+    c = osp.assemble(jets, m.simplices, m.ps_split)
+    q = random_unit(3000, 11)
+    J = osp.map_grad(m.query, c, q)
+    (d1, d2), (t1, t2) = _tangent_fd(lambda x: osp.map_eval(m.query, c, x), q)
+    for d, t in ((d1, t1), (d2, t2)):
+        Jt = np.einsum("cdn,nd->cn", J, t)
+        assert np.abs(Jt - d).max() < 1e-5 * np.abs(d).max()
+    # exact for a homogeneous quadratic map p(x) = (x^T A_c x)_c: the spline
+    # reproduces p, so the Jacobian is the analytic one
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((3, 3, 3))
+    A = A + A.transpose(0, 2, 1) + 6 * np.eye(3)[None]
+    c2 = osp.assemble(_quadratic_jets(m.vertices, A), m.simplices, m.ps_split)
+    X = np.array([np.einsum("ni,ij,nj->n", q, A[k], q) for k in range(3)])
+    G = np.array([(2 * q @ A[k]).T for k in range(3)])
+    r = np.sqrt((X ** 2).sum(0))
+    xg = np.einsum("kn,kdn->dn", X, G)
+    Aj = G / r - X[:, None] * xg[None] / r ** 3
+    ref = Aj - np.einsum("cdn,dn->cn", Aj, q.T)[:, None] * q.T[None]
+    assert np.abs(osp.map_grad(m.query, c2, q) - ref).max() < 1e-11 * np.abs(ref).max()
+    # tangent at both ends: J q = 0 and X^T J = 0
+    assert np.abs(np.einsum("cdn,nd->cn", J, q)).max() < 1e-12
+    Xn = osp.map_eval(m.query, c, q)
+    assert np.abs(np.einsum("cn,cdn->dn", Xn, J)).max() < 1e-12
+
+
+def test_scalar_gradient_against_finite_differences():
+    m = om.StructuredTriangulation(16)
+    V = m.vertices
+    f = lambda X: X[:, 0] ** 3 + X[:, 1] * X[:, 2] + np.sin(2 * X[:, 2])
+    gr = np.stack([3 * V[:, 0] ** 2, V[:, 2], V[:, 1] + 2 * np.cos(2 * V[:, 2])], 1)
+    gr = gr - np.einsum("ij,ij->i", gr, V)[:, None] * V
+    vals = np.zeros((4, len(V)))
+    vals[0] = f(V)
+    vals[1:] = gr.T
+    c = osp.assemble(vals, m.simplices, m.ps_split)
+    q = random_unit(2000, 12)
+    grad = np.array(osp.grad_eval(m.query, c, q))
+    (d1, d2), (t1, t2) = _tangent_fd(lambda x: osp.located_eval(m.query, c, x), q)
+    for d, t in ((d1, t1), (d2, t2)):
+        assert np.abs(np.einsum("dn,nd->n", grad, t) - d).max() < 1e-5 * np.abs(d).max()
