@@ -56,6 +56,9 @@ def parse():
                     help="points of the CPU-baseline sample (every k-th grid point)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-Legendre comparison run")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="instead: M seeded members (L in {128..512}, ico 5-7, 50 steps, remap every 10, "
+                         "FP32 synthesis), one HIP stream per member, against the same members on one stream")
     ap.add_argument("--small", action="store_true", help="quick check: L=64 ico-4 Nt=20 Lf=256")
     ap.add_argument("--save-maps", default=None, help="write the submap jets (torch.save) after the run")
     ap.add_argument("--compose-only", default=None,
@@ -69,6 +72,8 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the composition and the "
               "CMM step shard by points / vertices with no collectives (see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.ensemble:
+        return ensemble_bench(a)
     if a.small:
         a.L, a.ico, a.Nt, a.n_maps, a.Lf, a.Lw = 64, 4, 20, 5, 256, 128
     import numpy as np
@@ -190,6 +195,37 @@ def main():
         "mixed": mixed,
         "area_distortion": area,
     }
+    print(json.dumps(rec), flush=True)
+
+
+def ensemble_bench(a):
+    import gc
+    import torch
+    from cmm_s2.core.ensemble import Ensemble, random_members
+    torch.cuda.set_device(0)
+    res = {}
+    for mode in ("streams", "one_stream"):
+        t0 = time.perf_counter()
+        builders, params = random_members(a.ensemble)
+        ens = Ensemble(builders, streams=(mode == "streams"))
+        setup = time.perf_counter() - t0
+        n, sec = ens.timed_run()
+        res[mode] = (n, sec, setup)
+        log(f"ensemble {mode}: {len(params)} members, {n} member steps in {sec:.3f} s "
+            f"({n / sec:.1f} member-steps/s), set-up {setup:.1f} s")
+        del ens, builders
+        gc.collect()
+        torch.cuda.empty_cache()
+    n, sec, _ = res["streams"]
+    n1, sec1, _ = res["one_stream"]
+    rec = {"metric": "ensemble member-steps/s", "value": round(n / sec, 2), "unit": "member-steps/s",
+           "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "f64 maps, fp32 synthesis",
+           "config": {"workload": f"{a.ensemble} members, L uniform in {{128,192,256,384,512}}, icosahedral "
+                                  f"level uniform in {{5,6,7}} (seed 4), l^-3 initial vorticity l <= L/8, "
+                                  f"50 RK4 steps dt=1/L, remap every 10", "members": a.ensemble,
+                      "member_params": params},
+           "one_stream_value": round(n1 / sec1, 2), "stream_speedup": round((n / sec) / (n1 / sec1), 4),
+           "member_steps": n, "seconds": round(sec, 4)}
     print(json.dumps(rec), flush=True)
 
 

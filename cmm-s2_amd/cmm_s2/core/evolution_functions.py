@@ -101,10 +101,17 @@ class _Work:
 
 
 def _work(mesh):
-    w = getattr(mesh, "_cmm_work", None)
+    """Map-update buffers of ``mesh`` for the current stream (simulations on
+    different streams share the mesh tables, never these buffers)."""
+    ws = getattr(mesh, "_cmm_work", None)
+    if ws is None:
+        ws = {}
+        mesh._cmm_work = ws
+    key = B.stream()
+    w = ws.get(key)
     if w is None:
         w = _Work(mesh)
-        mesh._cmm_work = w
+        ws[key] = w
     return w
 
 

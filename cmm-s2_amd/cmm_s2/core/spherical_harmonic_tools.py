@@ -84,12 +84,15 @@ class SHT(object):
         return x, w
 
     def work(self, nf):
-        t = self._work.get(nf)
+        """Device workspace for nf fields on the current stream (transforms
+        on different streams share the plan tables, never the workspace)."""
+        key = (nf, B.stream())
+        t = self._work.get(key)
         if t is None:
             b = np.zeros(1, dtype=np.int64)
             N.call("cmm_sht_workspace_bytes", self._h, int(nf), N.lptr(b))
             t = B.empty(int(b[0]) // 8 + 1)
-            self._work[nf] = t
+            self._work[key] = t
         return t
 
     def inverse(self, flm, nf=None, out=None):
