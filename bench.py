@@ -28,10 +28,10 @@ for _p in (ROOT, os.path.join(ROOT, "cmm-s2_amd")):
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_tri_compose for the default workload (ico-7 map mesh,
-# 992 cells per cube-face edge, depth 5, L=2048 grid) from two rocprofv3 counter
-# passes, FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a 256 MiB device copy):
-# profiles/compose_L2048_depth5_ico7_pmc.txt
-MEASURED_TRAFFIC = {(7, 2048, 5, 992): 2.616e9}
+# 992 cells per cube-face edge, depth 5, L=2048 grid, tile rows) from two
+# rocprofv3 counter passes, FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a
+# 256 MiB device copy): profiles/compose_L2048_depth5_ico7_pmc.txt
+MEASURED_TRAFFIC = {(7, 2048, 5, 992, 8): 2.584e9, (7, 2048, 5, 992, 4): 2.394e9}
 
 
 def log(*a):
@@ -132,15 +132,20 @@ def main():
     n = q.shape[0]
     out = torch.empty_like(q)
     levels = sim.submaps.levels(depth)
+    gshape = (a.Lf + 1, 2 * a.Lf)  # ring-major MWSS grid: tiled traversal
     from cmm_s2.core.spherical_spline import compose_packed
+    from cmm_s2.core.spherical_spline import tune_grid_tiles
+    tiles = tune_grid_tiles(mesh, levels, q, gshape, out)
+    tile_rows = min(tiles, key=tiles.get)
+    log("tile rows -> ms per composition: " + ", ".join(f"{r}: {t * 1e3:.3f}" for r, t in tiles.items()))
     for _ in range(a.warmup):
-        compose_packed(mesh, levels, q, "N3", out)
+        compose_packed(mesh, levels, q, "N3", out, grid=gshape)
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
     t0 = time.perf_counter()
     for e0, e1 in ev:
         e0.record()
-        compose_packed(mesh, levels, q, "N3", out)
+        compose_packed(mesh, levels, q, "N3", out, grid=gshape)
         e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -168,7 +173,7 @@ def main():
     if not a.compose_only and not a.no_mixed:
         mixed = mixed_run(a, mesh, w0, wf, grid, s_step)
     cells = int(mesh.stats()["cells_per_face"])
-    traffic = MEASURED_TRAFFIC.get((a.ico, a.Lf, depth, cells))
+    traffic = MEASURED_TRAFFIC.get((a.ico, a.Lf, depth, cells, tile_rows))
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(mesh, sim, q, depth, a.cpu_sample)
@@ -180,7 +185,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"barotropic CMM L={a.L} ico-{a.ico}, {a.Nt} RK4 steps dt=1/{a.L}, remap "
                                f"every {a.n_maps}; composed map (depth {depth}) on the L={a.Lf} MWSS grid",
-                   "points": n, "depth": depth, "L": a.L, "ico": a.ico, "Lf": a.Lf, "Lw": a.Lw},
+                   "points": n, "depth": depth, "L": a.L, "ico": a.ico, "Lf": a.Lf, "Lw": a.Lw,
+                   "tile_rows": tile_rows},
         "s_per_step": None if s_step is None else round(s_step, 6),
         "remap_step_s": [round(x, 6) for x in remap_t],
         "bootstrap_s": round(t_boot, 4), "vorticity_grid_ms": round(t_vort * 1e3, 3),

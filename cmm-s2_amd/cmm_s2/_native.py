@@ -44,6 +44,8 @@ SIGNATURES = {
     "cmm_tri_eval": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp]),
     "cmm_tri_compose": (C.c_int, [vp, P_vp, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp, vp]),
     "cmm_tri_eval_grad": (C.c_int, [vp, vp, C.c_int, vp, i64, i64, i64, vp, vp]),
+    "cmm_tri_compose_grid": (C.c_int, [vp, P_vp, C.c_int, vp, i64, i64, C.c_int, i64, i64, vp, i64, i64, vp,
+                                       vp]),
     "cmm_tri_compose_grad": (C.c_int, [vp, P_vp, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp, vp]),
     "cmm_mwss_eval_grad": (C.c_int, [vp, vp, vp, i64, i64, i64, vp, vp]),
     "cmm_tri_stencil_jets": (C.c_int, [vp, vp, vp, i64, vp, i64, vp, dbl, vp, vp, vp]),

@@ -119,6 +119,16 @@ class BarotropicCMM(object):
         U.stepping = True
         self.identity = False
         self.count = 2
+        self.tune_tiles()
+
+    def tune_tiles(self):
+        """Pick the tile shape of the per-step sample-grid composition for
+        every depth the run will use (1 + remaps), timing the candidates on
+        the current map; results do not depend on the choice."""
+        from .spherical_spline import tune_grid_tiles
+        grid = (self.L + 1, 2 * self.L)
+        for depth in range(1, len(self.ns) + 2):
+            tune_grid_tiles(self.mesh, [self.curr.packed] * depth, self.samples, grid, out=self.X, reps=2)
 
     def step(self):
         """One loop iteration (spherical_simulations.py:263-294).  Returns True
@@ -126,7 +136,7 @@ class BarotropicCMM(object):
         t = self.tspan[self.count]
         self.curr = evol.advect_project_sphere(self.curr, evol.RK4_proj, t, self.dt, self.U,
                                                identity=self.identity)
-        self.submaps.compose(self.samples, "N3", out=self.X, pre=[self.curr.packed])
+        self.submaps.compose(self.samples, "N3", out=self.X, pre=[self.curr.packed], grid=(self.L + 1, 2 * self.L))
         self._velocity_from(self.X, 2)
         self.U.ts = self.U.ts + self.dt
         self.count += 1
@@ -179,7 +189,8 @@ class BarotropicCMM(object):
         figures/data_gen_experiments.py:122-144)."""
         g = grid if grid is not None else structure_spherical_triangulation(L_out)
         q = g.sample_points()
-        X = self.submaps.compose(q, "N3", pre=[] if self.identity else [self.curr.packed])
+        X = self.submaps.compose(q, "N3", pre=[] if self.identity else [self.curr.packed],
+                                 grid=(L_out + 1, 2 * L_out))
         return self.sampler(X, out=out, cos_grid=g.d_cos_theta, ring_len=2 * L_out).reshape(L_out + 1, 2 * L_out)
 
 

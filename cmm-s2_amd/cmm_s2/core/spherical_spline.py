@@ -192,11 +192,11 @@ class composite_sphere_diffeomorphism(object):
             N_c = self.Nc
         return [self.packed[i] for i in reversed(range(N_c))]
 
-    def compose(self, q, layout="N3", out=None, pre=(), N_c=None, inter=None):
+    def compose(self, q, layout="N3", out=None, pre=(), N_c=None, inter=None, grid=None):
         """Fused device composition: apply maps ``pre`` (packed buffers), then
         submaps N_c-1 .. 0."""
         lv = list(pre) + self.levels(N_c)
-        return compose_packed(self.mesh, lv, q, layout, out, inter)
+        return compose_packed(self.mesh, lv, q, layout, out, inter, grid)
 
     def __call__(self, q_pts, N_c=None):
         """spherical_spline.py:365-398: q (N,3) -> (N,3)."""
@@ -229,15 +229,54 @@ class composite_sphere_diffeomorphism(object):
         return [B.like(o, q_pts) for o in outs]
 
 
-def compose_packed(mesh, levels, q, layout="N3", out=None, inter=None):
+# tile rows chosen by tune_grid_tiles per (mesh, grid shape, depth)
+_tiles = {}
+
+
+def tune_grid_tiles(mesh, levels, q, grid, out=None, reps=3, candidates=(1, 2, 4, 8, 16)):
+    """Time the grid composition with each tile shape (rows x 64/rows) on
+    these inputs and keep the fastest for (mesh, grid shape, depth): a
+    measured choice of the launch shape, results are identical for every
+    candidate.  Returns {rows: seconds}."""
+    import time
+    n = q.shape[0]
+    if out is None:
+        out = B.empty(n, 3)
+    key = (mesh._h, int(grid[0]), int(grid[1]), len(levels))
+    times = {}
+    for rows in candidates:
+        _tiles[key] = rows
+        compose_packed(mesh, levels, q, "N3", out, grid=grid)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            compose_packed(mesh, levels, q, "N3", out, grid=grid)
+        torch.cuda.synchronize()
+        times[rows] = (time.perf_counter() - t0) / reps
+    _tiles[key] = min(times, key=times.get)
+    return times
+
+
+def compose_packed(mesh, levels, q, layout="N3", out=None, inter=None, grid=None):
+    """Fused composition through packed levels (application order).  ``grid``
+    = (nring, ncol) declares q a ring-major grid: same result, tiled
+    traversal."""
     n = q.shape[0] if layout == "N3" else q.shape[1]
     if out is None:
         out = B.empty(n, 3) if layout == "N3" else B.empty(3, n)
     sp, sc = B.point_strides(q, layout)
     osp, osc = B.point_strides(out, layout)
     arr = N.ptr_array([p.data_ptr() for p in levels])
-    N.call("cmm_tri_compose", mesh._h, arr, len(levels), q.data_ptr(), n, sp, sc, out.data_ptr(), osp,
-           osc, B.ptr(inter), B.stream())
+    if grid is not None:
+        nring, ncol = int(grid[0]), int(grid[1])
+        if nring * ncol != n:
+            raise N.CmmError(f"grid {grid} does not hold {n} points")
+        rows = _tiles.get((mesh._h, nring, ncol, len(levels)), 4)
+        N.call("cmm_tri_compose_grid", mesh._h, arr, len(levels), q.data_ptr(), nring, ncol, rows, sp, sc,
+               out.data_ptr(), osp, osc, B.ptr(inter), B.stream())
+    else:
+        N.call("cmm_tri_compose", mesh._h, arr, len(levels), q.data_ptr(), n, sp, sc, out.data_ptr(), osp,
+               osc, B.ptr(inter), B.stream())
     return out
 
 

@@ -453,10 +453,27 @@ __global__ __launch_bounds__(256) void k_tri_eval(cmm_tri_mesh M, const double* 
   for (int c = 0; c < ncomp; ++c) out[i * osp + c * osc] = r[c];
 }
 
+// Ring-major grid traversal: with ncol > 0 the points form nring x ncol
+// (row-major) and thread t walks them in tiles of tr x (64 / tr) (one wave
+// per tile), so a wave covers a compact patch instead of a long strip of one
+// ring and touches fewer triangles per level.
+__device__ inline int64_t grid_point(int64_t t, int64_t nring, int64_t ncol, int tr) {
+  const int tc = 64 / tr;
+  const int64_t ntc = (ncol + tc - 1) / tc;
+  const int64_t tile = t >> 6;
+  const int w = (int)(t & 63);
+  const int64_t r = (tile / ntc) * tr + w / tc, c = (tile % ntc) * tc + w % tc;
+  return (r < nring && c < ncol) ? r * ncol + c : -1;
+}
+
 __global__ __launch_bounds__(256) void k_tri_compose(cmm_tri_mesh M, LevelPtrs lv, int depth, const double* __restrict__ q,
                               int64_t n, int64_t sp, int64_t sc, double* __restrict__ out, int64_t osp,
-                              int64_t osc, double* __restrict__ inter) {
+                              int64_t osc, double* __restrict__ inter, int64_t nring, int64_t ncol, int tr) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ncol > 0) {
+    i = grid_point(i, nring, ncol, tr);
+    if (i < 0) return;
+  }
   if (i >= n) return;
   d3 x = load_pt(q, i, sp, sc);
   for (int l = 0; l < depth; ++l) {
@@ -686,8 +703,32 @@ extern "C" int cmm_tri_compose(const cmm_tri_mesh* m, const double* const* level
   }
   if (n == 0) return CMM_OK;
   hipLaunchKernelGGL(k_tri_compose, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, lv,
-                     depth, q, n, sp, sc, out, osp, osc, inter);
+                     depth, q, n, sp, sc, out, osp, osc, inter, (int64_t)0, (int64_t)0, 1);
   CMM_LAUNCH_CHECK("cmm_tri_compose");
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_compose_grid(const cmm_tri_mesh* m, const double* const* levels, int depth,
+                                    const double* q, int64_t nring, int64_t ncol, int tile_rows,
+                                    int64_t q_stride_pt, int64_t q_stride_comp, double* out, int64_t osp,
+                                    int64_t osc, double* inter, void* stream) {
+  if (tile_rows == 0) tile_rows = 4;
+  if (!m || depth < 0 || depth > kMaxDepth || (depth > 0 && !levels) || nring < 0 || ncol < 1 ||
+      tile_rows < 1 || tile_rows > 64 || (tile_rows & (tile_rows - 1)))
+    return fail(CMM_E_ARG, "cmm_tri_compose_grid: bad arguments");
+  const int64_t n = nring * ncol;
+  if (n > 0 && (!q || !out)) return fail(CMM_E_ARG, "cmm_tri_compose_grid: bad arguments");
+  LevelPtrs lv;
+  for (int l = 0; l < depth; ++l) {
+    if (!levels[l]) return fail(CMM_E_ARG, "cmm_tri_compose_grid: null level");
+    lv.p[l] = levels[l];
+  }
+  if (n == 0) return CMM_OK;
+  const int tc = 64 / tile_rows;
+  const int64_t threads = (nring + tile_rows - 1) / tile_rows * ((ncol + tc - 1) / tc) * 64;
+  hipLaunchKernelGGL(k_tri_compose, dim3(grid_for(threads, 256)), dim3(256), 0, as_stream(stream), *m, lv, depth, q,
+                     n, q_stride_pt, q_stride_comp, out, osp, osc, inter, nring, ncol, tile_rows);
+  CMM_LAUNCH_CHECK("cmm_tri_compose_grid");
   return CMM_OK;
 }
 

@@ -89,6 +89,16 @@ int cmm_tri_compose(const cmm_tri_mesh* mesh, const double* const* levels, int d
                     double* out, int64_t out_stride_pt, int64_t out_stride_comp,
                     double* inter, void* stream);
 
+/* cmm_tri_compose for points forming a ring-major nring x ncol grid (point
+ * (r, c) at index r ncol + c, e.g. the MWSS sample grid): identical results;
+ * the threads walk the grid in tiles of tile_rows x (64 / tile_rows) points
+ * (tile_rows a power of two <= 64, 0 = 4) so a wave covers a compact patch.
+ * No counterpart in the reference (a traversal order, not an operation). */
+int cmm_tri_compose_grid(const cmm_tri_mesh* mesh, const double* const* levels, int depth,
+                         const double* q, int64_t nring, int64_t ncol, int tile_rows, int64_t q_stride_pt,
+                         int64_t q_stride_comp, double* out, int64_t out_stride_pt,
+                         int64_t out_stride_comp, double* inter, void* stream);
+
 /* Differential of an evaluated spline.  ncomp 3: Jacobian of the normalised
  * map x -> p(x)/|p(x)| restricted to the tangent plane at x, out (3,3,n)
  * row-major [c][d] at out[(3c + d) n + i] (sphere_diffeomorphism.eval_grad,

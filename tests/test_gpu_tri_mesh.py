@@ -147,3 +147,22 @@ def test_locate_large_mesh_against_kdtree():
     clear = om.orientation_margin(pm.vertices, pm.simplices, t_o, q) > T_TIE
     assert (t[clear] == t_o[clear]).all()
     assert (om.orientation_margin(pm.vertices, pm.simplices, t, q) > -T_TIE).all()
+
+
+def test_tiled_grid_composition_is_identical():
+    """cmm_tri_compose_grid walks a ring-major grid in tiles: same bits as the
+    point-order composition, including ragged edges (ring count and length
+    not multiples of the tile)."""
+    from cmm_s2.core import mesh_functions as mf
+    from cmm_s2.core.spherical_spline import sphere_diffeomorphism, compose_packed
+    from helpers import perturbed_identity_jets
+    ico = mf.icosahedral_mesh(3)
+    pm = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
+    maps = [sphere_diffeomorphism(pm, torch.from_numpy(perturbed_identity_jets(pm.vertices, 40 + i)).cuda())
+            for i in range(3)]
+    for L in (24, 37):
+        grid = mf.structure_spherical_triangulation(L)
+        q = grid.sample_points()
+        a = compose_packed(pm, [m.packed for m in maps], q, "N3")
+        b = compose_packed(pm, [m.packed for m in maps], q, "N3", grid=(L + 1, 2 * L))
+        assert torch.equal(a, b)
