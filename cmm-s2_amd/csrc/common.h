@@ -290,6 +290,56 @@ __host__ __device__ inline void ps_coeffs(const PSGeom& P, const double f[3], co
   c[18] = P.G[9][0] * c[4] + P.G[9][1] * c[7] + P.G[9][2] * c[10];
 }
 
+// Transpose of ps_coeffs (a linear map of f, gr for fixed geometry): the
+// adjoints cb[19] of the coefficients -> adjoints added into fb[3], gb[3].
+__host__ __device__ inline void ps_coeffs_adj(const PSGeom& P, const double cb_in[19], double fb[3], d3 gb[3]) {
+  double c[19];
+  for (int k = 0; k < 19; ++k) c[k] = cb_in[k];
+  c[3] += P.E[0][0] * c[12];
+  c[8] += P.E[0][1] * c[12];
+  c[6] += P.E[1][1] * c[13];
+  c[11] += P.E[1][2] * c[13];
+  c[9] += P.E[2][2] * c[14];
+  c[5] += P.E[2][0] * c[14];
+  c[4] += P.E[0][0] * c[15] + P.E[2][0] * c[17] + P.G[9][0] * c[18];
+  c[7] += P.E[0][1] * c[15] + P.E[1][1] * c[16] + P.G[9][1] * c[18];
+  c[10] += P.E[1][2] * c[16] + P.E[2][2] * c[17] + P.G[9][2] * c[18];
+  for (int i = 0; i < 3; ++i) fb[i] += c[i];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int r = ps_div_row(k), i = k / 3;
+    const double t = P.Gi[k] * c[3 + k];
+    gb[i] = add(gb[i], scl(t / 2, P.H[r]));
+    fb[i] -= t * P.G[r][0];
+  }
+}
+
+// the 6 quadratic Bernstein-Bezier basis values of bb6
+__host__ __device__ inline void bb6_basis(d3 c, double b[6]) {
+  b[0] = c.x * c.x;
+  b[1] = c.y * c.y;
+  b[2] = c.z * c.z;
+  b[3] = 2 * (c.y * c.x);
+  b[4] = 2 * (c.z * c.y);
+  b[5] = 2 * (c.z * c.x);
+}
+
+// transpose of the (linear) sub-triangle map b -> sub_bary(s, b, ...)
+__host__ __device__ inline d3 sub_bary_T(int s, d3 cbar, double s12, double s23, double s31, double s4) {
+  const d3 c0 = sub_bary(s, 1.0, 0.0, 0.0, s12, s23, s31, s4);
+  const d3 c1 = sub_bary(s, 0.0, 1.0, 0.0, s12, s23, s31, s4);
+  const d3 c2 = sub_bary(s, 0.0, 0.0, 1.0, s12, s23, s31, s4);
+  return d3{dot(cbar, c0), dot(cbar, c1), dot(cbar, c2)};
+}
+
+// adjoint of x -> x / |x| at x (raw vector r): (I - u u^T) ybar / |r|
+__host__ __device__ inline d3 unit_adj(d3 r, d3 ybar) {
+  const double n = sqrt(r.x * r.x + r.y * r.y + r.z * r.z), in = 1 / n;
+  const d3 u = d3{r.x * in, r.y * in, r.z * in};
+  const double p = dot(u, ybar);
+  return d3{(ybar.x - p * u.x) * in, (ybar.y - p * u.y) * in, (ybar.z - p * u.z) * in};
+}
+
 // numpy's floor division of doubles (npy_divmod): NOT floor(a / b).
 __host__ __device__ inline double npy_floordiv(double a, double b) {
   double mod = fmod(a, b);

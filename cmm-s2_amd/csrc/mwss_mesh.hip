@@ -611,6 +611,327 @@ __global__ __launch_bounds__(256) void k_sample_vorticity(VortArgs A, cmm_mwss_m
 }
 
 // ---------------------------------------------------------------------------
+// adjoints (reverse mode).  Kernels marked "accumulate" add into their
+// outputs with atomics (the caller clears them); the others overwrite.
+// ---------------------------------------------------------------------------
+// transpose of rot_grad in g: r = P (g x s), P = I - s s^T / |s| symmetric
+__device__ inline d3 rot_grad_T(d3 rb, d3 s) {
+  const double c = dot(rb, s) / sqrt(dot(s, s));
+  const d3 ab = d3{rb.x - c * s.x, rb.y - c * s.y, rb.z - c * s.z};
+  return cross(s, ab);
+}
+
+// mesh vertex sampled by grid point (t, p), -1 for the pole samples p > 0
+__device__ inline int64_t grid_vertex(const cmm_mwss_mesh& M, int t, int p) {
+  if (t == 0) return p == 0 ? 0 : -1;
+  if (t == M.L) return p == 0 ? M.Nu - 1 : -1;
+  return 1 + (int64_t)(t - 1) * M.N + p;
+}
+
+// transpose of k_velocity_jets (NF = 12) / k_scalar_jets (NF = 4): every grid
+// point receives from at most one vertex, so one thread per grid point
+// overwrites all NF fields
+template <int NF>
+__global__ __launch_bounds__(256) void k_grid_jets_adj(cmm_mwss_mesh M, const double* __restrict__ vb,
+                                                       double* __restrict__ Fb) {
+  const int64_t G = (int64_t)(M.L + 1) * M.N;
+  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= G) return;
+  const int t = (int)(gi / M.N), p = (int)(gi % M.N);
+  const int64_t v = grid_vertex(M, t, p);
+  const int64_t Nu = M.Nu;
+  if (v < 0) {
+    for (int k = 0; k < NF; ++k) Fb[k * G + gi] = 0.0;
+    return;
+  }
+  const d3 s = ring_pt(M, t, p);
+  if (NF == 12) {
+    for (int i = 0; i < 3; ++i) {
+      Fb[i * G + gi] = vb[(i * 4) * Nu + v];
+      const d3 rb = d3{vb[(0 * 4 + 1 + i) * Nu + v], vb[(1 * 4 + 1 + i) * Nu + v], vb[(2 * 4 + 1 + i) * Nu + v]};
+      const d3 g = rot_grad_T(rb, s);
+      Fb[(3 + 3 * i) * G + gi] = g.x;
+      Fb[(4 + 3 * i) * G + gi] = g.y;
+      Fb[(5 + 3 * i) * G + gi] = g.z;
+    }
+  } else {
+    Fb[gi] = vb[v];
+    const d3 g = rot_grad_T(d3{vb[Nu + v], vb[2 * Nu + v], vb[3 * Nu + v]}, s);
+    Fb[G + gi] = g.x;
+    Fb[2 * G + gi] = g.y;
+    Fb[3 * G + gi] = g.z;
+  }
+}
+
+// transpose of k_mwss_assemble (accumulate into the jets vb): packed
+// adjoints -> 19 coefficient adjoints per component (sum over the
+// sub-triangles that share a coefficient) -> ps_coeffs^T -> the 3 vertices
+template <int NC>
+__global__ __launch_bounds__(256) void k_mwss_assemble_adj(cmm_mwss_mesh M, const double* __restrict__ pb,
+                                                           double* __restrict__ vb) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M.Tu) return;
+  const int N = M.N, Mr = M.M;
+  int row, col, half;
+  if (t < N) { row = 0; col = (int)t; half = 0; }
+  else if (t >= M.Tu - N) { row = Mr - 2; col = (int)(t - (M.Tu - N)); half = 0; }
+  else {
+    const int64_t u = t - N;
+    half = (int)(u % 2);
+    const int64_t k = u / 2;
+    row = 1 + (int)(k / N);
+    col = (int)(k % N);
+  }
+  int64_t vi[3];
+  d3 v[3];
+  mwss_tri_vertices(M, t, vi, v);
+  PSGeom P;
+  ps_unpack(M.rowgeom + ((int64_t)row * 2 + half) * kPSGeomLen, 1, P);
+  const double c = M.cp[col], sn = M.sp[col];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const d3 h = P.H[i];
+    P.H[i] = d3{c * h.x - sn * h.y, sn * h.x + c * h.y, h.z};
+  }
+  const int64_t Nu = M.Nu;
+  const double* src = pb + t * (36 * NC);
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc) {
+    double cb[19];
+#pragma unroll
+    for (int k = 0; k < 19; ++k) cb[k] = 0.0;
+#pragma unroll
+    for (int ss = 0; ss < 6; ++ss)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += src[(ss * NC + cc) * 6 + j];
+    double fb[3] = {0.0, 0.0, 0.0};
+    d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+    ps_coeffs_adj(P, cb, fb, gb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (NC == 1) {
+        unsafeAtomicAdd(vb + vi[i], fb[i]);
+        unsafeAtomicAdd(vb + Nu + vi[i], gb[i].x);
+        unsafeAtomicAdd(vb + 2 * Nu + vi[i], gb[i].y);
+        unsafeAtomicAdd(vb + 3 * Nu + vi[i], gb[i].z);
+      } else {
+        unsafeAtomicAdd(vb + (cc * 4) * Nu + vi[i], fb[i]);
+        unsafeAtomicAdd(vb + (0 * 4 + 1 + cc) * Nu + vi[i], gb[i].x);
+        unsafeAtomicAdd(vb + (1 * 4 + 1 + cc) * Nu + vi[i], gb[i].y);
+        unsafeAtomicAdd(vb + (2 * 4 + 1 + cc) * Nu + vi[i], gb[i].z);
+      }
+    }
+  }
+}
+
+// located velocity-spline point: triangle, sub-triangle, sub-triangle
+// barycentrics and the pieces of their position gradient
+struct VLoc {
+  bool ok;
+  int64_t tri;
+  int s;
+  d3 cb;
+  d3 w[3];  // macro dual basis: b_i = x . w_i
+  double se, s4;
+};
+
+__device__ __forceinline__ VLoc vel_locate(const cmm_mwss_mesh& M, d3 x) {
+  VLoc r;
+  MLoc o;
+  r.ok = mwss_locate(M, x, o);
+  if (!r.ok) {
+    r.tri = -1;
+    r.s = 0;
+    return r;
+  }
+  r.tri = o.tri;
+  r.s = split_index(o.b.x, o.b.y, o.b.z);
+  const d3 s3 = add(add(o.v[0], o.v[1]), o.v[2]);
+  r.s4 = norm3(s3);
+  if (r.s <= 1) r.se = norm3(add(o.v[1], o.v[2]));
+  else if (r.s <= 3) r.se = norm3(add(o.v[2], o.v[0]));
+  else r.se = norm3(add(o.v[0], o.v[1]));
+  r.cb = sub_bary(r.s, o.b.x, o.b.y, o.b.z, r.se, r.se, r.se, r.s4);
+  const double D = det3(o.v[0], o.v[1], o.v[2]);
+  r.w[0] = scl(1 / D, cross(o.v[1], o.v[2]));
+  r.w[1] = scl(1 / D, cross(o.v[2], o.v[0]));
+  r.w[2] = scl(1 / D, cross(o.v[0], o.v[1]));
+  return r;
+}
+
+// Adjoint of one velocity-spline evaluation v = sum_k w_k PS_k(x) of stage
+// `stage`: adds w_k vbar_c basis_j into the level adjoints lb[k] (atomics,
+// summed over the 4 lanes of a quad first when the whole quad sits in one
+// sub-triangle: the four stencil feet of a vertex are 1e-5 apart) and
+// returns xbar = (dv/dx)^T vbar.  Every lane of the quad must call it.
+__device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, double* const* lb) {
+  const VLoc r = vel_locate(V.mesh, x);
+  const LevelSet& ls = V.ls[stage];
+  const int lane = threadIdx.x & 63;
+  const int64_t key = r.ok ? r.tri * 6 + r.s : -1 - (int64_t)lane;
+  const int64_t k1 = __shfl_xor(key, 1), k2 = __shfl_xor(key, 2);
+  const bool e1 = key == k1, e2 = key == k2;
+  // every lane must execute the shuffle (no short-circuit: a lane that skips
+  // it leaves its partner reading an inactive lane)
+  const int e1p = __shfl_xor((int)e1, 2);
+  const bool same = e1 && e2 && e1p;
+  double bas[6];
+  bb6_basis(r.cb, bas);
+  double cv[18];
+  const double vv[3] = {vb.x, vb.y, vb.z};
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) cv[6 * c + j] = r.ok ? vv[c] * bas[j] : 0.0;
+  // quad sums in convergent code (every lane shuffles), then select
+#pragma unroll
+  for (int e = 0; e < 18; ++e) {
+    double t = cv[e] + __shfl_xor(cv[e], 1);
+    t += __shfl_xor(t, 2);
+    cv[e] = same ? t : cv[e];
+  }
+  const bool issue = r.ok && (!same || (lane & 3) == 0);
+  d3 cbar = d3{0, 0, 0};
+  if (r.ok) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (k < ls.nlev) {
+        const double w = ls.w[k];
+        const double* p = ls.p[k] + (r.tri * 6 + r.s) * 18;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cbar = add(cbar, scl(w * vv[c], bb6_db(r.cb, p + 6 * c)));
+        if (issue && w != 0.0) {
+          double* dst = lb[k] + (r.tri * 6 + r.s) * 18;
+#pragma unroll
+          for (int e = 0; e < 18; ++e) unsafeAtomicAdd(dst + e, w * cv[e]);
+        }
+      }
+    }
+  }
+  if (!r.ok) return d3{0, 0, 0};
+  const d3 bbar = sub_bary_T(r.s, cbar, r.se, r.se, r.se, r.s4);
+  return add(add(scl(bbar.x, r.w[0]), scl(bbar.y, r.w[1])), scl(bbar.z, r.w[2]));
+}
+
+struct LevelBars {
+  double* p[3];
+};
+
+// Adjoint of advect_point for the stencil feet (integrators Euler, improved
+// Euler, RK4): recompute the stages, then reverse them.  Point i = (vertex
+// i / 4, stencil k = i % 4) as k_advect_stencil<4>, so a quad of lanes holds
+// the four feet of one vertex.  ybar (3, 4 nv) with row stride yb_row; the
+// level adjoints accumulate.
+__global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ, double dt,
+                                                            const double* __restrict__ xs, int64_t xs_row, int64_t nv,
+                                                            const double* __restrict__ yb, int64_t yb_row,
+                                                            LevelBars LB, double* __restrict__ xb, int64_t xb_row) {
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // the last quad may reach past the end: clamp (its lanes then repeat a
+  // valid point and add nothing)
+  const bool live = i0 < 4 * nv;
+  const int64_t i = live ? i0 : 4 * nv - 1;
+  const int64_t v = i / 4;
+  const int k = (int)(i - v * 4);
+  const double* src = xs + (int64_t)k * nv + v;
+  const d3 xn = d3{src[0], src[xs_row], src[2 * xs_row]};
+  const double* bsrc = yb + (int64_t)k * nv + v;
+  d3 ybar = live ? d3{bsrc[0], bsrc[yb_row], bsrc[2 * yb_row]} : d3{0, 0, 0};
+  double* const* lb = LB.p;
+  d3 xbar;
+  if (integ == CMM_INT_EULER) {
+    const d3 v1 = vel_eval(V, 0, xn);
+    const d3 z = sub(xn, scl(dt, v1));
+    const d3 zb = unit_adj(z, ybar);
+    xbar = add(zb, vel_adj(V, 0, xn, scl(-dt, zb), lb));
+  } else if (integ == CMM_INT_IMPROVED_EULER) {
+    const d3 v1 = vel_eval(V, 0, xn);
+    const d3 z1 = sub(xn, scl(dt, v1));
+    const d3 s1 = proj_norm(z1);
+    const d3 v2 = vel_eval(V, 1, s1);
+    const d3 z = sub(xn, scl(dt / 2, add(v1, v2)));
+    const d3 zb = unit_adj(z, ybar);
+    const d3 v2b = scl(-dt / 2, zb);
+    d3 v1b = v2b;
+    const d3 s1b = vel_adj(V, 1, s1, v2b, lb);
+    const d3 z1b = unit_adj(z1, s1b);
+    v1b = add(v1b, scl(-dt, z1b));
+    xbar = add(add(zb, z1b), vel_adj(V, 0, xn, v1b, lb));
+  } else {
+  const d3 v1 = vel_eval(V, 0, xn);
+  const d3 z1 = d3{xn.x - dt * v1.x / 2, xn.y - dt * v1.y / 2, xn.z - dt * v1.z / 2};
+  const d3 s1 = proj_norm(z1);
+  const d3 v2 = vel_eval(V, 1, s1);
+  const d3 z2 = sub(xn, scl(dt / 2, v2));
+  const d3 s2 = proj_norm(z2);
+  const d3 v3 = vel_eval(V, 2, s2);
+  const d3 z3 = sub(xn, scl(dt, v3));
+  const d3 s3 = proj_norm(z3);
+  const d3 v4 = vel_eval(V, 3, s3);
+  const d3 acc = d3{v1.x + 2 * v2.x + 2 * v3.x + v4.x, v1.y + 2 * v2.y + 2 * v3.y + v4.y,
+                    v1.z + 2 * v2.z + 2 * v3.z + v4.z};
+  const d3 z4 = sub(xn, scl(dt / 6, acc));
+  const d3 ab = scl(-dt / 6, unit_adj(z4, ybar));
+  d3 v1b = ab, v2b = scl(2, ab), v3b = scl(2, ab);
+  const d3 v4b = ab;
+  const d3 s3b = vel_adj(V, 3, s3, v4b, lb);
+  v3b = add(v3b, scl(-dt, unit_adj(z3, s3b)));
+  const d3 s2b = vel_adj(V, 2, s2, v3b, lb);
+  v2b = add(v2b, scl(-dt / 2, unit_adj(z2, s2b)));
+  const d3 s1b = vel_adj(V, 1, s1, v2b, lb);
+  const d3 z1b = unit_adj(z1, s1b);
+  v1b = add(v1b, scl(-dt / 2, z1b));
+  // z4 = xn - ..., z3 = xn - ..., z2 = xn - ..., z1 = xn - ... all carry xn
+  const d3 z4b = unit_adj(z4, ybar), z3b = unit_adj(z3, s3b), z2b = unit_adj(z2, s2b);
+  xbar = add(add(add(z4b, z3b), add(z2b, z1b)), vel_adj(V, 0, xn, v1b, lb));
+  }
+  if (xb && live) {
+    double* dst = xb + (int64_t)k * nv + v;
+    dst[0] = xbar.x;
+    dst[xb_row] = xbar.y;
+    dst[2 * xb_row] = xbar.z;
+  }
+}
+
+// Adjoint of k_sample_vorticity for a spline w0: xbar (overwrite, may be
+// null) and the adjoint of the w0 coefficients (accumulate, may be null).
+// q = x / |x| and cos theta(x) = z / |x| as in the forward sampling.
+__global__ __launch_bounds__(256) void k_sample_vorticity_adj(cmm_mwss_mesh W, const double* __restrict__ wpacked,
+                                                              const double* __restrict__ X, int64_t n, int64_t sp,
+                                                              int64_t sc, const double* __restrict__ wbar, double rot,
+                                                              double* __restrict__ xbar, int64_t bsp, int64_t bsc,
+                                                              double* __restrict__ pbar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const d3 x = load_pt(X, i, sp, sc);
+  const double phi = npy_mod(atan2(x.y, x.x) + 2 * M_PI, 2 * M_PI);
+  const double theta = atan2(sqrt(x.y * x.y + x.x * x.x), x.z);
+  const d3 q = d3{sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
+  const double wb = wbar[i];
+  const VLoc r = vel_locate(W, q);
+  d3 gq = d3{0, 0, 0};
+  if (r.ok) {
+    const double* p = wpacked + (r.tri * 6 + r.s) * 6;
+    if (pbar) {
+      double bas[6];
+      bb6_basis(r.cb, bas);
+      double* dst = pbar + (r.tri * 6 + r.s) * 6;
+      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(dst + j, wb * bas[j]);
+    }
+    const d3 bb = sub_bary_T(r.s, bb6_db(r.cb, p), r.se, r.se, r.se, r.s4);
+    gq = add(add(scl(bb.x, r.w[0]), scl(bb.y, r.w[1])), scl(bb.z, r.w[2]));
+  }
+  if (!xbar) return;
+  const double nx = norm3(x), inx = 1 / nx;
+  // d(x / |x|)^T wb gq + wb 2 rot d(z / |x|)
+  d3 xb = unit_adj(x, scl(wb, gq));
+  const d3 u = scl(inx, x);
+  const double cz_ = 2 * rot * wb * inx;
+  xb = add(xb, d3{-cz_ * u.z * u.x, -cz_ * u.z * u.y, cz_ * (1 - u.z * u.z)});
+  store_pt(xbar, i, bsp, bsc, xb);
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" int cmm_mwss_locate(const cmm_mwss_mesh* m, const double* q, int64_t n, int64_t sp,
@@ -760,5 +1081,76 @@ extern "C" int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const 
   hipLaunchKernelGGL(k_sample_vorticity, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), A, W,
                      wpacked, X, n, sp, sc, cosg, out);
   CMM_LAUNCH_CHECK("cmm_sample_vorticity");
+  return CMM_OK;
+}
+
+// ---- adjoints -------------------------------------------------------------
+extern "C" int cmm_mwss_velocity_jets_adjoint(const cmm_mwss_mesh* m, const double* vals_bar, double* fields_bar,
+                                              void* stream) {
+  if (!m || !vals_bar || !fields_bar) return fail(CMM_E_ARG, "cmm_mwss_velocity_jets_adjoint: bad arguments");
+  const int64_t G = (int64_t)(m->L + 1) * m->N;
+  hipLaunchKernelGGL(k_grid_jets_adj<12>, dim3(grid_for(G, 256)), dim3(256), 0, as_stream(stream), *m, vals_bar,
+                     fields_bar);
+  CMM_LAUNCH_CHECK("cmm_mwss_velocity_jets_adjoint");
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_scalar_jets_adjoint(const cmm_mwss_mesh* m, const double* vals_bar, double* fields_bar,
+                                            void* stream) {
+  if (!m || !vals_bar || !fields_bar) return fail(CMM_E_ARG, "cmm_mwss_scalar_jets_adjoint: bad arguments");
+  const int64_t G = (int64_t)(m->L + 1) * m->N;
+  hipLaunchKernelGGL(k_grid_jets_adj<4>, dim3(grid_for(G, 256)), dim3(256), 0, as_stream(stream), *m, vals_bar,
+                     fields_bar);
+  CMM_LAUNCH_CHECK("cmm_mwss_scalar_jets_adjoint");
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_assemble_adjoint(const cmm_mwss_mesh* m, const double* packed_bar, int ncomp,
+                                         double* vals_bar, void* stream) {
+  if (!m || !packed_bar || !vals_bar || (ncomp != 1 && ncomp != 3))
+    return fail(CMM_E_ARG, "cmm_mwss_assemble_adjoint: bad arguments");
+  if (ncomp == 3)
+    hipLaunchKernelGGL(k_mwss_assemble_adj<3>, dim3(grid_for(m->Tu, 256)), dim3(256), 0, as_stream(stream), *m,
+                       packed_bar, vals_bar);
+  else
+    hipLaunchKernelGGL(k_mwss_assemble_adj<1>, dim3(grid_for(m->Tu, 256)), dim3(256), 0, as_stream(stream), *m,
+                       packed_bar, vals_bar);
+  CMM_LAUNCH_CHECK("cmm_mwss_assemble_adjoint");
+  return CMM_OK;
+}
+
+extern "C" int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integ, double dt, const double* st,
+                                          int64_t st_row, int64_t nv, const double* yst_bar, int64_t yb_row,
+                                          double* const* level_bars, double* st_bar, int64_t sb_row,
+                                          void* stream) {
+  if (!vel || !st || !yst_bar || !level_bars || nv < 1 || st_row < 4 * nv || yb_row < 4 * nv ||
+      (st_bar && sb_row < 4 * nv) ||
+      (integ != CMM_INT_EULER && integ != CMM_INT_IMPROVED_EULER && integ != CMM_INT_RK4))
+    return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint: bad arguments");
+  if (vel->kind != CMM_VEL_SPLINE) return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint: needs a spline velocity");
+  VelDev V;
+  int rc = velocity_device(vel, "cmm_advect_stencil_adjoint", V);
+  if (rc) return rc;
+  LevelBars LB;
+  for (int k = 0; k < 3; ++k) {
+    LB.p[k] = k < vel->nlev ? level_bars[k] : nullptr;
+    if (k < vel->nlev && !LB.p[k]) return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint: null level adjoint");
+  }
+  hipLaunchKernelGGL(k_advect_stencil_adj, dim3(grid_for(4 * nv, 128)), dim3(128), 0, as_stream(stream), V, integ,
+                     dt, st, st_row, nv, yst_bar, yb_row, LB, st_bar, sb_row);
+  CMM_LAUNCH_CHECK("cmm_advect_stencil_adjoint");
+  return CMM_OK;
+}
+
+extern "C" int cmm_sample_vorticity_adjoint(const cmm_mwss_mesh* wmesh, const double* wpacked, const double* X,
+                                            int64_t n, int64_t sp, int64_t sc, const double* wbar, double rot_rate,
+                                            double* xbar, int64_t bsp, int64_t bsc, double* wpacked_bar,
+                                            void* stream) {
+  if (!wmesh || !wpacked || n < 0 || (n > 0 && (!X || !wbar)))
+    return fail(CMM_E_ARG, "cmm_sample_vorticity_adjoint: bad arguments");
+  if (n == 0 || (!xbar && !wpacked_bar)) return CMM_OK;
+  hipLaunchKernelGGL(k_sample_vorticity_adj, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *wmesh, wpacked,
+                     X, n, sp, sc, wbar, rot_rate, xbar, bsp, bsc, wpacked_bar);
+  CMM_LAUNCH_CHECK("cmm_sample_vorticity_adjoint");
   return CMM_OK;
 }

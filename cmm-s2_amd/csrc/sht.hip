@@ -52,6 +52,12 @@ struct cmm_sht_plan {
   int* unit0 = nullptr;      // first unit of each m (L)
   double2* st_p = nullptr;   // recurrence state at each unit start (nunits, L/2 nodes)
   int* st_s = nullptr;
+  // transposed synthesis (adjoint): the same analysis schedule over the L+1
+  // rings, unit weights, per-order scale of the transposed real FFT
+  double2* st_pr = nullptr;  // (nunits, L/2 + 1 northern rings)
+  int* st_sr = nullptr;
+  double* w_r = nullptr;     // ones (L+1)
+  double* msc_r = nullptr;   // (L / pi) * (1 for m = 0, else 2)  (L)
   std::vector<double> hx_g, hw_g;
 };
 
@@ -131,6 +137,7 @@ extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
   (void)hipFree(p->x_g); (void)hipFree(p->w_g); (void)hipFree(p->dP_g); (void)hipFree(p->dS_g);
   (void)hipFree(p->K); (void)hipFree(p->tw); (void)hipFree(p->dinv); (void)hipFree(p->rec);
   (void)hipFree(p->units); (void)hipFree(p->unit0); (void)hipFree(p->st_p); (void)hipFree(p->st_s);
+  (void)hipFree(p->st_pr); (void)hipFree(p->st_sr); (void)hipFree(p->w_r); (void)hipFree(p->msc_r);
   delete p;
   return CMM_OK;
 }
@@ -201,6 +208,12 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
       (rc = upload(&p->dS_g, Sg.data(), Sg.size())) || (rc = upload(&p->K, K.data(), K.size())) ||
       (rc = upload(&p->tw, tw.data(), tw.size())) || (rc = upload(&p->dinv, dinv.data(), dinv.size())) ||
       (rc = upload(&p->rec, rec.data(), rec.size()))) {
+    cmm_sht_plan_destroy(p);
+    return rc;
+  }
+  std::vector<double> ones(L + 1, 1.0), msc(L);
+  for (int m = 0; m < L; ++m) msc[m] = (m == 0 ? 1.0 : 2.0) * L / M_PI;
+  if ((rc = upload(&p->w_r, ones.data(), ones.size())) || (rc = upload(&p->msc_r, msc.data(), msc.size()))) {
     cmm_sht_plan_destroy(p);
     return rc;
   }
@@ -706,7 +719,7 @@ constexpr int kFwdNpt = 2;
 
 // state (P_{l-2}, P_{l-1}, s) before the chunk at every segment start
 // l = m + kFwdSeg k, for every northern node: one thread per (m, node)
-__global__ __launch_bounds__(256) void k_leg_restart(int L, int nh, const double* __restrict__ xg,
+__global__ __launch_bounds__(256) void k_leg_restart(int L, int nh, int dstride, const double* __restrict__ xg,
                                                      const double* __restrict__ dP, const int* __restrict__ dS,
                                                      const double2* __restrict__ rec, const int* __restrict__ unit0,
                                                      double2* __restrict__ st_p, int* __restrict__ st_s) {
@@ -714,8 +727,8 @@ __global__ __launch_bounds__(256) void k_leg_restart(int L, int nh, const double
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nh) return;
   const double x = xg[j];
-  double pp = 0.0, pc = dP[(int64_t)m * L + j];
-  int s = dS[(int64_t)m * L + j];
+  double pp = 0.0, pc = dP[(int64_t)m * dstride + j];
+  int s = dS[(int64_t)m * dstride + j];
   const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
   int u = unit0[m];
   for (int lc = m; lc < L; lc += kFwdChunk) {
@@ -735,12 +748,17 @@ __global__ __launch_bounds__(256) void k_leg_restart(int L, int nh, const double
   }
 }
 
+// nn nodes symmetric about the equator (x_{nn-1-j} = -x_j), G [f][j][m] with
+// nn rows per field.  The analysis runs it over the L Gauss-Legendre nodes;
+// the transposed synthesis over the L+1 rings (unit weights, order scale
+// msc, no pole terms for m > 0, as the synthesis skips them).
 template <int C>
-__global__ __launch_bounds__(512) void k_leg_forward(int L, int nf, const int2* __restrict__ units,
+__global__ __launch_bounds__(512) void k_leg_forward(int L, int nn, int nf, const int2* __restrict__ units,
                                                      const double* __restrict__ xg, const double* __restrict__ wg,
                                                      const double2* __restrict__ st_p, const int* __restrict__ st_s,
                                                      const double2* __restrict__ rec, const double2* __restrict__ G,
-                                                     double2* __restrict__ flm) {
+                                                     double2* __restrict__ flm, const double* __restrict__ msc,
+                                                     int skip_poles) {
   constexpr int RQ = 64 / C;
   extern __shared__ double2 tile[];  // [C][T + 4]
   __shared__ double2 part[2][8][C];
@@ -749,8 +767,9 @@ __global__ __launch_bounds__(512) void k_leg_forward(int L, int nf, const int2* 
   const int u = blockIdx.x;
   const int m = units[u].x, ls = units[u].y;
   const int le = min(L, ls + kFwdSeg);
-  const int nh = (L + 1) / 2;  // northern nodes (incl. the middle one when L is odd)
+  const int nh = (nn + 1) / 2;  // northern nodes (incl. the middle one when nn is odd)
   const int T = blockDim.x, TP = T + 4, nw = T >> 6;
+  const double cm = msc ? msc[m] : 1.0;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   double x[kFwdNpt], pp[kFwdNpt], pc[kFwdNpt];
   int s[kFwdNpt];
@@ -761,16 +780,16 @@ __global__ __launch_bounds__(512) void k_leg_forward(int L, int nf, const int2* 
     const int jw = tid + q * T;
     act[q] = jw < nh;
     const int j = act[q] ? jw : 0;
-    const int jm = L - 1 - j;
+    const int jm = nn - 1 - j;
     x[q] = xg[j];
     const double2 ps = st_p[(int64_t)u * nh + j];
     pp[q] = ps.x;
     pc[q] = ps.y;
     s[q] = st_s[(int64_t)u * nh + j];
-    const double2 ga = G[((int64_t)fi * L + j) * L + m];
-    const double wj = act[q] ? wg[j] : 0.0;
+    const double2 ga = G[((int64_t)fi * nn + j) * L + m];
+    const double wj = (act[q] && !(skip_poles && m > 0 && j == 0)) ? cm * wg[j] : 0.0;
     if (jm != j) {
-      const double2 gb = G[((int64_t)fi * L + jm) * L + m];
+      const double2 gb = G[((int64_t)fi * nn + jm) * L + m];
       S[q] = double2{wj * (ga.x + gb.x), wj * (ga.y + gb.y)};
       D[q] = double2{wj * (ga.x - gb.x), wj * (ga.y - gb.y)};
     } else {
@@ -937,6 +956,172 @@ __global__ __launch_bounds__(128) void k_scalar_ops(int L, const double2* __rest
   for (int i = 0; i < 3; ++i) out[idx * 4 + 1 + i] = double2{u[i].re, u[i].im};
 }
 
+// ---------------------------------------------------------------------------
+// adjoints (transposes of the linear maps above, for reverse-mode gradients)
+// ---------------------------------------------------------------------------
+// Transposed Gauss-Legendre quadrature of the analysis:
+// G[f][j][m] = w_j sum_l Pbar_lm(x_j) flm[lm][f] at the L nodes, one thread per
+// northern node (folded with its mirror), one block per (m, field).
+__global__ __launch_bounds__(256) void k_leg_synth_gl(int L, int nf, const double* __restrict__ xg,
+                                                      const double* __restrict__ wg, const double* __restrict__ dP,
+                                                      const int* __restrict__ dS, const double2* __restrict__ rec,
+                                                      const double2* __restrict__ flm, double2* __restrict__ G) {
+  const int m = blockIdx.x, fi = blockIdx.y;
+  const int nh = (L + 1) / 2;
+  const int j = threadIdx.x;
+  if (j >= nh) return;
+  const double x = xg[j];
+  double pp = 0.0, pc = dP[(int64_t)m * L + j];
+  int s = dS[(int64_t)m * L + j];
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  double2 S = {0.0, 0.0}, D = {0.0, 0.0};
+  for (int l = m; l < L; ++l) {
+    if (l > m) {
+      const double2 ab = rec[lm0 + (l - m)];
+      leg_step(ab.x, ab.y, x, pp, pc);
+    }
+    leg_rescale(pp, pc, s);
+    const double v = s == 0 ? pc : 0.0;
+    const double2 a = flm[(lm0 + (l - m)) * nf + fi];
+    if ((l - m) & 1) {
+      D.x = fma(v, a.x, D.x);
+      D.y = fma(v, a.y, D.y);
+    } else {
+      S.x = fma(v, a.x, S.x);
+      S.y = fma(v, a.y, S.y);
+    }
+  }
+  const int jm = L - 1 - j;
+  double2* Gf = G + (int64_t)fi * L * L;
+  const double wj = wg[j];
+  if (jm != j) {
+    Gf[(int64_t)j * L + m] = double2{wj * (S.x + D.x), wj * (S.y + D.y)};
+    const double wm = wg[jm];
+    Gf[(int64_t)jm * L + m] = double2{wm * (S.x - D.x), wm * (S.y - D.y)};
+  } else {
+    Gf[(int64_t)j * L + m] = double2{wj * S.x, wj * S.y};
+  }
+}
+
+// Transposed resampling with the input scale of the transposed r2c FFT:
+// F[f][t][m] = c_m sum_j K_{m mod 2}[j][t] G[f][j][m], t = 0..L, c_0 = pi / L,
+// c_m = pi / (2 L) (k_fft_c2r doubles orders m > 0).  One lane per order of
+// one parity, one block per (ring, 64 orders, field).
+__global__ __launch_bounds__(64) void k_resample_adj(int L, int nqb, int tpad, const double* __restrict__ Kt,
+                                                     const double2* __restrict__ G, double2* __restrict__ F) {
+  const int p = (int)blockIdx.x / nqb, qb = (int)blockIdx.x % nqb;
+  const int t = blockIdx.y, fi = blockIdx.z;
+  const int m = p + 2 * (64 * qb + (int)threadIdx.x);
+  if (m >= L) return;
+  const double* K = Kt + ((int64_t)p * tpad + t) * L;
+  const double2* Gf = G + (int64_t)fi * L * L;
+  double2 acc = {0.0, 0.0};
+  for (int j = 0; j < L; ++j) {
+    const double k = K[j];
+    const double2 g = Gf[(int64_t)j * L + m];
+    acc.x = fma(k, g.x, acc.x);
+    acc.y = fma(k, g.y, acc.y);
+  }
+  const double c = (m == 0 ? 1.0 : 0.5) * M_PI / L;
+  F[((int64_t)fi * (L + 1) + t) * L + m] = double2{c * acc.x, c * acc.y};
+}
+
+__device__ inline void decode_lm(int L, int64_t idx, int& l, int& m) {
+  int lo = 0, hi = L - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) / 2;
+    if (lm_index(L, mid, mid) <= idx) lo = mid; else hi = mid - 1;
+  }
+  m = lo;
+  l = m + (int)(idx - lm_index(L, m, m));
+}
+
+// Transpose of angmom: the adjoint of the stored coefficient a(l, m), m >= 0,
+// from the adjoints o(l', m', k) of the three outputs at every stored (l', m')
+// that reads it: (l, m+1) through a(l, m'-1), (l, m-1) through a(l, m'+1),
+// (l, m) through a(l, m'), and for m = 1 also (l, 0) through
+// a(l, -1) = -conj(a(l, 1)).  Complex-scalar factors transpose to their
+// conjugates.
+template <class O>
+__device__ inline Cplx angmom_T(int l, int m, O o) {
+  Cplx r = cz();
+  auto add_mi = [&](double c, Cplx v) {  // r += (-i c) v
+    r.re += c * v.im;
+    r.im -= c * v.re;
+  };
+  if (m + 1 <= l) {
+    const double h = Lp(l, m + 1) / 2;
+    add_mi(h, o(l, m + 1, 0));
+    const Cplx v = o(l, m + 1, 1);
+    r.re += h * v.re;
+    r.im += h * v.im;
+  }
+  if (m >= 1) {
+    const double h = Lm(l, m - 1) / 2;
+    add_mi(h, o(l, m - 1, 0));
+    const Cplx v = o(l, m - 1, 1);
+    r.re -= h * v.re;
+    r.im -= h * v.im;
+  }
+  add_mi((double)m, o(l, m, 2));
+  if (m == 1) {
+    const double h = Lp(l, 0) / 2;
+    const Cplx v0 = o(l, 0, 0), v1 = o(l, 0, 1);
+    const Cplx t = Cplx{h * v0.im + h * v1.re, -h * v0.re + h * v1.im};
+    r.re -= t.re;  // transpose of a -> -conj(a)
+    r.im += t.im;
+  }
+  return r;
+}
+
+// stream operators, reverse step 1: ubar_i = obar_i + angmom^T of the
+// gradient outputs of u_i
+__global__ __launch_bounds__(128) void k_stream_ops_adj1(int L, const double2* __restrict__ ob,
+                                                         double2* __restrict__ ub) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)L * (L + 1) / 2) return;
+  int l, m;
+  decode_lm(L, idx, l, m);
+  for (int i = 0; i < 3; ++i) {
+    const Cplx g = angmom_T(l, m, [&](int ll, int mm, int k) {
+      const double2 v = ob[lm_index(L, ll, mm) * 12 + 3 + 3 * i + k];
+      return Cplx{v.x, v.y};
+    });
+    const double2 d = ob[idx * 12 + i];
+    ub[idx * 3 + i] = double2{d.x + g.re, d.y + g.im};
+  }
+}
+
+// reverse step 2: wbar = D_inv * angmom^T(ubar)
+__global__ __launch_bounds__(128) void k_stream_ops_adj2(int L, const double2* __restrict__ ub,
+                                                         const double* __restrict__ dinv, int apply_dinv,
+                                                         double2* __restrict__ wb) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)L * (L + 1) / 2) return;
+  int l, m;
+  decode_lm(L, idx, l, m);
+  const Cplx g = angmom_T(l, m, [&](int ll, int mm, int k) {
+    const double2 v = ub[lm_index(L, ll, mm) * 3 + k];
+    return Cplx{v.x, v.y};
+  });
+  const double dd = apply_dinv ? dinv[l] : 1.0;
+  wb[idx] = double2{dd * g.re, dd * g.im};
+}
+
+__global__ __launch_bounds__(128) void k_scalar_ops_adj(int L, const double2* __restrict__ ob,
+                                                        double2* __restrict__ wb) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)L * (L + 1) / 2) return;
+  int l, m;
+  decode_lm(L, idx, l, m);
+  const Cplx g = angmom_T(l, m, [&](int ll, int mm, int k) {
+    const double2 v = ob[lm_index(L, ll, mm) * 4 + 1 + k];
+    return Cplx{v.x, v.y};
+  });
+  const double2 d = ob[idx * 4];
+  wb[idx] = double2{d.x + g.re, d.y + g.im};
+}
+
 static_assert(kFwdSeg == kFwdSegHost, "segment length");
 
 static int build_restart(cmm_sht_plan* p, const std::vector<int2>& units, const std::vector<int>& unit0) {
@@ -949,8 +1134,17 @@ static int build_restart(cmm_sht_plan* p, const std::vector<int2>& units, const 
   if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: hipMalloc");
   e = hipMalloc(&p->st_s, sizeof(int) * (size_t)p->nunits * nh);
   if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: hipMalloc");
-  hipLaunchKernelGGL(k_leg_restart, dim3((nh + 255) / 256, L), dim3(256), 0, 0, L, nh, p->x_g, p->dP_g, p->dS_g,
+  hipLaunchKernelGGL(k_leg_restart, dim3((nh + 255) / 256, L), dim3(256), 0, 0, L, nh, L, p->x_g, p->dP_g, p->dS_g,
                      p->rec, p->unit0, p->st_p, p->st_s);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: restart kernel");
+  const int nhr = L / 2 + 1;
+  e = hipMalloc(&p->st_pr, sizeof(double2) * (size_t)p->nunits * nhr);
+  if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: hipMalloc");
+  e = hipMalloc(&p->st_sr, sizeof(int) * (size_t)p->nunits * nhr);
+  if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: hipMalloc");
+  hipLaunchKernelGGL(k_leg_restart, dim3((nhr + 255) / 256, L), dim3(256), 0, 0, L, nhr, L + 1, p->x_t, p->dP_t,
+                     p->dS_t, p->rec, p->unit0, p->st_pr, p->st_sr);
   e = hipGetLastError();
   if (e != hipSuccess) return hip_check(e, "cmm_sht_plan_create: restart kernel");
   e = hipDeviceSynchronize();
@@ -1015,8 +1209,8 @@ extern "C" int cmm_sht_forward(const cmm_sht_plan* p, const double* f, int nf, d
   const int T = std::max(64, ((nh + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
   if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_forward: L > 1024");
   size_t lds2 = (size_t)kFwdChunk * (T + 4) * sizeof(double2);
-  hipLaunchKernelGGL(k_leg_forward<kFwdChunk>, dim3(p->nunits, nf), dim3(T), lds2, st, L, nf, p->units, p->x_g,
-                     p->w_g, p->st_p, p->st_s, p->rec, G, reinterpret_cast<double2*>(flm));
+  hipLaunchKernelGGL(k_leg_forward<kFwdChunk>, dim3(p->nunits, nf), dim3(T), lds2, st, L, L, nf, p->units, p->x_g,
+                     p->w_g, p->st_p, p->st_s, p->rec, G, reinterpret_cast<double2*>(flm), nullptr, 0);
   CMM_LAUNCH_CHECK("cmm_sht_forward: legendre");
   return CMM_OK;
 }
@@ -1037,5 +1231,74 @@ extern "C" int cmm_sht_scalar_operators(const cmm_sht_plan* p, const double* w, 
   hipLaunchKernelGGL(k_scalar_ops, dim3(grid_for(total, 128)), dim3(128), 0, as_stream(stream), p->L,
                      reinterpret_cast<const double2*>(w), reinterpret_cast<double2*>(out));
   CMM_LAUNCH_CHECK("cmm_sht_scalar_operators");
+  return CMM_OK;
+}
+
+// ---- adjoints -------------------------------------------------------------
+extern "C" int cmm_sht_inverse_adjoint(const cmm_sht_plan* p, const double* fbar, int nf, double* flm_bar,
+                                       void* work, void* stream) {
+  if (!p || !fbar || !flm_bar || !work || nf < 1) return fail(CMM_E_ARG, "cmm_sht_inverse_adjoint: bad arguments");
+  const int L = p->L;
+  double2* F = reinterpret_cast<double2*>(work);
+  hipStream_t st = as_stream(stream);
+  const int pairs = (L + 2) / 2;
+  const size_t lds = (size_t)2 * p->N * sizeof(double2);
+  hipLaunchKernelGGL(k_fft_r2c, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, fbar, L, F);
+  CMM_LAUNCH_CHECK("cmm_sht_inverse_adjoint: fft");
+  const int nhr = L / 2 + 1;
+  const int T = std::max(64, ((nhr + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
+  if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_inverse_adjoint: L > 1024");
+  const size_t lds2 = (size_t)kFwdChunk * (T + 4) * sizeof(double2);
+  hipLaunchKernelGGL(k_leg_forward<kFwdChunk>, dim3(p->nunits, nf), dim3(T), lds2, st, L, L + 1, nf, p->units,
+                     p->x_t, p->w_r, p->st_pr, p->st_sr, p->rec, F, reinterpret_cast<double2*>(flm_bar), p->msc_r, 1);
+  CMM_LAUNCH_CHECK("cmm_sht_inverse_adjoint: legendre");
+  return CMM_OK;
+}
+
+extern "C" int cmm_sht_forward_adjoint(const cmm_sht_plan* p, const double* flm_bar, int nf, double* fbar,
+                                       void* work, void* stream) {
+  if (!p || !fbar || !flm_bar || !work || nf < 1) return fail(CMM_E_ARG, "cmm_sht_forward_adjoint: bad arguments");
+  const int L = p->L;
+  double2* F = reinterpret_cast<double2*>(work);
+  double2* G = F + (int64_t)nf * (L + 1) * L;
+  hipStream_t st = as_stream(stream);
+  const int nh = (L + 1) / 2;
+  const int T = (nh + 63) / 64 * 64;
+  if (T > 256) return fail(CMM_E_RANGE, "cmm_sht_forward_adjoint: L > 512");
+  hipLaunchKernelGGL(k_leg_synth_gl, dim3(L, nf), dim3(T), 0, st, L, nf, p->x_g, p->w_g, p->dP_g, p->dS_g, p->rec,
+                     reinterpret_cast<const double2*>(flm_bar), G);
+  CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: legendre");
+  const int nqb = ((L + 1) / 2 + 63) / 64;
+  hipLaunchKernelGGL(k_resample_adj, dim3(2 * nqb, L + 1, nf), dim3(64), 0, st, L, nqb, p->tpad, p->K, G, F);
+  CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: resample");
+  const int pairs = (L + 2) / 2;
+  const size_t lds = (size_t)2 * p->N * sizeof(double2);
+  hipLaunchKernelGGL(k_fft_c2r, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, F, L, fbar);
+  CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: fft");
+  return CMM_OK;
+}
+
+extern "C" int cmm_sht_stream_operators_adjoint(const cmm_sht_plan* p, const double* ops_bar, int apply_dinv,
+                                                double* w_bar, double* scratch, void* stream) {
+  if (!p || !ops_bar || !w_bar || !scratch) return fail(CMM_E_ARG, "cmm_sht_stream_operators_adjoint: bad arguments");
+  const int64_t total = (int64_t)p->L * (p->L + 1) / 2;
+  hipStream_t st = as_stream(stream);
+  double2* ub = reinterpret_cast<double2*>(scratch);
+  hipLaunchKernelGGL(k_stream_ops_adj1, dim3(grid_for(total, 128)), dim3(128), 0, st, p->L,
+                     reinterpret_cast<const double2*>(ops_bar), ub);
+  CMM_LAUNCH_CHECK("cmm_sht_stream_operators_adjoint");
+  hipLaunchKernelGGL(k_stream_ops_adj2, dim3(grid_for(total, 128)), dim3(128), 0, st, p->L, ub, p->dinv, apply_dinv,
+                     reinterpret_cast<double2*>(w_bar));
+  CMM_LAUNCH_CHECK("cmm_sht_stream_operators_adjoint");
+  return CMM_OK;
+}
+
+extern "C" int cmm_sht_scalar_operators_adjoint(const cmm_sht_plan* p, const double* ops_bar, double* w_bar,
+                                                void* stream) {
+  if (!p || !ops_bar || !w_bar) return fail(CMM_E_ARG, "cmm_sht_scalar_operators_adjoint: bad arguments");
+  const int64_t total = (int64_t)p->L * (p->L + 1) / 2;
+  hipLaunchKernelGGL(k_scalar_ops_adj, dim3(grid_for(total, 128)), dim3(128), 0, as_stream(stream), p->L,
+                     reinterpret_cast<const double2*>(ops_bar), reinterpret_cast<double2*>(w_bar));
+  CMM_LAUNCH_CHECK("cmm_sht_scalar_operators_adjoint");
   return CMM_OK;
 }

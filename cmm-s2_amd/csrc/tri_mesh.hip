@@ -652,6 +652,219 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets(cmm_tri_mesh M, const 
 }
 
 // ---------------------------------------------------------------------------
+// adjoints (reverse mode).  Coefficient and jet adjoints accumulate with
+// atomics (the caller clears them); position adjoints overwrite.
+// ---------------------------------------------------------------------------
+constexpr int kAdjMaxDepth = 8;
+
+struct LevelBarPtrs {
+  double* p[kAdjMaxDepth];
+};
+
+// geometry of triangle t as tri_locate leaves it, barycentrics of x
+__device__ inline void located_at(const double* __restrict__ geom, int t, d3 x, Located& L) {
+  load_geom_test(geom, t, L);
+  L.tri = t;
+  load_geom_rest(geom, L);
+  L.b0 = dot(x, L.w0);
+  L.b1 = dot(x, L.w1);
+  L.b2 = dot(x, L.w2);
+}
+
+// adjoint of the raw evaluation r = PS(b(x)) of one packed level at a located
+// point: adds rbar_c basis_j into pb (if non-null), returns xbar
+__device__ inline d3 eval_level3_adj(const double* __restrict__ packed, double* pb, const Located& L, d3 cb, int s,
+                                     d3 rb) {
+  const double* p = packed + ((int64_t)L.tri * 6 + s) * 18;
+  const double rv[3] = {rb.x, rb.y, rb.z};
+  if (pb) {
+    double bas[6];
+    bb6_basis(cb, bas);
+    double* dst = pb + ((int64_t)L.tri * 6 + s) * 18;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(dst + 6 * c + j, rv[c] * bas[j]);
+  }
+  d3 cbar = d3{0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) cbar = add(cbar, scl(rv[c], bb6_db(cb, p + 6 * c)));
+  const d3 bb = sub_bary_T(s, cbar, L.s12, L.s23, L.s31, L.s4);
+  return add(add(scl(bb.x, L.w0), scl(bb.y, L.w1)), scl(bb.z, L.w2));
+}
+
+// adjoint of k_tri_compose: xbar of the output (n,3) -> coefficient adjoints
+// of every level (null entries skipped) and, if qb, the input adjoint
+__global__ __launch_bounds__(256) void k_tri_compose_adj(cmm_tri_mesh M, LevelPtrs lv, LevelBarPtrs lb, int depth,
+                                                         const double* __restrict__ q, int64_t n, int64_t sp,
+                                                         int64_t sc, const double* __restrict__ ob,
+                                                         double* __restrict__ qb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  d3 xs[kAdjMaxDepth];
+  int ts[kAdjMaxDepth], ss[kAdjMaxDepth];
+  d3 x = load_pt(q, i, sp, sc);
+  bool ok = true;
+#pragma unroll
+  for (int l = 0; l < kAdjMaxDepth; ++l) {
+    if (l < depth) {
+      xs[l] = x;
+      Located L;
+      if (ok && tri_locate(M, M.geom, M.cells, M.over, x, L)) {
+        const int s = split_index(L.b0, L.b1, L.b2);
+        ts[l] = L.tri;
+        ss[l] = s;
+        const d3 r = eval_level3(lv.p[l], L, d3{L.b0, L.b1, L.b2}, s);
+        const double inv = 1 / sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+        x = d3{inv * r.x, inv * r.y, inv * r.z};
+      } else {
+        ok = false;
+        ts[l] = 0;
+        ss[l] = 0;
+      }
+    }
+  }
+  d3 xb = ok ? d3{ob[3 * i], ob[3 * i + 1], ob[3 * i + 2]} : d3{0, 0, 0};
+#pragma unroll
+  for (int l = kAdjMaxDepth - 1; l >= 0; --l) {
+    if (l < depth && ok) {
+      Located L;
+      located_at(M.geom, ts[l], xs[l], L);
+      const int s = ss[l];
+      const d3 cb = sub_bary(s, L.b0, L.b1, L.b2, L.s12, L.s23, L.s31, L.s4);
+      const d3 r = eval_level3(lv.p[l], L, d3{L.b0, L.b1, L.b2}, s);
+      xb = eval_level3_adj(lv.p[l], lb.p[l], L, cb, s, unit_adj(r, xb));
+    }
+  }
+  if (qb) {
+    qb[3 * i] = xb.x;
+    qb[3 * i + 1] = xb.y;
+    qb[3 * i + 2] = xb.z;
+  }
+}
+
+// transpose of k_tri_assemble<3> (accumulate into the jets vb (3,4,Nv))
+__global__ __launch_bounds__(256) void k_tri_assemble_adj(cmm_tri_mesh M, const double* __restrict__ pb,
+                                                          double* __restrict__ vb) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M.ntri) return;
+  const int vi[3] = {M.tris[3 * t], M.tris[3 * t + 1], M.tris[3 * t + 2]};
+  PSGeom P;
+  ps_unpack(M.psgeom + t, M.ntri, P);
+  const int64_t Nv = M.nv;
+  const double* src = pb + t * 108;
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    double cb[19];
+#pragma unroll
+    for (int k = 0; k < 19; ++k) cb[k] = 0.0;
+#pragma unroll
+    for (int ss = 0; ss < 6; ++ss)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += src[(ss * 3 + cc) * 6 + j];
+    double fb[3] = {0.0, 0.0, 0.0};
+    d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+    ps_coeffs_adj(P, cb, fb, gb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      unsafeAtomicAdd(vb + (cc * 4) * Nv + vi[i], fb[i]);
+      unsafeAtomicAdd(vb + (0 * 4 + 1 + cc) * Nv + vi[i], gb[i].x);
+      unsafeAtomicAdd(vb + (1 * 4 + 1 + cc) * Nv + vi[i], gb[i].y);
+      unsafeAtomicAdd(vb + (2 * 4 + 1 + cc) * Nv + vi[i], gb[i].z);
+    }
+  }
+}
+
+// adjoint of k_tri_stencil_jets: jet adjoints (3,4,Nv) -> stencil-feet
+// adjoints sfb (3, 4 Nv, overwrite) and, with a current map, its coefficient
+// adjoints curb (accumulate)
+__global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, const double* __restrict__ cur,
+                                                              const double* __restrict__ yv, int64_t yrow,
+                                                              const double* __restrict__ sf, int64_t srow,
+                                                              const double* __restrict__ tanv, double eps,
+                                                              const double* __restrict__ jb, double* __restrict__ sfb,
+                                                              int64_t sbrow, double* __restrict__ curb) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t Nv = M.nv;
+  if (v >= Nv) return;
+  d3 S[4], Sin[4];
+  for (int k = 0; k < 4; ++k) {
+    const int64_t col = k * Nv + v;
+    Sin[k] = d3{sf[col], sf[srow + col], sf[2 * srow + col]};
+    S[k] = Sin[k];
+  }
+  Located L;
+  int s = 0;
+  bool ok = true;
+  d3 cbk[4];
+  if (cur) {
+    const d3 y = d3{yv[v], yv[yrow + v], yv[2 * yrow + v]};
+    ok = tri_locate(M, M.geom, M.cells, M.over, y, L);
+    if (ok) {
+      s = split_index(L.b0, L.b1, L.b2);
+      for (int k = 0; k < 4; ++k) {
+        const d3 bq = d3{dot(Sin[k], L.w0), dot(Sin[k], L.w1), dot(Sin[k], L.w2)};
+        cbk[k] = sub_bary(s, bq.x, bq.y, bq.z, L.s12, L.s23, L.s31, L.s4);
+        S[k] = eval_level3(cur, L, bq, s);
+      }
+    }
+  }
+  d3 Sb[4] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+  if (ok) {
+    const d3 mu = d3{(S[0].x + S[1].x + S[2].x + S[3].x) / 4, (S[0].y + S[1].y + S[2].y + S[3].y) / 4,
+                     (S[0].z + S[1].z + S[2].z + S[3].z) / 4};
+    const d3 a1 = d3{tanv[3 * v], tanv[3 * v + 1], tanv[3 * v + 2]};
+    const d3 a2 = d3{tanv[3 * (Nv + v)], tanv[3 * (Nv + v) + 1], tanv[3 * (Nv + v) + 2]};
+    const double A1[3] = {a1.x, a1.y, a1.z}, A2[3] = {a2.x, a2.y, a2.z};
+    double d1b[3], d2b[3];
+    for (int c = 0; c < 3; ++c) {
+      d1b[c] = 0.0;
+      d2b[c] = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double g = jb[(d * 4 + 1 + c) * Nv + v];
+        d1b[c] += g * A1[d];
+        d2b[c] += g * A2[d];
+      }
+    }
+    const d3 mb = unit_adj(mu, d3{jb[(0 * 4) * Nv + v], jb[(1 * 4) * Nv + v], jb[(2 * 4) * Nv + v]});
+    const double e4 = 4 * eps;
+    const d3 q = scl(0.25, mb), D1 = d3{d1b[0] / e4, d1b[1] / e4, d1b[2] / e4},
+             D2 = d3{d2b[0] / e4, d2b[1] / e4, d2b[2] / e4};
+    Sb[0] = sub(sub(q, D1), D2);
+    Sb[1] = sub(add(q, D1), D2);
+    Sb[2] = add(sub(q, D1), D2);
+    Sb[3] = add(add(q, D1), D2);
+    if (cur) {
+      double acc[18];
+      for (int e = 0; e < 18; ++e) acc[e] = 0.0;
+      const double* p = cur + ((int64_t)L.tri * 6 + s) * 18;
+      for (int k = 0; k < 4; ++k) {
+        double bas[6];
+        bb6_basis(cbk[k], bas);
+        const double rv[3] = {Sb[k].x, Sb[k].y, Sb[k].z};
+        d3 cbar = d3{0, 0, 0};
+        for (int c = 0; c < 3; ++c) {
+          for (int j = 0; j < 6; ++j) acc[6 * c + j] += rv[c] * bas[j];
+          cbar = add(cbar, scl(rv[c], bb6_db(cbk[k], p + 6 * c)));
+        }
+        const d3 bb = sub_bary_T(s, cbar, L.s12, L.s23, L.s31, L.s4);
+        Sb[k] = add(add(scl(bb.x, L.w0), scl(bb.y, L.w1)), scl(bb.z, L.w2));
+      }
+      if (curb) {
+        double* dst = curb + ((int64_t)L.tri * 6 + s) * 18;
+        for (int e = 0; e < 18; ++e) unsafeAtomicAdd(dst + e, acc[e]);
+      }
+    }
+  }
+  for (int k = 0; k < 4; ++k) {
+    const int64_t col = k * Nv + v;
+    sfb[col] = Sb[k].x;
+    sfb[sbrow + col] = Sb[k].y;
+    sfb[2 * sbrow + col] = Sb[k].z;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" int cmm_tri_locate(const cmm_tri_mesh* m, const double* q, int64_t n, int64_t sp,
@@ -769,5 +982,49 @@ extern "C" int cmm_tri_stencil_jets(const cmm_tri_mesh* m, const double* cur, co
   hipLaunchKernelGGL(k_tri_stencil_jets, dim3(grid_for(m->nv, 128)), dim3(128), 0, as_stream(stream), *m,
                      cur, yv, yrow, sf, srow, tanv, eps, jets, st);
   CMM_LAUNCH_CHECK("cmm_tri_stencil_jets");
+  return CMM_OK;
+}
+
+// ---- adjoints -------------------------------------------------------------
+extern "C" int cmm_tri_compose_adjoint(const cmm_tri_mesh* m, const double* const* levels,
+                                       double* const* level_bars, int depth, const double* q, int64_t n,
+                                       int64_t sp, int64_t sc, const double* out_bar, double* q_bar, void* stream) {
+  if (!m || depth < 1 || n < 0 || !levels || !level_bars || (n > 0 && (!q || !out_bar)))
+    return fail(CMM_E_ARG, "cmm_tri_compose_adjoint: bad arguments");
+  if (depth > kAdjMaxDepth) return fail(CMM_E_RANGE, "cmm_tri_compose_adjoint: depth > 8");
+  LevelPtrs lv;
+  LevelBarPtrs lb;
+  for (int l = 0; l < depth; ++l) {
+    if (!levels[l]) return fail(CMM_E_ARG, "cmm_tri_compose_adjoint: null level");
+    lv.p[l] = levels[l];
+    lb.p[l] = level_bars[l];
+  }
+  for (int l = depth; l < kAdjMaxDepth; ++l) lb.p[l] = nullptr;
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_tri_compose_adj, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, lv, lb, depth,
+                     q, n, sp, sc, out_bar, q_bar);
+  CMM_LAUNCH_CHECK("cmm_tri_compose_adjoint");
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_assemble_adjoint(const cmm_tri_mesh* m, const double* packed_bar, double* vals_bar,
+                                        void* stream) {
+  if (!m || !packed_bar || !vals_bar) return fail(CMM_E_ARG, "cmm_tri_assemble_adjoint: bad arguments");
+  hipLaunchKernelGGL(k_tri_assemble_adj, dim3(grid_for(m->ntri, 256)), dim3(256), 0, as_stream(stream), *m,
+                     packed_bar, vals_bar);
+  CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
+  return CMM_OK;
+}
+
+extern "C" int cmm_tri_stencil_jets_adjoint(const cmm_tri_mesh* m, const double* cur, const double* yv,
+                                            int64_t yrow, const double* sf, int64_t srow, const double* tanv,
+                                            double eps, const double* jets_bar, double* sf_bar, int64_t sbrow,
+                                            double* cur_bar, void* stream) {
+  if (!m || !sf || !tanv || !jets_bar || !sf_bar || !(eps > 0) || (cur && !yv) || srow < 4 * m->nv ||
+      sbrow < 4 * m->nv)
+    return fail(CMM_E_ARG, "cmm_tri_stencil_jets_adjoint: bad arguments");
+  hipLaunchKernelGGL(k_tri_stencil_jets_adj, dim3(grid_for(m->nv, 128)), dim3(128), 0, as_stream(stream), *m, cur,
+                     yv, yrow, sf, srow, tanv, eps, jets_bar, sf_bar, sbrow, cur ? cur_bar : nullptr);
+  CMM_LAUNCH_CHECK("cmm_tri_stencil_jets_adjoint");
   return CMM_OK;
 }

@@ -273,6 +273,72 @@ int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const double* wpa
                          int64_t x_stride_comp, const double* cos_grid_theta, int ring_len,
                          double rot_rate, double* out, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Adjoints (reverse-mode transposes of the entry points above) for the      */
+/* gradient of a final-vorticity misfit with respect to the initial spectral */
+/* coefficients.  The reference has no gradient code; each entry point is    */
+/* the exact transpose of the forward entry point named in its comment, for  */
+/* the same location decisions.  "accumulates" = adds into the output with   */
+/* FP64 atomics (clear it first; summation order is not fixed); the other    */
+/* outputs are overwritten.                                                  */
+/* ------------------------------------------------------------------------ */
+/* transpose of cmm_sht_inverse: fbar (nf, L+1, 2L) -> flm_bar [lm][nf].
+ * work: cmm_sht_workspace_bytes(nf). */
+int cmm_sht_inverse_adjoint(const cmm_sht_plan* plan, const double* fbar, int nf, double* flm_bar,
+                            void* work, void* stream);
+/* transpose of cmm_sht_forward: flm_bar [lm][nf] -> fbar (nf, L+1, 2L); L <= 512. */
+int cmm_sht_forward_adjoint(const cmm_sht_plan* plan, const double* flm_bar, int nf, double* fbar,
+                            void* work, void* stream);
+/* transpose of cmm_sht_stream_operators: ops_bar [lm][12] -> omega_bar [lm];
+ * scratch: 3 * nlm complex. */
+int cmm_sht_stream_operators_adjoint(const cmm_sht_plan* plan, const double* ops_bar, int apply_dinv,
+                                     double* omega_bar, double* scratch, void* stream);
+/* transpose of cmm_sht_scalar_operators: ops_bar [lm][4] -> f_bar [lm]. */
+int cmm_sht_scalar_operators_adjoint(const cmm_sht_plan* plan, const double* ops_bar, double* f_bar,
+                                     void* stream);
+/* transposes of cmm_mwss_velocity_jets (12 fields) / cmm_mwss_scalar_jets
+ * (4 fields): vals_bar (.., Nu) -> fields_bar (nf, L+1, 2L). */
+int cmm_mwss_velocity_jets_adjoint(const cmm_mwss_mesh* mesh, const double* vals_bar, double* fields_bar,
+                                   void* stream);
+int cmm_mwss_scalar_jets_adjoint(const cmm_mwss_mesh* mesh, const double* vals_bar, double* fields_bar,
+                                 void* stream);
+/* transpose of cmm_mwss_assemble (packed only): packed_bar -> vals_bar
+ * (accumulates). */
+int cmm_mwss_assemble_adjoint(const cmm_mwss_mesh* mesh, const double* packed_bar, int ncomp,
+                              double* vals_bar, void* stream);
+/* transpose of cmm_tri_assemble (ncomp 3, packed only): accumulates vals_bar. */
+int cmm_tri_assemble_adjoint(const cmm_tri_mesh* mesh, const double* packed_bar, double* vals_bar,
+                             void* stream);
+/* adjoint of cmm_advect_stencil for the stencil feet (spline velocity;
+ * Euler, improved Euler or RK4): yst_bar (3, 4 nv) -> the packed adjoints of
+ * the nlev velocity levels (accumulate; level_bars[k] pairs with
+ * vel->levels[k]) and, if st_bar != NULL, the adjoint of the start points
+ * (3, 4 nv), overwritten. */
+int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integrator, double dt, const double* st,
+                               int64_t st_row, int64_t nv, const double* yst_bar, int64_t yst_bar_row,
+                               double* const* level_bars, double* st_bar, int64_t st_bar_row,
+                               void* stream);
+/* adjoint of cmm_tri_stencil_jets: jets_bar (3,4,Nv) -> stencil-feet
+ * adjoints sf_bar (3, 4 Nv) and, when cur != NULL, the current map's packed
+ * adjoints cur_bar (accumulates). */
+int cmm_tri_stencil_jets_adjoint(const cmm_tri_mesh* mesh, const double* cur, const double* yv,
+                                 int64_t yrow, const double* sf, int64_t srow, const double* tanv,
+                                 double eps, const double* jets_bar, double* sf_bar, int64_t sf_bar_row,
+                                 double* cur_bar, void* stream);
+/* adjoint of cmm_tri_compose (depth <= 8): out_bar (n,3) -> packed adjoints
+ * of the levels (accumulate; NULL entries skipped) and q_bar (n,3) or NULL. */
+int cmm_tri_compose_adjoint(const cmm_tri_mesh* mesh, const double* const* levels,
+                            double* const* level_bars, int depth, const double* q, int64_t n,
+                            int64_t q_stride_pt, int64_t q_stride_comp, const double* out_bar,
+                            double* q_bar, void* stream);
+/* adjoint of cmm_sample_vorticity for a spline vorticity: wbar (n) ->
+ * x_bar (strided, may be NULL) and wpacked_bar (accumulates, may be NULL). */
+int cmm_sample_vorticity_adjoint(const cmm_mwss_mesh* wmesh, const double* wpacked, const double* X,
+                                 int64_t n, int64_t x_stride_pt, int64_t x_stride_comp,
+                                 const double* wbar, double rot_rate, double* x_bar,
+                                 int64_t xb_stride_pt, int64_t xb_stride_comp, double* wpacked_bar,
+                                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif
