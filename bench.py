@@ -59,6 +59,11 @@ def parse():
     ap.add_argument("--ensemble", type=int, default=0,
                     help="instead: M seeded members (L in {128..512}, ico 5-7, 50 steps, remap every 10, "
                          "FP32 synthesis), one HIP stream per member, against the same members on one stream")
+    ap.add_argument("--adjoint", type=int, default=0,
+                    help="instead: gradient of the final-vorticity misfit for a batch of B initial fields "
+                         "(L=256, ico-6, 50 steps dt=1/256, remap every 10, l <= 64) with central-difference "
+                         "checks on 32 random directions")
+    ap.add_argument("--fd-dirs", type=int, default=32)
     ap.add_argument("--small", action="store_true", help="quick check: L=64 ico-4 Nt=20 Lf=256")
     ap.add_argument("--save-maps", default=None, help="write the submap jets (torch.save) after the run")
     ap.add_argument("--compose-only", default=None,
@@ -74,6 +79,8 @@ def main():
         sys.exit(2)
     if a.ensemble:
         return ensemble_bench(a)
+    if a.adjoint:
+        return adjoint_bench(a)
     if a.small:
         a.L, a.ico, a.Nt, a.n_maps, a.Lf, a.Lw = 64, 4, 20, 5, 256, 128
     import numpy as np
@@ -232,6 +239,75 @@ def ensemble_bench(a):
                       "member_params": params},
            "one_stream_value": round(n1 / sec1, 2), "stream_speedup": round((n / sec) / (n1 / sec1), 4),
            "member_steps": n, "seconds": round(sec, 4)}
+    print(json.dumps(rec), flush=True)
+
+
+def adjoint_bench(a):
+    """Gradient of J = sum_lm |(F omega_T - omega_target)_lm|^2 (all orders)
+    with respect to the initial coefficients (1 <= l <= 64) for a batch of
+    seeded fields: forward and reverse times per member, and central
+    differences with h = 1e-6 (and 1e-5) along seeded random unit directions
+    on the first member."""
+    import numpy as np
+    import torch
+    from cmm_s2.core import mesh_functions as mf, spherical_harmonic_tools as st, dynamical_fields as df
+    from cmm_s2.core.adjoint import MisfitProblem
+    torch.cuda.set_device(0)
+    L, ico, Nt, n_maps, lmax = 256, 6, 50, 10, 64
+    if a.small:
+        L, ico, Nt, n_maps, lmax = 64, 4, 20, 5, 16
+    im = mf.icosahedral_mesh(ico)
+    mesh = mf.spherical_triangulation(im.points, simplices=im.simplices)
+    plan = st.SHT.get(L)
+    target = plan.full_to_packed(df.lm_to_full(df.random_sh_coefficients(50, lmax, L), L))
+    prob = MisfitProblem(L, Nt, Nt / L, mesh, target, 2 * np.pi, n_maps, lmax=lmax)
+    members = []
+    ps = [torch.from_numpy(prob.params.from_coeffs(df.random_sh_coefficients(500 + k, lmax, L))).cuda()
+          for k in range(a.adjoint)]
+    prob.value_and_grad(ps[0])  # warm-up: plans, tile tuning, allocator
+    for k, p in enumerate(ps):
+        J, g = prob.value_and_grad(p)
+        t = dict(prob.timings)
+        members.append({"seed": 500 + k, "J": J, "grad_norm": float(g.norm()), "forward_s": round(t["forward_s"], 5),
+                        "reverse_s": round(t["reverse_s"], 5),
+                        "reverse_over_forward": round(t["reverse_s"] / t["forward_s"], 3)})
+        log(f"member {k}: J {J:.6e} |g| {float(g.norm()):.4e} forward {t['forward_s'] * 1e3:.1f} ms "
+            f"reverse {t['reverse_s'] * 1e3:.1f} ms ({t['reverse_s'] / t['forward_s']:.2f}x)")
+        if k == 0:
+            g0, J0 = g, J
+    rng = np.random.default_rng(51)
+    fd = []
+    for i in range(a.fd_dirs):
+        d = torch.from_numpy(rng.standard_normal(prob.params.n)).cuda()
+        d /= d.norm()
+        ad = float((g0 * d).sum())
+        row = {"adjoint": ad}
+        for h in (1e-6, 1e-5):
+            f = (prob.value(ps[0] + h * d) - prob.value(ps[0] - h * d)) / (2 * h)
+            row[f"fd_h{h:.0e}"] = f
+            row[f"rel_err_h{h:.0e}"] = abs(f - ad) / max(abs(ad), 1e-300)
+        fd.append(row)
+    e6 = np.array([r["rel_err_h1e-06"] for r in fd] or [np.nan])
+    e5 = np.array([r["rel_err_h1e-05"] for r in fd] or [np.nan])
+    ads = np.array([abs(r["adjoint"]) for r in fd] or [np.nan])
+    log(f"central differences on {len(fd)} directions: rel err h=1e-6 median {np.median(e6):.2e} max {e6.max():.2e}; "
+        f"h=1e-5 median {np.median(e5):.2e} max {e5.max():.2e}; |dJ.d| from {ads.min():.2e} to {ads.max():.2e}")
+    fwd = float(np.median([m["forward_s"] for m in members]))
+    rev = float(np.median([m["reverse_s"] for m in members]))
+    rec = {"metric": "adjoint cost / forward cost (gradient of final-vorticity misfit)",
+           "value": round(rev / fwd, 4), "unit": "x forward", "n_gpus": 1, "higher_is_better": False,
+           "data": "synthetic", "dtype": "f64",
+           "config": {"workload": f"barotropic CMM L={L} ico-{ico}, {Nt} RK4 steps dt=1/{L}, remap every {n_maps}, "
+                                  f"gradient w.r.t. the initial coefficients 1 <= l <= {lmax} "
+                                  f"({prob.params.n} reals), target seed 50, batch seeds 500..{499 + a.adjoint}, "
+                                  "checkpoint at every remap with recomputation", "members": a.adjoint},
+           "forward_s_median": round(fwd, 5), "reverse_s_median": round(rev, 5),
+           "forward_ms_per_step": round(fwd / Nt * 1e3, 4), "members": members,
+           "fd_check": {"directions": len(fd), "seed": 51,
+                        "rel_err_h1e-6": {"median": float(np.median(e6)), "max": float(e6.max())},
+                        "rel_err_h1e-5": {"median": float(np.median(e5)), "max": float(e5.max())},
+                        "abs_dir_deriv_range": [float(ads.min()), float(ads.max())], "J": J0,
+                        "rows": fd[:8]}}
     print(json.dumps(rec), flush=True)
 
 

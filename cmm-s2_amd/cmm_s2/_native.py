@@ -76,13 +76,14 @@ SIGNATURES = {
     "cmm_sht_scalar_operators_adjoint": (C.c_int, [vp, vp, vp, vp]),
     "cmm_mwss_velocity_jets_adjoint": (C.c_int, [vp, vp, vp, vp]),
     "cmm_mwss_scalar_jets_adjoint": (C.c_int, [vp, vp, vp, vp]),
-    "cmm_mwss_assemble_adjoint": (C.c_int, [vp, vp, C.c_int, vp, vp]),
-    "cmm_tri_assemble_adjoint": (C.c_int, [vp, vp, vp, vp]),
+    "cmm_mwss_assemble_adjoint": (C.c_int, [vp, vp, C.c_int, vp, vp, i64, vp]),
+    "cmm_tri_assemble_adjoint": (C.c_int, [vp, vp, vp, vp, i64, vp]),
     "cmm_advect_stencil_adjoint": (C.c_int, [C.POINTER(Velocity), C.c_int, dbl, vp, i64, i64, vp, i64, P_vp, vp,
-                                             i64, vp]),
-    "cmm_tri_stencil_jets_adjoint": (C.c_int, [vp, vp, vp, i64, vp, i64, vp, dbl, vp, vp, i64, vp, vp]),
-    "cmm_tri_compose_adjoint": (C.c_int, [vp, P_vp, P_vp, C.c_int, vp, i64, i64, i64, vp, vp, vp]),
-    "cmm_sample_vorticity_adjoint": (C.c_int, [vp, vp, vp, i64, i64, i64, vp, dbl, vp, i64, i64, vp, vp]),
+                                             i64, vp, i64, vp]),
+    "cmm_tri_stencil_jets_adjoint": (C.c_int, [vp, vp, vp, i64, vp, i64, vp, dbl, vp, vp, i64, vp, vp, i64, vp]),
+    "cmm_tri_compose_adjoint": (C.c_int, [vp, P_vp, P_vp, C.c_int, vp, i64, i64, i64, vp, vp, vp, i64, vp]),
+    "cmm_sample_vorticity_adjoint": (C.c_int, [vp, vp, vp, i64, i64, i64, vp, dbl, vp, i64, i64, vp, vp, i64, vp]),
+    "cmm_scatter_workspace_bytes": (C.c_int, [i64, P_i64]),
 }
 
 

@@ -139,6 +139,21 @@ class MisfitProblem(object):
         self._nu, self._tu = nu, tu
         self.w0 = None
         self.timings = {}
+        self._sw = None  # sorted scatter-add workspace (uint8)
+
+    def _ws(self, nbytes):
+        """(pointer, bytes) of the adjoint workspace (one buffer, grown on
+        demand; the calls that use it are ordered on one stream)."""
+        if self._sw is None or self._sw.numel() < nbytes:
+            self._sw = torch.empty(int(nbytes), dtype=torch.uint8, device=B.device())
+        return self._sw.data_ptr(), self._sw.numel()
+
+    def _scatter(self, records):
+        """workspace for ``records`` sorted scatter-add records"""
+        import ctypes
+        b = ctypes.c_int64(0)
+        N.call("cmm_scatter_workspace_bytes", int(records), ctypes.byref(b))
+        return self._ws(b.value)
 
     # ---- building blocks --------------------------------------------------
     def _omega0(self, p):
@@ -358,15 +373,17 @@ class MisfitProblem(object):
     def _sample_and_compose_adj(self, chain, X, wbar, rot):
         n = self.n
         xbar = B.empty(n, 3) if chain else None
+        wp, wb = self._scatter(n)
         N.call("cmm_sample_vorticity_adjoint", self.mesh_w._h, self.w0.packed.data_ptr(), X.data_ptr(), n,
                X.stride(0), X.stride(1), wbar.data_ptr(), float(rot), B.ptr(xbar), 3, 1, self.lam_w.data_ptr(),
-               B.stream())
+               wp, wb, B.stream())
         if chain:
             lv = N.ptr_array([self.maps[i].data_ptr() for i in chain])
             lb = N.ptr_array([self._lam_map(i).data_ptr() for i in chain])
             q = self.samples
+            wp, wb = self._scatter(len(chain) * n)
             N.call("cmm_tri_compose_adjoint", self.mesh._h, lv, lb, len(chain), q.data_ptr(), n, q.stride(0),
-                   q.stride(1), xbar.data_ptr(), None, B.stream())
+                   q.stride(1), xbar.data_ptr(), None, wp, wb, B.stream())
 
     def _reverse(self, tape):
         plan, L = self.plan, self.L
@@ -385,7 +402,9 @@ class MisfitProblem(object):
                 if lam is None:
                     continue
                 vb = B.zeros(12, self._nu)
-                N.call("cmm_mwss_assemble_adjoint", self.mesh_u._h, lam.data_ptr(), 3, vb.data_ptr(), B.stream())
+                wp, wb = self._ws(96 * 3 * self._tu)
+                N.call("cmm_mwss_assemble_adjoint", self.mesh_u._h, lam.data_ptr(), 3, vb.data_ptr(), wp, wb,
+                       B.stream())
                 fb = B.empty(12, L + 1, 2 * L)
                 N.call("cmm_mwss_velocity_jets_adjoint", self.mesh_u._h, vb.data_ptr(), fb.data_ptr(), B.stream())
                 ob = B.empty(nlm(L), 12, 2)
@@ -407,18 +426,22 @@ class MisfitProblem(object):
                 m = self.mesh
                 nv = m.nv
                 jb = B.zeros(3, 4, nv)
-                N.call("cmm_tri_assemble_adjoint", m._h, lam.data_ptr(), jb.data_ptr(), B.stream())
+                wp, wb = self._ws(288 * m.ntri)
+                N.call("cmm_tri_assemble_adjoint", m._h, lam.data_ptr(), jb.data_ptr(), wp, wb, B.stream())
                 sb = B.empty(3, 4 * nv)
                 cur = None if src is None else self.maps[src].data_ptr()
                 curb = None if src is None else self._lam_map(src).data_ptr()
+                wp, wb = self._scatter(4 * nv)
                 N.call("cmm_tri_stencil_jets_adjoint", m._h, cur, yv.data_ptr(), yv.stride(0), ys.data_ptr(),
                        ys.stride(0), m.d_tan.data_ptr(), self.EPS, jb.data_ptr(), sb.data_ptr(), sb.stride(0),
-                       curb, B.stream())
+                       curb, wp, wb, B.stream())
                 desc = self._desc(V, ikind, t)
                 lbs = N.ptr_array([self._lam_vel(V.ids[k]).data_ptr() for k in range(min(V.nVs, 3))])
                 xs = m.d_spts
+                # quad-aggregated atomics: faster than the sorted form for the
+                # four stencil feet of a vertex (they share sub-triangles)
                 N.call("cmm_advect_stencil_adjoint", desc, ikind, float(self.dt), xs.data_ptr(), xs.stride(0), nv,
-                       sb.data_ptr(), sb.stride(0), lbs, None, 0, B.stream())
+                       sb.data_ptr(), sb.stride(0), lbs, None, 0, None, 0, B.stream())
                 if mid not in self.submaps:
                     self.maps.pop(mid, None)
 
@@ -426,7 +449,8 @@ class MisfitProblem(object):
         mw, pw = self.mesh_w, self.plan_w
         Lw = self.L_w
         vb = B.zeros(4, mw.nv)
-        N.call("cmm_mwss_assemble_adjoint", mw._h, self.lam_w.data_ptr(), 1, vb.data_ptr(), B.stream())
+        wp, wb = self._ws(96 * mw.ntri)
+        N.call("cmm_mwss_assemble_adjoint", mw._h, self.lam_w.data_ptr(), 1, vb.data_ptr(), wp, wb, B.stream())
         fb = B.empty(4, Lw + 1, 2 * Lw)
         N.call("cmm_mwss_scalar_jets_adjoint", mw._h, vb.data_ptr(), fb.data_ptr(), B.stream())
         ob = B.empty(nlm(Lw), 4, 2)

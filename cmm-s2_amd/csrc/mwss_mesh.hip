@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.h"
+#include "scatter.h"
 
 using namespace cmm;
 
@@ -694,7 +695,6 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj(cmm_mwss_mesh M, cons
     P.H[i] = d3{c * h.x - sn * h.y, sn * h.x + c * h.y, h.z};
   }
   const int64_t Nu = M.Nu;
-  const double* src = pb + t * (36 * NC);
 #pragma unroll
   for (int cc = 0; cc < NC; ++cc) {
     double cb[19];
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj(cmm_mwss_mesh M, cons
 #pragma unroll
     for (int ss = 0; ss < 6; ++ss)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += src[(ss * NC + cc) * 6 + j];
+      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += pb[adj_index(6 * cc + j, ss, t, M.Tu)];
     double fb[3] = {0.0, 0.0, 0.0};
     d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
     ps_coeffs_adj(P, cb, fb, gb);
@@ -721,6 +721,131 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj(cmm_mwss_mesh M, cons
         unsafeAtomicAdd(vb + (2 * 4 + 1 + cc) * Nu + vi[i], gb[i].z);
       }
     }
+  }
+}
+
+// Two-phase transpose of k_mwss_assemble<NC> without atomics.  Phase 1, one
+// thread per triangle: the jet adjoints of its corners, corner i row r at
+// tmp[(4 NC i + r) Tu + t] (rows in the jets order).  Phase 2, one thread per
+// non-pole vertex: its (at most 6) incident (triangle, corner) pairs in closed
+// form, added into vb; one block per pole reduces the 2L cap triangles.
+template <int NC>
+__global__ __launch_bounds__(256) void k_mwss_assemble_adj_corners(cmm_mwss_mesh M, const double* __restrict__ pb,
+                                                                   double* __restrict__ tmp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M.Tu) return;
+  const int N = M.N, Mr = M.M;
+  int row, col, half;
+  if (t < N) { row = 0; col = (int)t; half = 0; }
+  else if (t >= M.Tu - N) { row = Mr - 2; col = (int)(t - (M.Tu - N)); half = 0; }
+  else {
+    const int64_t u = t - N;
+    half = (int)(u % 2);
+    const int64_t k = u / 2;
+    row = 1 + (int)(k / N);
+    col = (int)(k % N);
+  }
+  PSGeom P;
+  ps_unpack(M.rowgeom + ((int64_t)row * 2 + half) * kPSGeomLen, 1, P);
+  const double c = M.cp[col], sn = M.sp[col];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const d3 h = P.H[i];
+    P.H[i] = d3{c * h.x - sn * h.y, sn * h.x + c * h.y, h.z};
+  }
+  const int64_t Tu = M.Tu;
+  constexpr int R = 4 * NC;
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc) {
+    double cb[19];
+#pragma unroll
+    for (int k = 0; k < 19; ++k) cb[k] = 0.0;
+#pragma unroll
+    for (int ss = 0; ss < 6; ++ss)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += pb[adj_index(6 * cc + j, ss, t, M.Tu)];
+    double fb[3] = {0.0, 0.0, 0.0};
+    d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+    ps_coeffs_adj(P, cb, fb, gb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (NC == 1) {
+        tmp[(R * i + 0) * Tu + t] = fb[i];
+        tmp[(R * i + 1) * Tu + t] = gb[i].x;
+        tmp[(R * i + 2) * Tu + t] = gb[i].y;
+        tmp[(R * i + 3) * Tu + t] = gb[i].z;
+      } else {
+        tmp[(R * i + cc * 4) * Tu + t] = fb[i];
+        tmp[(R * i + 0 * 4 + 1 + cc) * Tu + t] = gb[i].x;
+        tmp[(R * i + 1 * 4 + 1 + cc) * Tu + t] = gb[i].y;
+        tmp[(R * i + 2 * 4 + 1 + cc) * Tu + t] = gb[i].z;
+      }
+    }
+  }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_mwss_assemble_adj_gather(cmm_mwss_mesh M, const double* __restrict__ tmp,
+                                                                  double* __restrict__ vb) {
+  const int64_t v = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // poles handled apart
+  if (v >= M.Nu - 1) return;
+  const int N = M.N, L = M.L;
+  const int r = 1 + (int)((v - 1) / N), c = (int)((v - 1) % N);
+  const int cm = (c + N - 1) % N;
+  const int64_t Tu = M.Tu;
+  auto mid = [&](int rr, int cc, int h) -> int64_t { return N + 2 * ((int64_t)(rr - 1) * N + cc) + h; };
+  int64_t ts[6];
+  int cs[6];
+  int k = 0;
+  if (r == 1) { ts[k] = c; cs[k++] = 1; ts[k] = cm; cs[k++] = 2; }
+  if (r == L - 1) { ts[k] = Tu - N + c; cs[k++] = 0; ts[k] = Tu - N + cm; cs[k++] = 2; }
+  if (r <= L - 2) {
+    ts[k] = mid(r, c, 0); cs[k++] = 0;
+    ts[k] = mid(r, c, 1); cs[k++] = 0;
+    ts[k] = mid(r, cm, 0); cs[k++] = 2;
+  }
+  if (r >= 2) {
+    ts[k] = mid(r - 1, cm, 0); cs[k++] = 1;
+    ts[k] = mid(r - 1, cm, 1); cs[k++] = 2;
+    ts[k] = mid(r - 1, c, 1); cs[k++] = 1;
+  }
+  constexpr int R = 4 * NC;
+  double acc[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) acc[q] = 0.0;
+  for (int e = 0; e < k; ++e) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) acc[q] += tmp[(R * cs[e] + q) * Tu + ts[e]];
+  }
+  const int64_t Nu = M.Nu;
+#pragma unroll
+  for (int q = 0; q < R; ++q) vb[q * Nu + v] += acc[q];
+}
+
+// pole p (0 north: corner 0 of triangles 0..N-1; 1 south: corner 1 of the
+// last N triangles), one block each
+template <int NC>
+__global__ __launch_bounds__(256) void k_mwss_assemble_adj_poles(cmm_mwss_mesh M, const double* __restrict__ tmp,
+                                                                 double* __restrict__ vb) {
+  constexpr int R = 4 * NC;
+  __shared__ double red[256];
+  const int p = blockIdx.x;
+  const int N = M.N;
+  const int64_t Tu = M.Tu;
+  const int64_t t0 = p == 0 ? 0 : Tu - N;
+  const int corner = p == 0 ? 0 : 1;
+  const int64_t v = p == 0 ? 0 : M.Nu - 1;
+  for (int q = 0; q < R; ++q) {
+    double a = 0.0;
+    for (int c = threadIdx.x; c < N; c += blockDim.x) a += tmp[(R * corner + q) * Tu + t0 + c];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) vb[q * M.Nu + v] += red[0];
+    __syncthreads();
   }
 }
 
@@ -764,9 +889,34 @@ __device__ __forceinline__ VLoc vel_locate(const cmm_mwss_mesh& M, d3 x) {
 // summed over the 4 lanes of a quad first when the whole quad sits in one
 // sub-triangle: the four stencil feet of a vertex are 1e-5 apart) and
 // returns xbar = (dv/dx)^T vbar.  Every lane of the quad must call it.
-__device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, double* const* lb) {
+__device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, double* const* lb,
+                                      const ScatterWork* sw, int64_t slot) {
   const VLoc r = vel_locate(V.mesh, x);
   const LevelSet& ls = V.ls[stage];
+  if (sw) {
+    // record for the sorted scatter-add (class = stage, weights applied
+    // there); slot < 0: a lane past the end, nothing to write
+    if (slot >= 0) {
+      if (r.ok)
+        scatter_put(*sw, slot, (uint32_t)(r.tri * 6 + r.s), vb, r.cb, stage);
+      else
+        scatter_empty(*sw, slot);
+    }
+    if (!r.ok) return d3{0, 0, 0};
+    d3 cbar = d3{0, 0, 0};
+    const double vv[3] = {vb.x, vb.y, vb.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (k < ls.nlev) {
+        const double w = ls.w[k];
+        const double* p = ls.p[k] + (r.tri * 6 + r.s) * 18;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cbar = add(cbar, scl(w * vv[c], bb6_db(r.cb, p + 6 * c)));
+      }
+    }
+    const d3 bbar = sub_bary_T(r.s, cbar, r.se, r.se, r.se, r.s4);
+    return add(add(scl(bbar.x, r.w[0]), scl(bbar.y, r.w[1])), scl(bbar.z, r.w[2]));
+  }
   const int lane = threadIdx.x & 63;
   const int64_t key = r.ok ? r.tri * 6 + r.s : -1 - (int64_t)lane;
   const int64_t k1 = __shfl_xor(key, 1), k2 = __shfl_xor(key, 2);
@@ -801,9 +951,8 @@ __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, d
 #pragma unroll
         for (int c = 0; c < 3; ++c) cbar = add(cbar, scl(w * vv[c], bb6_db(r.cb, p + 6 * c)));
         if (issue && w != 0.0) {
-          double* dst = lb[k] + (r.tri * 6 + r.s) * 18;
 #pragma unroll
-          for (int e = 0; e < 18; ++e) unsafeAtomicAdd(dst + e, w * cv[e]);
+          for (int e = 0; e < 18; ++e) unsafeAtomicAdd(lb[k] + adj_index(e, r.s, r.tri, V.mesh.Tu), w * cv[e]);
         }
       }
     }
@@ -825,7 +974,8 @@ struct LevelBars {
 __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ, double dt,
                                                             const double* __restrict__ xs, int64_t xs_row, int64_t nv,
                                                             const double* __restrict__ yb, int64_t yb_row,
-                                                            LevelBars LB, double* __restrict__ xb, int64_t xb_row) {
+                                                            LevelBars LB, double* __restrict__ xb, int64_t xb_row,
+                                                            ScatterWork swk, int rec) {
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // the last quad may reach past the end: clamp (its lanes then repeat a
   // valid point and add nothing)
@@ -838,12 +988,19 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
   const double* bsrc = yb + (int64_t)k * nv + v;
   d3 ybar = live ? d3{bsrc[0], bsrc[yb_row], bsrc[2 * yb_row]} : d3{0, 0, 0};
   double* const* lb = LB.p;
+  // record slots 4 i + stage (unused stages marked empty)
+  const ScatterWork* sw = rec ? &swk : nullptr;
+  const int64_t s0 = 4 * i0;
+  if (rec && live) {
+    const int used = integ == CMM_INT_EULER ? 1 : (integ == CMM_INT_IMPROVED_EULER ? 2 : 4);
+    for (int st = used; st < 4; ++st) scatter_empty(swk, s0 + st);
+  }
   d3 xbar;
   if (integ == CMM_INT_EULER) {
     const d3 v1 = vel_eval(V, 0, xn);
     const d3 z = sub(xn, scl(dt, v1));
     const d3 zb = unit_adj(z, ybar);
-    xbar = add(zb, vel_adj(V, 0, xn, scl(-dt, zb), lb));
+    xbar = add(zb, vel_adj(V, 0, xn, scl(-dt, zb), lb, sw, live ? s0 : -1));
   } else if (integ == CMM_INT_IMPROVED_EULER) {
     const d3 v1 = vel_eval(V, 0, xn);
     const d3 z1 = sub(xn, scl(dt, v1));
@@ -853,10 +1010,10 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
     const d3 zb = unit_adj(z, ybar);
     const d3 v2b = scl(-dt / 2, zb);
     d3 v1b = v2b;
-    const d3 s1b = vel_adj(V, 1, s1, v2b, lb);
+    const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, live ? s0 + 1 : -1);
     const d3 z1b = unit_adj(z1, s1b);
     v1b = add(v1b, scl(-dt, z1b));
-    xbar = add(add(zb, z1b), vel_adj(V, 0, xn, v1b, lb));
+    xbar = add(add(zb, z1b), vel_adj(V, 0, xn, v1b, lb, sw, live ? s0 : -1));
   } else {
   const d3 v1 = vel_eval(V, 0, xn);
   const d3 z1 = d3{xn.x - dt * v1.x / 2, xn.y - dt * v1.y / 2, xn.z - dt * v1.z / 2};
@@ -874,16 +1031,16 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
   const d3 ab = scl(-dt / 6, unit_adj(z4, ybar));
   d3 v1b = ab, v2b = scl(2, ab), v3b = scl(2, ab);
   const d3 v4b = ab;
-  const d3 s3b = vel_adj(V, 3, s3, v4b, lb);
+  const d3 s3b = vel_adj(V, 3, s3, v4b, lb, sw, live ? s0 + 3 : -1);
   v3b = add(v3b, scl(-dt, unit_adj(z3, s3b)));
-  const d3 s2b = vel_adj(V, 2, s2, v3b, lb);
+  const d3 s2b = vel_adj(V, 2, s2, v3b, lb, sw, live ? s0 + 2 : -1);
   v2b = add(v2b, scl(-dt / 2, unit_adj(z2, s2b)));
-  const d3 s1b = vel_adj(V, 1, s1, v2b, lb);
+  const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, live ? s0 + 1 : -1);
   const d3 z1b = unit_adj(z1, s1b);
   v1b = add(v1b, scl(-dt / 2, z1b));
   // z4 = xn - ..., z3 = xn - ..., z2 = xn - ..., z1 = xn - ... all carry xn
   const d3 z4b = unit_adj(z4, ybar), z3b = unit_adj(z3, s3b), z2b = unit_adj(z2, s2b);
-  xbar = add(add(add(z4b, z3b), add(z2b, z1b)), vel_adj(V, 0, xn, v1b, lb));
+  xbar = add(add(add(z4b, z3b), add(z2b, z1b)), vel_adj(V, 0, xn, v1b, lb, sw, live ? s0 : -1));
   }
   if (xb && live) {
     double* dst = xb + (int64_t)k * nv + v;
@@ -900,7 +1057,7 @@ __global__ __launch_bounds__(256) void k_sample_vorticity_adj(cmm_mwss_mesh W, c
                                                               const double* __restrict__ X, int64_t n, int64_t sp,
                                                               int64_t sc, const double* __restrict__ wbar, double rot,
                                                               double* __restrict__ xbar, int64_t bsp, int64_t bsc,
-                                                              double* __restrict__ pbar) {
+                                                              double* __restrict__ pbar, ScatterWork sw, int rec) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const d3 x = load_pt(X, i, sp, sc);
@@ -910,13 +1067,18 @@ __global__ __launch_bounds__(256) void k_sample_vorticity_adj(cmm_mwss_mesh W, c
   const double wb = wbar[i];
   const VLoc r = vel_locate(W, q);
   d3 gq = d3{0, 0, 0};
+  if (rec) {
+    if (r.ok)
+      scatter_put(sw, i, (uint32_t)(r.tri * 6 + r.s), d3{wb, 0.0, 0.0}, r.cb, 0);
+    else
+      scatter_empty(sw, i);
+  }
   if (r.ok) {
     const double* p = wpacked + (r.tri * 6 + r.s) * 6;
-    if (pbar) {
+    if (pbar && !rec) {
       double bas[6];
       bb6_basis(r.cb, bas);
-      double* dst = pbar + (r.tri * 6 + r.s) * 6;
-      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(dst + j, wb * bas[j]);
+      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(pbar + adj_index(j, r.s, r.tri, W.Tu), wb * bas[j]);
     }
     const d3 bb = sub_bary_T(r.s, bb6_db(r.cb, p), r.se, r.se, r.se, r.s4);
     gq = add(add(scl(bb.x, r.w[0]), scl(bb.y, r.w[1])), scl(bb.z, r.w[2]));
@@ -1106,9 +1268,26 @@ extern "C" int cmm_mwss_scalar_jets_adjoint(const cmm_mwss_mesh* m, const double
 }
 
 extern "C" int cmm_mwss_assemble_adjoint(const cmm_mwss_mesh* m, const double* packed_bar, int ncomp,
-                                         double* vals_bar, void* stream) {
+                                         double* vals_bar, void* work, int64_t work_bytes, void* stream) {
   if (!m || !packed_bar || !vals_bar || (ncomp != 1 && ncomp != 3))
     return fail(CMM_E_ARG, "cmm_mwss_assemble_adjoint: bad arguments");
+  if (work) {
+    if (work_bytes < 96 * ncomp * m->Tu) return fail(CMM_E_ARG, "cmm_mwss_assemble_adjoint: workspace too small");
+    double* tmp = static_cast<double*>(work);
+    hipStream_t st = as_stream(stream);
+    const int64_t ni = m->Nu - 2;
+    if (ncomp == 3) {
+      hipLaunchKernelGGL(k_mwss_assemble_adj_corners<3>, dim3(grid_for(m->Tu, 256)), dim3(256), 0, st, *m, packed_bar, tmp);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<3>, dim3(grid_for(ni, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_poles<3>, dim3(2), dim3(256), 0, st, *m, tmp, vals_bar);
+    } else {
+      hipLaunchKernelGGL(k_mwss_assemble_adj_corners<1>, dim3(grid_for(m->Tu, 256)), dim3(256), 0, st, *m, packed_bar, tmp);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<1>, dim3(grid_for(ni, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_poles<1>, dim3(2), dim3(256), 0, st, *m, tmp, vals_bar);
+    }
+    CMM_LAUNCH_CHECK("cmm_mwss_assemble_adjoint");
+    return CMM_OK;
+  }
   if (ncomp == 3)
     hipLaunchKernelGGL(k_mwss_assemble_adj<3>, dim3(grid_for(m->Tu, 256)), dim3(256), 0, as_stream(stream), *m,
                        packed_bar, vals_bar);
@@ -1121,8 +1300,8 @@ extern "C" int cmm_mwss_assemble_adjoint(const cmm_mwss_mesh* m, const double* p
 
 extern "C" int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integ, double dt, const double* st,
                                           int64_t st_row, int64_t nv, const double* yst_bar, int64_t yb_row,
-                                          double* const* level_bars, double* st_bar, int64_t sb_row,
-                                          void* stream) {
+                                          double* const* level_bars, double* st_bar, int64_t sb_row, void* work,
+                                          int64_t work_bytes, void* stream) {
   if (!vel || !st || !yst_bar || !level_bars || nv < 1 || st_row < 4 * nv || yb_row < 4 * nv ||
       (st_bar && sb_row < 4 * nv) ||
       (integ != CMM_INT_EULER && integ != CMM_INT_IMPROVED_EULER && integ != CMM_INT_RK4))
@@ -1136,21 +1315,56 @@ extern "C" int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integ, do
     LB.p[k] = k < vel->nlev ? level_bars[k] : nullptr;
     if (k < vel->nlev && !LB.p[k]) return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint: null level adjoint");
   }
-  hipLaunchKernelGGL(k_advect_stencil_adj, dim3(grid_for(4 * nv, 128)), dim3(128), 0, as_stream(stream), V, integ,
-                     dt, st, st_row, nv, yst_bar, yb_row, LB, st_bar, sb_row);
+  ScatterWork sw;
+  const int64_t nrec = 16 * nv;  // 4 stages per stencil foot
+  if (work) {
+    rc = scatter_layout(work, work_bytes, nrec, vel->mesh->Tu * 6, sw);
+    if (rc) return rc;
+  }
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_advect_stencil_adj, dim3(grid_for(4 * nv, 128)), dim3(128), 0, s, V, integ, dt, st, st_row, nv,
+                     yst_bar, yb_row, LB, st_bar, sb_row, sw, work ? 1 : 0);
   CMM_LAUNCH_CHECK("cmm_advect_stencil_adjoint");
+  if (work) {
+    ScatterTargets t;
+    t.mode = 1;
+    t.nc = 3;
+    t.nout = vel->nlev;
+    t.ntri = vel->mesh->Tu;
+    for (int k = 0; k < vel->nlev; ++k) {
+      t.T[k] = level_bars[k];
+      for (int g = 0; g < 4; ++g) t.W[g][k] = vel->stage_w[g][k];
+    }
+    return scatter_reduce(sw, nrec, t, s);
+  }
   return CMM_OK;
 }
 
 extern "C" int cmm_sample_vorticity_adjoint(const cmm_mwss_mesh* wmesh, const double* wpacked, const double* X,
                                             int64_t n, int64_t sp, int64_t sc, const double* wbar, double rot_rate,
-                                            double* xbar, int64_t bsp, int64_t bsc, double* wpacked_bar,
-                                            void* stream) {
+                                            double* xbar, int64_t bsp, int64_t bsc, double* wpacked_bar, void* work,
+                                            int64_t work_bytes, void* stream) {
   if (!wmesh || !wpacked || n < 0 || (n > 0 && (!X || !wbar)))
     return fail(CMM_E_ARG, "cmm_sample_vorticity_adjoint: bad arguments");
   if (n == 0 || (!xbar && !wpacked_bar)) return CMM_OK;
-  hipLaunchKernelGGL(k_sample_vorticity_adj, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *wmesh, wpacked,
-                     X, n, sp, sc, wbar, rot_rate, xbar, bsp, bsc, wpacked_bar);
+  ScatterWork sw;
+  const bool rec = work && wpacked_bar;
+  if (rec) {
+    int rc = scatter_layout(work, work_bytes, n, wmesh->Tu * 6, sw);
+    if (rc) return rc;
+  }
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_sample_vorticity_adj, dim3(grid_for(n, 256)), dim3(256), 0, s, *wmesh, wpacked, X, n, sp, sc,
+                     wbar, rot_rate, xbar, bsp, bsc, wpacked_bar, sw, rec ? 1 : 0);
   CMM_LAUNCH_CHECK("cmm_sample_vorticity_adjoint");
+  if (rec) {
+    ScatterTargets t;
+    t.mode = 1;
+    t.nc = 1;
+    t.nout = 1;
+    t.ntri = wmesh->Tu;
+    t.T[0] = wpacked_bar;
+    return scatter_reduce(sw, n, t, s);
+  }
   return CMM_OK;
 }

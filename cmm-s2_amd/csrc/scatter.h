@@ -1,0 +1,81 @@
+// Sorted scatter-add of Bernstein-Bezier adjoint records (internal).
+//
+// The adjoints of spline evaluation at scattered points add
+//   a_c * basis_j(cb)   (c < NC components, j < 6 basis functions)
+// into the 6 NC coefficients of the located sub-triangle.  Device-wide FP64
+// atomics for that run at ~15 G/s here (they are performed beyond the
+// per-XCD L2), so the adjoint kernels write one record per (point, use)
+// instead: key = target block, a (3 doubles), cb (3 doubles), class.  A radix
+// sort groups equal keys and one thread per key adds its group with plain
+// stores (one writer per target block): deterministic, no atomics.
+#pragma once
+#include "common.h"
+
+namespace cmm {
+
+constexpr uint32_t kScatterEmpty = 0xffffffffu;
+
+// one record: 64 bytes, one cache line
+struct alignas(64) ScatterRec {
+  double a[3];
+  double cb[3];
+  int32_t cls;
+  int32_t pad[3];
+};
+
+struct ScatterWork {
+  uint32_t* keys = nullptr;    // (n) written by the producer; `empty` = unused slot
+  uint32_t* idx = nullptr;     // (n) slot index (written by the producer)
+  uint32_t* keys_s = nullptr;  // sorted
+  uint32_t* idx_s = nullptr;
+  ScatterRec* rec = nullptr;   // (n)
+  void* temp = nullptr;
+  size_t temp_bytes = 0;
+  int64_t n = 0;
+  uint32_t empty = kScatterEmpty;  // key of an unused slot: 1 + the largest key
+  int end_bit = 32;                // key bits the sort looks at
+};
+
+// where a group of records goes
+struct ScatterTargets {
+  // mode 0: key = class * span + block, target T[class] (classes < 8), one
+  //         output per key, weight 1
+  // mode 1: key = block, nout targets T[t] with weight W[record class][t]
+  int mode = 0;
+  int nc = 3;
+  int nout = 1;
+  int64_t span = 0;
+  int64_t ntri = 0;  // triangles of the target mesh (planar adjoint layout, adj_index); block = 6 t + s
+  double* T[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double W[4][3] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
+};
+
+// bytes of a workspace for n records (0 on error)
+int64_t scatter_bytes(int64_t n);
+// carve a caller workspace for n records with keys < key_range; fails if it
+// is too small
+int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range, ScatterWork& w);
+// sort the n records by key and add every group into its target
+int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hipStream_t st);
+
+__device__ inline void scatter_put(const ScatterWork& w, int64_t slot, uint32_t key, d3 a, d3 cb, int cls) {
+  w.keys[slot] = key;
+  w.idx[slot] = (uint32_t)slot;
+  ScatterRec r;
+  r.a[0] = a.x;
+  r.a[1] = a.y;
+  r.a[2] = a.z;
+  r.cb[0] = cb.x;
+  r.cb[1] = cb.y;
+  r.cb[2] = cb.z;
+  r.cls = cls;
+  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  w.rec[slot] = r;
+}
+
+__device__ inline void scatter_empty(const ScatterWork& w, int64_t slot) {
+  w.keys[slot] = w.empty;
+  w.idx[slot] = (uint32_t)slot;
+}
+
+}  // namespace cmm

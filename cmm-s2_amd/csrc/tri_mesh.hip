@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "common.h"
+#include "scatter.h"
 
 namespace cmm {
 
@@ -51,6 +52,8 @@ struct cmm_tri_mesh {
   double* verts = nullptr;
   int32_t* tris = nullptr;
   double* psgeom = nullptr;  // (kPSGeomLen, Ntri) Powell-Sabin geometry, structure of arrays
+  int32_t* vstart = nullptr; // (Nv + 1) incidence lists: vertex -> 3 t + corner
+  int32_t* vinc = nullptr;   // (3 Ntri)
   int device = 0;
 };
 
@@ -267,6 +270,14 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
     }
   }
   if (over.empty()) over.push_back(0);
+  // vertex -> (triangle, corner) incidence (transposed assembly)
+  std::vector<int32_t> vstart((size_t)nv + 1, 0), vinc((size_t)ntri * 3);
+  for (int64_t k = 0; k < ntri * 3; ++k) ++vstart[(size_t)tris[k] + 1];
+  for (int64_t v = 0; v < nv; ++v) vstart[v + 1] += vstart[v];
+  {
+    std::vector<int32_t> fill(vstart.begin(), vstart.end() - 1);
+    for (int64_t k = 0; k < ntri * 3; ++k) vinc[fill[tris[k]]++] = (int32_t)k;
+  }
 
   cmm_tri_mesh* m = new cmm_tri_mesh();
   m->nv = nv; m->ntri = ntri; m->R = R;
@@ -279,7 +290,9 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
   hipError_t e;
   if ((e = hipMalloc(&m->geom, bg)) != hipSuccess || (e = hipMalloc(&m->cells, bc)) != hipSuccess ||
       (e = hipMalloc(&m->over, bo)) != hipSuccess || (e = hipMalloc(&m->verts, bv)) != hipSuccess ||
-      (e = hipMalloc(&m->tris, bt)) != hipSuccess || (e = hipMalloc(&m->psgeom, bp)) != hipSuccess) {
+      (e = hipMalloc(&m->tris, bt)) != hipSuccess || (e = hipMalloc(&m->psgeom, bp)) != hipSuccess ||
+      (e = hipMalloc(&m->vstart, vstart.size() * 4)) != hipSuccess ||
+      (e = hipMalloc(&m->vinc, vinc.size() * 4)) != hipSuccess) {
     cmm_tri_mesh_destroy(m);
     return hip_check(e, "cmm_tri_mesh_create: hipMalloc");
   }
@@ -288,7 +301,9 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
       (e = hipMemcpy(m->over, over.data(), bo, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(m->verts, vertices, bv, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(m->tris, tris.data(), bt, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(m->psgeom, psg.data(), bp, hipMemcpyHostToDevice)) != hipSuccess) {
+      (e = hipMemcpy(m->psgeom, psg.data(), bp, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->vstart, vstart.data(), vstart.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->vinc, vinc.data(), vinc.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) {
     cmm_tri_mesh_destroy(m);
     return hip_check(e, "cmm_tri_mesh_create: hipMemcpy");
   }
@@ -304,6 +319,8 @@ extern "C" int cmm_tri_mesh_destroy(cmm_tri_mesh* m) {
   (void)hipFree(m->verts);
   (void)hipFree(m->tris);
   (void)hipFree(m->psgeom);
+  (void)hipFree(m->vstart);
+  (void)hipFree(m->vinc);
   delete m;
   return CMM_OK;
 }
@@ -673,18 +690,17 @@ __device__ inline void located_at(const double* __restrict__ geom, int t, d3 x, 
 
 // adjoint of the raw evaluation r = PS(b(x)) of one packed level at a located
 // point: adds rbar_c basis_j into pb (if non-null), returns xbar
-__device__ inline d3 eval_level3_adj(const double* __restrict__ packed, double* pb, const Located& L, d3 cb, int s,
-                                     d3 rb) {
+__device__ inline d3 eval_level3_adj(const double* __restrict__ packed, double* pb, int64_t ntri, const Located& L,
+                                     d3 cb, int s, d3 rb) {
   const double* p = packed + ((int64_t)L.tri * 6 + s) * 18;
   const double rv[3] = {rb.x, rb.y, rb.z};
   if (pb) {
     double bas[6];
     bb6_basis(cb, bas);
-    double* dst = pb + ((int64_t)L.tri * 6 + s) * 18;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(dst + 6 * c + j, rv[c] * bas[j]);
+      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(pb + adj_index(6 * c + j, s, L.tri, ntri), rv[c] * bas[j]);
   }
   d3 cbar = d3{0, 0, 0};
 #pragma unroll
@@ -695,10 +711,12 @@ __device__ inline d3 eval_level3_adj(const double* __restrict__ packed, double* 
 
 // adjoint of k_tri_compose: xbar of the output (n,3) -> coefficient adjoints
 // of every level (null entries skipped) and, if qb, the input adjoint
+// With rec (a workspace) the coefficient adjoints become records
+// (slot l n + i, key l 6 Ntri + 6 tri + s) for the sorted scatter-add.
 __global__ __launch_bounds__(256) void k_tri_compose_adj(cmm_tri_mesh M, LevelPtrs lv, LevelBarPtrs lb, int depth,
                                                          const double* __restrict__ q, int64_t n, int64_t sp,
                                                          int64_t sc, const double* __restrict__ ob,
-                                                         double* __restrict__ qb) {
+                                                         double* __restrict__ qb, ScatterWork sw, int rec) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   d3 xs[kAdjMaxDepth];
@@ -725,15 +743,27 @@ __global__ __launch_bounds__(256) void k_tri_compose_adj(cmm_tri_mesh M, LevelPt
     }
   }
   d3 xb = ok ? d3{ob[3 * i], ob[3 * i + 1], ob[3 * i + 2]} : d3{0, 0, 0};
+  const int64_t span = M.ntri * 6;
 #pragma unroll
   for (int l = kAdjMaxDepth - 1; l >= 0; --l) {
-    if (l < depth && ok) {
-      Located L;
-      located_at(M.geom, ts[l], xs[l], L);
-      const int s = ss[l];
-      const d3 cb = sub_bary(s, L.b0, L.b1, L.b2, L.s12, L.s23, L.s31, L.s4);
-      const d3 r = eval_level3(lv.p[l], L, d3{L.b0, L.b1, L.b2}, s);
-      xb = eval_level3_adj(lv.p[l], lb.p[l], L, cb, s, unit_adj(r, xb));
+    if (l < depth) {
+      if (ok) {
+        Located L;
+        located_at(M.geom, ts[l], xs[l], L);
+        const int s = ss[l];
+        const d3 cb = sub_bary(s, L.b0, L.b1, L.b2, L.s12, L.s23, L.s31, L.s4);
+        const d3 r = eval_level3(lv.p[l], L, d3{L.b0, L.b1, L.b2}, s);
+        const d3 rb = unit_adj(r, xb);
+        if (rec) {
+          if (lb.p[l])
+            scatter_put(sw, (int64_t)l * n + i, (uint32_t)(l * span + (int64_t)L.tri * 6 + s), rb, cb, l);
+          else
+            scatter_empty(sw, (int64_t)l * n + i);
+        }
+        xb = eval_level3_adj(lv.p[l], rec ? nullptr : lb.p[l], M.ntri, L, cb, s, rb);
+      } else if (rec) {
+        scatter_empty(sw, (int64_t)l * n + i);
+      }
     }
   }
   if (qb) {
@@ -752,7 +782,6 @@ __global__ __launch_bounds__(256) void k_tri_assemble_adj(cmm_tri_mesh M, const 
   PSGeom P;
   ps_unpack(M.psgeom + t, M.ntri, P);
   const int64_t Nv = M.nv;
-  const double* src = pb + t * 108;
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
     double cb[19];
@@ -761,7 +790,7 @@ __global__ __launch_bounds__(256) void k_tri_assemble_adj(cmm_tri_mesh M, const 
 #pragma unroll
     for (int ss = 0; ss < 6; ++ss)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += src[(ss * 3 + cc) * 6 + j];
+      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += pb[adj_index(6 * cc + j, ss, t, M.ntri)];
     double fb[3] = {0.0, 0.0, 0.0};
     d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
     ps_coeffs_adj(P, cb, fb, gb);
@@ -775,6 +804,62 @@ __global__ __launch_bounds__(256) void k_tri_assemble_adj(cmm_tri_mesh M, const 
   }
 }
 
+// Two-phase transpose of k_tri_assemble<3> without atomics.  Phase 1, one
+// thread per triangle: the jet adjoints of its three corners, corner i row r
+// at tmp[(12 i + r) Ntri + t] (rows in the jets order: c 4 + 0 = value of
+// component c, d 4 + 1 + c = its derivative along axis d).  Phase 2, one
+// thread per vertex: the sum over its incidence list, added into vb (the only
+// writer of that vertex).  Deterministic.
+__global__ __launch_bounds__(256) void k_tri_assemble_adj_corners(cmm_tri_mesh M, const double* __restrict__ pb,
+                                                                  double* __restrict__ tmp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M.ntri) return;
+  PSGeom P;
+  ps_unpack(M.psgeom + t, M.ntri, P);
+  const int64_t nt = M.ntri;
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    double cb[19];
+#pragma unroll
+    for (int k = 0; k < 19; ++k) cb[k] = 0.0;
+#pragma unroll
+    for (int ss = 0; ss < 6; ++ss)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += pb[adj_index(6 * cc + j, ss, t, M.ntri)];
+    double fb[3] = {0.0, 0.0, 0.0};
+    d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+    ps_coeffs_adj(P, cb, fb, gb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      tmp[(12 * i + cc * 4) * nt + t] = fb[i];
+      tmp[(12 * i + 0 * 4 + 1 + cc) * nt + t] = gb[i].x;
+      tmp[(12 * i + 1 * 4 + 1 + cc) * nt + t] = gb[i].y;
+      tmp[(12 * i + 2 * 4 + 1 + cc) * nt + t] = gb[i].z;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tri_assemble_adj_gather(cmm_tri_mesh M, const double* __restrict__ tmp,
+                                                                 double* __restrict__ vb) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= M.nv) return;
+  double acc[12];
+#pragma unroll
+  for (int r = 0; r < 12; ++r) acc[r] = 0.0;
+  const int64_t nt = M.ntri;
+  const int b = M.vstart[v], e = M.vstart[v + 1];
+  for (int k = b; k < e; ++k) {
+    const int tc = M.vinc[k];
+    const int64_t t = tc / 3;
+    const int i = tc - 3 * (int)t;
+#pragma unroll
+    for (int r = 0; r < 12; ++r) acc[r] += tmp[(12 * i + r) * nt + t];
+  }
+  const int64_t Nv = M.nv;
+#pragma unroll
+  for (int r = 0; r < 12; ++r) vb[r * Nv + v] += acc[r];
+}
+
 // adjoint of k_tri_stencil_jets: jet adjoints (3,4,Nv) -> stencil-feet
 // adjoints sfb (3, 4 Nv, overwrite) and, with a current map, its coefficient
 // adjoints curb (accumulate)
@@ -783,7 +868,8 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, co
                                                               const double* __restrict__ sf, int64_t srow,
                                                               const double* __restrict__ tanv, double eps,
                                                               const double* __restrict__ jb, double* __restrict__ sfb,
-                                                              int64_t sbrow, double* __restrict__ curb) {
+                                                              int64_t sbrow, double* __restrict__ curb, ScatterWork sw,
+                                                              int rec) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t Nv = M.nv;
   if (v >= Nv) return;
@@ -810,6 +896,8 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, co
     }
   }
   d3 Sb[4] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+  if (rec && !(ok && cur))
+    for (int k = 0; k < 4; ++k) scatter_empty(sw, 4 * v + k);
   if (ok) {
     const d3 mu = d3{(S[0].x + S[1].x + S[2].x + S[3].x) / 4, (S[0].y + S[1].y + S[2].y + S[3].y) / 4,
                      (S[0].z + S[1].z + S[2].z + S[3].z) / 4};
@@ -839,6 +927,7 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, co
       for (int e = 0; e < 18; ++e) acc[e] = 0.0;
       const double* p = cur + ((int64_t)L.tri * 6 + s) * 18;
       for (int k = 0; k < 4; ++k) {
+        if (rec) scatter_put(sw, 4 * v + k, (uint32_t)((int64_t)L.tri * 6 + s), Sb[k], cbk[k], 0);
         double bas[6];
         bb6_basis(cbk[k], bas);
         const double rv[3] = {Sb[k].x, Sb[k].y, Sb[k].z};
@@ -850,9 +939,8 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, co
         const d3 bb = sub_bary_T(s, cbar, L.s12, L.s23, L.s31, L.s4);
         Sb[k] = add(add(scl(bb.x, L.w0), scl(bb.y, L.w1)), scl(bb.z, L.w2));
       }
-      if (curb) {
-        double* dst = curb + ((int64_t)L.tri * 6 + s) * 18;
-        for (int e = 0; e < 18; ++e) unsafeAtomicAdd(dst + e, acc[e]);
+      if (curb && !rec) {
+        for (int e = 0; e < 18; ++e) unsafeAtomicAdd(curb + adj_index(e, s, L.tri, M.ntri), acc[e]);
       }
     }
   }
@@ -988,7 +1076,8 @@ extern "C" int cmm_tri_stencil_jets(const cmm_tri_mesh* m, const double* cur, co
 // ---- adjoints -------------------------------------------------------------
 extern "C" int cmm_tri_compose_adjoint(const cmm_tri_mesh* m, const double* const* levels,
                                        double* const* level_bars, int depth, const double* q, int64_t n,
-                                       int64_t sp, int64_t sc, const double* out_bar, double* q_bar, void* stream) {
+                                       int64_t sp, int64_t sc, const double* out_bar, double* q_bar, void* work,
+                                       int64_t work_bytes, void* stream) {
   if (!m || depth < 1 || n < 0 || !levels || !level_bars || (n > 0 && (!q || !out_bar)))
     return fail(CMM_E_ARG, "cmm_tri_compose_adjoint: bad arguments");
   if (depth > kAdjMaxDepth) return fail(CMM_E_RANGE, "cmm_tri_compose_adjoint: depth > 8");
@@ -1001,17 +1090,42 @@ extern "C" int cmm_tri_compose_adjoint(const cmm_tri_mesh* m, const double* cons
   }
   for (int l = depth; l < kAdjMaxDepth; ++l) lb.p[l] = nullptr;
   if (n == 0) return CMM_OK;
-  hipLaunchKernelGGL(k_tri_compose_adj, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, lv, lb, depth,
-                     q, n, sp, sc, out_bar, q_bar);
+  ScatterWork sw;
+  const int64_t nrec = (int64_t)depth * n;
+  if (work) {
+    int rc = scatter_layout(work, work_bytes, nrec, (int64_t)depth * m->ntri * 6, sw);
+    if (rc) return rc;
+  }
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_tri_compose_adj, dim3(grid_for(n, 256)), dim3(256), 0, st, *m, lv, lb, depth, q, n, sp, sc,
+                     out_bar, q_bar, sw, work ? 1 : 0);
   CMM_LAUNCH_CHECK("cmm_tri_compose_adjoint");
+  if (work) {
+    ScatterTargets t;
+    t.mode = 0;
+    t.nc = 3;
+    t.span = m->ntri * 6;
+    t.ntri = m->ntri;
+    for (int l = 0; l < depth; ++l) t.T[l] = level_bars[l];
+    return scatter_reduce(sw, nrec, t, st);
+  }
   return CMM_OK;
 }
 
 extern "C" int cmm_tri_assemble_adjoint(const cmm_tri_mesh* m, const double* packed_bar, double* vals_bar,
-                                        void* stream) {
+                                        void* work, int64_t work_bytes, void* stream) {
   if (!m || !packed_bar || !vals_bar) return fail(CMM_E_ARG, "cmm_tri_assemble_adjoint: bad arguments");
-  hipLaunchKernelGGL(k_tri_assemble_adj, dim3(grid_for(m->ntri, 256)), dim3(256), 0, as_stream(stream), *m,
-                     packed_bar, vals_bar);
+  hipStream_t st = as_stream(stream);
+  if (!work) {
+    hipLaunchKernelGGL(k_tri_assemble_adj, dim3(grid_for(m->ntri, 256)), dim3(256), 0, st, *m, packed_bar, vals_bar);
+    CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
+    return CMM_OK;
+  }
+  if (work_bytes < 36 * 8 * m->ntri) return fail(CMM_E_ARG, "cmm_tri_assemble_adjoint: workspace too small");
+  double* tmp = static_cast<double*>(work);
+  hipLaunchKernelGGL(k_tri_assemble_adj_corners, dim3(grid_for(m->ntri, 256)), dim3(256), 0, st, *m, packed_bar, tmp);
+  CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
+  hipLaunchKernelGGL(k_tri_assemble_adj_gather, dim3(grid_for(m->nv, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
   CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
   return CMM_OK;
 }
@@ -1019,12 +1133,28 @@ extern "C" int cmm_tri_assemble_adjoint(const cmm_tri_mesh* m, const double* pac
 extern "C" int cmm_tri_stencil_jets_adjoint(const cmm_tri_mesh* m, const double* cur, const double* yv,
                                             int64_t yrow, const double* sf, int64_t srow, const double* tanv,
                                             double eps, const double* jets_bar, double* sf_bar, int64_t sbrow,
-                                            double* cur_bar, void* stream) {
+                                            double* cur_bar, void* work, int64_t work_bytes, void* stream) {
   if (!m || !sf || !tanv || !jets_bar || !sf_bar || !(eps > 0) || (cur && !yv) || srow < 4 * m->nv ||
       sbrow < 4 * m->nv)
     return fail(CMM_E_ARG, "cmm_tri_stencil_jets_adjoint: bad arguments");
-  hipLaunchKernelGGL(k_tri_stencil_jets_adj, dim3(grid_for(m->nv, 128)), dim3(128), 0, as_stream(stream), *m, cur,
-                     yv, yrow, sf, srow, tanv, eps, jets_bar, sf_bar, sbrow, cur ? cur_bar : nullptr);
+  ScatterWork sw;
+  const bool rec = work && cur && cur_bar;
+  if (rec) {
+    int rc = scatter_layout(work, work_bytes, 4 * m->nv, m->ntri * 6, sw);
+    if (rc) return rc;
+  }
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_tri_stencil_jets_adj, dim3(grid_for(m->nv, 128)), dim3(128), 0, st, *m, cur, yv, yrow, sf,
+                     srow, tanv, eps, jets_bar, sf_bar, sbrow, cur ? cur_bar : nullptr, sw, rec ? 1 : 0);
   CMM_LAUNCH_CHECK("cmm_tri_stencil_jets_adjoint");
+  if (rec) {
+    ScatterTargets t;
+    t.mode = 1;
+    t.nc = 3;
+    t.nout = 1;
+    t.ntri = m->ntri;
+    t.T[0] = cur_bar;
+    return scatter_reduce(sw, 4 * m->nv, t, st);
+  }
   return CMM_OK;
 }

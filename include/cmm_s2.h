@@ -282,6 +282,18 @@ int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const double* wpa
 /* FP64 atomics (clear it first; summation order is not fixed); the other    */
 /* outputs are overwritten.                                                  */
 /* ------------------------------------------------------------------------ */
+/* Coefficient adjoints ("packed_bar", "level_bars", "cur_bar", ...) have the
+ * size of the packed array they belong to, (Ntri, 6, ncomp, 6), in a planar
+ * layout: element (triangle t, sub-triangle s, component c, basis j) at
+ * ((6 c + j) * 6 + s) * Ntri + t.
+ * The coefficient adjoints of the four scattering entry points below
+ * (composition, RK feet, stencil projection, vorticity sampling) either
+ * accumulate with FP64 atomics (work == NULL) or go through a sorted
+ * scatter-add in a caller workspace of cmm_scatter_workspace_bytes(records)
+ * bytes (records: depth * n for the composition, 16 nv for the RK feet,
+ * 4 Nv for the stencil projection, n for the sampling): deterministic and
+ * several times faster. */
+int cmm_scatter_workspace_bytes(int64_t records, int64_t* bytes);
 /* transpose of cmm_sht_inverse: fbar (nf, L+1, 2L) -> flm_bar [lm][nf].
  * work: cmm_sht_workspace_bytes(nf). */
 int cmm_sht_inverse_adjoint(const cmm_sht_plan* plan, const double* fbar, int nf, double* flm_bar,
@@ -303,12 +315,15 @@ int cmm_mwss_velocity_jets_adjoint(const cmm_mwss_mesh* mesh, const double* vals
 int cmm_mwss_scalar_jets_adjoint(const cmm_mwss_mesh* mesh, const double* vals_bar, double* fields_bar,
                                  void* stream);
 /* transpose of cmm_mwss_assemble (packed only): packed_bar -> vals_bar
- * (accumulates). */
+ * (accumulates).  With a workspace of 96 * ncomp * Tu bytes it runs as a
+ * per-triangle pass and a per-vertex gather (no atomics, deterministic);
+ * work == NULL uses atomics. */
 int cmm_mwss_assemble_adjoint(const cmm_mwss_mesh* mesh, const double* packed_bar, int ncomp,
-                              double* vals_bar, void* stream);
-/* transpose of cmm_tri_assemble (ncomp 3, packed only): accumulates vals_bar. */
+                              double* vals_bar, void* work, int64_t work_bytes, void* stream);
+/* transpose of cmm_tri_assemble (ncomp 3, packed only): accumulates vals_bar;
+ * workspace 288 * Ntri bytes for the gather form, or NULL for atomics. */
 int cmm_tri_assemble_adjoint(const cmm_tri_mesh* mesh, const double* packed_bar, double* vals_bar,
-                             void* stream);
+                             void* work, int64_t work_bytes, void* stream);
 /* adjoint of cmm_advect_stencil for the stencil feet (spline velocity;
  * Euler, improved Euler or RK4): yst_bar (3, 4 nv) -> the packed adjoints of
  * the nlev velocity levels (accumulate; level_bars[k] pairs with
@@ -316,28 +331,28 @@ int cmm_tri_assemble_adjoint(const cmm_tri_mesh* mesh, const double* packed_bar,
  * (3, 4 nv), overwritten. */
 int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integrator, double dt, const double* st,
                                int64_t st_row, int64_t nv, const double* yst_bar, int64_t yst_bar_row,
-                               double* const* level_bars, double* st_bar, int64_t st_bar_row,
-                               void* stream);
+                               double* const* level_bars, double* st_bar, int64_t st_bar_row, void* work,
+                               int64_t work_bytes, void* stream);
 /* adjoint of cmm_tri_stencil_jets: jets_bar (3,4,Nv) -> stencil-feet
  * adjoints sf_bar (3, 4 Nv) and, when cur != NULL, the current map's packed
  * adjoints cur_bar (accumulates). */
 int cmm_tri_stencil_jets_adjoint(const cmm_tri_mesh* mesh, const double* cur, const double* yv,
                                  int64_t yrow, const double* sf, int64_t srow, const double* tanv,
                                  double eps, const double* jets_bar, double* sf_bar, int64_t sf_bar_row,
-                                 double* cur_bar, void* stream);
+                                 double* cur_bar, void* work, int64_t work_bytes, void* stream);
 /* adjoint of cmm_tri_compose (depth <= 8): out_bar (n,3) -> packed adjoints
  * of the levels (accumulate; NULL entries skipped) and q_bar (n,3) or NULL. */
 int cmm_tri_compose_adjoint(const cmm_tri_mesh* mesh, const double* const* levels,
                             double* const* level_bars, int depth, const double* q, int64_t n,
                             int64_t q_stride_pt, int64_t q_stride_comp, const double* out_bar,
-                            double* q_bar, void* stream);
+                            double* q_bar, void* work, int64_t work_bytes, void* stream);
 /* adjoint of cmm_sample_vorticity for a spline vorticity: wbar (n) ->
  * x_bar (strided, may be NULL) and wpacked_bar (accumulates, may be NULL). */
 int cmm_sample_vorticity_adjoint(const cmm_mwss_mesh* wmesh, const double* wpacked, const double* X,
                                  int64_t n, int64_t x_stride_pt, int64_t x_stride_comp,
                                  const double* wbar, double rot_rate, double* x_bar,
                                  int64_t xb_stride_pt, int64_t xb_stride_comp, double* wpacked_bar,
-                                 void* stream);
+                                 void* work, int64_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,26 @@ def _close(a, b, tol):
     assert abs(a - b) <= tol * max(abs(a), abs(b), 1e-300), (a, b)
 
 
+def _planar(pk, nc):
+    """packed (Ntri, 6, nc, 6) [t][s][c][j] -> the planar adjoint layout
+    [c][j][s][t] of the coefficient adjoints"""
+    ntri = pk.shape[0]
+    return pk.reshape(ntri, 6, nc, 6).permute(2, 3, 1, 0).contiguous()
+
+
+def _work(records):
+    """(pointer, bytes) of a sorted scatter-add workspace, None = atomics."""
+    import ctypes
+    from cmm_s2 import _native as N
+    if records is None:
+        return None, 0
+    b = ctypes.c_int64(0)
+    N.call("cmm_scatter_workspace_bytes", int(records), ctypes.byref(b))
+    t = torch.empty(b.value, dtype=torch.uint8, device="cuda")
+    _work.keep = t
+    return t.data_ptr(), b.value
+
+
 @pytest.mark.parametrize("L", [16, 24, 64])
 def test_sht_transposes(L):
     from cmm_s2 import _native as N
@@ -91,18 +111,30 @@ def test_jets_and_assembly_transposes():
         pb = _rand(mu.ntri, 36 * nc, seed=20 + nc)
         pk = torch.empty(mu.ntri, 36 * nc, dtype=torch.float64, device="cuda")
         N.call("cmm_mwss_assemble", mu._h, vals.data_ptr(), nc, None, pk.data_ptr(), stream())
-        vb = torch.zeros_like(vals)
-        N.call("cmm_mwss_assemble_adjoint", mu._h, pb.data_ptr(), nc, vb.data_ptr(), stream())
-        _close(_dot(pk, pb), _dot(vals, vb), 1e-12)
+        outs = []
+        for gather in (False, True):  # atomics, then the two-phase gather
+            vb = torch.zeros_like(vals)
+            ws = torch.empty(96 * nc * mu.ntri, dtype=torch.uint8, device="cuda") if gather else None
+            N.call("cmm_mwss_assemble_adjoint", mu._h, pb.data_ptr(), nc, vb.data_ptr(),
+                   None if ws is None else ws.data_ptr(), 0 if ws is None else ws.numel(), stream())
+            _close(_dot(_planar(pk, nc), pb), _dot(vals, vb), 1e-12)
+            outs.append(vb)
+        assert torch.allclose(outs[0], outs[1], rtol=1e-12, atol=1e-12)
     ico = mf.icosahedral_mesh(3)
     mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
     vals = _rand(12, mesh.nv, seed=31)
     pb = _rand(mesh.ntri, 108, seed=32)
     pk = torch.empty(mesh.ntri, 108, dtype=torch.float64, device="cuda")
     N.call("cmm_tri_assemble", mesh._h, vals.data_ptr(), 3, None, pk.data_ptr(), stream())
-    vb = torch.zeros_like(vals)
-    N.call("cmm_tri_assemble_adjoint", mesh._h, pb.data_ptr(), vb.data_ptr(), stream())
-    _close(_dot(pk, pb), _dot(vals, vb), 1e-12)
+    outs = []
+    for gather in (False, True):
+        vb = torch.zeros_like(vals)
+        ws = torch.empty(288 * mesh.ntri, dtype=torch.uint8, device="cuda") if gather else None
+        N.call("cmm_tri_assemble_adjoint", mesh._h, pb.data_ptr(), vb.data_ptr(),
+               None if ws is None else ws.data_ptr(), 0 if ws is None else ws.numel(), stream())
+        _close(_dot(_planar(pk, 3), pb), _dot(vals, vb), 1e-12)
+        outs.append(vb)
+    assert torch.allclose(outs[0], outs[1], rtol=1e-12, atol=1e-12)
 
 
 def _ico(level):
@@ -123,11 +155,12 @@ def _jets_bar(mesh, pb):
     from cmm_s2 import _native as N
     from cmm_s2.core._backend import stream
     jb = torch.zeros(3, 4, mesh.nv, dtype=torch.float64, device="cuda")
-    N.call("cmm_tri_assemble_adjoint", mesh._h, pb.data_ptr(), jb.data_ptr(), stream())
+    N.call("cmm_tri_assemble_adjoint", mesh._h, pb.data_ptr(), jb.data_ptr(), None, 0, stream())
     return jb
 
 
-def test_compose_adjoint_fd():
+@pytest.mark.parametrize("scatter", ["atomics", "sorted"])
+def test_compose_adjoint_fd(scatter):
     """d/dh <obar, compose(levels + h delta)> over the jets of every level."""
     from cmm_s2 import _native as N
     from cmm_s2.core._backend import stream
@@ -150,9 +183,10 @@ def test_compose_adjoint_fd():
     lv = [_packed(mesh, j) for j in jets]
     lb = [torch.zeros_like(p) for p in lv]
     qb = torch.empty_like(q)
+    wp, wb = _work(None if scatter == "atomics" else depth * 20000)
     N.call("cmm_tri_compose_adjoint", mesh._h, N.ptr_array([p.data_ptr() for p in lv]),
            N.ptr_array([p.data_ptr() for p in lb]), depth, q.data_ptr(), 20000, 3, 1, ob.data_ptr(),
-           qb.data_ptr(), stream())
+           qb.data_ptr(), wp, wb, stream())
     ad = sum(_dot(_jets_bar(mesh, lb[k]), dirs[k]) for k in range(depth))
     h = 1e-6
     fd = (J(h) - J(-h)) / (2 * h)
@@ -164,7 +198,8 @@ def test_compose_adjoint_fd():
     _close(_dot(qb, t), fdq, 1e-6)
 
 
-def test_sample_vorticity_adjoint_fd():
+@pytest.mark.parametrize("scatter", ["atomics", "sorted"])
+def test_sample_vorticity_adjoint_fd(scatter):
     from cmm_s2 import _native as N
     from cmm_s2.core._backend import stream
     from cmm_s2.core import mesh_functions as mf, dynamical_fields as df, spherical_harmonic_tools as st
@@ -181,8 +216,9 @@ def test_sample_vorticity_adjoint_fd():
     wb = _rand(n, seed=5)
     xb = torch.empty_like(X)
     pb = torch.zeros_like(w0.packed)
+    wp, wsz = _work(None if scatter == "atomics" else n)
     N.call("cmm_sample_vorticity_adjoint", mu._h, w0.packed.data_ptr(), X.data_ptr(), n, 3, 1, wb.data_ptr(), rot,
-           xb.data_ptr(), 3, 1, pb.data_ptr(), stream())
+           xb.data_ptr(), 3, 1, pb.data_ptr(), wp, wsz, stream())
     h = 1e-6
     d = _rand(n, 3, seed=6)
     f = lambda Y: _dot(smp(Y, cos_grid=cosg, ring_len=100), wb)
@@ -193,7 +229,7 @@ def test_sample_vorticity_adjoint_fd():
     w0.packed.copy_(base + dp)
     f1 = f(X)
     w0.packed.copy_(base)
-    _close(_dot(pb, dp), f1 - f(X), 1e-9)
+    _close(_dot(pb, _planar(dp, 1)), f1 - f(X), 1e-9)
 
 
 def _spline_velocity(L, seed):
@@ -218,8 +254,9 @@ def _spline_velocity(L, seed):
     return mu, lv, vals
 
 
+@pytest.mark.parametrize("scatter", ["atomics", "sorted"])
 @pytest.mark.parametrize("integ", ["euler", "improved_euler", "rk4"])
-def test_advect_stencil_adjoint_fd(integ):
+def test_advect_stencil_adjoint_fd(integ, scatter):
     from cmm_s2 import _native as N
     from cmm_s2.core._backend import stream
     from cmm_s2.core.evolution_functions import _STAGES
@@ -256,13 +293,15 @@ def test_advect_stencil_adjoint_fd(integ):
         return _dot(ys, yb)
 
     lb = [torch.zeros_like(p) for p in lv]
+    xb = torch.empty(3, 4 * nv, dtype=torch.float64, device="cuda")
+    wp, wsz = _work(None if scatter == "atomics" else 16 * nv)
     N.call("cmm_advect_stencil_adjoint", desc(lv), kind, dt, xs.data_ptr(), xs.stride(0), nv, yb.data_ptr(),
-           yb.stride(0), N.ptr_array([p.data_ptr() for p in lb]), None, 0, stream())
+           yb.stride(0), N.ptr_array([p.data_ptr() for p in lb]), xb.data_ptr(), xb.stride(0), wp, wsz, stream())
     dv = [_rand(12, mu.nv, seed=80 + k) for k in range(3)]
     ad = 0.0
     for k in range(nvs):
         vb = torch.zeros_like(vals[k])
-        N.call("cmm_mwss_assemble_adjoint", mu._h, lb[k].data_ptr(), 3, vb.data_ptr(), stream())
+        N.call("cmm_mwss_assemble_adjoint", mu._h, lb[k].data_ptr(), 3, vb.data_ptr(), None, 0, stream())
         ad += _dot(vb, dv[k])
 
     def perturbed(h):
@@ -277,10 +316,22 @@ def test_advect_stencil_adjoint_fd(integ):
     h = 1e-6
     fd = (J(perturbed(h)) - J(perturbed(-h))) / (2 * h)
     _close(ad, fd, 1e-6)
+    # start-point adjoint along a smooth direction
+    dx = torch.stack([torch.cos(xs[2]), torch.sin(2 * xs[0]), torch.cos(xs[1])]).contiguous()
+
+    def Jx(x):
+        ys = torch.empty(3, 4 * nv, dtype=torch.float64, device="cuda")
+        N.call("cmm_advect_stencil", desc(lv), kind, dt, None, 0, x.data_ptr(), x.stride(0), nv, None, 0,
+               ys.data_ptr(), ys.stride(0), stream())
+        return _dot(ys, yb)
+
+    h = 1e-6
+    _close(_dot(xb, dx), (Jx((xs + h * dx).contiguous()) - Jx((xs - h * dx).contiguous())) / (2 * h), 1e-6)
 
 
+@pytest.mark.parametrize("scatter", ["atomics", "sorted"])
 @pytest.mark.parametrize("identity", [True, False])
-def test_stencil_jets_adjoint_fd(identity):
+def test_stencil_jets_adjoint_fd(identity, scatter):
     from cmm_s2 import _native as N
     from cmm_s2.core._backend import stream
     mesh = _ico(4)
@@ -307,9 +358,10 @@ def test_stencil_jets_adjoint_fd(identity):
     pk = _packed(mesh, cur_j)
     sb = torch.empty_like(ys)
     curb = torch.zeros_like(pk)
+    wp, wsz = _work(None if scatter == "atomics" else 4 * nv)
     N.call("cmm_tri_stencil_jets_adjoint", mesh._h, None if identity else pk.data_ptr(), yv.data_ptr(), yv.stride(0),
            ys.data_ptr(), ys.stride(0), mesh.d_tan.data_ptr(), eps, jb.data_ptr(), sb.data_ptr(), sb.stride(0),
-           curb.data_ptr(), stream())
+           curb.data_ptr(), wp, wsz, stream())
     # a smooth direction for the feet (a per-point random one would be
     # amplified by the 1 / (4 eps) differences); the jets are quadratic in the
     # feet up to the normalised mean, so a central difference with h = 1e-5
