@@ -373,10 +373,10 @@ class MisfitProblem(object):
     def _sample_and_compose_adj(self, chain, X, wbar, rot):
         n = self.n
         xbar = B.empty(n, 3) if chain else None
-        wp, wb = self._scatter(n)
+        # atomics (wave-summed at the pole rings): cheaper than a sort here
         N.call("cmm_sample_vorticity_adjoint", self.mesh_w._h, self.w0.packed.data_ptr(), X.data_ptr(), n,
                X.stride(0), X.stride(1), wbar.data_ptr(), float(rot), B.ptr(xbar), 3, 1, self.lam_w.data_ptr(),
-               wp, wb, B.stream())
+               None, 0, B.stream())
         if chain:
             lv = N.ptr_array([self.maps[i].data_ptr() for i in chain])
             lb = N.ptr_array([self._lam_map(i).data_ptr() for i in chain])
@@ -431,17 +431,16 @@ class MisfitProblem(object):
                 sb = B.empty(3, 4 * nv)
                 cur = None if src is None else self.maps[src].data_ptr()
                 curb = None if src is None else self._lam_map(src).data_ptr()
-                wp, wb = self._scatter(4 * nv)
+                # atomics: the four feet of a vertex are summed per thread first
                 N.call("cmm_tri_stencil_jets_adjoint", m._h, cur, yv.data_ptr(), yv.stride(0), ys.data_ptr(),
                        ys.stride(0), m.d_tan.data_ptr(), self.EPS, jb.data_ptr(), sb.data_ptr(), sb.stride(0),
-                       curb, wp, wb, B.stream())
+                       curb, None, 0, B.stream())
                 desc = self._desc(V, ikind, t)
                 lbs = N.ptr_array([self._lam_vel(V.ids[k]).data_ptr() for k in range(min(V.nVs, 3))])
                 xs = m.d_spts
-                # quad-aggregated atomics: faster than the sorted form for the
-                # four stencil feet of a vertex (they share sub-triangles)
+                wp, wb = self._scatter(16 * nv)
                 N.call("cmm_advect_stencil_adjoint", desc, ikind, float(self.dt), xs.data_ptr(), xs.stride(0), nv,
-                       sb.data_ptr(), sb.stride(0), lbs, None, 0, None, 0, B.stream())
+                       sb.data_ptr(), sb.stride(0), lbs, None, 0, wp, wb, B.stream())
                 if mid not in self.submaps:
                     self.maps.pop(mid, None)
 

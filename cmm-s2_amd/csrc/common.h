@@ -314,15 +314,6 @@ __host__ __device__ inline void ps_coeffs_adj(const PSGeom& P, const double cb_i
   }
 }
 
-// Adjoint coefficient layout (planar): element (triangle t, sub-triangle s,
-// component c, basis j) of an adjoint of a packed (Ntri, 6, NC, 6) array at
-// ((6 c + j) 6 + s) Ntri + t, so consecutive triangles are consecutive
-// addresses (coalesced transposed assembly) and one point's 6 NC atomic adds
-// go to 6 NC different cache lines.
-__host__ __device__ inline int64_t adj_index(int e, int s, int64_t t, int64_t ntri) {
-  return ((int64_t)e * 6 + s) * ntri + t;
-}
-
 // the 6 quadratic Bernstein-Bezier basis values of bb6
 __host__ __device__ inline void bb6_basis(d3 c, double b[6]) {
   b[0] = c.x * c.x;

@@ -695,6 +695,7 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj(cmm_mwss_mesh M, cons
     P.H[i] = d3{c * h.x - sn * h.y, sn * h.x + c * h.y, h.z};
   }
   const int64_t Nu = M.Nu;
+  const double* src = pb + t * (36 * NC);
 #pragma unroll
   for (int cc = 0; cc < NC; ++cc) {
     double cb[19];
@@ -703,7 +704,7 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj(cmm_mwss_mesh M, cons
 #pragma unroll
     for (int ss = 0; ss < 6; ++ss)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += pb[adj_index(6 * cc + j, ss, t, M.Tu)];
+      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += src[(ss * NC + cc) * 6 + j];
     double fb[3] = {0.0, 0.0, 0.0};
     d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
     ps_coeffs_adj(P, cb, fb, gb);
@@ -730,14 +731,18 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj(cmm_mwss_mesh M, cons
 // non-pole vertex: its (at most 6) incident (triangle, corner) pairs in closed
 // form, added into vb; one block per pole reduces the 2L cap triangles.
 template <int NC>
-__global__ __launch_bounds__(256) void k_mwss_assemble_adj_corners(cmm_mwss_mesh M, const double* __restrict__ pb,
-                                                                   double* __restrict__ tmp) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= M.Tu) return;
+__global__ __launch_bounds__(64) void k_mwss_assemble_adj_corners(cmm_mwss_mesh M, const double* __restrict__ pb,
+                                                                  double* __restrict__ tmp) {
+  // one wave per 64 triangles; each component's 36 adjoints per triangle are
+  // staged through LDS so the packed blocks are read coalesced
+  __shared__ double sh[64 * 37];
+  const int lane = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * 64, t = t0 + lane;
+  const bool live = t < M.Tu;
   const int N = M.N, Mr = M.M;
-  int row, col, half;
+  int row = 0, col = 0, half = 0;
   if (t < N) { row = 0; col = (int)t; half = 0; }
-  else if (t >= M.Tu - N) { row = Mr - 2; col = (int)(t - (M.Tu - N)); half = 0; }
+  else if (t >= M.Tu - N) { row = Mr - 2; col = (int)(live ? t - (M.Tu - N) : 0); half = 0; }
   else {
     const int64_t u = t - N;
     half = (int)(u % 2);
@@ -757,30 +762,39 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj_corners(cmm_mwss_mesh
   constexpr int R = 4 * NC;
 #pragma unroll
   for (int cc = 0; cc < NC; ++cc) {
-    double cb[19];
+    for (int q = lane; q < 64 * 36; q += 64) {
+      const int tt = q / 36, r = q - 36 * tt, ss = r / 6, j = r - 6 * ss;
+      const int64_t T = t0 + tt;
+      sh[tt * 37 + r] = T < Tu ? pb[T * (36 * NC) + (ss * NC + cc) * 6 + j] : 0.0;
+    }
+    __syncthreads();
+    if (live) {
+      double cb[19];
 #pragma unroll
-    for (int k = 0; k < 19; ++k) cb[k] = 0.0;
+      for (int k = 0; k < 19; ++k) cb[k] = 0.0;
 #pragma unroll
-    for (int ss = 0; ss < 6; ++ss)
+      for (int ss = 0; ss < 6; ++ss)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += pb[adj_index(6 * cc + j, ss, t, M.Tu)];
-    double fb[3] = {0.0, 0.0, 0.0};
-    d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
-    ps_coeffs_adj(P, cb, fb, gb);
+        for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += sh[lane * 37 + ss * 6 + j];
+      double fb[3] = {0.0, 0.0, 0.0};
+      d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+      ps_coeffs_adj(P, cb, fb, gb);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      if (NC == 1) {
-        tmp[(R * i + 0) * Tu + t] = fb[i];
-        tmp[(R * i + 1) * Tu + t] = gb[i].x;
-        tmp[(R * i + 2) * Tu + t] = gb[i].y;
-        tmp[(R * i + 3) * Tu + t] = gb[i].z;
-      } else {
-        tmp[(R * i + cc * 4) * Tu + t] = fb[i];
-        tmp[(R * i + 0 * 4 + 1 + cc) * Tu + t] = gb[i].x;
-        tmp[(R * i + 1 * 4 + 1 + cc) * Tu + t] = gb[i].y;
-        tmp[(R * i + 2 * 4 + 1 + cc) * Tu + t] = gb[i].z;
+      for (int i = 0; i < 3; ++i) {
+        if (NC == 1) {
+          tmp[(R * i + 0) * Tu + t] = fb[i];
+          tmp[(R * i + 1) * Tu + t] = gb[i].x;
+          tmp[(R * i + 2) * Tu + t] = gb[i].y;
+          tmp[(R * i + 3) * Tu + t] = gb[i].z;
+        } else {
+          tmp[(R * i + cc * 4) * Tu + t] = fb[i];
+          tmp[(R * i + 0 * 4 + 1 + cc) * Tu + t] = gb[i].x;
+          tmp[(R * i + 1 * 4 + 1 + cc) * Tu + t] = gb[i].y;
+          tmp[(R * i + 2 * 4 + 1 + cc) * Tu + t] = gb[i].z;
+        }
       }
     }
+    __syncthreads();
   }
 }
 
@@ -893,30 +907,6 @@ __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, d
                                       const ScatterWork* sw, int64_t slot) {
   const VLoc r = vel_locate(V.mesh, x);
   const LevelSet& ls = V.ls[stage];
-  if (sw) {
-    // record for the sorted scatter-add (class = stage, weights applied
-    // there); slot < 0: a lane past the end, nothing to write
-    if (slot >= 0) {
-      if (r.ok)
-        scatter_put(*sw, slot, (uint32_t)(r.tri * 6 + r.s), vb, r.cb, stage);
-      else
-        scatter_empty(*sw, slot);
-    }
-    if (!r.ok) return d3{0, 0, 0};
-    d3 cbar = d3{0, 0, 0};
-    const double vv[3] = {vb.x, vb.y, vb.z};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      if (k < ls.nlev) {
-        const double w = ls.w[k];
-        const double* p = ls.p[k] + (r.tri * 6 + r.s) * 18;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) cbar = add(cbar, scl(w * vv[c], bb6_db(r.cb, p + 6 * c)));
-      }
-    }
-    const d3 bbar = sub_bary_T(r.s, cbar, r.se, r.se, r.se, r.s4);
-    return add(add(scl(bbar.x, r.w[0]), scl(bbar.y, r.w[1])), scl(bbar.z, r.w[2]));
-  }
   const int lane = threadIdx.x & 63;
   const int64_t key = r.ok ? r.tri * 6 + r.s : -1 - (int64_t)lane;
   const int64_t k1 = __shfl_xor(key, 1), k2 = __shfl_xor(key, 2);
@@ -941,6 +931,16 @@ __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, d
     cv[e] = same ? t : cv[e];
   }
   const bool issue = r.ok && (!same || (lane & 3) == 0);
+  if (sw) {
+    // value record for the sorted scatter-add (class = stage; the stage
+    // weights are applied there); slot < 0: a lane past the end
+    if (slot >= 0) {
+      if (issue)
+        scatter_put_values(*sw, slot, (uint32_t)(r.tri * 6 + r.s), cv, stage);
+      else
+        scatter_empty(*sw, slot);
+    }
+  }
   d3 cbar = d3{0, 0, 0};
   if (r.ok) {
 #pragma unroll
@@ -950,9 +950,10 @@ __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, d
         const double* p = ls.p[k] + (r.tri * 6 + r.s) * 18;
 #pragma unroll
         for (int c = 0; c < 3; ++c) cbar = add(cbar, scl(w * vv[c], bb6_db(r.cb, p + 6 * c)));
-        if (issue && w != 0.0) {
+        if (!sw && issue && w != 0.0) {
+          double* dst = lb[k] + (r.tri * 6 + r.s) * 18;
 #pragma unroll
-          for (int e = 0; e < 18; ++e) unsafeAtomicAdd(lb[k] + adj_index(e, r.s, r.tri, V.mesh.Tu), w * cv[e]);
+          for (int e = 0; e < 18; ++e) unsafeAtomicAdd(dst + e, w * cv[e]);
         }
       }
     }
@@ -1073,13 +1074,31 @@ __global__ __launch_bounds__(256) void k_sample_vorticity_adj(cmm_mwss_mesh W, c
     else
       scatter_empty(sw, i);
   }
+  if (pbar && !rec) {
+    // atomics; a full wave on one sub-triangle (the 2L samples of a pole
+    // ring are one point) is summed first and added once
+    double bas[6];
+    bb6_basis(r.cb, bas);
+    const int64_t key = r.ok ? r.tri * 6 + r.s : -1 - (int64_t)(threadIdx.x & 63);
+    const bool full = __ballot(1) == ~0ull;
+    const int64_t k0 = __shfl(key, 0);
+    const bool uniform = full && __all(key == k0) && r.ok;
+    double v[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) v[j] = r.ok ? wb * bas[j] : 0.0;
+    if (uniform) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) v[j] += __shfl_xor(v[j], o);
+    }
+    if (r.ok && (!uniform || (threadIdx.x & 63) == 0)) {
+      double* dst = pbar + (r.tri * 6 + r.s) * 6;
+      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(dst + j, v[j]);
+    }
+  }
   if (r.ok) {
     const double* p = wpacked + (r.tri * 6 + r.s) * 6;
-    if (pbar && !rec) {
-      double bas[6];
-      bb6_basis(r.cb, bas);
-      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(pbar + adj_index(j, r.s, r.tri, W.Tu), wb * bas[j]);
-    }
     const d3 bb = sub_bary_T(r.s, bb6_db(r.cb, p), r.se, r.se, r.se, r.s4);
     gq = add(add(scl(bb.x, r.w[0]), scl(bb.y, r.w[1])), scl(bb.z, r.w[2]));
   }
@@ -1277,11 +1296,11 @@ extern "C" int cmm_mwss_assemble_adjoint(const cmm_mwss_mesh* m, const double* p
     hipStream_t st = as_stream(stream);
     const int64_t ni = m->Nu - 2;
     if (ncomp == 3) {
-      hipLaunchKernelGGL(k_mwss_assemble_adj_corners<3>, dim3(grid_for(m->Tu, 256)), dim3(256), 0, st, *m, packed_bar, tmp);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_corners<3>, dim3(grid_for(m->Tu, 64)), dim3(64), 0, st, *m, packed_bar, tmp);
       hipLaunchKernelGGL(k_mwss_assemble_adj_gather<3>, dim3(grid_for(ni, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
       hipLaunchKernelGGL(k_mwss_assemble_adj_poles<3>, dim3(2), dim3(256), 0, st, *m, tmp, vals_bar);
     } else {
-      hipLaunchKernelGGL(k_mwss_assemble_adj_corners<1>, dim3(grid_for(m->Tu, 256)), dim3(256), 0, st, *m, packed_bar, tmp);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_corners<1>, dim3(grid_for(m->Tu, 64)), dim3(64), 0, st, *m, packed_bar, tmp);
       hipLaunchKernelGGL(k_mwss_assemble_adj_gather<1>, dim3(grid_for(ni, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
       hipLaunchKernelGGL(k_mwss_assemble_adj_poles<1>, dim3(2), dim3(256), 0, st, *m, tmp, vals_bar);
     }
@@ -1327,10 +1346,9 @@ extern "C" int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integ, do
   CMM_LAUNCH_CHECK("cmm_advect_stencil_adjoint");
   if (work) {
     ScatterTargets t;
-    t.mode = 1;
+    t.mode = 2;
     t.nc = 3;
     t.nout = vel->nlev;
-    t.ntri = vel->mesh->Tu;
     for (int k = 0; k < vel->nlev; ++k) {
       t.T[k] = level_bars[k];
       for (int g = 0; g < 4; ++g) t.W[g][k] = vel->stage_w[g][k];
@@ -1362,7 +1380,6 @@ extern "C" int cmm_sample_vorticity_adjoint(const cmm_mwss_mesh* wmesh, const do
     t.mode = 1;
     t.nc = 1;
     t.nout = 1;
-    t.ntri = wmesh->Tu;
     t.T[0] = wpacked_bar;
     return scatter_reduce(sw, n, t, s);
   }

@@ -15,7 +15,7 @@ namespace cmm {
 
 constexpr uint32_t kScatterEmpty = 0xffffffffu;
 
-// one record: 64 bytes, one cache line
+// one record: 64 bytes, one cache line: the sums are a_c basis_j(cb)
 struct alignas(64) ScatterRec {
   double a[3];
   double cb[3];
@@ -23,12 +23,22 @@ struct alignas(64) ScatterRec {
   int32_t pad[3];
 };
 
+// a record that carries its 18 sums (NC = 3) directly: partial sums formed
+// before the scatter (e.g. over a quad of lanes)
+struct alignas(16) ScatterRecV {
+  double v[18];
+  int32_t cls;
+  int32_t pad[3];
+};
+constexpr int64_t kScatterRecBytes = sizeof(ScatterRecV);  // slot size (both kinds)
+
 struct ScatterWork {
   uint32_t* keys = nullptr;    // (n) written by the producer; `empty` = unused slot
   uint32_t* idx = nullptr;     // (n) slot index (written by the producer)
   uint32_t* keys_s = nullptr;  // sorted
   uint32_t* idx_s = nullptr;
-  ScatterRec* rec = nullptr;   // (n)
+  ScatterRec* rec = nullptr;   // (n) slots of kScatterRecBytes
+  ScatterRecV* recv = nullptr; // the same slots as value records
   void* temp = nullptr;
   size_t temp_bytes = 0;
   int64_t n = 0;
@@ -41,11 +51,11 @@ struct ScatterTargets {
   // mode 0: key = class * span + block, target T[class] (classes < 8), one
   //         output per key, weight 1
   // mode 1: key = block, nout targets T[t] with weight W[record class][t]
+  // mode 2: as mode 1 with value records (ScatterRecV, NC = 3)
   int mode = 0;
   int nc = 3;
   int nout = 1;
   int64_t span = 0;
-  int64_t ntri = 0;  // triangles of the target mesh (planar adjoint layout, adj_index); block = 6 t + s
   double* T[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double W[4][3] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
 };
@@ -71,6 +81,18 @@ __device__ inline void scatter_put(const ScatterWork& w, int64_t slot, uint32_t 
   r.cls = cls;
   r.pad[0] = r.pad[1] = r.pad[2] = 0;
   w.rec[slot] = r;
+}
+
+__device__ inline void scatter_put_values(const ScatterWork& w, int64_t slot, uint32_t key, const double v[18],
+                                          int cls) {
+  w.keys[slot] = key;
+  w.idx[slot] = (uint32_t)slot;
+  ScatterRecV r;
+#pragma unroll
+  for (int e = 0; e < 18; ++e) r.v[e] = v[e];
+  r.cls = cls;
+  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  w.recv[slot] = r;
 }
 
 __device__ inline void scatter_empty(const ScatterWork& w, int64_t slot) {

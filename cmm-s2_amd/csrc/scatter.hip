@@ -23,7 +23,7 @@ int64_t scatter_bytes(int64_t n) {
   if (n < 1 || n > (int64_t)0x7fffffff) return 0;
   const size_t tb = sort_temp_bytes(n);
   if (!tb) return 0;
-  return 4 * align256(4 * n) + align256(64 * n) + align256((int64_t)tb);
+  return 4 * align256(4 * n) + align256(kScatterRecBytes * n) + align256((int64_t)tb);
 }
 
 int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range, ScatterWork& w) {
@@ -44,7 +44,9 @@ int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range,
   w.idx = reinterpret_cast<uint32_t*>(take(4 * n));
   w.keys_s = reinterpret_cast<uint32_t*>(take(4 * n));
   w.idx_s = reinterpret_cast<uint32_t*>(take(4 * n));
-  w.rec = reinterpret_cast<ScatterRec*>(take(64 * n));
+  char* recs = take(kScatterRecBytes * n);
+  w.rec = reinterpret_cast<ScatterRec*>(recs);
+  w.recv = reinterpret_cast<ScatterRecV*>(recs);
   w.empty = (uint32_t)key_range;
   w.end_bit = 1;
   while (w.end_bit < 32 && ((int64_t)1 << w.end_bit) <= key_range) ++w.end_bit;
@@ -61,19 +63,21 @@ int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range,
 // block is never written both ways.
 constexpr int kChunk = 16;
 
-// add the 6 NC sums of block (triangle t, sub-triangle s) into a planar
-// adjoint array
+// add the 6 NC sums of one block into a packed-layout adjoint array
 template <int NC>
-__device__ inline void add_block(double* T, int64_t blk, int64_t ntri, const double* acc, bool atomic) {
-  const int64_t t = blk / 6;
-  const int s = (int)(blk - 6 * t);
+__device__ inline void add_block(double* base, const double* acc, bool atomic) {
+  if (atomic) {
 #pragma unroll
-  for (int e = 0; e < NC * 6; ++e) {
-    double* d = T + adj_index(e, s, t, ntri);
-    if (atomic)
-      unsafeAtomicAdd(d, acc[e]);
-    else
-      *d += acc[e];
+    for (int e = 0; e < NC * 6; ++e) unsafeAtomicAdd(base + e, acc[e]);
+  } else {
+    double2* dst = reinterpret_cast<double2*>(base);
+#pragma unroll
+    for (int e = 0; e < NC * 3; ++e) {
+      double2 v = dst[e];
+      v.x += acc[2 * e];
+      v.y += acc[2 * e + 1];
+      dst[e] = v;
+    }
   }
 }
 
@@ -94,6 +98,19 @@ __global__ __launch_bounds__(256) void k_scatter_reduce(ScatterWork w, int64_t n
   const int64_t stop = (j / kChunk + 1) * kChunk;  // next chunk boundary
   int64_t r = j;
   for (; r < n && r < stop && w.keys_s[r] == key; ++r) {
+    if (MODE == 2) {
+      const ScatterRecV& rv = w.recv[w.idx_s[r]];
+      const int cl = rv.cls;
+#pragma unroll
+      for (int o = 0; o < NO; ++o) {
+        if (o < t.nout) {
+          const double wt = t.W[cl][o];
+#pragma unroll
+          for (int e = 0; e < NC * 6; ++e) acc[o][e] += wt * rv.v[e];
+        }
+      }
+      continue;
+    }
     const ScatterRec rc = w.rec[w.idx_s[r]];
     const d3 cb = d3{rc.cb[0], rc.cb[1], rc.cb[2]};
     const double a[3] = {rc.a[0], rc.a[1], rc.a[2]};
@@ -123,11 +140,11 @@ __global__ __launch_bounds__(256) void k_scatter_reduce(ScatterWork w, int64_t n
   if (MODE == 0) {
     const int cl = (int)(key / (uint64_t)t.span);
     const int64_t blk = (int64_t)(key - (uint64_t)cl * t.span);
-    add_block<NC>(t.T[cl], blk, t.ntri, acc[0], atomic);
+    add_block<NC>(t.T[cl] + blk * (NC * 6), acc[0], atomic);
   } else {
 #pragma unroll
     for (int o = 0; o < NO; ++o)
-      if (o < t.nout) add_block<NC>(t.T[o], (int64_t)key, t.ntri, acc[o], atomic);
+      if (o < t.nout) add_block<NC>(t.T[o] + (int64_t)key * (NC * 6), acc[o], atomic);
   }
 }
 
@@ -144,11 +161,14 @@ int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hip
       hipLaunchKernelGGL((k_scatter_reduce<3, 0>), dim3(grid), dim3(256), 0, st, w, n, t);
     else
       hipLaunchKernelGGL((k_scatter_reduce<1, 0>), dim3(grid), dim3(256), 0, st, w, n, t);
-  } else {
+  } else if (t.mode == 1) {
     if (t.nc == 3)
       hipLaunchKernelGGL((k_scatter_reduce<3, 1>), dim3(grid), dim3(256), 0, st, w, n, t);
     else
       hipLaunchKernelGGL((k_scatter_reduce<1, 1>), dim3(grid), dim3(256), 0, st, w, n, t);
+  } else {
+    if (t.nc != 3) return fail(CMM_E_ARG, "scatter: value records need 3 components");
+    hipLaunchKernelGGL((k_scatter_reduce<3, 2>), dim3(grid), dim3(256), 0, st, w, n, t);
   }
   CMM_LAUNCH_CHECK("scatter: reduce");
   return CMM_OK;

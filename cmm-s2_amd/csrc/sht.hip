@@ -58,6 +58,8 @@ struct cmm_sht_plan {
   int* st_sr = nullptr;
   double* w_r = nullptr;     // ones (L+1)
   double* msc_r = nullptr;   // (L / pi) * (1 for m = 0, else 2)  (L)
+  double* K2 = nullptr;      // (2, jpad, tcols) K transposed for the adjoint resampling: K2[p][j][t] = K[p][t][j]
+  int jpad = 0, tcols = 0;
   std::vector<double> hx_g, hw_g;
 };
 
@@ -138,6 +140,7 @@ extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
   (void)hipFree(p->K); (void)hipFree(p->tw); (void)hipFree(p->dinv); (void)hipFree(p->rec);
   (void)hipFree(p->units); (void)hipFree(p->unit0); (void)hipFree(p->st_p); (void)hipFree(p->st_s);
   (void)hipFree(p->st_pr); (void)hipFree(p->st_sr); (void)hipFree(p->w_r); (void)hipFree(p->msc_r);
+  (void)hipFree(p->K2);
   delete p;
   return CMM_OK;
 }
@@ -213,7 +216,16 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
   }
   std::vector<double> ones(L + 1, 1.0), msc(L);
   for (int m = 0; m < L; ++m) msc[m] = (m == 0 ? 1.0 : 2.0) * L / M_PI;
-  if ((rc = upload(&p->w_r, ones.data(), ones.size())) || (rc = upload(&p->msc_r, msc.data(), msc.size()))) {
+  // transposed resampling matrices, zero rows j >= L and columns t > L
+  p->jpad = (L + 3) / 4 * 4;
+  p->tcols = (L + 1 + 15) / 16 * 16;
+  std::vector<double> K2((size_t)2 * p->jpad * p->tcols, 0.0);
+  for (int par = 0; par < 2; ++par)
+    for (int t = 0; t <= L; ++t)
+      for (int j = 0; j < L; ++j)
+        K2[((size_t)par * p->jpad + j) * p->tcols + t] = K[(size_t)par * L * tpad + (size_t)t * L + j];
+  if ((rc = upload(&p->w_r, ones.data(), ones.size())) || (rc = upload(&p->msc_r, msc.data(), msc.size())) ||
+      (rc = upload(&p->K2, K2.data(), K2.size()))) {
     cmm_sht_plan_destroy(p);
     return rc;
   }
@@ -966,24 +978,37 @@ __global__ __launch_bounds__(256) void k_leg_synth_gl(int L, int nf, const doubl
                                                       const double* __restrict__ wg, const double* __restrict__ dP,
                                                       const int* __restrict__ dS, const double2* __restrict__ rec,
                                                       const double2* __restrict__ flm, double2* __restrict__ G) {
+  // recurrence pairs and this order's coefficients of one field staged in
+  // LDS (L <= 512: 2 x 512 x 16 bytes)
+  __shared__ double2 srec[512], scf[512];
   const int m = blockIdx.x, fi = blockIdx.y;
   const int nh = (L + 1) / 2;
   const int j = threadIdx.x;
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  for (int i = threadIdx.x; i < L - m; i += blockDim.x) {
+    srec[i] = rec[lm0 + i];
+    scf[i] = flm[(lm0 + i) * nf + fi];
+  }
+  __syncthreads();
   if (j >= nh) return;
   const double x = xg[j];
   double pp = 0.0, pc = dP[(int64_t)m * L + j];
   int s = dS[(int64_t)m * L + j];
-  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
   double2 S = {0.0, 0.0}, D = {0.0, 0.0};
-  for (int l = m; l < L; ++l) {
-    if (l > m) {
-      const double2 ab = rec[lm0 + (l - m)];
-      leg_step(ab.x, ab.y, x, pp, pc);
-    }
+  leg_rescale(pp, pc, s);
+  {
+    const double v = s == 0 ? pc : 0.0;
+    S.x = v * scf[0].x;
+    S.y = v * scf[0].y;
+  }
+#pragma unroll 4
+  for (int i = 1; i < L - m; ++i) {
+    const double2 ab = srec[i];
+    leg_step(ab.x, ab.y, x, pp, pc);
     leg_rescale(pp, pc, s);
     const double v = s == 0 ? pc : 0.0;
-    const double2 a = flm[(lm0 + (l - m)) * nf + fi];
-    if ((l - m) & 1) {
+    const double2 a = scf[i];
+    if (i & 1) {
       D.x = fma(v, a.x, D.x);
       D.y = fma(v, a.y, D.y);
     } else {
@@ -1005,25 +1030,68 @@ __global__ __launch_bounds__(256) void k_leg_synth_gl(int L, int nf, const doubl
 
 // Transposed resampling with the input scale of the transposed r2c FFT:
 // F[f][t][m] = c_m sum_j K_{m mod 2}[j][t] G[f][j][m], t = 0..L, c_0 = pi / L,
-// c_m = pi / (2 L) (k_fft_c2r doubles orders m > 0).  One lane per order of
-// one parity, one block per (ring, 64 orders, field).
-__global__ __launch_bounds__(64) void k_resample_adj(int L, int nqb, int tpad, const double* __restrict__ Kt,
-                                                     const double2* __restrict__ G, double2* __restrict__ F) {
-  const int p = (int)blockIdx.x / nqb, qb = (int)blockIdx.x % nqb;
-  const int t = blockIdx.y, fi = blockIdx.z;
-  const int m = p + 2 * (64 * qb + (int)threadIdx.x);
-  if (m >= L) return;
-  const double* K = Kt + ((int64_t)p * tpad + t) * L;
-  const double2* Gf = G + (int64_t)fi * L * L;
-  double2 acc = {0.0, 0.0};
-  for (int j = 0; j < L; ++j) {
-    const double k = K[j];
-    const double2 g = Gf[(int64_t)j * L + m];
-    acc.x = fma(k, g.x, acc.x);
-    acc.y = fma(k, g.y, acc.y);
+// c_m = pi / (2 L) (k_fft_c2r doubles orders m > 0).  The layout of
+// k_resample with the roles of t and j exchanged: one wave per (16 rings) x
+// (16 orders of one parity, re/im) x field, K2[p][j][t] = K[p][t][j] zero
+// padded (rows j >= L, columns t > L), FP64 matrix cores.
+__global__ __launch_bounds__(64) void k_resample_adj(int L, int nmg, int jpad, int tcols, const double* __restrict__ K2,
+                                                     const double* __restrict__ G, double* __restrict__ F) {
+  const int lane = threadIdx.x;
+  const int bk = lane >> 4, bj = lane & 15;
+  const int fi = blockIdx.z;
+  const int t0 = blockIdx.y * 16;
+  const int p = (int)blockIdx.x / nmg, qb = (int)blockIdx.x % nmg;
+  const double* K = K2 + (int64_t)p * jpad * tcols;
+  const double* Gf = G + (int64_t)fi * L * L * 2;
+  int mcol[2], mcc[2];
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt) {
+    mcol[nt] = p + 2 * (16 * qb + 8 * nt + (bj >> 1));
+    mcc[nt] = min(mcol[nt], L - 1);
   }
-  const double c = (m == 0 ? 1.0 : 0.5) * M_PI / L;
-  F[((int64_t)fi * (L + 1) + t) * L + m] = double2{c * acc.x, c * acc.y};
+  const int ta = t0 + bj;  // A row (output ring) of this lane, < tcols
+  auto load = [&](int s0, double (&a)[kResSteps], double (&b)[kResSteps][2]) {
+#pragma unroll
+    for (int u = 0; u < kResSteps; ++u) {
+      const int jw = 4 * (s0 + u) + bk;
+      a[u] = K[(int64_t)min(jw, jpad - 1) * tcols + ta];
+      const int j = min(jw, L - 1);
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) b[u][nt] = Gf[((int64_t)j * L + mcc[nt]) * 2 + (bj & 1)];
+    }
+  };
+  dx4 acc[2] = {dx4{0.0, 0.0, 0.0, 0.0}, dx4{0.0, 0.0, 0.0, 0.0}};
+  const int nsteps = jpad / 4;
+  auto mma = [&](const double (&a)[kResSteps], const double (&b)[kResSteps][2], int s0) {
+#pragma unroll
+    for (int u = 0; u < kResSteps; ++u)
+      if (s0 + u < nsteps)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][nt], acc[nt], 0, 0, 0);
+  };
+  double aA[kResSteps], bA[kResSteps][2], aB[kResSteps], bB[kResSteps][2];
+  load(0, aA, bA);
+  for (int s0 = 0;;) {
+    load(s0 + kResSteps, aB, bB);
+    mma(aA, bA, s0);
+    s0 += kResSteps;
+    if (s0 >= nsteps) break;
+    load(s0 + kResSteps, aA, bA);
+    mma(aB, bB, s0);
+    s0 += kResSteps;
+    if (s0 >= nsteps) break;
+  }
+  double* Ff = F + (int64_t)fi * (L + 1) * L * 2;
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt) {
+    if (mcol[nt] >= L) continue;
+    const double c = (mcol[nt] == 0 ? 1.0 : 0.5) * M_PI / L;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int t = t0 + 4 * e + bk;
+      if (t <= L) Ff[((int64_t)t * L + mcol[nt]) * 2 + (bj & 1)] = c * acc[nt][e];
+    }
+  }
 }
 
 __device__ inline void decode_lm(int L, int64_t idx, int& l, int& m) {
@@ -1268,8 +1336,9 @@ extern "C" int cmm_sht_forward_adjoint(const cmm_sht_plan* p, const double* flm_
   hipLaunchKernelGGL(k_leg_synth_gl, dim3(L, nf), dim3(T), 0, st, L, nf, p->x_g, p->w_g, p->dP_g, p->dS_g, p->rec,
                      reinterpret_cast<const double2*>(flm_bar), G);
   CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: legendre");
-  const int nqb = ((L + 1) / 2 + 63) / 64;
-  hipLaunchKernelGGL(k_resample_adj, dim3(2 * nqb, L + 1, nf), dim3(64), 0, st, L, nqb, p->tpad, p->K, G, F);
+  const int nmg = ((L + 1) / 2 + 15) / 16;
+  hipLaunchKernelGGL(k_resample_adj, dim3(2 * nmg, p->tcols / 16, nf), dim3(64), 0, st, L, nmg, p->jpad, p->tcols,
+                     p->K2, reinterpret_cast<const double*>(G), reinterpret_cast<double*>(F));
   CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: resample");
   const int pairs = (L + 2) / 2;
   const size_t lds = (size_t)2 * p->N * sizeof(double2);

@@ -690,17 +690,18 @@ __device__ inline void located_at(const double* __restrict__ geom, int t, d3 x, 
 
 // adjoint of the raw evaluation r = PS(b(x)) of one packed level at a located
 // point: adds rbar_c basis_j into pb (if non-null), returns xbar
-__device__ inline d3 eval_level3_adj(const double* __restrict__ packed, double* pb, int64_t ntri, const Located& L,
-                                     d3 cb, int s, d3 rb) {
+__device__ inline d3 eval_level3_adj(const double* __restrict__ packed, double* pb, const Located& L, d3 cb, int s,
+                                     d3 rb) {
   const double* p = packed + ((int64_t)L.tri * 6 + s) * 18;
   const double rv[3] = {rb.x, rb.y, rb.z};
   if (pb) {
     double bas[6];
     bb6_basis(cb, bas);
+    double* dst = pb + ((int64_t)L.tri * 6 + s) * 18;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(pb + adj_index(6 * c + j, s, L.tri, ntri), rv[c] * bas[j]);
+      for (int j = 0; j < 6; ++j) unsafeAtomicAdd(dst + 6 * c + j, rv[c] * bas[j]);
   }
   d3 cbar = d3{0, 0, 0};
 #pragma unroll
@@ -760,7 +761,7 @@ __global__ __launch_bounds__(256) void k_tri_compose_adj(cmm_tri_mesh M, LevelPt
           else
             scatter_empty(sw, (int64_t)l * n + i);
         }
-        xb = eval_level3_adj(lv.p[l], rec ? nullptr : lb.p[l], M.ntri, L, cb, s, rb);
+        xb = eval_level3_adj(lv.p[l], rec ? nullptr : lb.p[l], L, cb, s, rb);
       } else if (rec) {
         scatter_empty(sw, (int64_t)l * n + i);
       }
@@ -782,6 +783,7 @@ __global__ __launch_bounds__(256) void k_tri_assemble_adj(cmm_tri_mesh M, const 
   PSGeom P;
   ps_unpack(M.psgeom + t, M.ntri, P);
   const int64_t Nv = M.nv;
+  const double* src = pb + t * 108;
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
     double cb[19];
@@ -790,7 +792,7 @@ __global__ __launch_bounds__(256) void k_tri_assemble_adj(cmm_tri_mesh M, const 
 #pragma unroll
     for (int ss = 0; ss < 6; ++ss)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += pb[adj_index(6 * cc + j, ss, t, M.ntri)];
+      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += src[(ss * 3 + cc) * 6 + j];
     double fb[3] = {0.0, 0.0, 0.0};
     d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
     ps_coeffs_adj(P, cb, fb, gb);
@@ -810,32 +812,44 @@ __global__ __launch_bounds__(256) void k_tri_assemble_adj(cmm_tri_mesh M, const 
 // component c, d 4 + 1 + c = its derivative along axis d).  Phase 2, one
 // thread per vertex: the sum over its incidence list, added into vb (the only
 // writer of that vertex).  Deterministic.
-__global__ __launch_bounds__(256) void k_tri_assemble_adj_corners(cmm_tri_mesh M, const double* __restrict__ pb,
-                                                                  double* __restrict__ tmp) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= M.ntri) return;
-  PSGeom P;
-  ps_unpack(M.psgeom + t, M.ntri, P);
+__global__ __launch_bounds__(64) void k_tri_assemble_adj_corners(cmm_tri_mesh M, const double* __restrict__ pb,
+                                                                 double* __restrict__ tmp) {
+  // one wave per 64 triangles, packed blocks staged through LDS per component
+  __shared__ double sh[64 * 37];
+  const int lane = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * 64, t = t0 + lane;
+  const bool live = t < M.ntri;
   const int64_t nt = M.ntri;
+  PSGeom P;
+  ps_unpack(M.psgeom + (live ? t : 0), M.ntri, P);
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
-    double cb[19];
-#pragma unroll
-    for (int k = 0; k < 19; ++k) cb[k] = 0.0;
-#pragma unroll
-    for (int ss = 0; ss < 6; ++ss)
-#pragma unroll
-      for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += pb[adj_index(6 * cc + j, ss, t, M.ntri)];
-    double fb[3] = {0.0, 0.0, 0.0};
-    d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
-    ps_coeffs_adj(P, cb, fb, gb);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      tmp[(12 * i + cc * 4) * nt + t] = fb[i];
-      tmp[(12 * i + 0 * 4 + 1 + cc) * nt + t] = gb[i].x;
-      tmp[(12 * i + 1 * 4 + 1 + cc) * nt + t] = gb[i].y;
-      tmp[(12 * i + 2 * 4 + 1 + cc) * nt + t] = gb[i].z;
+    for (int q = lane; q < 64 * 36; q += 64) {
+      const int tt = q / 36, r = q - 36 * tt, ss = r / 6, j = r - 6 * ss;
+      const int64_t T = t0 + tt;
+      sh[tt * 37 + r] = T < nt ? pb[T * 108 + (ss * 3 + cc) * 6 + j] : 0.0;
     }
+    __syncthreads();
+    if (live) {
+      double cb[19];
+#pragma unroll
+      for (int k = 0; k < 19; ++k) cb[k] = 0.0;
+#pragma unroll
+      for (int ss = 0; ss < 6; ++ss)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += sh[lane * 37 + ss * 6 + j];
+      double fb[3] = {0.0, 0.0, 0.0};
+      d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+      ps_coeffs_adj(P, cb, fb, gb);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        tmp[(12 * i + cc * 4) * nt + t] = fb[i];
+        tmp[(12 * i + 0 * 4 + 1 + cc) * nt + t] = gb[i].x;
+        tmp[(12 * i + 1 * 4 + 1 + cc) * nt + t] = gb[i].y;
+        tmp[(12 * i + 2 * 4 + 1 + cc) * nt + t] = gb[i].z;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -940,7 +954,8 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, co
         Sb[k] = add(add(scl(bb.x, L.w0), scl(bb.y, L.w1)), scl(bb.z, L.w2));
       }
       if (curb && !rec) {
-        for (int e = 0; e < 18; ++e) unsafeAtomicAdd(curb + adj_index(e, s, L.tri, M.ntri), acc[e]);
+        double* dst = curb + ((int64_t)L.tri * 6 + s) * 18;
+        for (int e = 0; e < 18; ++e) unsafeAtomicAdd(dst + e, acc[e]);
       }
     }
   }
@@ -1105,7 +1120,6 @@ extern "C" int cmm_tri_compose_adjoint(const cmm_tri_mesh* m, const double* cons
     t.mode = 0;
     t.nc = 3;
     t.span = m->ntri * 6;
-    t.ntri = m->ntri;
     for (int l = 0; l < depth; ++l) t.T[l] = level_bars[l];
     return scatter_reduce(sw, nrec, t, st);
   }
@@ -1123,7 +1137,7 @@ extern "C" int cmm_tri_assemble_adjoint(const cmm_tri_mesh* m, const double* pac
   }
   if (work_bytes < 36 * 8 * m->ntri) return fail(CMM_E_ARG, "cmm_tri_assemble_adjoint: workspace too small");
   double* tmp = static_cast<double*>(work);
-  hipLaunchKernelGGL(k_tri_assemble_adj_corners, dim3(grid_for(m->ntri, 256)), dim3(256), 0, st, *m, packed_bar, tmp);
+  hipLaunchKernelGGL(k_tri_assemble_adj_corners, dim3(grid_for(m->ntri, 64)), dim3(64), 0, st, *m, packed_bar, tmp);
   CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
   hipLaunchKernelGGL(k_tri_assemble_adj_gather, dim3(grid_for(m->nv, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
   CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
@@ -1152,7 +1166,6 @@ extern "C" int cmm_tri_stencil_jets_adjoint(const cmm_tri_mesh* m, const double*
     t.mode = 1;
     t.nc = 3;
     t.nout = 1;
-    t.ntri = m->ntri;
     t.T[0] = cur_bar;
     return scatter_reduce(sw, 4 * m->nv, t, st);
   }

@@ -283,9 +283,7 @@ int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const double* wpa
 /* outputs are overwritten.                                                  */
 /* ------------------------------------------------------------------------ */
 /* Coefficient adjoints ("packed_bar", "level_bars", "cur_bar", ...) have the
- * size of the packed array they belong to, (Ntri, 6, ncomp, 6), in a planar
- * layout: element (triangle t, sub-triangle s, component c, basis j) at
- * ((6 c + j) * 6 + s) * Ntri + t.
+ * layout of the packed array they belong to, (Ntri, 6, ncomp, 6).
  * The coefficient adjoints of the four scattering entry points below
  * (composition, RK feet, stencil projection, vorticity sampling) either
  * accumulate with FP64 atomics (work == NULL) or go through a sorted

@@ -28,13 +28,6 @@ def _close(a, b, tol):
     assert abs(a - b) <= tol * max(abs(a), abs(b), 1e-300), (a, b)
 
 
-def _planar(pk, nc):
-    """packed (Ntri, 6, nc, 6) [t][s][c][j] -> the planar adjoint layout
-    [c][j][s][t] of the coefficient adjoints"""
-    ntri = pk.shape[0]
-    return pk.reshape(ntri, 6, nc, 6).permute(2, 3, 1, 0).contiguous()
-
-
 def _work(records):
     """(pointer, bytes) of a sorted scatter-add workspace, None = atomics."""
     import ctypes
@@ -117,7 +110,7 @@ def test_jets_and_assembly_transposes():
             ws = torch.empty(96 * nc * mu.ntri, dtype=torch.uint8, device="cuda") if gather else None
             N.call("cmm_mwss_assemble_adjoint", mu._h, pb.data_ptr(), nc, vb.data_ptr(),
                    None if ws is None else ws.data_ptr(), 0 if ws is None else ws.numel(), stream())
-            _close(_dot(_planar(pk, nc), pb), _dot(vals, vb), 1e-12)
+            _close(_dot(pk, pb), _dot(vals, vb), 1e-12)
             outs.append(vb)
         assert torch.allclose(outs[0], outs[1], rtol=1e-12, atol=1e-12)
     ico = mf.icosahedral_mesh(3)
@@ -132,7 +125,7 @@ def test_jets_and_assembly_transposes():
         ws = torch.empty(288 * mesh.ntri, dtype=torch.uint8, device="cuda") if gather else None
         N.call("cmm_tri_assemble_adjoint", mesh._h, pb.data_ptr(), vb.data_ptr(),
                None if ws is None else ws.data_ptr(), 0 if ws is None else ws.numel(), stream())
-        _close(_dot(_planar(pk, 3), pb), _dot(vals, vb), 1e-12)
+        _close(_dot(pk, pb), _dot(vals, vb), 1e-12)
         outs.append(vb)
     assert torch.allclose(outs[0], outs[1], rtol=1e-12, atol=1e-12)
 
@@ -229,7 +222,7 @@ def test_sample_vorticity_adjoint_fd(scatter):
     w0.packed.copy_(base + dp)
     f1 = f(X)
     w0.packed.copy_(base)
-    _close(_dot(pb, _planar(dp, 1)), f1 - f(X), 1e-9)
+    _close(_dot(pb, dp), f1 - f(X), 1e-9)
 
 
 def _spline_velocity(L, seed):
