@@ -115,6 +115,10 @@ class MisfitProblem(object):
         self.L_w = L if L_w is None else L_w
         self.params = SpectralParams(self.L_w, lmax)
         self.ns = [i for i in range(1, Nt) if i % n_maps == 0] + [Nt]
+        if self.ns[0] <= 2:
+            # the loop tests for a remap from step 3 on (spherical_simulations.py:
+            # 263-294), so a remap at step <= 2 never fills its submap slot
+            raise N.CmmError("the first remap must come after step 2 (n_maps >= 3, Nt >= 3)")
         if len(self.ns) + 1 > 8:
             raise N.CmmError("the adjoint composes at most 8 maps (remaps + current)")
         self.tspan = np.linspace(0, T, Nt, endpoint=False)
