@@ -178,7 +178,7 @@ def main():
     area = area_distortion(sim, q)
     mixed = None
     if not a.compose_only and not a.no_mixed:
-        mixed = mixed_run(a, mesh, w0, wf, grid, s_step)
+        mixed = mixed_run(a, mesh, w0, wf, grid, s_step, sim)
     cells = int(mesh.stats()["cells_per_face"])
     traffic = MEASURED_TRAFFIC.get((a.ico, a.Lf, depth, cells, tile_rows))
     cpu = None
@@ -330,10 +330,14 @@ def area_distortion(sim, q):
     return out
 
 
-def mixed_run(a, mesh, w0, wf64, grid, s64):
+def mixed_run(a, mesh, w0, wf64, grid, s64, sim64):
     """The same CMM run with the Legendre synthesis on FP32 matrix cores
-    (FP64 recurrence, FP64-compensated partial sums): seconds per steady step
-    and the relative max-norm vorticity difference on the fine grid."""
+    (FP64 recurrence, FP64-compensated partial sums): seconds per steady step,
+    the relative max-norm vorticity difference on the fine grid, and the
+    relative max-norm difference of every intermediate output against the
+    FP64 run: psi_lm, u and grad u on the grid and the velocity coefficients
+    of the last step, the map positions (jet values at the vertices) of
+    every submap."""
     import numpy as np
     import torch
     from cmm_s2.core.spherical_simulations import BarotropicCMM
@@ -350,10 +354,24 @@ def mixed_run(a, mesh, w0, wf64, grid, s64):
     s32 = statistics.median(st[1:] if len(st) > 2 else st)
     w32 = sim.vorticity_on_grid(a.Lf, grid=grid)
     err = float((w32 - wf64).abs().max() / wf64.abs().max())
+    rel = lambda x, y: float((x - y).abs().max() / y.abs().max())
+    dinv = torch.tensor([0.0] + [-1.0 / (l * (l + 1)) for l in range(1, a.L)], dtype=torch.float64,
+                        device="cuda")[torch.from_numpy(sim.plan.l_of).cuda()]
+    per = {"psi_lm": rel(sim.w_lm[:, 0] * dinv[:, None], sim64.w_lm[:, 0] * dinv[:, None]),
+           "u_grid": rel(sim.fields[0:3], sim64.fields[0:3]),
+           "grad_u_grid": rel(sim.fields[3:12], sim64.fields[3:12]),
+           "U_coefficients": rel(sim.U.level_packed(2), sim64.U.level_packed(2)),
+           "map_positions_at_remaps": [rel(sim.submaps.vals[i][:, 0], sim64.submaps.vals[i][:, 0])
+                                       for i in range(sim64.submaps.Nc)],
+           "final_vorticity": err}
+    log("fp32 Legendre, per output rel. max-norm diff vs fp64: " +
+        ", ".join(f"{k} {v:.2e}" if not isinstance(v, list) else f"{k} " + "/".join(f"{x:.1e}" for x in v)
+                  for k, v in per.items()))
     log(f"fp32 Legendre: steady step {s32 * 1e3:.3f} ms (fp64 {s64 * 1e3:.3f} ms), vorticity rel err {err:.3e}")
     return {"legendre": "fp32 matrix cores, fp64 recurrence, fp64-compensated 32-degree partial sums",
             "s_per_step": round(s32, 6), "fp64_s_per_step": round(s64, 6),
             "step_speedup_vs_fp64": round(s64 / s32, 4), "vorticity_max_rel_err": err,
+            "per_output_rel_err_vs_fp64_gpu": per,
             "target_rel_err": 1e-10, "meets_target": bool(err <= 1e-10 and s32 < s64)}
 
 
