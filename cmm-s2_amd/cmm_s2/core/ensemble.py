@@ -6,6 +6,13 @@ priority), so the work of small members overlaps on the chip while the
 host walks the members in lockstep; members share read-only tables (map
 meshes, SHT plans of equal bandlimit).  A member's results do not depend on
 the others: running it alone gives the same bits.
+
+``capture()`` records one whole run of every member (bootstrap and all
+steps, forked over the member streams) as a single HIP graph; ``replay()``
+re-runs it from the members' initial fields with one graph launch, without
+the host's per-step work.  Every kernel argument is fixed at capture (buffer
+addresses, times, depths), so a replay writes the bits of the eager run; the
+initial vorticity buffers may be rewritten in place between replays.
 """
 import time
 
@@ -21,6 +28,7 @@ class Ensemble(object):
         with its member's stream current, so the member's buffers belong to
         that stream."""
         n = len(builders)
+        self.own_streams = bool(streams)
         self.streams = [torch.cuda.Stream() for _ in range(n)] if streams else \
             [torch.cuda.current_stream()] * n
         self.members = []
@@ -51,6 +59,63 @@ class Ensemble(object):
             pass
         torch.cuda.synchronize()
         return [m.submaps for m in self.members]
+
+    def capture(self):
+        """Run once eagerly (allocates every buffer, tunes the tile shapes),
+        then record a full run as a HIP graph.  Returns the graph."""
+        self.run()
+        for m in self.members:
+            m.reset()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        cap = torch.cuda.Stream()
+        eager = self.streams
+        if not self.own_streams:  # one stream: record on the capture stream itself
+            self.streams = [cap] * len(self.members)
+        try:
+            with torch.cuda.graph(g, stream=cap):
+                for s in set(self.streams):
+                    s.wait_stream(cap)
+                for m, s in zip(self.members, self.streams):
+                    with torch.cuda.stream(s):
+                        m.bootstrap(tune=False)
+                while self.step():
+                    pass
+                for s in set(self.streams):
+                    cap.wait_stream(s)
+        finally:
+            self.streams = eager
+        torch.cuda.synchronize()
+        self.graph = g
+        return g
+
+    def replay(self):
+        """One captured run of every member (asynchronous)."""
+        self.graph.replay()
+
+    def timed_replay(self, reps=1):
+        """(member steps, seconds) of ``reps`` back-to-back replays."""
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            self.graph.replay()
+        torch.cuda.synchronize()
+        return reps * sum(m.Nt for m in self.members), time.perf_counter() - t0
+
+    def timed_full_run(self):
+        """(member steps, seconds) of an eager run including the bootstrap
+        (the work a replay does), tile shapes already tuned."""
+        for m in self.members:
+            m.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for m, s in zip(self.members, self.streams):
+            with torch.cuda.stream(s):
+                m.bootstrap(tune=False)
+        while self.step():
+            pass
+        torch.cuda.synchronize()
+        return sum(m.Nt for m in self.members), time.perf_counter() - t0
 
     def timed_run(self):
         """Bootstrap, then time the remaining lockstep steps: returns

@@ -100,8 +100,24 @@ class BarotropicCMM(object):
         project_stream_lm(self.plan, self.w_lm, self.U, level, apply_dinv=True, fields=self.fields,
                           ops=self.ops)
 
-    def bootstrap(self):
-        """spherical_simulations.py:216-261."""
+    def reset(self):
+        """Return the host-side loop state to that of a fresh object (step
+        count, current map, velocity time levels, submap count).  Device
+        buffers keep their contents: a run writes every value before it reads
+        it, so a run after reset() gives the bits of a first run."""
+        self.curr = self.map0
+        self.identity = True
+        self.count = 0
+        self.submaps.Nc = 0
+        U = self.U
+        U._order = [0, 1, 2]
+        U.ts = np.array(self.tspan[0:3], dtype=np.float64)
+        U.stepping = False
+        U.nVs = 1
+
+    def bootstrap(self, tune=True):
+        """spherical_simulations.py:216-261.  ``tune`` times the composition
+        tile shapes (host synchronisation; pass False under stream capture)."""
         U, dt = self.U, self.dt
         self._velocity_from(self.samples, 0, rot=0.0)
         int0 = evol.advect_project_sphere(self.map0, evol.Euler_step_proj, 0, dt, U, identity=True)
@@ -119,7 +135,8 @@ class BarotropicCMM(object):
         U.stepping = True
         self.identity = False
         self.count = 2
-        self.tune_tiles()
+        if tune:
+            self.tune_tiles()
 
     def tune_tiles(self):
         """Pick the tile shape of the per-step sample-grid composition for

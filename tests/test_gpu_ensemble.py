@@ -36,3 +36,85 @@ def test_random_members_are_seeded():
     _, p2 = random_members(64)
     assert p1 == p2 and len(set(p1)) > 5
     assert {L for L, _ in p1} <= {128, 192, 256, 384, 512} and {i for _, i in p1} <= {5, 6, 7}
+
+
+def _meshes():
+    from cmm_s2.core import mesh_functions as mf
+    out = {}
+    for k in (2, 3):
+        ico = mf.icosahedral_mesh(k)
+        out[k] = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
+    return out
+
+
+def _state(m):
+    """Every buffer a run leaves results in."""
+    nc = m.submaps.Nc
+    return [m.submaps.packed[:nc].clone(), m.submaps.vals[:nc].clone(), m.curr.packed.clone(),
+            m.U._packed.clone(), m.U._vals.clone(), m.X.clone(), m.w.clone(), m.fields.clone()]
+
+
+def _clobber(m):
+    for t in (m.submaps.packed, m.submaps.vals, m.curr.packed, m.U._packed, m.U._vals, m.X, m.w, m.w_lm,
+              m.ops, m.fields):
+        t.fill_(float("nan"))
+
+
+@pytest.mark.parametrize("own_streams", [True, False])
+def test_captured_run_replays_the_eager_bits(own_streams):
+    """A full ensemble run recorded as one HIP graph: every replay (outputs
+    overwritten with NaN before it, eager work between replays) writes the
+    bits of the eager run."""
+    from cmm_s2.core import dynamical_fields as df
+    from cmm_s2.core.ensemble import Ensemble
+    from cmm_s2.core.spherical_simulations import BarotropicCMM
+    meshes = _meshes()
+    specs = [(16, 2, 12, df.rossby_wave), (24, 3, 10, df.rossby_wave), (16, 2, 9, df.rossby_wave.at(t=0.3))]
+    mk = lambda L, i, Nt, w: (lambda: BarotropicCMM(L, Nt, Nt / L, meshes[i], w, 2 * np.pi, n_maps=4))
+    ens = Ensemble([mk(*s) for s in specs], streams=own_streams)
+    ens.run()
+    ref = [_state(m) for m in ens.members]
+    ens.capture()
+    for rep in range(3):
+        for m in ens.members:
+            _clobber(m)
+        ens.replay()
+        torch.cuda.synchronize()
+        for k, (m, r) in enumerate(zip(ens.members, ref)):
+            got = _state(m)
+            for j, (a, b) in enumerate(zip(got, r)):
+                assert torch.equal(a, b), (rep, k, j)
+        other = mk(16, 2, 6, df.rossby_wave)()  # eager work between replays
+        other.run()
+        torch.cuda.synchronize()
+
+
+def test_replay_with_new_initial_fields():
+    """The initial vorticity spline is a graph input: rewriting its
+    coefficients in place and replaying gives the run of the new field."""
+    from cmm_s2.core import dynamical_fields as df, mesh_functions as mf, spherical_harmonic_tools as st
+    from cmm_s2.core.ensemble import Ensemble
+    from cmm_s2.core.spherical_simulations import BarotropicCMM
+    meshes = _meshes()
+
+    def field(seed, L):
+        c = df.random_sh_coefficients(seed, 6, L)
+        return st.project_onto_S12(df.lm_to_full(c, L), L, mf.structure_spherical_triangulation(L))
+
+    specs = [(24, 3, 10), (32, 2, 9)]
+    w_a = [field(40 + k, L) for k, (L, _, _) in enumerate(specs)]
+    w_b = [field(60 + k, L) for k, (L, _, _) in enumerate(specs)]
+    ens = Ensemble([(lambda L=L, i=i, Nt=Nt, w=w: BarotropicCMM(L, Nt, Nt / L, meshes[i], w, 2 * np.pi, n_maps=4))
+                    for (L, i, Nt), w in zip(specs, w_a)])
+    ens.capture()
+    for w, wb in zip(w_a, w_b):
+        w.packed.copy_(wb.packed)
+    ens.replay()
+    torch.cuda.synchronize()
+    for (L, i, Nt), m, wb in zip(specs, ens.members, w_b):
+        solo = BarotropicCMM(L, Nt, Nt / L, meshes[i], wb, 2 * np.pi, n_maps=4)
+        solo.run()
+        assert m.submaps.Nc == solo.submaps.Nc == 3
+        assert torch.equal(m.submaps.packed, solo.submaps.packed)
+        assert torch.equal(m.curr.packed, solo.curr.packed)
+        assert torch.equal(m.X, solo.X)
