@@ -178,7 +178,7 @@ def main():
     area = area_distortion(sim, q)
     mixed = None
     if not a.compose_only and not a.no_mixed:
-        mixed = mixed_run(a, mesh, w0, wf, grid, s_step, sim)
+        mixed = {mode: mixed_run(a, mesh, w0, wf, grid, s_step, sim, mode) for mode in _MIXED}
     cells = int(mesh.stats()["cells_per_face"])
     traffic = MEASURED_TRAFFIC.get((a.ico, a.Lf, depth, cells, tile_rows))
     cpu = None
@@ -350,18 +350,25 @@ def area_distortion(sim, q):
     return out
 
 
-def mixed_run(a, mesh, w0, wf64, grid, s64, sim64):
-    """The same CMM run with the Legendre synthesis on FP32 matrix cores
-    (FP64 recurrence, FP64-compensated partial sums): seconds per steady step,
-    the relative max-norm vorticity difference on the fine grid, and the
-    relative max-norm difference of every intermediate output against the
-    FP64 run: psi_lm, u and grad u on the grid and the velocity coefficients
-    of the last step, the map positions (jet values at the vertices) of
-    every submap."""
+_MIXED = {"fp32": "fp32 matrix cores, fp64 recurrence, fp64-compensated 32-degree partial sums",
+          "bf16x3": "split-bf16, 3 fixed-point digits per operand (6 digit products) on bf16 matrix cores, "
+                    "exact fp32 sums of the two leading weights, fp64 flush every 256 degrees per parity",
+          "bf16x5": "split-bf16, 5 fixed-point digits per operand (15 digit products) on bf16 matrix cores, "
+                    "exact fp32 sums of the two leading weights, fp64 flush every 256 degrees per parity"}
+
+
+def mixed_run(a, mesh, w0, wf64, grid, s64, sim64, mode):
+    """The same CMM run with the Legendre synthesis in a lower-precision form
+    (``mode``: fp32 or split-bf16 with 3 / 5 digits; FP64 recurrence):
+    seconds per steady step, the relative max-norm vorticity difference on the
+    fine grid, and the relative max-norm difference of every intermediate
+    output against the FP64 run: psi_lm, u and grad u on the grid and the
+    velocity coefficients of the last step, the map positions (jet values at
+    the vertices) of every submap."""
     import numpy as np
     import torch
     from cmm_s2.core.spherical_simulations import BarotropicCMM
-    sim = BarotropicCMM(a.L, a.Nt, a.Nt / a.L, mesh, w0, 2 * np.pi, a.n_maps, legendre="fp32")
+    sim = BarotropicCMM(a.L, a.Nt, a.Nt / a.L, mesh, w0, 2 * np.pi, a.n_maps, legendre=mode)
     sim.bootstrap()
     torch.cuda.synchronize()
     st = []
@@ -384,11 +391,11 @@ def mixed_run(a, mesh, w0, wf64, grid, s64, sim64):
            "map_positions_at_remaps": [rel(sim.submaps.vals[i][:, 0], sim64.submaps.vals[i][:, 0])
                                        for i in range(sim64.submaps.Nc)],
            "final_vorticity": err}
-    log("fp32 Legendre, per output rel. max-norm diff vs fp64: " +
+    log(f"{mode} Legendre, per output rel. max-norm diff vs fp64: " +
         ", ".join(f"{k} {v:.2e}" if not isinstance(v, list) else f"{k} " + "/".join(f"{x:.1e}" for x in v)
                   for k, v in per.items()))
-    log(f"fp32 Legendre: steady step {s32 * 1e3:.3f} ms (fp64 {s64 * 1e3:.3f} ms), vorticity rel err {err:.3e}")
-    return {"legendre": "fp32 matrix cores, fp64 recurrence, fp64-compensated 32-degree partial sums",
+    log(f"{mode} Legendre: steady step {s32 * 1e3:.3f} ms (fp64 {s64 * 1e3:.3f} ms), vorticity rel err {err:.3e}")
+    return {"legendre": _MIXED[mode],
             "s_per_step": round(s32, 6), "fp64_s_per_step": round(s64, 6),
             "step_speedup_vs_fp64": round(s64 / s32, 4), "vorticity_max_rel_err": err,
             "per_output_rel_err_vs_fp64_gpu": per,

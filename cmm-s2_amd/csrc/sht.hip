@@ -27,6 +27,8 @@ namespace {
 constexpr int kScaleExp = 600;
 constexpr int kMaxFFT = 2048;
 constexpr int kFwdSegHost = 128;  // degrees per forward work unit (kFwdSeg)
+constexpr int kOzJ = 16;          // split-BF16 synthesis: degrees of one parity per chunk
+constexpr int kOzMaxS = 6;        // split-BF16 synthesis: most digits per operand
 }
 
 struct cmm_sht_plan {
@@ -46,7 +48,12 @@ struct cmm_sht_plan {
   double2* tw = nullptr;     // exp(-2 pi i n / N), n < N
   double* dinv = nullptr;    // D_inv[l] (L)
   double2* rec = nullptr;    // recurrence (a_lm, a_lm b_lm) per packed lm
-  int legendre = 0;          // CMM_LEGENDRE_FP64 / CMM_LEGENDRE_FP32 for the synthesis
+  int legendre = 0;          // CMM_LEGENDRE_* arithmetic of the synthesis
+  // split-BF16 synthesis: chunk offsets offJ[m] = sum_{m' < m} ceil((L - m') / 32)
+  // (L + 1), their total, and the exponent bounding |Pbar| (2^eA > max |Pbar|)
+  int* offJ = nullptr;
+  int64_t nJtot = 0;
+  int eA = 0;
   int nunits = 0;            // forward work units (m, first degree of a segment)
   int2* units = nullptr;
   int* unit0 = nullptr;      // first unit of each m (L)
@@ -135,7 +142,7 @@ static int upload(T** dst, const T* src, size_t n) {
 
 extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
   if (!p) return CMM_OK;
-  (void)hipFree(p->x_t); (void)hipFree(p->dP_t); (void)hipFree(p->dS_t);
+  (void)hipFree(p->x_t); (void)hipFree(p->dP_t); (void)hipFree(p->dS_t); (void)hipFree(p->offJ);
   (void)hipFree(p->x_g); (void)hipFree(p->w_g); (void)hipFree(p->dP_g); (void)hipFree(p->dS_g);
   (void)hipFree(p->K); (void)hipFree(p->tw); (void)hipFree(p->dinv); (void)hipFree(p->rec);
   (void)hipFree(p->units); (void)hipFree(p->unit0); (void)hipFree(p->st_p); (void)hipFree(p->st_s);
@@ -214,6 +221,14 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
     cmm_sht_plan_destroy(p);
     return rc;
   }
+  std::vector<int> offJ(L + 1, 0);
+  for (int m = 0; m < L; ++m) offJ[m + 1] = offJ[m] + (L - m + 31) / 32;
+  p->nJtot = offJ[L];
+  p->eA = std::ilogb(1.01 * std::sqrt((2.0 * L - 1.0) / (4.0 * M_PI))) + 1;
+  if ((rc = upload(&p->offJ, offJ.data(), offJ.size()))) {
+    cmm_sht_plan_destroy(p);
+    return rc;
+  }
   std::vector<double> ones(L + 1, 1.0), msc(L);
   for (int m = 0; m < L; ++m) msc[m] = (m == 0 ? 1.0 : 2.0) * L / M_PI;
   // transposed resampling matrices, zero rows j >= L and columns t > L
@@ -244,7 +259,7 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
 }
 
 extern "C" int cmm_sht_set_legendre(cmm_sht_plan* p, int mode) {
-  if (!p || (mode != CMM_LEGENDRE_FP64 && mode != CMM_LEGENDRE_FP32))
+  if (!p || mode < CMM_LEGENDRE_FP64 || mode > CMM_LEGENDRE_BF16X6)
     return fail(CMM_E_ARG, "cmm_sht_set_legendre: bad arguments");
   p->legendre = mode;
   return CMM_OK;
@@ -256,10 +271,22 @@ extern "C" int cmm_sht_plan_gl(const cmm_sht_plan* p, double* x, double* w) {
   return CMM_OK;
 }
 
+// Workspace: the Fourier coefficients F (nf, L+1, L) complex, then either the
+// analysis intermediate G (nf, L, L) complex or, during a split-BF16
+// synthesis, the coefficient digits of one group of <= 16 fields (room for
+// kOzMaxS digits whatever the plan's mode) and their exponents (L, 32).
+static int64_t sht_f_bytes(const cmm_sht_plan* p, int nf) { return (int64_t)nf * (p->L + 1) * p->L * 16; }
+static int64_t sht_oz_dstride(const cmm_sht_plan* p, int nfp) { return p->nJtot * 2 * (2 * nfp) * kOzJ; }
+static int64_t sht_work_bytes(const cmm_sht_plan* p, int nf) {
+  const int64_t L = p->L;
+  const int64_t g = (int64_t)nf * L * L * 16;
+  const int64_t oz = kOzMaxS * sht_oz_dstride(p, std::min(nf, 16)) * 2 + 256 + L * 32 * 4;
+  return sht_f_bytes(p, nf) + std::max(g, oz);
+}
+
 extern "C" int cmm_sht_workspace_bytes(const cmm_sht_plan* p, int nf, int64_t* bytes) {
   if (!p || nf < 1 || !bytes) return fail(CMM_E_ARG, "cmm_sht_workspace_bytes: bad arguments");
-  int64_t L = p->L;
-  *bytes = (int64_t)nf * ((L + 1) * L + L * L) * 16;
+  *bytes = sht_work_bytes(p, nf);
   return CMM_OK;
 }
 
@@ -636,6 +663,285 @@ __global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, in
         if (L - tr != tr) F[(((int64_t)(f0 + f) * (L + 1) + (L - tr)) * L + m) * 2 + (col & 1)] = ev - od;
       }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 synthesis: fixed-point slicing of both operands (the Ozaki
+// scheme).  Each operand is a power of two times S base-256 digits,
+//   Pbar_lm(x_t) = 2^(eA - 8) sum_i a_i,   flm[col] = 2^(eB[m][col] - 8) sum_j b_j,
+// with a_1, b_1 integers in [-256, 256] and a_i = d_i 2^(-8 (i - 1)),
+// |d_i| <= 128 (balanced digits, rounded to nearest at every step).  A digit
+// has at most 8 significant bits, so it is exact in bf16, and a product
+// a_i b_j is an exact multiple of 2^(-8 (i + j - 2)) below 2^16 in magnitude.
+// The products are summed by v_mfma_f32_16x16x16_bf16 into four FP32
+// accumulators by weight: i + j = 2, 3, 4 (multiples of 1, 2^-8, 2^-16 with
+// per-degree magnitudes below 2^16, 2^8 and 1.25) and i + j >= 5.  Over at
+// most 192 degrees of one parity the first three stay below 2^24 units, so
+// their FP32 sums are exact; the fourth carries the FP32 rounding of a
+// quantity 2^-24 below the result.  Products with i + j > S + 1 are left out
+// (each below 2^(-8 S) of the scale).  Every 192 degrees per parity the four
+// sums are scaled and added into the FP64 output.  eA bounds |Pbar| over all degrees (orthonormal functions:
+// |Pbar_lm| <= sqrt((2 l + 1) / 4 pi)); eB is the exponent of the largest
+// coefficient of each order and column.  The recurrence stays FP64.
+// ---------------------------------------------------------------------------
+typedef short bf4 __attribute__((ext_vector_type(4)));
+constexpr int kOzRings = 64;  // rings per wave
+constexpr int kOzFlush = 12;  // chunks between FP64 flushes (192 degrees per parity)
+constexpr int kOzAcc = 4;     // FP32 accumulators by digit weight
+constexpr int kOzRow = kOzJ / 2 + 2;  // dwords per tile row (16 bf16 + padding)
+
+// bf16 bit pattern of a float whose significand has at most 8 bits (exact)
+__device__ inline uint32_t bf16_bits(float f) { return __float_as_uint(f) >> 16; }
+
+// the S digits of v (|v| <= 1) multiplied by their weights, as bf16 bits
+template <int S>
+__device__ inline void oz_digits(double v, uint32_t (&d)[S]) {
+  double t = v * 256.0;
+  double s = rint(t);
+  d[0] = bf16_bits((float)s);
+  double r = t - s;  // exact, |r| <= 1/2
+  double w = 1.0;
+#pragma unroll
+  for (int k = 1; k < S; ++k) {
+    t = r * 256.0;
+    s = rint(t);
+    r = t - s;
+    w *= 0x1p-8;
+    d[k] = bf16_bits((float)(s * w));
+  }
+}
+
+// exponents eB[m][col] (stride 32) with max_l |flm| < 2^eB for the columns
+// col < ncols (field f0 + col / 2, real / imaginary part) of order
+// m = blockIdx.x; 32 partial maxima per column, loads unrolled by 4
+__global__ __launch_bounds__(1024) void k_oz_exponents(int L, int nf, int f0, int ncols,
+                                                       const double* __restrict__ flm, int* __restrict__ eB) {
+  __shared__ double mx[32][33];
+  const int m = blockIdx.x;
+  const int col = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  const int cc = col < ncols ? col : 0;
+  const double* base = flm + (lm0 * nf + f0 + (cc >> 1)) * 2 + (cc & 1);
+  const int64_t st = (int64_t)nf * 2;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int l = m + g;
+  for (; l + 96 < L; l += 128) {
+    a0 = fmax(a0, fabs(base[(l - m) * st]));
+    a1 = fmax(a1, fabs(base[(l + 32 - m) * st]));
+    a2 = fmax(a2, fabs(base[(l + 64 - m) * st]));
+    a3 = fmax(a3, fabs(base[(l + 96 - m) * st]));
+  }
+  for (; l < L; l += 32) a0 = fmax(a0, fabs(base[(l - m) * st]));
+  mx[g][col] = fmax(fmax(a0, a1), fmax(a2, a3));
+  __syncthreads();
+  if (g != 0 || col >= ncols) return;
+  double amax = 0.0;
+#pragma unroll
+  for (int i = 0; i < 32; ++i) amax = fmax(amax, mx[i][col]);
+  eB[(int64_t)m * 32 + col] = amax > 0.0 ? ilogb(amax) + 1 : 0;
+}
+
+// B operands: the digits of the coefficients of order m = blockIdx.x and
+// chunk c = blockIdx.y (16 degrees of each parity), columns col < ncols,
+// laid out [digit][((offJ[m] * 2 + p * nJ + c) * ncols + col) * 16 + jj] for
+// degree l = m + p + 2 (16 c + jj) (zero for l >= L).  One thread per
+// (parity, column, 4 degrees), the 4 degrees fastest (coalesced stores).
+template <int S>
+__global__ __launch_bounds__(256) void k_oz_coeffs(int L, int nf, int f0, int ncols, const int* __restrict__ offJ,
+                                                   int64_t dstride, const double* __restrict__ flm,
+                                                   const int* __restrict__ eB, uint16_t* __restrict__ Bd) {
+  const int m = blockIdx.x, c = blockIdx.y;
+  const int nJ = (L - m + 31) / 32;
+  if (c >= nJ) return;
+  const int j4 = threadIdx.x & 3, col = (threadIdx.x >> 2) & 31, p = threadIdx.x >> 7;
+  if (col >= ncols) return;
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  const double sc = ldexp(1.0, -eB[(int64_t)m * 32 + col]);
+  uint32_t d[4][S];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int l = m + p + 2 * (kOzJ * c + 4 * j4 + u);
+    const double v = l < L ? flm[((lm0 + (l - m)) * nf + f0 + (col >> 1)) * 2 + (col & 1)] * sc : 0.0;
+    oz_digits<S>(v, d[u]);
+  }
+  const int64_t at = (int64_t)offJ[m] * 2 * ncols * kOzJ + ((int64_t)(p * nJ + c) * ncols + col) * kOzJ + 4 * j4;
+#pragma unroll
+  for (int k = 0; k < S; ++k)
+    *reinterpret_cast<uint2*>(Bd + k * dstride + at) = uint2{d[0][k] | (d[1][k] << 16), d[2][k] | (d[3][k] << 16)};
+}
+
+// The S digits of v * 256 (|v| < 1) as floats that are exact in bf16, given
+// V = v 2^24, from two integers: H = rint(V) (|H| <= 2^24) and
+// Lo = rint((V - H) 2^24) (|Lo| <= 2^23).  With a bias of 128 in its two low bytes, an integer
+// X = d0 2^16 + d1 2^8 + d2 has bytes d1 + 128, d2 + 128 and d0 = (X + 0x8080)
+// >> 16, so d1, d2 are in [-128, 127] and |d0| <= 256:
+//   v 256 = d0 + d1 2^-8 + d2 2^-16 + e0 2^-24 + e1 2^-32 + e2 2^-40.
+template <int S>
+__device__ inline void oz_digits_fixed(double V, float (&d)[S]) {
+  const double Hd = rint(V);
+  const int hb = (int)Hd + 0x8080;
+  d[0] = (float)(hb >> 16);
+  if (S > 1) d[1] = fmaf((float)((hb >> 8) & 0xff), 0x1p-8f, -0x1p-1f);
+  if (S > 2) d[2] = fmaf((float)(hb & 0xff), 0x1p-16f, -0x1p-9f);
+  if (S > 3) {
+    const int lb = (int)rint((V - Hd) * 0x1p24) + 0x8080;
+    d[3] = (float)(lb >> 16) * 0x1p-24f;
+    if (S > 4) d[4] = fmaf((float)((lb >> 8) & 0xff), 0x1p-32f, -0x1p-25f);
+    if (S > 5) d[5] = fmaf((float)(lb & 0xff), 0x1p-40f, -0x1p-33f);
+  }
+}
+
+// two floats exact in bf16 -> one dword of bf16 bits (lo in the low half)
+__device__ inline uint32_t pack_bf16(float lo, float hi) {
+  return (__float_as_uint(lo) >> 16) | (__float_as_uint(hi) & 0xffff0000u);
+}
+
+// inverse Legendre F[f][t][m] on bf16 matrix cores.  A block is two waves on
+// the same 64 northern rings of one order: wave w sums the degrees with
+// l - m = w (mod 2).  Both run the FP64 recurrence; per chunk of 32 degrees
+// each slices its 16 values into a tile [digit][ring][degree] in LDS, then
+// multiplies with the coefficient digits (32 x 32 x 16 per instruction,
+// A = 32 rings x 16 degrees, B = 16 degrees x 32 columns = 16 fields re / im),
+// loaded at the start of the chunk.  At a flush the two parity sums meet in
+// LDS and give ring t (even + odd) and L - t (even - odd).  Blocks are mapped
+// so that the ring blocks of one order run on one XCD (its L2 holds the
+// coefficient digits), orders ascending.  (Overlapping the matrix
+// instructions of one chunk with the recurrence of the next, with two tiles
+// per wave, measured slower: one wave per SIMD.)
+// Operand layouts (v_mfma_f32_32x32x16_bf16, measured): A lane l holds
+// A[l % 32][8 (l / 32) + j], B lane l holds B[8 (l / 32) + j][l % 32],
+// D lane l element r holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float fx16 __attribute__((ext_vector_type(16)));
+constexpr int kOzRow2 = kOzJ / 2 + 4;  // dwords per tile row (16 bf16, 16-byte aligned rows)
+template <int S>
+__global__ __launch_bounds__(128, 2) void k_leg_inverse_oz(int L, int nf, int f0, int nfp, int nrb, int eA,
+                                                           const double* __restrict__ x_t,
+                                                           const double* __restrict__ dP, const int* __restrict__ dS,
+                                                           const double2* __restrict__ rec,
+                                                           const int* __restrict__ offJ, int64_t dstride,
+                                                           const uint16_t* __restrict__ Bd,
+                                                           const int* __restrict__ eB, double* __restrict__ F) {
+  constexpr int kTile = 2 * S * kOzRings * kOzRow2;  // dwords
+  constexpr int kComb = 2 * kOzRings * 33 * 2;       // dwords of the combine buffer
+  __shared__ __align__(16) uint32_t lds[kTile > kComb ? kTile : kComb];
+  extern __shared__ double2 rsh[];  // recurrence pairs of this order
+  const int b = (int)blockIdx.x;
+  const int slot = b >> 3;
+  const int m = 8 * (slot / nrb) + (b & 7), rb = slot % nrb;
+  if (m >= L) return;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int nh = L / 2 + 1;
+  const int tfirst = m == 0 ? 0 : 1;
+  const int nr = nh - tfirst;
+  if (rb * kOzRings >= nr) return;
+  if (m > 0 && rb == 0 && threadIdx.x < nfp) {
+    const double2 z = {0.0, 0.0};
+    double2* F2 = reinterpret_cast<double2*>(F);
+    F2[((int64_t)(f0 + threadIdx.x) * (L + 1) + 0) * L + m] = z;
+    F2[((int64_t)(f0 + threadIdx.x) * (L + 1) + L) * L + m] = z;
+  }
+  const int r = rb * kOzRings + lane;
+  const bool valid = r < nr;
+  const int t = tfirst + (valid ? r : 0);
+  const double x = x_t[t];
+  double pp = 0.0, pc = dP[(int64_t)m * (L + 1) + t];
+  int s = dS[(int64_t)m * (L + 1) + t];
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  const int nJ = (L - m + 31) / 32;
+  for (int i = threadIdx.x; i < 32 * nJ; i += 128) rsh[i] = rec[lm0 + min(i, L - 1 - m)];
+  const int hi = lane >> 5, lc = lane & 31;
+  const int ncols = 2 * nfp;
+  const int64_t bat = (int64_t)offJ[m] * 2 * ncols * kOzJ + (int64_t)min(lc, ncols - 1) * kOzJ + 8 * hi;
+  fx16 acc[kOzAcc][2];
+  auto zero = [&]() {
+#pragma unroll
+    for (int q = 0; q < kOzAcc; ++q)
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) acc[q][mt] = fx16{};
+  };
+  zero();
+  const double scA = ldexp(1.0, 24 - eA);  // Pbar -> V = (Pbar / 2^eA) 2^24
+  uint32_t* tile = lds + w * S * kOzRings * kOzRow2;
+  double* comb = reinterpret_cast<double*>(lds);  // [parity][ring][33]
+  auto flush = [&](bool first) {
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        comb[(w * kOzRings + 32 * mt + 8 * (e >> 2) + 4 * hi + (e & 3)) * 33 + lc] =
+            (((double)acc[0][mt][e] + (double)acc[1][mt][e]) + (double)acc[2][mt][e]) + (double)acc[3][mt][e];
+    zero();
+    __syncthreads();
+    double2* F2 = reinterpret_cast<double2*>(F);
+    for (int e = threadIdx.x; e < kOzRings * nfp; e += 128) {
+      const int row = e & (kOzRings - 1), f = e / kOzRings;
+      const int rr = rb * kOzRings + row;
+      if (rr >= nr) continue;
+      const int tr = tfirst + rr;
+      const double sr = ldexp(1.0, eA + eB[(int64_t)m * 32 + 2 * f] - 16);
+      const double si = ldexp(1.0, eA + eB[(int64_t)m * 32 + 2 * f + 1] - 16);
+      const double er = comb[row * 33 + 2 * f], ei = comb[row * 33 + 2 * f + 1];
+      const double orr = comb[(kOzRings + row) * 33 + 2 * f], oi = comb[(kOzRings + row) * 33 + 2 * f + 1];
+      const double2 va = {(er + orr) * sr, (ei + oi) * si}, vb = {(er - orr) * sr, (ei - oi) * si};
+      const int64_t ia = ((int64_t)(f0 + f) * (L + 1) + tr) * L + m;
+      F2[ia] = first ? va : double2{F2[ia].x + va.x, F2[ia].y + va.y};
+      if (L - tr != tr) {
+        const int64_t ib = ((int64_t)(f0 + f) * (L + 1) + (L - tr)) * L + m;
+        F2[ib] = first ? vb : double2{F2[ib].x + vb.x, F2[ib].y + vb.y};
+      }
+    }
+    __syncthreads();
+  };
+  bf8 bcur[S];
+  __syncthreads();  // rsh staged
+  for (int c = 0; c < nJ; ++c) {
+    // coefficient digits of this chunk, consumed after the recurrence below
+    const int64_t at = bat + (int64_t)(w * nJ + c) * ncols * kOzJ;
+#pragma unroll
+    for (int k = 0; k < S; ++k) bcur[k] = *reinterpret_cast<const bf8*>(Bd + k * dstride + at);
+    // recurrence of degrees l0 .. l0 + 31; this wave keeps l0 + 2 jj + w
+    leg_rescale(pp, pc, s);
+    const double f = ((s == 0) & valid) ? scA : 0.0;
+    const int l0 = m + 32 * c;
+    float lo[S];
+#pragma unroll
+    for (int i = 0; i < 32; i += 2) {
+      const double2 ab0 = rsh[l0 - m + i], ab1 = rsh[l0 - m + i + 1];
+      if (!(c == 0 && i == 0)) leg_step(ab0.x, ab0.y, x, pp, pc);
+      const double v0 = pc;
+      leg_step(ab1.x, ab1.y, x, pp, pc);
+      const double v = w == 0 ? v0 : pc;
+      const int l = l0 + i + w;
+      float d[S];
+      oz_digits_fixed<S>(l < L ? v * f : 0.0, d);
+      const int jj = i >> 1;
+      if (jj & 1) {
+#pragma unroll
+        for (int k = 0; k < S; ++k) tile[(k * kOzRings + lane) * kOzRow2 + (jj >> 1)] = pack_bf16(lo[k], d[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < S; ++k) lo[k] = d[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      bf8 av[2];
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+        av[mt] = *reinterpret_cast<const bf8*>(&tile[(i * kOzRings + 32 * mt + lc) * kOzRow2 + 4 * hi]);
+#pragma unroll
+      for (int j = 0; i + j < S; ++j) {
+        const int q = i + j < kOzAcc - 1 ? i + j : kOzAcc - 1;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+          acc[q][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[mt], bcur[j], acc[q][mt], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    if ((c + 1) % kOzFlush == 0 || c + 1 == nJ) flush(c < kOzFlush);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1231,6 +1537,38 @@ static FFTPlanDev fft_plan(const cmm_sht_plan* p) {
   return P;
 }
 
+template <int S>
+static int oz_inverse_s(const cmm_sht_plan* p, const double* flm, int nf, int f0, int nfp, double2* F, char* oz,
+                        hipStream_t st) {
+  const int L = p->L;
+  const int64_t dstride = sht_oz_dstride(p, nfp);
+  uint16_t* Bd = reinterpret_cast<uint16_t*>(oz);
+  int* eB = reinterpret_cast<int*>(oz + (kOzMaxS * sht_oz_dstride(p, std::min(nf, 16)) * 2 + 255) / 256 * 256);
+  hipLaunchKernelGGL(k_oz_exponents, dim3(L), dim3(1024), 0, st, L, nf, f0, 2 * nfp, flm, eB);
+  CMM_LAUNCH_CHECK("cmm_sht_inverse: coefficient exponents");
+  hipLaunchKernelGGL(k_oz_coeffs<S>, dim3(L, (L + 31) / 32), dim3(256), 0, st, L, nf, f0, 2 * nfp, p->offJ, dstride,
+                     flm, eB, Bd);
+  CMM_LAUNCH_CHECK("cmm_sht_inverse: coefficient digits");
+  const int nh = L / 2 + 1;
+  const int nrb = (nh + kOzRings - 1) / kOzRings;
+  const dim3 grid((unsigned)(8 * ((L + 7) / 8) * nrb)), block(128);
+  const size_t lds_rec = (size_t)32 * ((L + 31) / 32) * sizeof(double2);
+  hipLaunchKernelGGL(k_leg_inverse_oz<S>, grid, block, lds_rec, st, L, nf, f0, nfp, nrb, p->eA, p->x_t, p->dP_t, p->dS_t, p->rec,
+                     p->offJ, dstride, Bd, eB, reinterpret_cast<double*>(F));
+  CMM_LAUNCH_CHECK("cmm_sht_inverse: split-bf16 legendre");
+  return CMM_OK;
+}
+
+static int oz_inverse(const cmm_sht_plan* p, const double* flm, int nf, int f0, int nfp, double2* F, char* oz,
+                      hipStream_t st) {
+  switch (p->legendre) {
+    case CMM_LEGENDRE_BF16X3: return oz_inverse_s<3>(p, flm, nf, f0, nfp, F, oz, st);
+    case CMM_LEGENDRE_BF16X4: return oz_inverse_s<4>(p, flm, nf, f0, nfp, F, oz, st);
+    case CMM_LEGENDRE_BF16X5: return oz_inverse_s<5>(p, flm, nf, f0, nfp, F, oz, st);
+    default: return oz_inverse_s<6>(p, flm, nf, f0, nfp, F, oz, st);
+  }
+}
+
 extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf, double* f, void* work,
                                void* stream) {
   if (!p || !flm || !f || !work || nf < 1) return fail(CMM_E_ARG, "cmm_sht_inverse: bad arguments");
@@ -1242,14 +1580,24 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   const dim3 grid((unsigned)(L * nrb)), block(64);
   // staged recurrence pairs: at most ceil(L / 8) + 2 chunks of 8 degrees
   const size_t lds_rec = (size_t)((L + kInvDeg - 1) / kInvDeg + 2) * kInvDeg * sizeof(double2);
-  for (int f0 = 0; f0 < nf;) {
-    const int nfp = std::min(16, nf - f0);
-    auto kern = p->legendre == CMM_LEGENDRE_FP32 ? (nfp > 8 ? k_leg_inverse<2, true> : k_leg_inverse<1, true>)
-                                                 : (nfp > 8 ? k_leg_inverse<2, false> : k_leg_inverse<1, false>);
-    hipLaunchKernelGGL(kern, grid, block, lds_rec, st, L, nf, f0, nfp, nrb, p->x_t, p->dP_t, p->dS_t, p->rec, flm,
-                       reinterpret_cast<double*>(F));
-    CMM_LAUNCH_CHECK("cmm_sht_inverse: legendre");
-    f0 += nfp;
+  if (p->legendre >= CMM_LEGENDRE_BF16X3) {
+    char* oz = reinterpret_cast<char*>(work) + sht_f_bytes(p, nf);
+    for (int f0 = 0; f0 < nf;) {
+      const int nfp = std::min(16, nf - f0);
+      const int rc = oz_inverse(p, flm, nf, f0, nfp, F, oz, st);
+      if (rc) return rc;
+      f0 += nfp;
+    }
+  } else {
+    for (int f0 = 0; f0 < nf;) {
+      const int nfp = std::min(16, nf - f0);
+      auto kern = p->legendre == CMM_LEGENDRE_FP32 ? (nfp > 8 ? k_leg_inverse<2, true> : k_leg_inverse<1, true>)
+                                                   : (nfp > 8 ? k_leg_inverse<2, false> : k_leg_inverse<1, false>);
+      hipLaunchKernelGGL(kern, grid, block, lds_rec, st, L, nf, f0, nfp, nrb, p->x_t, p->dP_t, p->dS_t, p->rec, flm,
+                         reinterpret_cast<double*>(F));
+      CMM_LAUNCH_CHECK("cmm_sht_inverse: legendre");
+      f0 += nfp;
+    }
   }
   int pairs = (L + 2) / 2;
   size_t lds = (size_t)2 * p->N * sizeof(double2);
