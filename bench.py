@@ -63,6 +63,8 @@ def parse():
                     help="instead: gradient of the final-vorticity misfit for a batch of B initial fields "
                          "(L=256, ico-6, 50 steps dt=1/256, remap every 10, l <= 64) with central-difference "
                          "checks on 32 random directions")
+    ap.add_argument("--keep-gib", type=float, default=48.0,
+                    help="device memory for kept remap-segment tapes in the --adjoint comparison run")
     ap.add_argument("--fd-dirs", type=int, default=32)
     ap.add_argument("--small", action="store_true", help="quick check: L=64 ico-4 Nt=20 Lf=256")
     ap.add_argument("--save-maps", default=None, help="write the submap jets (torch.save) after the run")
@@ -295,6 +297,22 @@ def adjoint_bench(a):
             f"reverse {t['reverse_s'] * 1e3:.1f} ms ({t['reverse_s'] / t['forward_s']:.2f}x)")
         if k == 0:
             g0, J0 = g, J
+    # the same gradients with the tapes of the remap segments kept in device
+    # memory (budget a.keep_gib) instead of recomputed from the checkpoints
+    budget = int(a.keep_gib * (1 << 30))
+    kept = []
+    for k, p in enumerate(ps):
+        J, g = prob.value_and_grad(p, keep_bytes=budget)
+        t = dict(prob.timings)
+        kept.append({"seed": 500 + k, "forward_s": round(t["forward_s"], 5), "reverse_s": round(t["reverse_s"], 5),
+                     "reverse_over_forward": round(t["reverse_s"] / t["forward_s"], 3)})
+        if k == 0:
+            kept_diff = float((g - g0).norm() / g0.norm())
+    nkept = sum(1 for k in range(len(prob.ns)) if prob._segment_bytes(k) <= budget)
+    kfwd = float(np.median([m["forward_s"] for m in kept]))
+    krev = float(np.median([m["reverse_s"] for m in kept]))
+    log(f"kept tapes ({a.keep_gib} GiB budget): forward {kfwd * 1e3:.1f} ms reverse {krev * 1e3:.1f} ms "
+        f"({krev / kfwd:.2f}x), gradient rel. diff vs recomputation {kept_diff:.1e}")
     rng = np.random.default_rng(51)
     fd = []
     for i in range(a.fd_dirs):
@@ -323,6 +341,10 @@ def adjoint_bench(a):
                                   "checkpoint at every remap with recomputation", "members": a.adjoint},
            "forward_s_median": round(fwd, 5), "reverse_s_median": round(rev, 5),
            "forward_ms_per_step": round(fwd / Nt * 1e3, 4), "members": members,
+           "kept_tapes": {"budget_gib": a.keep_gib, "segment_gib": round(prob._segment_bytes(1) / (1 << 30), 3),
+                          "forward_s_median": round(kfwd, 5), "reverse_s_median": round(krev, 5),
+                          "reverse_over_forward": round(krev / kfwd, 4),
+                          "grad_rel_diff_vs_recompute": kept_diff, "members": kept},
            "fd_check": {"directions": len(fd), "seed": 51,
                         "rel_err_h1e-6": {"median": float(np.median(e6)), "max": float(e6.max())},
                         "rel_err_h1e-5": {"median": float(np.median(e5)), "max": float(e5.max())},
@@ -352,14 +374,16 @@ def area_distortion(sim, q):
 
 _MIXED = {"fp32": "fp32 matrix cores, fp64 recurrence, fp64-compensated 32-degree partial sums",
           "bf16x3": "split-bf16, 3 fixed-point digits per operand (6 digit products) on bf16 matrix cores, "
-                    "exact fp32 sums of the two leading weights, fp64 flush every 256 degrees per parity",
+                    "exact fp32 sums of the three leading weights, fp64 flush every 192 degrees per parity",
           "bf16x5": "split-bf16, 5 fixed-point digits per operand (15 digit products) on bf16 matrix cores, "
-                    "exact fp32 sums of the two leading weights, fp64 flush every 256 degrees per parity"}
+                    "exact fp32 sums of the three leading weights, fp64 flush every 192 degrees per parity",
+          "bf16x6": "split-bf16, 6 fixed-point digits per operand (21 digit products) on bf16 matrix cores, "
+                    "exact fp32 sums of the three leading weights, fp64 flush every 192 degrees per parity"}
 
 
 def mixed_run(a, mesh, w0, wf64, grid, s64, sim64, mode):
     """The same CMM run with the Legendre synthesis in a lower-precision form
-    (``mode``: fp32 or split-bf16 with 3 / 5 digits; FP64 recurrence):
+    (``mode``: fp32 or split-bf16 with 3 / 5 / 6 digits; FP64 recurrence):
     seconds per steady step, the relative max-norm vorticity difference on the
     fine grid, and the relative max-norm difference of every intermediate
     output against the FP64 run: psi_lm, u and grad u on the grid and the

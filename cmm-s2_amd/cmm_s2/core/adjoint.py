@@ -274,48 +274,82 @@ class MisfitProblem(object):
         return J
 
     def _state(self):
+        """Loop state at a remap.  Velocity levels and maps are written once
+        and never modified, so the checkpoint holds references, not copies."""
         return dict(count=self.count, curr=self.curr, V=self.V.copy(), submaps=list(self.submaps),
-                    U={k: self.U[k].clone() for k in self.V.ids}, next_vid=self.next_vid,
-                    next_mid=self.next_mid)
+                    U={k: self.U[k] for k in self.V.ids}, next_vid=self.next_vid, next_mid=self.next_mid)
 
     def _restore(self, st):
         self.count, self.curr, self.V = st["count"], st["curr"], st["V"].copy()
         self.submaps = list(st["submaps"])
-        self.U = {k: v.clone() for k, v in st["U"].items()}
+        self.U = dict(st["U"])
         self.next_vid, self.next_mid = st["next_vid"], st["next_mid"]
 
-    def _run(self, p, checkpoints=None):
+    def _segment_bytes(self, k):
+        """Device bytes of the tape of remap segment k (its velocity levels,
+        maps, sample positions and feet)."""
+        steps = self.ns[k] - (self.ns[k - 1] if k > 0 else 0) + (2 if k == 0 else 0)
+        per = self._tu * 864 + self.mesh.ntri * 864 + self.n * 24 + self.mesh.nv * 120
+        return steps * per
+
+    def _run(self, p, checkpoints=None, keep_bytes=0):
+        """The forward loop.  With ``checkpoints`` (a list) the state at every
+        remap is appended; segments whose tape fits in ``keep_bytes`` (from
+        the last segment backwards) keep it, with their velocity levels and
+        maps, in self._kept for a reverse sweep without recomputation."""
         self._omega0(p)
         self.V = _Velocity()
         self.U, self.maps, self.submaps = {}, {}, []
         self.curr = None
-        self._bootstrap(None)
+        self._kept = {}
+        keep = set()
+        budget = keep_bytes
+        for k in range(len(self.ns) - 1, -1, -1):
+            if budget >= self._segment_bytes(k):
+                budget -= self._segment_bytes(k)
+                keep.add(k)
+        seg = 0
+        tape = [] if 0 in keep else None
+        self._bootstrap(tape)
         if checkpoints is not None:
             checkpoints.append(None)  # segment 0 starts from scratch
         while self.count < self.Nt:
             prev = self.curr
-            remap = self._step(None)
-            # free maps no later step reads: the previous current map
-            if prev is not None and prev not in self.submaps:
-                self.maps.pop(prev, None)
-            for k in list(self.U):
-                if k not in self.V.ids:
-                    del self.U[k]
-            if remap and self.count < self.Nt and checkpoints is not None:
-                checkpoints.append(self._state())
+            remap = self._step(tape)
+            if remap and tape is not None:
+                self._kept[seg] = (tape, dict(self.U), dict(self.maps))
+            if tape is None or remap:
+                # free what no later step reads: the previous current map and
+                # the velocity levels that left the time window
+                if prev is not None and prev not in self.submaps:
+                    self.maps.pop(prev, None)
+                for k in list(self.maps):
+                    if k not in self.submaps and k != self.curr:
+                        del self.maps[k]
+                for k in list(self.U):
+                    if k not in self.V.ids:
+                        del self.U[k]
+            if remap:
+                seg += 1
+                tape = [] if seg in keep else None
+                if self.count < self.Nt and checkpoints is not None:
+                    checkpoints.append(self._state())
 
     # ---- public ------------------------------------------------------------
     def value(self, p):
         self._run(p)
         return self._objective(None)
 
-    def forward(self, p):
-        """J(p); keeps the remap checkpoints for ``backward``."""
+    def forward(self, p, keep_bytes=0):
+        """J(p); keeps the remap checkpoints for ``backward``.  Remap
+        segments whose tape fits in ``keep_bytes`` of device memory (taken
+        from the last segment backwards) keep it, so the reverse sweep skips
+        their recomputation; 0 recomputes every segment from its checkpoint."""
         import time
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         self._cps = []
-        self._run(p, checkpoints=self._cps)
+        self._run(p, checkpoints=self._cps, keep_bytes=keep_bytes)
         J = self._objective(None)
         torch.cuda.synchronize()
         self.timings = {"forward_s": time.perf_counter() - t0}
@@ -334,7 +368,17 @@ class MisfitProblem(object):
         tape = []
         self._objective(tape)
         self._reverse(tape)
+        kept = self._kept
+        self._kept = {}
         for k in range(len(cps) - 1, -1, -1):
+            if k in kept:
+                tape, U, maps = kept.pop(k)
+                self.U = U
+                self.maps = dict(final_maps)
+                self.maps.update(maps)
+                self._reverse(tape)
+                del tape, U, maps
+                continue
             tape = []
             self.maps = dict(final_maps)
             if cps[k] is None:
@@ -355,8 +399,8 @@ class MisfitProblem(object):
         self._cps = None
         return g
 
-    def value_and_grad(self, p):
-        J = self.forward(p)
+    def value_and_grad(self, p, keep_bytes=0):
+        J = self.forward(p, keep_bytes)
         return J, self.backward()
 
     # ---- reverse sweep -----------------------------------------------------

@@ -408,6 +408,22 @@ def test_loop_gradient_fd_small():
     assert float((g2 - g).norm()) <= 1e-12 * float(g.norm())
 
 
+def test_kept_tapes_match_recomputation():
+    """Keeping the tape of every remap segment, of the last one only, or of
+    none (recomputation from the checkpoints) gives the same J and dJ/dp (to
+    the atomic summation order, ~1e-13)."""
+    from cmm_s2.core import dynamical_fields as df
+    prob = _small_problem()
+    p = torch.from_numpy(prob.params.from_coeffs(df.random_sh_coefficients(502, 8, 32))).cuda()
+    J0, g0 = prob.value_and_grad(p)
+    last = prob._segment_bytes(len(prob.ns) - 1)
+    for budget in (1 << 40, last):
+        J, g = prob.value_and_grad(p, keep_bytes=budget)
+        assert J == J0
+        assert float((g - g0).norm()) <= 1e-12 * float(g0.norm()), budget
+    assert not prob._kept
+
+
 def test_loop_value_matches_driver():
     """J recomputed from BarotropicCMM's own run of the same loop."""
     from cmm_s2.core import dynamical_fields as df, spherical_harmonic_tools as st
