@@ -1651,6 +1651,132 @@ extern "C" int cmm_sht_scalar_operators(const cmm_sht_plan* p, const double* w, 
 }
 
 // ---- adjoints -------------------------------------------------------------
+// Transposed synthesis (adjoint of the inverse Legendre stage) on FP64 matrix
+// cores: flm_bar[lm][f] = msc[m] sum_t Pbar_lm(x_t) G[f][t][m] over the L + 1
+// MWSS rings (the pole rings skipped for m > 0), folded by parity:
+// S_t = G_t + G_{L-t} for even l - m, D_t = G_t - G_{L-t} for odd.  One block
+// per (order m, chunk c of 32 degrees; blockIdx.x = offJ[m] + c) and group of
+// <= 16 fields; wave w takes northern rings 64 w .. 64 w + 63 (lane = ring),
+// restarts the recurrence from the analysis restart state of its 128-degree
+// unit, writes the 16 even-degree values of the chunk into an LDS tile
+// [degree][ring] and multiplies (v_mfma_f64_16x16x4f64: A = 16 degrees x 4
+// rings from the tile, B = 4 rings x 16 columns of S from G), then the odd
+// degrees with D.  The waves' partial sums meet in LDS.
+template <int NT>
+__global__ __launch_bounds__(320) void k_leg_adj_mfma(int L, int nf, int f0, int nfp, const int* __restrict__ offJ,
+                                                      const int* __restrict__ unit0, const double* __restrict__ xr,
+                                                      const double2* __restrict__ st_p, const int* __restrict__ st_s,
+                                                      const double2* __restrict__ rec, const double* __restrict__ G,
+                                                      const double* __restrict__ msc, double* __restrict__ flm) {
+  __shared__ double tile[5][16][65];  // per wave: 16 degrees x 64 rings (+1 padding)
+  const int b = (int)blockIdx.x;
+  int lo = 0, hi = L;  // largest m with offJ[m] <= b
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offJ[mid] <= b) lo = mid; else hi = mid;
+  }
+  const int m = lo, c = b - offJ[m];
+  const int nn = L + 1, nh = L / 2 + 1;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int j = 64 * w + lane;
+  const bool act = j < nh;
+  const int jj = act ? j : 0;
+  const int lc = m + 32 * c;
+  const int su = (32 * c) / kFwdSeg;
+  const int u = unit0[m] + su;
+  const int ls = m + kFwdSeg * su;
+  const double x = xr[jj];
+  const double2 ps = st_p[(int64_t)u * nh + jj];
+  double pp = ps.x, pc = ps.y;
+  int s = st_s[(int64_t)u * nh + jj];
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  // advance from the unit start to the chunk (rescale every 16 degrees, as the analysis)
+  for (int l = ls; l < lc; l += 16) {
+    leg_rescale(pp, pc, s);
+#pragma unroll 4
+    for (int i = 0; i < 16; ++i) {
+      const int ll = l + i;
+      if (ll > m) {
+        const double2 ab = rec[lm0 + (ll - m)];
+        leg_step(ab.x, ab.y, x, pp, pc);
+      }
+    }
+  }
+  double odd[16];
+  bool live = false;
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    if ((i & 15) == 0) {
+      leg_rescale(pp, pc, s);
+      live = (s == 0) & act;
+    }
+    const int ll = lc + i;
+    if (ll > m) {
+      const double2 ab = rec[lm0 + (min(ll, L - 1) - m)];
+      leg_step(ab.x, ab.y, x, pp, pc);
+    }
+    const double pv = (live & (ll < L)) ? pc : 0.0;
+    if (i & 1) odd[i >> 1] = pv;
+    else tile[w][i >> 1][lane] = pv;
+  }
+  const double cm = msc[m];
+  // B operand (S for p = 0, D for p = 1) of ring 64 w + 4 k + lane / 16, column 16 nt + lane % 16
+  auto bval = [&](int k, int nt, int p) {
+    const int jr = 64 * w + 4 * k + (lane >> 4);
+    const int col = 16 * nt + (lane & 15);
+    const int f = min(col >> 1, nfp - 1);
+    const bool ok = jr < nh && !(m > 0 && jr == 0);
+    const int jc = ok ? jr : 0;
+    const int jm = nn - 1 - jc;
+    const double ga = G[(((int64_t)(f0 + f) * nn + jc) * L + m) * 2 + (col & 1)];
+    const double gb = G[(((int64_t)(f0 + f) * nn + jm) * L + m) * 2 + (col & 1)];
+    const double wgt = ok ? cm : 0.0;
+    return jm == jc ? wgt * ga : wgt * (p == 0 ? ga + gb : ga - gb);
+  };
+  dx4 acc[2][NT];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[p][nt] = dx4{0.0, 0.0, 0.0, 0.0};
+  __syncthreads();
+#pragma unroll 4
+  for (int k = 0; k < 16; ++k) {
+    const double a = tile[w][lane & 15][4 * k + (lane >> 4)];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+      acc[0][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bval(k, nt, 0), acc[0][nt], 0, 0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 16; ++i) tile[w][i][lane] = odd[i];
+  __syncthreads();
+#pragma unroll 4
+  for (int k = 0; k < 16; ++k) {
+    const double a = tile[w][lane & 15][4 * k + (lane >> 4)];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+      acc[1][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bval(k, nt, 1), acc[1][nt], 0, 0, 0);
+  }
+  __syncthreads();
+  double* red = &tile[w][0][0];  // [p][nt][e][lane]
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) red[((p * NT + nt) * 4 + e) * 64 + lane] = acc[p][nt][e];
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  for (int o = threadIdx.x; o < 2 * NT * 4 * 64; o += blockDim.x) {
+    double v = 0.0;
+    for (int ww = 0; ww < nw; ++ww) v += (&tile[ww][0][0])[o];
+    const int ln = o & 63, e = (o >> 6) & 3, pn = o >> 8, nt = pn % NT, p = pn / NT;
+    const int row = 4 * e + (ln >> 4), col = 16 * nt + (ln & 15);
+    const int l = lc + 2 * row + p, f = col >> 1;
+    if (l < L && f < nfp) flm[((lm0 + (l - m)) * nf + f0 + f) * 2 + (col & 1)] = v;
+  }
+}
+
 extern "C" int cmm_sht_inverse_adjoint(const cmm_sht_plan* p, const double* fbar, int nf, double* flm_bar,
                                        void* work, void* stream) {
   if (!p || !fbar || !flm_bar || !work || nf < 1) return fail(CMM_E_ARG, "cmm_sht_inverse_adjoint: bad arguments");
@@ -1662,6 +1788,17 @@ extern "C" int cmm_sht_inverse_adjoint(const cmm_sht_plan* p, const double* fbar
   hipLaunchKernelGGL(k_fft_r2c, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, fbar, L, F);
   CMM_LAUNCH_CHECK("cmm_sht_inverse_adjoint: fft");
   const int nhr = L / 2 + 1;
+  if (nhr <= 5 * 64) {  // matrix-core form: every field of a group shares the recurrence
+    const int nw = (nhr + 63) / 64;
+    for (int f0 = 0; f0 < nf; f0 += 16) {
+      const int nfp = std::min(16, nf - f0);
+      auto kern = nfp > 8 ? k_leg_adj_mfma<2> : k_leg_adj_mfma<1>;
+      hipLaunchKernelGGL(kern, dim3((unsigned)p->nJtot), dim3(64 * nw), 0, st, L, nf, f0, nfp, p->offJ, p->unit0,
+                         p->x_t, p->st_pr, p->st_sr, p->rec, reinterpret_cast<const double*>(F), p->msc_r, flm_bar);
+      CMM_LAUNCH_CHECK("cmm_sht_inverse_adjoint: legendre");
+    }
+    return CMM_OK;
+  }
   const int T = std::max(64, ((nhr + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
   if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_inverse_adjoint: L > 1024");
   const size_t lds2 = (size_t)kFwdChunk * (T + 4) * sizeof(double2);

@@ -41,7 +41,7 @@ def _work(records):
     return t.data_ptr(), b.value
 
 
-@pytest.mark.parametrize("L", [16, 24, 64])
+@pytest.mark.parametrize("L", [16, 24, 64, 256, 512])
 def test_sht_transposes(L):
     from cmm_s2 import _native as N
     from cmm_s2.core._backend import stream
