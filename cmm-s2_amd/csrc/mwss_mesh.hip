@@ -731,18 +731,32 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj(cmm_mwss_mesh M, cons
 // non-pole vertex: its (at most 6) incident (triangle, corner) pairs in closed
 // form, added into vb; one block per pole reduces the 2L cap triangles.
 template <int NC>
-__global__ __launch_bounds__(64) void k_mwss_assemble_adj_corners(cmm_mwss_mesh M, const double* __restrict__ pb,
-                                                                  double* __restrict__ tmp) {
-  // one wave per 64 triangles; each component's 36 adjoints per triangle are
-  // staged through LDS so the packed blocks are read coalesced
-  __shared__ double sh[64 * 37];
-  const int lane = threadIdx.x;
-  const int64_t t0 = (int64_t)blockIdx.x * 64, t = t0 + lane;
-  const bool live = t < M.Tu;
+__global__ __launch_bounds__(64 * NC) void k_mwss_assemble_adj_corners(cmm_mwss_mesh M, const double* __restrict__ pb,
+                                                                       double* __restrict__ tmp) {
+  // 64 triangles per block: their packed blocks (contiguous) are staged
+  // through LDS with coalesced 16-byte loads, then thread (component cc,
+  // triangle tt) forms that component's corner adjoints
+  constexpr int W = 36 * NC;  // doubles per triangle
+  __shared__ double sh[64 * (W + 1)];
+  const int64_t t0 = (int64_t)blockIdx.x * 64;
+  const int64_t Tu = M.Tu;
+  const int64_t nvalid = min((int64_t)64, Tu - t0) * W;  // doubles of this block in pb
+  const double2* src = reinterpret_cast<const double2*>(pb + t0 * W);
+#pragma unroll 4
+  for (int q = threadIdx.x; q < 32 * W; q += 64 * NC) {
+    const int e = 2 * q, tt = e / W, r = e - W * tt;
+    const double2 v = e < nvalid ? src[q] : double2{0.0, 0.0};
+    sh[tt * (W + 1) + r] = v.x;
+    sh[tt * (W + 1) + r + 1] = v.y;
+  }
+  __syncthreads();
+  const int tt = threadIdx.x & 63, cc = threadIdx.x >> 6;
+  const int64_t t = t0 + tt;
+  if (t >= Tu) return;
   const int N = M.N, Mr = M.M;
   int row = 0, col = 0, half = 0;
   if (t < N) { row = 0; col = (int)t; half = 0; }
-  else if (t >= M.Tu - N) { row = Mr - 2; col = (int)(live ? t - (M.Tu - N) : 0); half = 0; }
+  else if (t >= Tu - N) { row = Mr - 2; col = (int)(t - (Tu - N)); half = 0; }
   else {
     const int64_t u = t - N;
     half = (int)(u % 2);
@@ -758,50 +772,75 @@ __global__ __launch_bounds__(64) void k_mwss_assemble_adj_corners(cmm_mwss_mesh 
     const d3 h = P.H[i];
     P.H[i] = d3{c * h.x - sn * h.y, sn * h.x + c * h.y, h.z};
   }
-  const int64_t Tu = M.Tu;
   constexpr int R = 4 * NC;
+  double cb[19];
 #pragma unroll
-  for (int cc = 0; cc < NC; ++cc) {
-    for (int q = lane; q < 64 * 36; q += 64) {
-      const int tt = q / 36, r = q - 36 * tt, ss = r / 6, j = r - 6 * ss;
-      const int64_t T = t0 + tt;
-      sh[tt * 37 + r] = T < Tu ? pb[T * (36 * NC) + (ss * NC + cc) * 6 + j] : 0.0;
+  for (int k = 0; k < 19; ++k) cb[k] = 0.0;
+#pragma unroll
+  for (int ss = 0; ss < 6; ++ss)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += sh[tt * (W + 1) + (ss * NC + cc) * 6 + j];
+  double fb[3] = {0.0, 0.0, 0.0};
+  d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+  ps_coeffs_adj(P, cb, fb, gb);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (NC == 1) {
+      tmp[(R * i + 0) * Tu + t] = fb[i];
+      tmp[(R * i + 1) * Tu + t] = gb[i].x;
+      tmp[(R * i + 2) * Tu + t] = gb[i].y;
+      tmp[(R * i + 3) * Tu + t] = gb[i].z;
+    } else {
+      tmp[(R * i + cc * 4) * Tu + t] = fb[i];
+      tmp[(R * i + 0 * 4 + 1 + cc) * Tu + t] = gb[i].x;
+      tmp[(R * i + 1 * 4 + 1 + cc) * Tu + t] = gb[i].y;
+      tmp[(R * i + 2 * 4 + 1 + cc) * Tu + t] = gb[i].z;
     }
-    __syncthreads();
-    if (live) {
-      double cb[19];
-#pragma unroll
-      for (int k = 0; k < 19; ++k) cb[k] = 0.0;
-#pragma unroll
-      for (int ss = 0; ss < 6; ++ss)
-#pragma unroll
-        for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += sh[lane * 37 + ss * 6 + j];
-      double fb[3] = {0.0, 0.0, 0.0};
-      d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
-      ps_coeffs_adj(P, cb, fb, gb);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        if (NC == 1) {
-          tmp[(R * i + 0) * Tu + t] = fb[i];
-          tmp[(R * i + 1) * Tu + t] = gb[i].x;
-          tmp[(R * i + 2) * Tu + t] = gb[i].y;
-          tmp[(R * i + 3) * Tu + t] = gb[i].z;
-        } else {
-          tmp[(R * i + cc * 4) * Tu + t] = fb[i];
-          tmp[(R * i + 0 * 4 + 1 + cc) * Tu + t] = gb[i].x;
-          tmp[(R * i + 1 * 4 + 1 + cc) * Tu + t] = gb[i].y;
-          tmp[(R * i + 2 * 4 + 1 + cc) * Tu + t] = gb[i].z;
-        }
-      }
-    }
-    __syncthreads();
   }
 }
 
+// pole p (0 north: corner 0 of triangles 0..N-1; 1 south: corner 1 of the
+// last N triangles), reduced by one block of 256 threads
+template <int NC>
+__device__ inline void mwss_pole_reduce(const cmm_mwss_mesh& M, const double* __restrict__ tmp,
+                                        double* __restrict__ vb, int p) {
+  constexpr int R = 4 * NC;
+  __shared__ double red[R][4];
+  const int N = M.N;
+  const int64_t Tu = M.Tu;
+  const int64_t t0 = p == 0 ? 0 : Tu - N;
+  const int corner = p == 0 ? 0 : 1;
+  const int64_t v = p == 0 ? 0 : M.Nu - 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double a[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) a[q] = 0.0;
+  for (int c = threadIdx.x; c < N; c += 256)
+#pragma unroll
+    for (int q = 0; q < R; ++q) a[q] += tmp[(R * corner + q) * Tu + t0 + c];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a[q] += __shfl_xor(a[q], o);
+    if (lane == 0) red[q][w] = a[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < R)
+    vb[threadIdx.x * M.Nu + v] += ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) +
+                                  red[threadIdx.x][3];
+}
+
+// blocks 0 .. ceil((Nu - 2) / 256) - 1: the interior vertices; the last two
+// blocks: the poles
 template <int NC>
 __global__ __launch_bounds__(256) void k_mwss_assemble_adj_gather(cmm_mwss_mesh M, const double* __restrict__ tmp,
                                                                   double* __restrict__ vb) {
-  const int64_t v = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // poles handled apart
+  const int nbi = (int)((M.Nu - 2 + 255) / 256);
+  if ((int)blockIdx.x >= nbi) {
+    mwss_pole_reduce<NC>(M, tmp, vb, (int)blockIdx.x - nbi);
+    return;
+  }
+  const int64_t v = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= M.Nu - 1) return;
   const int N = M.N, L = M.L;
   const int r = 1 + (int)((v - 1) / N), c = (int)((v - 1) % N);
@@ -827,40 +866,16 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj_gather(cmm_mwss_mesh 
   double acc[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) acc[q] = 0.0;
-  for (int e = 0; e < k; ++e) {
 #pragma unroll
-    for (int q = 0; q < R; ++q) acc[q] += tmp[(R * cs[e] + q) * Tu + ts[e]];
+  for (int e = 0; e < 6; ++e) {  // fixed trip count: every load of the vertex in flight together
+    if (e < k) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) acc[q] += tmp[(R * cs[e] + q) * Tu + ts[e]];
+    }
   }
   const int64_t Nu = M.Nu;
 #pragma unroll
   for (int q = 0; q < R; ++q) vb[q * Nu + v] += acc[q];
-}
-
-// pole p (0 north: corner 0 of triangles 0..N-1; 1 south: corner 1 of the
-// last N triangles), one block each
-template <int NC>
-__global__ __launch_bounds__(256) void k_mwss_assemble_adj_poles(cmm_mwss_mesh M, const double* __restrict__ tmp,
-                                                                 double* __restrict__ vb) {
-  constexpr int R = 4 * NC;
-  __shared__ double red[256];
-  const int p = blockIdx.x;
-  const int N = M.N;
-  const int64_t Tu = M.Tu;
-  const int64_t t0 = p == 0 ? 0 : Tu - N;
-  const int corner = p == 0 ? 0 : 1;
-  const int64_t v = p == 0 ? 0 : M.Nu - 1;
-  for (int q = 0; q < R; ++q) {
-    double a = 0.0;
-    for (int c = threadIdx.x; c < N; c += blockDim.x) a += tmp[(R * corner + q) * Tu + t0 + c];
-    red[threadIdx.x] = a;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) vb[q * M.Nu + v] += red[0];
-    __syncthreads();
-  }
 }
 
 // located velocity-spline point: triangle, sub-triangle, sub-triangle
@@ -1296,13 +1311,13 @@ extern "C" int cmm_mwss_assemble_adjoint(const cmm_mwss_mesh* m, const double* p
     hipStream_t st = as_stream(stream);
     const int64_t ni = m->Nu - 2;
     if (ncomp == 3) {
-      hipLaunchKernelGGL(k_mwss_assemble_adj_corners<3>, dim3(grid_for(m->Tu, 64)), dim3(64), 0, st, *m, packed_bar, tmp);
-      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<3>, dim3(grid_for(ni, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
-      hipLaunchKernelGGL(k_mwss_assemble_adj_poles<3>, dim3(2), dim3(256), 0, st, *m, tmp, vals_bar);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_corners<3>, dim3(grid_for(m->Tu, 64)), dim3(192), 0, st, *m, packed_bar, tmp);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<3>, dim3(grid_for(ni, 256) + 2), dim3(256), 0, st, *m, tmp,
+                         vals_bar);
     } else {
       hipLaunchKernelGGL(k_mwss_assemble_adj_corners<1>, dim3(grid_for(m->Tu, 64)), dim3(64), 0, st, *m, packed_bar, tmp);
-      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<1>, dim3(grid_for(ni, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
-      hipLaunchKernelGGL(k_mwss_assemble_adj_poles<1>, dim3(2), dim3(256), 0, st, *m, tmp, vals_bar);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<1>, dim3(grid_for(ni, 256) + 2), dim3(256), 0, st, *m, tmp,
+                         vals_bar);
     }
     CMM_LAUNCH_CHECK("cmm_mwss_assemble_adjoint");
     return CMM_OK;

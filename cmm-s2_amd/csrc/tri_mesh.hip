@@ -812,44 +812,46 @@ __global__ __launch_bounds__(256) void k_tri_assemble_adj(cmm_tri_mesh M, const 
 // component c, d 4 + 1 + c = its derivative along axis d).  Phase 2, one
 // thread per vertex: the sum over its incidence list, added into vb (the only
 // writer of that vertex).  Deterministic.
-__global__ __launch_bounds__(64) void k_tri_assemble_adj_corners(cmm_tri_mesh M, const double* __restrict__ pb,
-                                                                 double* __restrict__ tmp) {
-  // one wave per 64 triangles, packed blocks staged through LDS per component
-  __shared__ double sh[64 * 37];
-  const int lane = threadIdx.x;
-  const int64_t t0 = (int64_t)blockIdx.x * 64, t = t0 + lane;
-  const bool live = t < M.ntri;
+__global__ __launch_bounds__(192) void k_tri_assemble_adj_corners(cmm_tri_mesh M, const double* __restrict__ pb,
+                                                                  double* __restrict__ tmp) {
+  // 64 triangles per block: their packed blocks (contiguous) are staged
+  // through LDS with coalesced 16-byte loads, then thread (component cc,
+  // triangle tt) forms that component's corner adjoints
+  constexpr int W = 108;
+  __shared__ double sh[64 * (W + 1)];
+  const int64_t t0 = (int64_t)blockIdx.x * 64;
   const int64_t nt = M.ntri;
+  const int64_t nvalid = min((int64_t)64, nt - t0) * W;
+  const double2* src = reinterpret_cast<const double2*>(pb + t0 * W);
+#pragma unroll 4
+  for (int q = threadIdx.x; q < 32 * W; q += 192) {
+    const int e = 2 * q, tt = e / W, r = e - W * tt;
+    const double2 v = e < nvalid ? src[q] : double2{0.0, 0.0};
+    sh[tt * (W + 1) + r] = v.x;
+    sh[tt * (W + 1) + r + 1] = v.y;
+  }
+  __syncthreads();
+  const int tt = threadIdx.x & 63, cc = threadIdx.x >> 6;
+  const int64_t t = t0 + tt;
+  if (t >= nt) return;
   PSGeom P;
-  ps_unpack(M.psgeom + (live ? t : 0), M.ntri, P);
+  ps_unpack(M.psgeom + t, M.ntri, P);
+  double cb[19];
 #pragma unroll
-  for (int cc = 0; cc < 3; ++cc) {
-    for (int q = lane; q < 64 * 36; q += 64) {
-      const int tt = q / 36, r = q - 36 * tt, ss = r / 6, j = r - 6 * ss;
-      const int64_t T = t0 + tt;
-      sh[tt * 37 + r] = T < nt ? pb[T * 108 + (ss * 3 + cc) * 6 + j] : 0.0;
-    }
-    __syncthreads();
-    if (live) {
-      double cb[19];
+  for (int k = 0; k < 19; ++k) cb[k] = 0.0;
 #pragma unroll
-      for (int k = 0; k < 19; ++k) cb[k] = 0.0;
+  for (int ss = 0; ss < 6; ++ss)
 #pragma unroll
-      for (int ss = 0; ss < 6; ++ss)
+    for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += sh[tt * (W + 1) + (ss * 3 + cc) * 6 + j];
+  double fb[3] = {0.0, 0.0, 0.0};
+  d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+  ps_coeffs_adj(P, cb, fb, gb);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) cb[cs_table(ss, j)] += sh[lane * 37 + ss * 6 + j];
-      double fb[3] = {0.0, 0.0, 0.0};
-      d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
-      ps_coeffs_adj(P, cb, fb, gb);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        tmp[(12 * i + cc * 4) * nt + t] = fb[i];
-        tmp[(12 * i + 0 * 4 + 1 + cc) * nt + t] = gb[i].x;
-        tmp[(12 * i + 1 * 4 + 1 + cc) * nt + t] = gb[i].y;
-        tmp[(12 * i + 2 * 4 + 1 + cc) * nt + t] = gb[i].z;
-      }
-    }
-    __syncthreads();
+  for (int i = 0; i < 3; ++i) {
+    tmp[(12 * i + cc * 4) * nt + t] = fb[i];
+    tmp[(12 * i + 0 * 4 + 1 + cc) * nt + t] = gb[i].x;
+    tmp[(12 * i + 1 * 4 + 1 + cc) * nt + t] = gb[i].y;
+    tmp[(12 * i + 2 * 4 + 1 + cc) * nt + t] = gb[i].z;
   }
 }
 
@@ -862,12 +864,17 @@ __global__ __launch_bounds__(256) void k_tri_assemble_adj_gather(cmm_tri_mesh M,
   for (int r = 0; r < 12; ++r) acc[r] = 0.0;
   const int64_t nt = M.ntri;
   const int b = M.vstart[v], e = M.vstart[v + 1];
-  for (int k = b; k < e; ++k) {
-    const int tc = M.vinc[k];
-    const int64_t t = tc / 3;
-    const int i = tc - 3 * (int)t;
+  for (int k0 = b; k0 < e; k0 += 6) {  // 6 incident corners per pass, their loads in flight together
 #pragma unroll
-    for (int r = 0; r < 12; ++r) acc[r] += tmp[(12 * i + r) * nt + t];
+    for (int k = k0; k < k0 + 6; ++k) {
+      if (k < e) {
+        const int tc = M.vinc[k];
+        const int64_t t = tc / 3;
+        const int i = tc - 3 * (int)t;
+#pragma unroll
+        for (int r = 0; r < 12; ++r) acc[r] += tmp[(12 * i + r) * nt + t];
+      }
+    }
   }
   const int64_t Nv = M.nv;
 #pragma unroll
@@ -1137,7 +1144,7 @@ extern "C" int cmm_tri_assemble_adjoint(const cmm_tri_mesh* m, const double* pac
   }
   if (work_bytes < 36 * 8 * m->ntri) return fail(CMM_E_ARG, "cmm_tri_assemble_adjoint: workspace too small");
   double* tmp = static_cast<double*>(work);
-  hipLaunchKernelGGL(k_tri_assemble_adj_corners, dim3(grid_for(m->ntri, 64)), dim3(64), 0, st, *m, packed_bar, tmp);
+  hipLaunchKernelGGL(k_tri_assemble_adj_corners, dim3(grid_for(m->ntri, 64)), dim3(192), 0, st, *m, packed_bar, tmp);
   CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
   hipLaunchKernelGGL(k_tri_assemble_adj_gather, dim3(grid_for(m->nv, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
   CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
