@@ -1,7 +1,7 @@
 // Sorted scatter-add of Bernstein-Bezier adjoint records: radix sort by
 // target block, then one thread per distinct key sums its group and adds it
 // to the target with plain loads and stores.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "scatter.h"
 
@@ -10,11 +10,24 @@ namespace cmm {
 namespace {
 inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
+// Onesweep radix sort at every size (rocPRIM's default switches to a merge
+// sort of ~17 launches below 2^20 items, the record counts here) with 6 key
+// bits per pass and 2048 items per block: measured on 655k / 786k pairs of
+// 21-22-bit keys, 125 / 134 us against 145 / 150 us for the default (and
+// 221 us with 8 bits per pass).
+using SortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<256, 12>, rocprim::kernel_config<256, 8>, 6>, 0>;
+
+hipError_t sort_pairs(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
+                      uint32_t* vout, int64_t n, int end_bit, hipStream_t st) {
+  return rocprim::radix_sort_pairs<SortConfig>(temp, bytes, kin, kout, vin, vout, (unsigned int)n, 0u,
+                                               (unsigned int)end_bit, st);
+}
+
 size_t sort_temp_bytes(int64_t n) {
   size_t bytes = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                    (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32,
-                                                    (hipStream_t)0);
+  hipError_t e = sort_pairs(nullptr, bytes, nullptr, nullptr, nullptr, nullptr, n, 32, (hipStream_t)0);
   return e == hipSuccess ? bytes : 0;
 }
 }  // namespace
@@ -152,8 +165,7 @@ int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hip
   if (n < 1) return CMM_OK;
   if (n > w.n) return fail(CMM_E_ARG, "scatter: more records than the workspace holds");
   size_t tb = w.temp_bytes;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(w.temp, tb, w.keys, w.keys_s, w.idx, w.idx_s, (int)n, 0, w.end_bit,
-                                                    st);
+  hipError_t e = sort_pairs(w.temp, tb, w.keys, w.keys_s, w.idx, w.idx_s, n, w.end_bit, st);
   if (e != hipSuccess) return hip_check(e, "scatter: radix sort");
   const int grid = grid_for(n, 256);
   if (t.mode == 0) {
