@@ -84,7 +84,7 @@ SIGNATURES = {
     "cmm_tri_stencil_jets_adjoint": (C.c_int, [vp, vp, vp, i64, vp, i64, vp, dbl, vp, vp, i64, vp, vp, i64, vp]),
     "cmm_tri_compose_adjoint": (C.c_int, [vp, P_vp, P_vp, C.c_int, vp, i64, i64, i64, vp, vp, vp, i64, vp]),
     "cmm_sample_vorticity_adjoint": (C.c_int, [vp, vp, vp, i64, i64, i64, vp, dbl, vp, i64, i64, vp, vp, i64, vp]),
-    "cmm_scatter_workspace_bytes": (C.c_int, [i64, P_i64]),
+    "cmm_scatter_workspace_bytes": (C.c_int, [i64, i64, P_i64]),
 }
 
 

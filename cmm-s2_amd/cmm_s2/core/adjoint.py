@@ -143,7 +143,7 @@ class MisfitProblem(object):
         self._nu, self._tu = nu, tu
         self.w0 = None
         self.timings = {}
-        self._sw = None  # sorted scatter-add workspace (uint8)
+        self._sw = None  # adjoint workspace (uint8)
 
     def _ws(self, nbytes):
         """(pointer, bytes) of the adjoint workspace (one buffer, grown on
@@ -152,11 +152,12 @@ class MisfitProblem(object):
             self._sw = torch.empty(int(nbytes), dtype=torch.uint8, device=B.device())
         return self._sw.data_ptr(), self._sw.numel()
 
-    def _scatter(self, records):
-        """workspace for ``records`` sorted scatter-add records"""
+    def _scatter(self, records, key_range):
+        """workspace for ``records`` grouped scatter-add records with keys
+        below ``key_range``"""
         import ctypes
         b = ctypes.c_int64(0)
-        N.call("cmm_scatter_workspace_bytes", int(records), ctypes.byref(b))
+        N.call("cmm_scatter_workspace_bytes", int(records), int(key_range), ctypes.byref(b))
         return self._ws(b.value)
 
     # ---- building blocks --------------------------------------------------
@@ -429,7 +430,7 @@ class MisfitProblem(object):
             lv = N.ptr_array([self.maps[i].data_ptr() for i in chain])
             lb = N.ptr_array([self._lam_map(i).data_ptr() for i in chain])
             q = self.samples
-            wp, wb = self._scatter(len(chain) * n)
+            wp, wb = self._scatter(len(chain) * n, 6 * len(chain) * self.mesh.ntri)
             N.call("cmm_tri_compose_adjoint", self.mesh._h, lv, lb, len(chain), q.data_ptr(), n, q.stride(0),
                    q.stride(1), xbar.data_ptr(), None, wp, wb, B.stream())
 
@@ -486,7 +487,7 @@ class MisfitProblem(object):
                 desc = self._desc(V, ikind, t)
                 lbs = N.ptr_array([self._lam_vel(V.ids[k]).data_ptr() for k in range(min(V.nVs, 3))])
                 xs = m.d_spts
-                wp, wb = self._scatter(16 * nv)
+                wp, wb = self._scatter(16 * nv, 6 * self._tu)
                 N.call("cmm_advect_stencil_adjoint", desc, ikind, float(self.dt), xs.data_ptr(), xs.stride(0), nv,
                        sb.data_ptr(), sb.stride(0), lbs, None, 0, wp, wb, B.stream())
                 if mid not in self.submaps:

@@ -1,13 +1,14 @@
-// Sorted scatter-add of Bernstein-Bezier adjoint records (internal).
+// Grouped scatter-add of Bernstein-Bezier adjoint records (internal).
 //
 // The adjoints of spline evaluation at scattered points add
 //   a_c * basis_j(cb)   (c < NC components, j < 6 basis functions)
 // into the 6 NC coefficients of the located sub-triangle.  Device-wide FP64
 // atomics for that run at ~15 G/s here (they are performed beyond the
 // per-XCD L2), so the adjoint kernels write one record per (point, use)
-// instead: key = target block, a (3 doubles), cb (3 doubles), class.  A radix
-// sort groups equal keys and one thread per key adds its group with plain
-// stores (one writer per target block): deterministic, no atomics.
+// instead: key = target block, a (3 doubles), cb (3 doubles), class.  A
+// counting sort (int32 histogram, one scan, placement by counter) groups
+// equal keys and one thread per group adds it with plain stores (one writer
+// per target block, no FP64 atomics).
 #pragma once
 #include "common.h"
 
@@ -35,15 +36,16 @@ constexpr int64_t kScatterRecBytes = sizeof(ScatterRecV);  // slot size (both ki
 struct ScatterWork {
   uint32_t* keys = nullptr;    // (n) written by the producer; `empty` = unused slot
   uint32_t* idx = nullptr;     // (n) slot index (written by the producer)
-  uint32_t* keys_s = nullptr;  // sorted
+  uint32_t* keys_s = nullptr;  // grouped by key (empty slots last)
   uint32_t* idx_s = nullptr;
+  uint32_t* count = nullptr;   // (empty + 1) records per key, then placement counters
+  uint32_t* offs = nullptr;    // (empty + 1) first grouped position of each key
   ScatterRec* rec = nullptr;   // (n) slots of kScatterRecBytes
   ScatterRecV* recv = nullptr; // the same slots as value records
-  void* temp = nullptr;
+  void* temp = nullptr;        // scan temporary storage
   size_t temp_bytes = 0;
   int64_t n = 0;
   uint32_t empty = kScatterEmpty;  // key of an unused slot: 1 + the largest key
-  int end_bit = 32;                // key bits the sort looks at
 };
 
 // where a group of records goes
@@ -60,12 +62,12 @@ struct ScatterTargets {
   double W[4][3] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
 };
 
-// bytes of a workspace for n records (0 on error)
-int64_t scatter_bytes(int64_t n);
+// bytes of a workspace for n records with keys < key_range (0 on error)
+int64_t scatter_bytes(int64_t n, int64_t key_range);
 // carve a caller workspace for n records with keys < key_range; fails if it
 // is too small
 int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range, ScatterWork& w);
-// sort the n records by key and add every group into its target
+// group the n records by key and add every group into its target
 int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hipStream_t st);
 
 __device__ inline void scatter_put(const ScatterWork& w, int64_t slot, uint32_t key, d3 a, d3 cb, int cls) {

@@ -1,7 +1,7 @@
-// Sorted scatter-add of Bernstein-Bezier adjoint records: radix sort by
+// Grouped scatter-add of Bernstein-Bezier adjoint records: counting sort by
 // target block, then one thread per distinct key sums its group and adds it
 // to the target with plain loads and stores.
-#include <rocprim/device/device_radix_sort.hpp>
+#include <hipcub/hipcub.hpp>
 
 #include "scatter.h"
 
@@ -10,39 +10,26 @@ namespace cmm {
 namespace {
 inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
-// Onesweep radix sort at every size (rocPRIM's default switches to a merge
-// sort of ~17 launches below 2^20 items, the record counts here) with 6 key
-// bits per pass and 2048 items per block: measured on 655k / 786k pairs of
-// 21-22-bit keys, 125 / 134 us against 145 / 150 us for the default (and
-// 221 us with 8 bits per pass).
-using SortConfig = rocprim::radix_sort_config<
-    rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<256, 12>, rocprim::kernel_config<256, 8>, 6>, 0>;
-
-hipError_t sort_pairs(void* temp, size_t& bytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
-                      uint32_t* vout, int64_t n, int end_bit, hipStream_t st) {
-  return rocprim::radix_sort_pairs<SortConfig>(temp, bytes, kin, kout, vin, vout, (unsigned int)n, 0u,
-                                               (unsigned int)end_bit, st);
-}
-
-size_t sort_temp_bytes(int64_t n) {
+size_t scan_temp_bytes(int64_t bins) {
   size_t bytes = 0;
-  hipError_t e = sort_pairs(nullptr, bytes, nullptr, nullptr, nullptr, nullptr, n, 32, (hipStream_t)0);
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                  (int)bins, (hipStream_t)0);
   return e == hipSuccess ? bytes : 0;
 }
 }  // namespace
 
-int64_t scatter_bytes(int64_t n) {
-  if (n < 1 || n > (int64_t)0x7fffffff) return 0;
-  const size_t tb = sort_temp_bytes(n);
+int64_t scatter_bytes(int64_t n, int64_t key_range) {
+  if (n < 1 || n > (int64_t)0x7fffffff || key_range < 1 || key_range >= (int64_t)kScatterEmpty) return 0;
+  const int64_t bins = key_range + 1;
+  const size_t tb = scan_temp_bytes(bins);
   if (!tb) return 0;
-  return 4 * align256(4 * n) + align256(kScatterRecBytes * n) + align256((int64_t)tb);
+  return 4 * align256(4 * n) + 2 * align256(4 * bins) + align256(kScatterRecBytes * n) + align256((int64_t)tb);
 }
 
 int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range, ScatterWork& w) {
   if (key_range < 1 || key_range >= (int64_t)kScatterEmpty)
     return fail(CMM_E_RANGE, "scatter workspace: keys do not fit 32 bits");
-  const int64_t need = scatter_bytes(n);
+  const int64_t need = scatter_bytes(n, key_range);
   if (!need) return fail(CMM_E_RANGE, "scatter workspace: bad record count");
   if (!work || work_bytes < need)
     return fail(CMM_E_ARG, "scatter workspace too small: need " + std::to_string(need) + " bytes");
@@ -52,20 +39,56 @@ int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range,
     p += align256(b);
     return q;
   };
+  const int64_t bins = key_range + 1;
   w.n = n;
   w.keys = reinterpret_cast<uint32_t*>(take(4 * n));
   w.idx = reinterpret_cast<uint32_t*>(take(4 * n));
   w.keys_s = reinterpret_cast<uint32_t*>(take(4 * n));
   w.idx_s = reinterpret_cast<uint32_t*>(take(4 * n));
+  w.count = reinterpret_cast<uint32_t*>(take(4 * bins));
+  w.offs = reinterpret_cast<uint32_t*>(take(4 * bins));
   char* recs = take(kScatterRecBytes * n);
   w.rec = reinterpret_cast<ScatterRec*>(recs);
   w.recv = reinterpret_cast<ScatterRecV*>(recs);
   w.empty = (uint32_t)key_range;
-  w.end_bit = 1;
-  while (w.end_bit < 32 && ((int64_t)1 << w.end_bit) <= key_range) ++w.end_bit;
-  w.temp_bytes = sort_temp_bytes(n);
+  w.temp_bytes = scan_temp_bytes(bins);
   w.temp = take((int64_t)w.temp_bytes);
   return CMM_OK;
+}
+
+// zero the per-key counters, mark every grouped position empty
+__global__ __launch_bounds__(256) void k_scatter_init(uint32_t* __restrict__ count, int64_t bins,
+                                                      uint32_t* __restrict__ keys_s, int64_t n, uint32_t empty) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < bins) count[i] = 0u;
+  if (i < n) keys_s[i] = empty;
+}
+
+// records per key; empty slots (and keys above the empty key, none are
+// written) are not counted: they stay at the end, marked by k_scatter_init
+__global__ __launch_bounds__(256) void k_scatter_count(const uint32_t* __restrict__ keys, int64_t n, uint32_t empty,
+                                                       uint32_t* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  if (k < empty) __hip_atomic_fetch_add(count + k, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// record i goes to the next free position of its key's range (the counters
+// hold the group sizes on entry and are left at zero)
+__global__ __launch_bounds__(256) void k_scatter_place(const uint32_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ idx, int64_t n, uint32_t empty,
+                                                       const uint32_t* __restrict__ offs,
+                                                       uint32_t* __restrict__ count, uint32_t* __restrict__ keys_s,
+                                                       uint32_t* __restrict__ idx_s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  if (k >= empty) return;
+  const uint32_t pos = offs[k] + __hip_atomic_fetch_add(count + k, 0xffffffffu, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT) - 1u;
+  keys_s[pos] = k;
+  idx_s[pos] = idx[i];
 }
 
 // Records per reducing thread: a group (records of one key) is cut into
@@ -164,10 +187,18 @@ __global__ __launch_bounds__(256) void k_scatter_reduce(ScatterWork w, int64_t n
 int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hipStream_t st) {
   if (n < 1) return CMM_OK;
   if (n > w.n) return fail(CMM_E_ARG, "scatter: more records than the workspace holds");
-  size_t tb = w.temp_bytes;
-  hipError_t e = sort_pairs(w.temp, tb, w.keys, w.keys_s, w.idx, w.idx_s, n, w.end_bit, st);
-  if (e != hipSuccess) return hip_check(e, "scatter: radix sort");
+  const int64_t bins = (int64_t)w.empty + 1;
   const int grid = grid_for(n, 256);
+  hipLaunchKernelGGL(k_scatter_init, dim3(grid_for(std::max(bins, n), 256)), dim3(256), 0, st, w.count, bins,
+                     w.keys_s, n, w.empty);
+  hipLaunchKernelGGL(k_scatter_count, dim3(grid), dim3(256), 0, st, w.keys, n, w.empty, w.count);
+  CMM_LAUNCH_CHECK("scatter: count");
+  size_t tb = w.temp_bytes;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(w.temp, tb, w.count, w.offs, (int)bins, st);
+  if (e != hipSuccess) return hip_check(e, "scatter: scan");
+  hipLaunchKernelGGL(k_scatter_place, dim3(grid), dim3(256), 0, st, w.keys, w.idx, n, w.empty, w.offs, w.count,
+                     w.keys_s, w.idx_s);
+  CMM_LAUNCH_CHECK("scatter: place");
   if (t.mode == 0) {
     if (t.nc == 3)
       hipLaunchKernelGGL((k_scatter_reduce<3, 0>), dim3(grid), dim3(256), 0, st, w, n, t);
@@ -188,10 +219,10 @@ int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hip
 
 }  // namespace cmm
 
-extern "C" int cmm_scatter_workspace_bytes(int64_t records, int64_t* bytes) {
-  if (!bytes || records < 1) return cmm::fail(CMM_E_ARG, "cmm_scatter_workspace_bytes: bad arguments");
-  const int64_t b = cmm::scatter_bytes(records);
-  if (!b) return cmm::fail(CMM_E_RANGE, "cmm_scatter_workspace_bytes: record count out of range");
+extern "C" int cmm_scatter_workspace_bytes(int64_t records, int64_t key_range, int64_t* bytes) {
+  if (!bytes || records < 1 || key_range < 1) return cmm::fail(CMM_E_ARG, "cmm_scatter_workspace_bytes: bad arguments");
+  const int64_t b = cmm::scatter_bytes(records, key_range);
+  if (!b) return cmm::fail(CMM_E_RANGE, "cmm_scatter_workspace_bytes: record count or key range out of range");
   *bytes = b;
   return CMM_OK;
 }

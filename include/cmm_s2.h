@@ -297,12 +297,14 @@ int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const double* wpa
  * layout of the packed array they belong to, (Ntri, 6, ncomp, 6).
  * The coefficient adjoints of the four scattering entry points below
  * (composition, RK feet, stencil projection, vorticity sampling) either
- * accumulate with FP64 atomics (work == NULL) or go through a sorted
- * scatter-add in a caller workspace of cmm_scatter_workspace_bytes(records)
- * bytes (records: depth * n for the composition, 16 nv for the RK feet,
- * 4 Nv for the stencil projection, n for the sampling): deterministic and
- * several times faster. */
-int cmm_scatter_workspace_bytes(int64_t records, int64_t* bytes);
+ * accumulate with FP64 atomics (work == NULL) or go through a grouped
+ * scatter-add (counting sort by target block, one writer per block) in a
+ * caller workspace of cmm_scatter_workspace_bytes(records, key_range) bytes
+ * (records / key_range: depth * n / depth * 6 Ntri for the composition,
+ * 16 Nv / 6 Tu of the velocity mesh for the RK feet, 4 Nv / 6 Ntri for the
+ * stencil projection, n / 6 Tu of the vorticity mesh for the sampling):
+ * several times faster than the FP64 atomics. */
+int cmm_scatter_workspace_bytes(int64_t records, int64_t key_range, int64_t* bytes);
 /* transpose of cmm_sht_inverse: fbar (nf, L+1, 2L) -> flm_bar [lm][nf].
  * work: cmm_sht_workspace_bytes(nf). */
 int cmm_sht_inverse_adjoint(const cmm_sht_plan* plan, const double* fbar, int nf, double* flm_bar,

@@ -28,14 +28,15 @@ def _close(a, b, tol):
     assert abs(a - b) <= tol * max(abs(a), abs(b), 1e-300), (a, b)
 
 
-def _work(records):
-    """(pointer, bytes) of a sorted scatter-add workspace, None = atomics."""
+def _work(records, key_range=None):
+    """(pointer, bytes) of a grouped scatter-add workspace for ``records``
+    records with keys below ``key_range``, None = atomics."""
     import ctypes
     from cmm_s2 import _native as N
     if records is None:
         return None, 0
     b = ctypes.c_int64(0)
-    N.call("cmm_scatter_workspace_bytes", int(records), ctypes.byref(b))
+    N.call("cmm_scatter_workspace_bytes", int(records), int(key_range), ctypes.byref(b))
     t = torch.empty(b.value, dtype=torch.uint8, device="cuda")
     _work.keep = t
     return t.data_ptr(), b.value
@@ -152,7 +153,7 @@ def _jets_bar(mesh, pb):
     return jb
 
 
-@pytest.mark.parametrize("scatter", ["atomics", "sorted"])
+@pytest.mark.parametrize("scatter", ["atomics", "grouped"])
 def test_compose_adjoint_fd(scatter):
     """d/dh <obar, compose(levels + h delta)> over the jets of every level."""
     from cmm_s2 import _native as N
@@ -176,7 +177,7 @@ def test_compose_adjoint_fd(scatter):
     lv = [_packed(mesh, j) for j in jets]
     lb = [torch.zeros_like(p) for p in lv]
     qb = torch.empty_like(q)
-    wp, wb = _work(None if scatter == "atomics" else depth * 20000)
+    wp, wb = _work(None if scatter == "atomics" else depth * 20000, 6 * depth * mesh.ntri)
     N.call("cmm_tri_compose_adjoint", mesh._h, N.ptr_array([p.data_ptr() for p in lv]),
            N.ptr_array([p.data_ptr() for p in lb]), depth, q.data_ptr(), 20000, 3, 1, ob.data_ptr(),
            qb.data_ptr(), wp, wb, stream())
@@ -191,7 +192,7 @@ def test_compose_adjoint_fd(scatter):
     _close(_dot(qb, t), fdq, 1e-6)
 
 
-@pytest.mark.parametrize("scatter", ["atomics", "sorted"])
+@pytest.mark.parametrize("scatter", ["atomics", "grouped"])
 def test_sample_vorticity_adjoint_fd(scatter):
     from cmm_s2 import _native as N
     from cmm_s2.core._backend import stream
@@ -209,7 +210,7 @@ def test_sample_vorticity_adjoint_fd(scatter):
     wb = _rand(n, seed=5)
     xb = torch.empty_like(X)
     pb = torch.zeros_like(w0.packed)
-    wp, wsz = _work(None if scatter == "atomics" else n)
+    wp, wsz = _work(None if scatter == "atomics" else n, 6 * mu.ntri)
     N.call("cmm_sample_vorticity_adjoint", mu._h, w0.packed.data_ptr(), X.data_ptr(), n, 3, 1, wb.data_ptr(), rot,
            xb.data_ptr(), 3, 1, pb.data_ptr(), wp, wsz, stream())
     h = 1e-6
@@ -247,7 +248,7 @@ def _spline_velocity(L, seed):
     return mu, lv, vals
 
 
-@pytest.mark.parametrize("scatter", ["atomics", "sorted"])
+@pytest.mark.parametrize("scatter", ["atomics", "grouped"])
 @pytest.mark.parametrize("integ", ["euler", "improved_euler", "rk4"])
 def test_advect_stencil_adjoint_fd(integ, scatter):
     from cmm_s2 import _native as N
@@ -287,7 +288,7 @@ def test_advect_stencil_adjoint_fd(integ, scatter):
 
     lb = [torch.zeros_like(p) for p in lv]
     xb = torch.empty(3, 4 * nv, dtype=torch.float64, device="cuda")
-    wp, wsz = _work(None if scatter == "atomics" else 16 * nv)
+    wp, wsz = _work(None if scatter == "atomics" else 16 * nv, 6 * mu.ntri)
     N.call("cmm_advect_stencil_adjoint", desc(lv), kind, dt, xs.data_ptr(), xs.stride(0), nv, yb.data_ptr(),
            yb.stride(0), N.ptr_array([p.data_ptr() for p in lb]), xb.data_ptr(), xb.stride(0), wp, wsz, stream())
     dv = [_rand(12, mu.nv, seed=80 + k) for k in range(3)]
@@ -322,7 +323,7 @@ def test_advect_stencil_adjoint_fd(integ, scatter):
     _close(_dot(xb, dx), (Jx((xs + h * dx).contiguous()) - Jx((xs - h * dx).contiguous())) / (2 * h), 1e-6)
 
 
-@pytest.mark.parametrize("scatter", ["atomics", "sorted"])
+@pytest.mark.parametrize("scatter", ["atomics", "grouped"])
 @pytest.mark.parametrize("identity", [True, False])
 def test_stencil_jets_adjoint_fd(identity, scatter):
     from cmm_s2 import _native as N
@@ -351,7 +352,7 @@ def test_stencil_jets_adjoint_fd(identity, scatter):
     pk = _packed(mesh, cur_j)
     sb = torch.empty_like(ys)
     curb = torch.zeros_like(pk)
-    wp, wsz = _work(None if scatter == "atomics" else 4 * nv)
+    wp, wsz = _work(None if scatter == "atomics" else 4 * nv, 6 * mesh.ntri)
     N.call("cmm_tri_stencil_jets_adjoint", mesh._h, None if identity else pk.data_ptr(), yv.data_ptr(), yv.stride(0),
            ys.data_ptr(), ys.stride(0), mesh.d_tan.data_ptr(), eps, jb.data_ptr(), sb.data_ptr(), sb.stride(0),
            curb.data_ptr(), wp, wsz, stream())
