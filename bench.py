@@ -63,6 +63,9 @@ def parse():
                     help="instead: gradient of the final-vorticity misfit for a batch of B initial fields "
                          "(L=256, ico-6, 50 steps dt=1/256, remap every 10, l <= 64) with central-difference "
                          "checks on 32 random directions")
+    ap.add_argument("--synthesis", action="store_true",
+                    help="time the 12-field Legendre synthesis (inverse SHT) in every arithmetic form at L=512 "
+                         "and L=1024 against FP64")
     ap.add_argument("--keep-gib", type=float, default=48.0,
                     help="device memory for kept remap-segment tapes in the --adjoint comparison run")
     ap.add_argument("--fd-dirs", type=int, default=32)
@@ -81,6 +84,8 @@ def main():
         sys.exit(2)
     if a.ensemble:
         return ensemble_bench(a)
+    if a.synthesis:
+        return synthesis_bench(a)
     if a.adjoint:
         return adjoint_bench(a)
     if a.small:
@@ -261,6 +266,44 @@ def ensemble_bench(a):
            "eager_streams_value": round(n / sec, 2), "eager_one_stream_value": round(n1 / sec1, 2),
            "graph_speedup_vs_eager_full_run": round(secf / (secg / 3), 4),
            "member_steps": ng, "seconds": round(secg, 4), "capture_seconds": round(cap, 2)}
+    print(json.dumps(rec), flush=True)
+
+
+def synthesis_bench(a):
+    """The velocity projection's inverse SHT (12 fields, l^-1.5 spectrum,
+    seed 7) in every Legendre form: HIP-event time per transform (20 reps,
+    FFT included) and the relative max-norm difference from the FP64 form."""
+    import numpy as np
+    import torch
+    from cmm_s2.core import spherical_harmonic_tools as st
+    torch.cuda.set_device(0)
+    forms = ("fp64", "fp32", "bf16x3", "bf16x4", "bf16x5", "bf16x6")
+    res = {}
+    for L in (512, 1024):
+        p64 = st.SHT.get(L, "fp64")
+        g = torch.Generator(device="cuda").manual_seed(7)
+        ls = torch.from_numpy(p64.l_of.astype(np.float64)).cuda()
+        flm = torch.randn(st.nlm(L), 12, 2, dtype=torch.float64, device="cuda", generator=g)
+        flm *= (1.0 + ls)[:, None, None] ** -1.5
+        ref = p64.inverse(flm)
+        for form in forms:
+            p = st.SHT.get(L, form)
+            out = p.inverse(flm)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                p.inverse(flm, out=out)
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) / 20 * 1e3
+            err = float((out - ref).abs().max() / ref.abs().max())
+            res[f"L{L}_{form}"] = {"us": round(us, 2), "rel_err_vs_fp64": err}
+            log(f"L={L} {form:7s} 12-field synthesis {us:8.1f} us, rel. max-norm diff vs fp64 {err:.2e}")
+    rec = {"metric": "12-field inverse SHT time per Legendre form", "value": res["L512_bf16x6"]["us"], "unit": "us",
+           "n_gpus": 1, "higher_is_better": False, "data": "synthetic", "dtype": "f64 / f32 / split-bf16",
+           "config": {"workload": "inverse MWSS SHT of 12 fields (the velocity projection), L=512 and L=1024"},
+           "forms": res}
     print(json.dumps(rec), flush=True)
 
 
