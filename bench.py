@@ -186,6 +186,7 @@ def main():
     mixed = None
     if not a.compose_only and not a.no_mixed:
         mixed = {mode: mixed_run(a, mesh, w0, wf, grid, s_step, sim, mode) for mode in _MIXED}
+        mixed["fp64_initial_field_perturbed_2^-50"] = sensitivity_run(a, mesh, w0, wf, grid)
     cells = int(mesh.stats()["cells_per_face"])
     traffic = MEASURED_TRAFFIC.get((a.ico, a.Lf, depth, cells, tile_rows))
     cpu = None
@@ -277,7 +278,7 @@ def synthesis_bench(a):
     import torch
     from cmm_s2.core import spherical_harmonic_tools as st
     torch.cuda.set_device(0)
-    forms = ("fp64", "fp32", "bf16x3", "bf16x4", "bf16x5", "bf16x6")
+    forms = ("fp64", "fp32", "bf16x3", "bf16x4", "bf16x5", "bf16x6", "bf16x7")
     res = {}
     for L in (512, 1024):
         p64 = st.SHT.get(L, "fp64")
@@ -417,16 +418,35 @@ def area_distortion(sim, q):
 
 _MIXED = {"fp32": "fp32 matrix cores, fp64 recurrence, fp64-compensated 32-degree partial sums",
           "bf16x3": "split-bf16, 3 fixed-point digits per operand (6 digit products) on bf16 matrix cores, "
-                    "exact fp32 sums of the three leading weights, fp64 flush every 192 degrees per parity",
+                    "exact fp32 sums of the four leading weights, fp64 flush every 192 degrees per parity",
           "bf16x5": "split-bf16, 5 fixed-point digits per operand (15 digit products) on bf16 matrix cores, "
-                    "exact fp32 sums of the three leading weights, fp64 flush every 192 degrees per parity",
+                    "exact fp32 sums of the four leading weights, fp64 flush every 192 degrees per parity",
           "bf16x6": "split-bf16, 6 fixed-point digits per operand (21 digit products) on bf16 matrix cores, "
-                    "exact fp32 sums of the three leading weights, fp64 flush every 192 degrees per parity"}
+                    "exact fp32 sums of the four leading weights, fp64 flush every 192 degrees per parity",
+          "bf16x7": "split-bf16, 7 fixed-point digits per operand (28 digit products) on bf16 matrix cores, "
+                    "exact fp32 sums of the four leading weights, fp64 flush every 192 degrees per parity"}
+
+
+def sensitivity_run(a, mesh, w0, wf64, grid):
+    """The FP64 run again from the initial vorticity spline with every jet
+    value scaled by (1 + 2^-50): the final vorticity difference that a
+    rounding-level change of the input alone produces in this run (the floor
+    of any comparison of two arithmetics that round differently)."""
+    import numpy as np
+    from cmm_s2.core.spherical_simulations import BarotropicCMM
+    from cmm_s2.core.spherical_spline import spherical_spline
+    wp = spherical_spline(w0.mesh, w0.vals * (1.0 + 2.0 ** -50))
+    sim = BarotropicCMM(a.L, a.Nt, a.Nt / a.L, mesh, wp, 2 * np.pi, a.n_maps)
+    sim.run()
+    w = sim.vorticity_on_grid(a.Lf, grid=grid)
+    err = float((w - wf64).abs().max() / wf64.abs().max())
+    log(f"fp64 with the initial field perturbed by 2^-50 relative: final vorticity rel. diff {err:.3e}")
+    return {"perturbation": "initial vorticity jets x (1 + 2^-50)", "vorticity_max_rel_err": err}
 
 
 def mixed_run(a, mesh, w0, wf64, grid, s64, sim64, mode):
     """The same CMM run with the Legendre synthesis in a lower-precision form
-    (``mode``: fp32 or split-bf16 with 3 / 5 / 6 digits; FP64 recurrence):
+    (``mode``: fp32 or split-bf16 with 3 / 5 / 6 / 7 digits; FP64 recurrence):
     seconds per steady step, the relative max-norm vorticity difference on the
     fine grid, and the relative max-norm difference of every intermediate
     output against the FP64 run: psi_lm, u and grad u on the grid and the

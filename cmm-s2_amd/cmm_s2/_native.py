@@ -22,7 +22,7 @@ P_i64 = C.POINTER(C.c_int64)
 VEL_SPLINE, VEL_SBR, VEL_DEFORM_ROT, VEL_RH4 = 0, 1, 2, 3
 INT_EULER, INT_IMPROVED_EULER, INT_RK4, INT_RK2, INT_EVAL = 0, 1, 2, 3, 4
 LEGENDRE_FP64, LEGENDRE_FP32 = 0, 1
-LEGENDRE_BF16X3, LEGENDRE_BF16X4, LEGENDRE_BF16X5, LEGENDRE_BF16X6 = 2, 3, 4, 5
+LEGENDRE_BF16X3, LEGENDRE_BF16X4, LEGENDRE_BF16X5, LEGENDRE_BF16X6, LEGENDRE_BF16X7 = 2, 3, 4, 5, 6
 VORT_SPLINE, VORT_ROSSBY, VORT_ZONAL_JET, VORT_GAUSSIAN = 0, 1, 2, 3
 VORT_PERTURBED_JET, VORT_MULTI_JET, VORT_PERTURBED_ROSSBY = 4, 5, 6
 

@@ -36,7 +36,8 @@ def _lm_tables(L):
 # arithmetic of the Legendre synthesis: "fp64" (reference precision) or
 # "fp32" (FP32 matrix cores, FP64 recurrence, FP64-compensated partial sums)
 _LEGENDRE = {"fp64": N.LEGENDRE_FP64, "fp32": N.LEGENDRE_FP32, "bf16x3": N.LEGENDRE_BF16X3,
-             "bf16x4": N.LEGENDRE_BF16X4, "bf16x5": N.LEGENDRE_BF16X5, "bf16x6": N.LEGENDRE_BF16X6}
+             "bf16x4": N.LEGENDRE_BF16X4, "bf16x5": N.LEGENDRE_BF16X5, "bf16x6": N.LEGENDRE_BF16X6,
+             "bf16x7": N.LEGENDRE_BF16X7}
 
 
 class SHT(object):

@@ -28,7 +28,7 @@ constexpr int kScaleExp = 600;
 constexpr int kMaxFFT = 2048;
 constexpr int kFwdSegHost = 128;  // degrees per forward work unit (kFwdSeg)
 constexpr int kOzJ = 16;          // split-BF16 synthesis: degrees of one parity per chunk
-constexpr int kOzMaxS = 6;        // split-BF16 synthesis: most digits per operand
+constexpr int kOzMaxS = 7;        // split-BF16 synthesis: most digits per operand
 }
 
 struct cmm_sht_plan {
@@ -259,7 +259,7 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
 }
 
 extern "C" int cmm_sht_set_legendre(cmm_sht_plan* p, int mode) {
-  if (!p || mode < CMM_LEGENDRE_FP64 || mode > CMM_LEGENDRE_BF16X6)
+  if (!p || mode < CMM_LEGENDRE_FP64 || mode > CMM_LEGENDRE_BF16X7)
     return fail(CMM_E_ARG, "cmm_sht_set_legendre: bad arguments");
   p->legendre = mode;
   return CMM_OK;
@@ -673,21 +673,27 @@ __global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, in
 // |d_i| <= 128 (balanced digits, rounded to nearest at every step).  A digit
 // has at most 8 significant bits, so it is exact in bf16, and a product
 // a_i b_j is an exact multiple of 2^(-8 (i + j - 2)) below 2^16 in magnitude.
-// The products are summed by v_mfma_f32_16x16x16_bf16 into four FP32
-// accumulators by weight: i + j = 2, 3, 4 (multiples of 1, 2^-8, 2^-16 with
-// per-degree magnitudes below 2^16, 2^8 and 1.25) and i + j >= 5.  Over at
-// most 192 degrees of one parity the first three stay below 2^24 units, so
-// their FP32 sums are exact; the fourth carries the FP32 rounding of a
-// quantity 2^-24 below the result.  Products with i + j > S + 1 are left out
-// (each below 2^(-8 S) of the scale).  Every 192 degrees per parity the four
-// sums are scaled and added into the FP64 output.  eA bounds |Pbar| over all degrees (orthonormal functions:
+// The products are summed on bf16 matrix cores into FP32 accumulators by
+// weight: i + j = 2, 3, 4, 5 (multiples of 1, 2^-8, 2^-16, 2^-24 with
+// per-degree magnitudes below 2^16, 2^9, 1.25 and 2^-8 + 2^-9) and, from five
+// digits on, i + j >= 6.  Over at most 192 degrees of one parity the first
+// three stay below 2^24 units, so their FP32 sums are exact, the fourth up to
+// 170 degrees (beyond, it rounds at half a unit, 2^-49 of the scale); the
+// last carries the FP32 rounding of a quantity 2^-32 below the result.  (With
+// i + j = 5 and the smaller weights in one accumulator its rounding measured
+// ~7e-13 of the scale in a CPU emulation: the floor of five and more digits.)
+// Products with i + j > S + 1 are left out (each below 2^(-8 S) of the
+// scale).  Every 192 degrees per parity the sums are scaled and added into
+// the FP64 output.  eA bounds |Pbar| over all degrees (orthonormal functions:
 // |Pbar_lm| <= sqrt((2 l + 1) / 4 pi)); eB is the exponent of the largest
 // coefficient of each order and column.  The recurrence stays FP64.
 // ---------------------------------------------------------------------------
 typedef short bf4 __attribute__((ext_vector_type(4)));
 constexpr int kOzRings = 64;  // rings per wave
 constexpr int kOzFlush = 12;  // chunks between FP64 flushes (192 degrees per parity)
-constexpr int kOzAcc = 4;     // FP32 accumulators by digit weight
+// FP32 accumulators by digit weight for S digits
+template <int S>
+constexpr int oz_acc() { return S >= 5 ? 5 : 4; }
 constexpr int kOzRow = kOzJ / 2 + 2;  // dwords per tile row (16 bf16 + padding)
 
 // bf16 bit pattern of a float whose significand has at most 8 bits (exact)
@@ -775,7 +781,8 @@ __global__ __launch_bounds__(256) void k_oz_coeffs(int L, int nf, int f0, int nc
 // Lo = rint((V - H) 2^24) (|Lo| <= 2^23).  With a bias of 128 in its two low bytes, an integer
 // X = d0 2^16 + d1 2^8 + d2 has bytes d1 + 128, d2 + 128 and d0 = (X + 0x8080)
 // >> 16, so d1, d2 are in [-128, 127] and |d0| <= 256:
-//   v 256 = d0 + d1 2^-8 + d2 2^-16 + e0 2^-24 + e1 2^-32 + e2 2^-40.
+//   v 256 = d0 + d1 2^-8 + d2 2^-16 + e0 2^-24 + e1 2^-32 + e2 2^-40 (+ f 2^-48),
+// the seventh digit f = rint(((V - H) 2^24 - Lo) 2^8) (|f| <= 128).
 template <int S>
 __device__ inline void oz_digits_fixed(double V, float (&d)[S]) {
   const double Hd = rint(V);
@@ -784,10 +791,13 @@ __device__ inline void oz_digits_fixed(double V, float (&d)[S]) {
   if (S > 1) d[1] = fmaf((float)((hb >> 8) & 0xff), 0x1p-8f, -0x1p-1f);
   if (S > 2) d[2] = fmaf((float)(hb & 0xff), 0x1p-16f, -0x1p-9f);
   if (S > 3) {
-    const int lb = (int)rint((V - Hd) * 0x1p24) + 0x8080;
+    const double R = (V - Hd) * 0x1p24;
+    const double Ld = rint(R);
+    const int lb = (int)Ld + 0x8080;
     d[3] = (float)(lb >> 16) * 0x1p-24f;
     if (S > 4) d[4] = fmaf((float)((lb >> 8) & 0xff), 0x1p-32f, -0x1p-25f);
     if (S > 5) d[5] = fmaf((float)(lb & 0xff), 0x1p-40f, -0x1p-33f);
+    if (S > 6) d[6] = (float)(rint((R - Ld) * 256.0) * 0x1p-48);
   }
 }
 
@@ -814,8 +824,10 @@ __device__ inline uint32_t pack_bf16(float lo, float hi) {
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float fx16 __attribute__((ext_vector_type(16)));
 constexpr int kOzRow2 = kOzJ / 2 + 4;  // dwords per tile row (16 bf16, 16-byte aligned rows)
+// (six and more digits need the register file of one wave per SIMD)
 template <int S>
-__global__ __launch_bounds__(128, 2) void k_leg_inverse_oz(int L, int nf, int f0, int nfp, int nrb, int eA,
+__global__ __launch_bounds__(128, S >= 6 ? 1 : 2) void k_leg_inverse_oz(int L, int nf, int f0, int nfp, int nrb,
+                                                                        int eA,
                                                            const double* __restrict__ x_t,
                                                            const double* __restrict__ dP, const int* __restrict__ dS,
                                                            const double2* __restrict__ rec,
@@ -853,10 +865,11 @@ __global__ __launch_bounds__(128, 2) void k_leg_inverse_oz(int L, int nf, int f0
   const int hi = lane >> 5, lc = lane & 31;
   const int ncols = 2 * nfp;
   const int64_t bat = (int64_t)offJ[m] * 2 * ncols * kOzJ + (int64_t)min(lc, ncols - 1) * kOzJ + 8 * hi;
-  fx16 acc[kOzAcc][2];
+  constexpr int NQ = oz_acc<S>();
+  fx16 acc[NQ][2];
   auto zero = [&]() {
 #pragma unroll
-    for (int q = 0; q < kOzAcc; ++q)
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
       for (int mt = 0; mt < 2; ++mt) acc[q][mt] = fx16{};
   };
@@ -868,9 +881,12 @@ __global__ __launch_bounds__(128, 2) void k_leg_inverse_oz(int L, int nf, int f0
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-      for (int e = 0; e < 16; ++e)
-        comb[(w * kOzRings + 32 * mt + 8 * (e >> 2) + 4 * hi + (e & 3)) * 33 + lc] =
-            (((double)acc[0][mt][e] + (double)acc[1][mt][e]) + (double)acc[2][mt][e]) + (double)acc[3][mt][e];
+      for (int e = 0; e < 16; ++e) {
+        double v = (double)acc[0][mt][e];
+#pragma unroll
+        for (int q = 1; q < NQ; ++q) v += (double)acc[q][mt][e];
+        comb[(w * kOzRings + 32 * mt + 8 * (e >> 2) + 4 * hi + (e & 3)) * 33 + lc] = v;
+      }
     zero();
     __syncthreads();
     double2* F2 = reinterpret_cast<double2*>(F);
@@ -933,7 +949,7 @@ __global__ __launch_bounds__(128, 2) void k_leg_inverse_oz(int L, int nf, int f0
         av[mt] = *reinterpret_cast<const bf8*>(&tile[(i * kOzRings + 32 * mt + lc) * kOzRow2 + 4 * hi]);
 #pragma unroll
       for (int j = 0; i + j < S; ++j) {
-        const int q = i + j < kOzAcc - 1 ? i + j : kOzAcc - 1;
+        const int q = i + j < NQ - 1 ? i + j : NQ - 1;
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
           acc[q][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[mt], bcur[j], acc[q][mt], 0, 0, 0);
@@ -1565,7 +1581,8 @@ static int oz_inverse(const cmm_sht_plan* p, const double* flm, int nf, int f0, 
     case CMM_LEGENDRE_BF16X3: return oz_inverse_s<3>(p, flm, nf, f0, nfp, F, oz, st);
     case CMM_LEGENDRE_BF16X4: return oz_inverse_s<4>(p, flm, nf, f0, nfp, F, oz, st);
     case CMM_LEGENDRE_BF16X5: return oz_inverse_s<5>(p, flm, nf, f0, nfp, F, oz, st);
-    default: return oz_inverse_s<6>(p, flm, nf, f0, nfp, F, oz, st);
+    case CMM_LEGENDRE_BF16X6: return oz_inverse_s<6>(p, flm, nf, f0, nfp, F, oz, st);
+    default: return oz_inverse_s<7>(p, flm, nf, f0, nfp, F, oz, st);
   }
 }
 
@@ -1669,6 +1686,7 @@ __global__ __launch_bounds__(320) void k_leg_adj_mfma(int L, int nf, int f0, int
                                                       const double2* __restrict__ rec, const double* __restrict__ G,
                                                       const double* __restrict__ msc, double* __restrict__ flm) {
   __shared__ double tile[5][16][65];  // per wave: 16 degrees x 64 rings (+1 padding)
+  __shared__ double2 rs[kFwdSeg];     // recurrence pairs of degrees ls .. lc + 31
   const int b = (int)blockIdx.x;
   int lo = 0, hi = L;  // largest m with offJ[m] <= b
   while (hi - lo > 1) {
@@ -1678,28 +1696,46 @@ __global__ __launch_bounds__(320) void k_leg_adj_mfma(int L, int nf, int f0, int
   const int m = lo, c = b - offJ[m];
   const int nn = L + 1, nh = L / 2 + 1;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int j = 64 * w + lane;
-  const bool act = j < nh;
-  const int jj = act ? j : 0;
   const int lc = m + 32 * c;
   const int su = (32 * c) / kFwdSeg;
   const int u = unit0[m] + su;
   const int ls = m + kFwdSeg * su;
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  // the ring pair values of this wave's B operands, loaded under the recurrence:
+  // ring 64 w + 4 k + lane / 16, column 16 nt + lane % 16
+  double ga[16][NT], gb[16][NT];
+  bool gok[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int jr = 64 * w + 4 * k + (lane >> 4);
+    gok[k] = jr < nh && !(m > 0 && jr == 0);
+    const int jc = gok[k] ? jr : 0;
+    const int jm = nn - 1 - jc;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int col = 16 * nt + (lane & 15);
+      const int f = min(col >> 1, nfp - 1);
+      ga[k][nt] = G[(((int64_t)(f0 + f) * nn + jc) * L + m) * 2 + (col & 1)];
+      gb[k][nt] = jm != jc ? G[(((int64_t)(f0 + f) * nn + jm) * L + m) * 2 + (col & 1)] : 0.0;
+    }
+  }
+  for (int i = threadIdx.x; i < lc + 32 - ls; i += blockDim.x) rs[i] = rec[lm0 + (min(ls + i, L - 1) - m)];
+  const int j = 64 * w + lane;
+  const bool act = j < nh;
+  const int jj = act ? j : 0;
   const double x = xr[jj];
   const double2 ps = st_p[(int64_t)u * nh + jj];
   double pp = ps.x, pc = ps.y;
   int s = st_s[(int64_t)u * nh + jj];
-  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  __syncthreads();
   // advance from the unit start to the chunk (rescale every 16 degrees, as the analysis)
   for (int l = ls; l < lc; l += 16) {
     leg_rescale(pp, pc, s);
-#pragma unroll 4
+#pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int ll = l + i;
-      if (ll > m) {
-        const double2 ab = rec[lm0 + (ll - m)];
-        leg_step(ab.x, ab.y, x, pp, pc);
-      }
+      const double2 ab = rs[ll - ls];
+      if (ll > m) leg_step(ab.x, ab.y, x, pp, pc);
     }
   }
   double odd[16];
@@ -1711,51 +1747,41 @@ __global__ __launch_bounds__(320) void k_leg_adj_mfma(int L, int nf, int f0, int
       live = (s == 0) & act;
     }
     const int ll = lc + i;
-    if (ll > m) {
-      const double2 ab = rec[lm0 + (min(ll, L - 1) - m)];
-      leg_step(ab.x, ab.y, x, pp, pc);
-    }
+    const double2 ab = rs[ll - ls];
+    if (ll > m) leg_step(ab.x, ab.y, x, pp, pc);
     const double pv = (live & (ll < L)) ? pc : 0.0;
     if (i & 1) odd[i >> 1] = pv;
     else tile[w][i >> 1][lane] = pv;
   }
   const double cm = msc[m];
-  // B operand (S for p = 0, D for p = 1) of ring 64 w + 4 k + lane / 16, column 16 nt + lane % 16
-  auto bval = [&](int k, int nt, int p) {
-    const int jr = 64 * w + 4 * k + (lane >> 4);
-    const int col = 16 * nt + (lane & 15);
-    const int f = min(col >> 1, nfp - 1);
-    const bool ok = jr < nh && !(m > 0 && jr == 0);
-    const int jc = ok ? jr : 0;
-    const int jm = nn - 1 - jc;
-    const double ga = G[(((int64_t)(f0 + f) * nn + jc) * L + m) * 2 + (col & 1)];
-    const double gb = G[(((int64_t)(f0 + f) * nn + jm) * L + m) * 2 + (col & 1)];
-    const double wgt = ok ? cm : 0.0;
-    return jm == jc ? wgt * ga : wgt * (p == 0 ? ga + gb : ga - gb);
-  };
   dx4 acc[2][NT];
 #pragma unroll
   for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[p][nt] = dx4{0.0, 0.0, 0.0, 0.0};
   __syncthreads();
-#pragma unroll 4
+#pragma unroll
   for (int k = 0; k < 16; ++k) {
     const double a = tile[w][lane & 15][4 * k + (lane >> 4)];
+    const double wgt = gok[k] ? cm : 0.0;
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
-      acc[0][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bval(k, nt, 0), acc[0][nt], 0, 0, 0);
+      acc[0][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, wgt * (ga[k][nt] + gb[k][nt]), acc[0][nt], 0, 0, 0);
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < 16; ++i) tile[w][i][lane] = odd[i];
   __syncthreads();
-#pragma unroll 4
+#pragma unroll
   for (int k = 0; k < 16; ++k) {
     const double a = tile[w][lane & 15][4 * k + (lane >> 4)];
+    const int jr = 64 * w + 4 * k + (lane >> 4);
+    const double wgt = gok[k] ? cm : 0.0;
+    const bool mid = (nn - 1 - jr) == jr;  // the equator ring: its own mirror
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
-      acc[1][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bval(k, nt, 1), acc[1][nt], 0, 0, 0);
+      acc[1][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, wgt * (mid ? ga[k][nt] : ga[k][nt] - gb[k][nt]),
+                                                        acc[1][nt], 0, 0, 0);
   }
   __syncthreads();
   double* red = &tile[w][0][0];  // [p][nt][e][lane]

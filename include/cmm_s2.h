@@ -222,19 +222,20 @@ int cmm_sht_plan_destroy(cmm_sht_plan* plan);
  * FP64 only.  FP32 runs the contraction on FP32 matrix cores with FP32
  * operands, the recurrence in FP64 and FP32 partial sums of 32 degrees per
  * parity added into FP64 accumulators; relative error ~1e-7 of the field.
- * BF16XS (S = 3..6) splits both operands into S bf16 fixed-point digits
+ * BF16XS (S = 3..7) splits both operands into S bf16 fixed-point digits
  * (a power of two per ring bound / per order and column) and sums the
  * S (S + 1) / 2 leading digit products on bf16 matrix cores in FP32
- * accumulators that are exact for the three leading weights, flushed into
+ * accumulators that are exact for the four leading weights, flushed into
  * FP64 every 192 degrees per parity; the truncation error is ~2^(4 - 8 S)
  * of the largest term (S = 3: ~1e-6, S = 4: ~1e-8, S = 5: ~4e-11,
- * S = 6: ~1e-12). */
+ * S = 6: ~1e-13, S = 7: ~1e-14). */
 #define CMM_LEGENDRE_FP64 0
 #define CMM_LEGENDRE_FP32 1
 #define CMM_LEGENDRE_BF16X3 2
 #define CMM_LEGENDRE_BF16X4 3
 #define CMM_LEGENDRE_BF16X5 4
 #define CMM_LEGENDRE_BF16X6 5
+#define CMM_LEGENDRE_BF16X7 6
 int cmm_sht_set_legendre(cmm_sht_plan* plan, int mode);
 /* host copies of the plan's Gauss-Legendre nodes / weights (L each) */
 int cmm_sht_plan_gl(const cmm_sht_plan* plan, double* x, double* w);
