@@ -183,6 +183,9 @@ def main():
         f"{mpts:.1f} Mpts/s; vorticity grid {t_vort * 1e3:.2f} ms; |w| max {float(wf.abs().max()):.4f}")
 
     area = area_distortion(sim, q)
+    cons = sim.conservation_errors(1024)
+    log(f"absolute vorticity of the composed map on the L=1024 grid: enstrophy rel. change "
+        f"{cons['enstrophy_rel_err']:.3e}, energy rel. change {cons['energy_rel_err']:.3e} over the run")
     mixed = None
     if not a.compose_only and not a.no_mixed:
         mixed = {mode: mixed_run(a, mesh, w0, wf, grid, s_step, sim, mode) for mode in _MIXED}
@@ -215,6 +218,9 @@ def main():
         "cpu_baseline": cpu,
         "mixed": mixed,
         "area_distortion": area,
+        "conservation_L1024": {"enstrophy_rel_err": cons["enstrophy_rel_err"], "energy_rel_err": cons["energy_rel_err"],
+                               "note": "absolute vorticity w0(X) + 2 rot cos theta(X) on the L=1024 MWSS grid against "
+                                       "t = 0 (figures/data_gen_convergence.py:58-95)"},
     }
     print(json.dumps(rec), flush=True)
 

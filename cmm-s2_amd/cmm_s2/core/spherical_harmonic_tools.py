@@ -263,6 +263,36 @@ def project_onto_S12_lm(plan, f_lm, mesh):
     return spherical_spline(mesh, vals)
 
 
+def enstrophy_energy(f, L, plan=None):
+    """Enstrophy sum_lm |f_lm|^2 and energy 0.5 sum_l E_l of a real field f
+    on the MWSS grid (L+1, 2L) (or a batch (nf, L+1, 2L)), over all orders
+    (m < 0 by conjugate symmetry), with E_0 = |f_00|^2 and E_l = sum_m |f_lm|^2 /
+    (l (l+1)): the diagnostics of figures/data_gen_convergence.py:58-61, 91-95
+    (pysh.forward with Reality=False, energy_spectrum of
+    spherical_harmonic_tools.py:157-163), from the device forward transform.
+    Returns two floats (or two arrays for a batch)."""
+    plan = SHT.get(L) if plan is None else plan
+    fd = B.to_dev(f)
+    single = fd.dim() == 2
+    a = plan.forward(fd.reshape(-1, L + 1, 2 * L))  # (nlm, nf, 2)
+    key = ("ens", plan._h)
+    w = _diag_weights.get(key)
+    if w is None:
+        l = plan.l_of.astype(np.float64)
+        mult = np.where(plan.m_of == 0, 1.0, 2.0)
+        inv = np.where(l > 0, 1.0 / np.maximum(l * (l + 1), 1.0), 1.0)
+        w = (torch.from_numpy(mult).to(a.device), torch.from_numpy(mult * inv).to(a.device))
+        _diag_weights[key] = w
+    p2 = (a * a).sum(-1)  # (nlm, nf)
+    ens = (w[0][:, None] * p2).sum(0)
+    en = 0.5 * (w[1][:, None] * p2).sum(0)
+    ens, en = ens.cpu().numpy(), en.cpu().numpy()
+    return (float(ens[0]), float(en[0])) if single else (ens, en)
+
+
+_diag_weights = {}
+
+
 def energy_spectrum(alms, L):
     """spherical_harmonic_tools.py:157-163."""
     a = np.asarray(alms.detach().cpu() if isinstance(alms, torch.Tensor) else alms)

@@ -200,6 +200,27 @@ class BarotropicCMM(object):
         from . import checkpoint as ck
         return ck.load_state(cls, path, mesh, vorticity)
 
+    def conservation_errors(self, L_out, grid=None, true_vorticity=None):
+        """Relative enstrophy and energy errors of the absolute vorticity
+        w0(X) + 2 rot cos theta(X) on the MWSS grid of bandlimit L_out
+        against the same quantities of w0 + 2 rot cos theta at t = 0, and,
+        with ``true_vorticity`` (the exact relative vorticity on that grid),
+        the relative max-norm error: figures/data_gen_convergence.py:58-95."""
+        from .spherical_harmonic_tools import enstrophy_energy
+        g = grid if grid is not None else structure_spherical_triangulation(L_out)
+        q = g.sample_points()
+        cos = g.d_cos_theta[:, None]
+        w0 = self.sampler(q, cos_grid=g.d_cos_theta, ring_len=2 * L_out).reshape(L_out + 1, 2 * L_out)
+        wT = self.vorticity_on_grid(L_out, grid=g)
+        e0, n0 = enstrophy_energy(w0 + 2 * self.rot * cos, L_out)
+        eT, nT = enstrophy_energy(wT + 2 * self.rot * cos, L_out)
+        out = {"enstrophy_rel_err": abs(eT - e0) / e0, "energy_rel_err": abs(nT - n0) / n0,
+               "enstrophy": eT, "energy": nT, "enstrophy_0": e0, "energy_0": n0}
+        if true_vorticity is not None:
+            t = B.to_dev(true_vorticity).reshape(L_out + 1, 2 * L_out)
+            out["max_rel_err"] = float((wT - t).abs().max() / t.abs().max())
+        return out
+
     def vorticity_on_grid(self, L_out, out=None, grid=None):
         """Relative vorticity w0(X) + 2 rot (cos theta(X) - cos theta) of the
         composed map on the MWSS grid of bandlimit L_out (the upsampling of

@@ -357,3 +357,17 @@ def energy_spectrum(alms, L):
     for ell in range(1, L):
         outs.append(sum(alms[ell, :] * alms[ell, :].conjugate()) / (ell * (ell + 1)))
     return np.array(outs)
+
+
+def enstrophy(samples, L):
+    """figures/data_gen_convergence.py:59-60, 88-89: |sum f_lm conj(f_lm)|
+    over the full coefficient set of a real field (its m < 0 half by
+    conjugate symmetry, as pysh.forward with Reality=False returns it)."""
+    a = sht_forward(samples, L)
+    return float(np.abs(np.sum(a * a.conjugate())))
+
+
+def energy(samples, L):
+    """figures/data_gen_convergence.py:61, 92: 0.5 |sum energy_spectrum|."""
+    a = sht_forward(samples, L)
+    return float(0.5 * np.abs(np.sum(energy_spectrum(coeff_array(a, L), L))))

@@ -231,3 +231,46 @@ def test_split_bf16_legendre_synthesis_error(L, nf):
         if prev is not None:
             assert max(errs) < prev, (mode, max(errs), prev)
         prev = max(errs)
+
+
+def test_enstrophy_energy_against_oracle():
+    """Device enstrophy / energy (forward transform + weighted sums) against
+    the oracle's restatement of figures/data_gen_convergence.py:58-95 on a
+    random real field, and on a batch."""
+    from oracle import sht as osht
+    from cmm_s2.core import spherical_harmonic_tools as st
+    rng = np.random.default_rng(8)
+    L = 24
+    fs = np.stack([rng.standard_normal((L + 1, 2 * L)) for _ in range(3)])
+    for k in range(3):
+        e, n = st.enstrophy_energy(fs[k], L)
+        e0, n0 = osht.enstrophy(fs[k], L), osht.energy(fs[k], L)
+        assert abs(e - e0) <= 1e-12 * e0 and abs(n - n0) <= 1e-12 * n0, (k, e, e0, n, n0)
+    eb, nb = st.enstrophy_energy(fs, L)
+    for k in range(3):
+        assert abs(eb[k] - osht.enstrophy(fs[k], L)) <= 1e-12 * eb[k]
+
+
+def test_conservation_errors_of_a_steady_run():
+    """BarotropicCMM.conservation_errors on the steady zonal jet: at t = 0
+    the absolute-vorticity enstrophy and energy equal the oracle's on the same
+    grid, and after the run the relative errors are small (the exact solution
+    is steady, so max_rel_err is the map error)."""
+    from oracle import sht as osht, fields as ofl, meshes as om
+    from cmm_s2.core import dynamical_fields as df, mesh_functions as mf
+    from cmm_s2.core.spherical_simulations import BarotropicCMM
+    ico = mf.icosahedral_mesh(4)
+    mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
+    L, Lo = 32, 48
+    sim = BarotropicCMM(L, 12, 12 / L, mesh, df.zonal_jet, 2 * np.pi, n_maps=4)
+    sim.run()
+    th = om.mwss_thetas(Lo)
+    ph = np.arange(2 * Lo) * np.pi / Lo
+    PH, TH = np.meshgrid(ph, th)
+    w_true = ofl.zonal_jet(PH, TH)
+    r = sim.conservation_errors(Lo, true_vorticity=w_true)
+    wabs = w_true + 2 * 2 * np.pi * np.cos(TH)
+    assert abs(r["enstrophy_0"] - osht.enstrophy(wabs, Lo)) <= 1e-10 * r["enstrophy_0"]
+    assert abs(r["energy_0"] - osht.energy(wabs, Lo)) <= 1e-10 * r["energy_0"]
+    # measured: 4.9e-5, 8.7e-5 and 1.7e-2 (the jet's steep flanks on ico-4 / L = 32)
+    assert r["enstrophy_rel_err"] < 1e-3 and r["energy_rel_err"] < 1e-3 and r["max_rel_err"] < 5e-2, r

@@ -185,3 +185,29 @@ def test_driver_rossby_haurwitz_wave_travels():
         qp, qt = g.cart2sphere(q.T)
         errs.append(np.abs(w - ofl.rossby_wave(qp, qt, t=T)).max())
     assert errs[1] < errs[0] / 4 and errs[1] < 2e-2, errs
+
+
+def test_enstrophy_energy_closed_forms():
+    """The oracle's diagnostics of figures/data_gen_convergence.py:58-95 on
+    single spherical harmonics c Y_lm + conj: enstrophy = (1 or 2) |c|^2 over
+    all orders, energy = half of it over l (l + 1) (the l = 0 term undivided),
+    and both add over orthogonal modes."""
+    from oracle import sht as osht
+    L = 12
+    tot_e = tot_n = 0.0
+    flm_all = np.zeros((L, L), complex)
+    for (l, m, c) in [(0, 0, 1.3), (3, 0, 0.7), (5, 2, 0.4 - 0.9j), (11, 11, 0.5j)]:
+        flm = np.zeros((L, L), complex)
+        flm[l, m] = c
+        flm_all[l, m] = c
+        f = osht.inverse_lm(flm, L)
+        mult = 1 if m == 0 else 2
+        e_true = mult * abs(c) ** 2
+        n_true = 0.5 * e_true / (l * (l + 1) if l > 0 else 1)
+        assert abs(osht.enstrophy(f, L) - e_true) < 1e-13 * e_true
+        assert abs(osht.energy(f, L) - n_true) < 1e-13 * n_true
+        tot_e += e_true
+        tot_n += n_true
+    f = osht.inverse_lm(flm_all, L)
+    assert abs(osht.enstrophy(f, L) - tot_e) < 1e-13 * tot_e
+    assert abs(osht.energy(f, L) - tot_n) < 1e-13 * tot_n
