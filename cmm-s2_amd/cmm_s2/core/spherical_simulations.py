@@ -200,6 +200,28 @@ class BarotropicCMM(object):
         from . import checkpoint as ck
         return ck.load_state(cls, path, mesh, vorticity)
 
+    def zoom_vorticity(self, x0=3.22055, y0=1.1963, ks=(-2, -4, -7, -10, -13), n=1500, same_center=True):
+        """Absolute vorticity w0(X) + 2 rot cos theta(X) of the composed map on
+        zoom windows (figures/zoom_sampling_figure.py:63-90): for each k an
+        n x n (phi, theta) grid of half-width 2^k, phi = linspace(x0 - 2^k,
+        x0 + 2^k, n) along columns.  The reference centres theta on x0 as
+        well (its `ys` repeats `x0`); ``same_center=False`` centres it on y0.
+        Returns a device tensor (len(ks), n, n) [window, theta row, phi col]."""
+        nw = len(ks)
+        pts = []
+        for k in ks:
+            h = 2.0 ** k
+            xs = np.linspace(x0 - h, x0 + h, n, endpoint=True)
+            yc = x0 if same_center else y0
+            ys = np.linspace(yc - h, yc + h, n, endpoint=True)
+            PH, TH = np.meshgrid(xs, ys)
+            pts.append(np.stack([np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)], -1).reshape(-1, 3))
+        q = B.to_dev(np.concatenate(pts))
+        X = self.submaps.compose(q, "N3", pre=[] if self.identity else [self.curr.packed])
+        cos_q = q[:, 2].contiguous()  # cos theta of each evaluation point, one point per "ring"
+        w = self.sampler(X, cos_grid=cos_q, ring_len=1)  # w0(X) + 2 rot (cos theta(X) - cos theta(q))
+        return (w + 2 * self.rot * cos_q).reshape(nw, n, n)
+
     def conservation_errors(self, L_out, grid=None, true_vorticity=None):
         """Relative enstrophy and energy errors of the absolute vorticity
         w0(X) + 2 rot cos theta(X) on the MWSS grid of bandlimit L_out

@@ -71,3 +71,35 @@ def test_plain_cmm_rh4_ico5(frame):
         away = np.abs(q[sub][:, 2]) < np.cos(0.5)
         assert err[away].max() < tol
         assert err.max() < 1e-7
+
+
+def test_zoom_vorticity_matches_oracle():
+    """zoom_vorticity (figures/zoom_sampling_figure.py:63-90) against the
+    oracle: the same windows through the oracle's composition of the run's
+    submaps (scipy find_simplex + NumPy PS evaluation) and the analytic
+    vorticity, with the reference's theta centre (x0) and with y0."""
+    from oracle import meshes as om, splines as osp, fields as ofl
+    from cmm_s2.core import dynamical_fields as df, mesh_functions as mf
+    from cmm_s2.core.spherical_simulations import BarotropicCMM
+    ico = mf.icosahedral_mesh(3)
+    mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
+    sim = BarotropicCMM(24, 12, 12 / 24, mesh, df.rossby_wave, 2 * np.pi, n_maps=4)
+    sim.run()
+    assert sim.identity  # the run ends with a remap: the chain is the submaps
+    oms = om.SphericalTriangulation(mesh.vertices)
+    vals = sim.submaps.vals[:sim.submaps.Nc].cpu().numpy()
+    levels = [osp.assemble(vals[i], oms.simplices, oms.ps_split) for i in range(sim.submaps.Nc)]
+    ks, n, x0, y0 = (-2, -5), 20, 3.22055, 1.1963
+    for same in (True, False):
+        got = sim.zoom_vorticity(x0, y0, ks, n, same_center=same).cpu().numpy()
+        for j, k in enumerate(ks):
+            h = 2.0 ** k
+            xs = np.linspace(x0 - h, x0 + h, n)
+            ys = np.linspace((x0 if same else y0) - h, (x0 if same else y0) + h, n)
+            PH, TH = np.meshgrid(xs, ys)
+            q = np.stack([np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)], -1).reshape(-1, 3)
+            X = osp.compose_eval(oms.query, levels, q)
+            ph = np.mod(np.arctan2(X[:, 1], X[:, 0]) + 2 * np.pi, 2 * np.pi)
+            th = np.arctan2(np.hypot(X[:, 0], X[:, 1]), X[:, 2])
+            ref = (ofl.rossby_wave(ph, th) + 2 * 2 * np.pi * np.cos(th)).reshape(n, n)
+            assert np.abs(got[j] - ref).max() <= 1e-11 * np.abs(ref).max(), (same, k)
