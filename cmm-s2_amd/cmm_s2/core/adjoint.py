@@ -341,12 +341,16 @@ class MisfitProblem(object):
         self._run(p)
         return self._objective(None)
 
-    def forward(self, p, keep_bytes=0):
+    def forward(self, p, keep_bytes=None):
         """J(p); keeps the remap checkpoints for ``backward``.  Remap
         segments whose tape fits in ``keep_bytes`` of device memory (taken
         from the last segment backwards) keep it, so the reverse sweep skips
-        their recomputation; 0 recomputes every segment from its checkpoint."""
+        their recomputation; None keeps the last segment's (the reverse
+        sweep holds two segment tapes at a time anyway), 0 recomputes every
+        segment from its checkpoint."""
         import time
+        if keep_bytes is None:
+            keep_bytes = self._segment_bytes(len(self.ns) - 1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         self._cps = []
@@ -356,9 +360,42 @@ class MisfitProblem(object):
         self.timings = {"forward_s": time.perf_counter() - t0}
         return J
 
-    def backward(self):
+    def _record(self, k, final_maps):
+        """Recompute remap segment k from its checkpoint with the tape on:
+        (tape, velocity levels, maps), the form a kept segment has."""
+        tape = []
+        self.maps = dict(final_maps)
+        cp = self._cps[k]
+        if cp is None:
+            self.V = _Velocity()
+            self.U, self.submaps = {}, []
+            self.curr = None
+            self._bootstrap(tape)
+        else:
+            self._restore(cp)
+        while self.count < self.ns[k]:
+            self._step(tape)
+        return tape, dict(self.U), dict(self.maps)
+
+    @staticmethod
+    def _hand_over(seg, stream):
+        """Tensors of a segment recorded on another stream are used on
+        ``stream``: the allocator must not reuse them before that work ends."""
+        tape, U, maps = seg
+        for t in list(U.values()) + list(maps.values()):
+            t.record_stream(stream)
+        for op in tape:
+            for x in op:
+                if isinstance(x, torch.Tensor):
+                    x.record_stream(stream)
+
+    def backward(self, overlap=True):
         """dJ/dp at the point of the last ``forward``: recompute each remap
-        segment from its checkpoint with the tape on and walk it backwards."""
+        segment from its checkpoint with the tape on and walk it backwards.
+        With ``overlap`` the recomputation of segment k-1 runs on a second
+        HIP stream (default priority) while segment k is reversed on the
+        current one: the two are independent, and the reverse of segment k-1
+        waits for its recording only."""
         import time
         cps = self._cps
         torch.cuda.synchronize()
@@ -371,28 +408,36 @@ class MisfitProblem(object):
         self._reverse(tape)
         kept = self._kept
         self._kept = {}
-        for k in range(len(cps) - 1, -1, -1):
-            if k in kept:
-                tape, U, maps = kept.pop(k)
-                self.U = U
-                self.maps = dict(final_maps)
-                self.maps.update(maps)
-                self._reverse(tape)
-                del tape, U, maps
-                continue
-            tape = []
+        main = torch.cuda.current_stream()
+        side = None
+        if overlap:
+            side = getattr(self, "_side_stream", None)
+            if side is None:
+                side = torch.cuda.Stream()
+                self._side_stream = side
+        K = len(cps) - 1
+        nxt = kept.pop(K) if K in kept else self._record(K, final_maps)
+        for k in range(K, -1, -1):
+            seg, nxt, ready = nxt, None, None
+            if k > 0:
+                if k - 1 in kept:
+                    nxt = kept.pop(k - 1)
+                elif side is not None:
+                    with torch.cuda.stream(side):
+                        nxt = self._record(k - 1, final_maps)
+                        ready = torch.cuda.Event()
+                        ready.record(side)
+                    self._hand_over(nxt, main)
+            tape, U, maps = seg
+            self.U = U
             self.maps = dict(final_maps)
-            if cps[k] is None:
-                self.V = _Velocity()
-                self.U, self.submaps = {}, []
-                self.curr = None
-                self._bootstrap(tape)
-            else:
-                self._restore(cps[k])
-            while self.count < self.ns[k]:
-                self._step(tape)
+            self.maps.update(maps)
             self._reverse(tape)
-            del tape
+            del seg, tape, U, maps
+            if ready is not None:
+                main.wait_event(ready)
+            elif k > 0 and nxt is None:
+                nxt = self._record(k - 1, final_maps)
         g = self._omega0_adjoint()
         torch.cuda.synchronize()
         self.timings["reverse_s"] = time.perf_counter() - t0
@@ -400,9 +445,9 @@ class MisfitProblem(object):
         self._cps = None
         return g
 
-    def value_and_grad(self, p, keep_bytes=0):
+    def value_and_grad(self, p, keep_bytes=None, overlap=True):
         J = self.forward(p, keep_bytes)
-        return J, self.backward()
+        return J, self.backward(overlap)
 
     # ---- reverse sweep -----------------------------------------------------
     def _lam_map(self, mid):

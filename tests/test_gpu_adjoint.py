@@ -411,14 +411,17 @@ def test_loop_gradient_fd_small():
 
 def test_kept_tapes_match_recomputation():
     """Keeping the tape of every remap segment, of the last one only, or of
-    none (recomputation from the checkpoints) gives the same J and dJ/dp (to
-    the atomic summation order, ~1e-13)."""
+    none (recomputation from the checkpoints, on a second stream or on the
+    same one) gives the same J and dJ/dp (to the atomic summation order,
+    ~1e-13)."""
     from cmm_s2.core import dynamical_fields as df
     prob = _small_problem()
     p = torch.from_numpy(prob.params.from_coeffs(df.random_sh_coefficients(502, 8, 32))).cuda()
-    J0, g0 = prob.value_and_grad(p)
+    J0, g0 = prob.value_and_grad(p, keep_bytes=0)  # every segment recomputed
+    Js, gs = prob.value_and_grad(p, keep_bytes=0, overlap=False)  # ... on the same stream
+    assert Js == J0 and float((gs - g0).norm()) <= 1e-12 * float(g0.norm())
     last = prob._segment_bytes(len(prob.ns) - 1)
-    for budget in (1 << 40, last):
+    for budget in (1 << 40, last, None):
         J, g = prob.value_and_grad(p, keep_bytes=budget)
         assert J == J0
         assert float((g - g0).norm()) <= 1e-12 * float(g0.norm()), budget
