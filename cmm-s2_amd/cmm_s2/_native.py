@@ -81,6 +81,9 @@ SIGNATURES = {
     "cmm_tri_assemble_adjoint": (C.c_int, [vp, vp, vp, vp, i64, vp]),
     "cmm_advect_stencil_adjoint": (C.c_int, [C.POINTER(Velocity), C.c_int, dbl, vp, i64, i64, vp, i64, P_vp, vp,
                                              i64, vp, i64, vp]),
+    "cmm_advect_stencil_adjoint_corners": (C.c_int, [C.POINTER(Velocity), C.c_int, dbl, vp, i64, i64, vp, i64, P_vp,
+                                                     P_vp, vp, i64, vp, i64, vp]),
+    "cmm_mwss_corner_jets_adjoint": (C.c_int, [vp, vp, vp, vp, vp]),
     "cmm_tri_stencil_jets_adjoint": (C.c_int, [vp, vp, vp, i64, vp, i64, vp, dbl, vp, vp, i64, vp, vp, i64, vp]),
     "cmm_tri_compose_adjoint": (C.c_int, [vp, P_vp, P_vp, C.c_int, vp, i64, i64, i64, vp, vp, vp, i64, vp]),
     "cmm_sample_vorticity_adjoint": (C.c_int, [vp, vp, vp, i64, i64, i64, vp, dbl, vp, i64, i64, vp, vp, i64, vp]),

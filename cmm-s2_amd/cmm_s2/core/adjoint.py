@@ -458,9 +458,11 @@ class MisfitProblem(object):
         return t
 
     def _lam_vel(self, vid):
+        """adjoint of velocity ``vid`` as sparse corner jets: (Tu, 36)
+        values, valid where the (Tu,) flags are set"""
         t = self.lam_U.get(vid)
         if t is None:
-            t = B.zeros(self._tu, 108)
+            t = (B.empty(self._tu, 36), torch.zeros(self._tu, dtype=torch.uint8, device=B.device()))
             self.lam_U[vid] = t
         return t
 
@@ -496,9 +498,8 @@ class MisfitProblem(object):
                 if lam is None:
                     continue
                 vb = B.zeros(12, self._nu)
-                wp, wb = self._ws(96 * 3 * self._tu)
-                N.call("cmm_mwss_assemble_adjoint", self.mesh_u._h, lam.data_ptr(), 3, vb.data_ptr(), wp, wb,
-                       B.stream())
+                N.call("cmm_mwss_corner_jets_adjoint", self.mesh_u._h, lam[0].data_ptr(), lam[1].data_ptr(),
+                       vb.data_ptr(), B.stream())
                 fb = B.empty(12, L + 1, 2 * L)
                 N.call("cmm_mwss_velocity_jets_adjoint", self.mesh_u._h, vb.data_ptr(), fb.data_ptr(), B.stream())
                 ob = B.empty(nlm(L), 12, 2)
@@ -530,11 +531,13 @@ class MisfitProblem(object):
                        ys.stride(0), m.d_tan.data_ptr(), self.EPS, jb.data_ptr(), sb.data_ptr(), sb.stride(0),
                        curb, None, 0, B.stream())
                 desc = self._desc(V, ikind, t)
-                lbs = N.ptr_array([self._lam_vel(V.ids[k]).data_ptr() for k in range(min(V.nVs, 3))])
+                lams = [self._lam_vel(V.ids[k]) for k in range(min(V.nVs, 3))]
+                lcs = N.ptr_array([c.data_ptr() for c, _ in lams])
+                lts = N.ptr_array([f.data_ptr() for _, f in lams])
                 xs = m.d_spts
-                wp, wb = self._scatter(16 * nv, 6 * self._tu)
-                N.call("cmm_advect_stencil_adjoint", desc, ikind, float(self.dt), xs.data_ptr(), xs.stride(0), nv,
-                       sb.data_ptr(), sb.stride(0), lbs, None, 0, wp, wb, B.stream())
+                wp, wb = self._scatter(16 * nv, self._tu)
+                N.call("cmm_advect_stencil_adjoint_corners", desc, ikind, float(self.dt), xs.data_ptr(), xs.stride(0),
+                       nv, sb.data_ptr(), sb.stride(0), lcs, lts, None, 0, wp, wb, B.stream())
                 if mid not in self.submaps:
                     self.maps.pop(mid, None)
 

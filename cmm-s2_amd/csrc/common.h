@@ -220,7 +220,7 @@ __host__ __device__ inline void ps_pack(const PSGeom& P, double* p, int64_t stri
   for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) p[(k++) * stride] = P.E[i][j];
   for (int i = 0; i < 9; ++i) p[(k++) * stride] = P.Gi[i];
 }
-__host__ __device__ inline void ps_unpack(const double* __restrict__ p, int64_t stride, PSGeom& P) {
+__host__ __device__ __forceinline__ void ps_unpack(const double* __restrict__ p, int64_t stride, PSGeom& P) {
   int k = 0;
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -292,7 +292,7 @@ __host__ __device__ inline void ps_coeffs(const PSGeom& P, const double f[3], co
 
 // Transpose of ps_coeffs (a linear map of f, gr for fixed geometry): the
 // adjoints cb[19] of the coefficients -> adjoints added into fb[3], gb[3].
-__host__ __device__ inline void ps_coeffs_adj(const PSGeom& P, const double cb_in[19], double fb[3], d3 gb[3]) {
+__host__ __device__ __forceinline__ void ps_coeffs_adj(const PSGeom& P, const double cb_in[19], double fb[3], d3 gb[3]) {
   double c[19];
   for (int k = 0; k < 19; ++k) c[k] = cb_in[k];
   c[3] += P.E[0][0] * c[12];

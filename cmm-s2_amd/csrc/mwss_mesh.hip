@@ -799,11 +799,28 @@ __global__ __launch_bounds__(64 * NC) void k_mwss_assemble_adj_corners(cmm_mwss_
   }
 }
 
+// Corner jets of the transposed assembly, in one of two layouts: the
+// per-triangle pass of a packed adjoint writes corner i row r of triangle t
+// at p[(R i + r) Tu + t] for every triangle; the sparse velocity adjoint
+// (k_mwss_corner_reduce) at p[3 R t + R i + r] for the triangles with
+// touched[t] != 0 only (the others hold no adjoint and are never read).
+template <int NC>
+struct CornerSrc {
+  static constexpr int R = 4 * NC;
+  const double* p;
+  const uint8_t* touched;  // null: the first layout
+  int64_t Tu;
+  __device__ bool has(int64_t t) const { return !touched || touched[t] != 0; }
+  __device__ double at(int64_t t, int i, int r) const {
+    return touched ? p[(3 * R) * t + R * i + r] : p[(R * i + r) * Tu + t];
+  }
+};
+
 // pole p (0 north: corner 0 of triangles 0..N-1; 1 south: corner 1 of the
 // last N triangles), reduced by one block of 256 threads
 template <int NC>
-__device__ inline void mwss_pole_reduce(const cmm_mwss_mesh& M, const double* __restrict__ tmp,
-                                        double* __restrict__ vb, int p) {
+__device__ inline void mwss_pole_reduce(const cmm_mwss_mesh& M, const CornerSrc<NC>& src, double* __restrict__ vb,
+                                        int p) {
   constexpr int R = 4 * NC;
   __shared__ double red[R][4];
   const int N = M.N;
@@ -816,8 +833,9 @@ __device__ inline void mwss_pole_reduce(const cmm_mwss_mesh& M, const double* __
 #pragma unroll
   for (int q = 0; q < R; ++q) a[q] = 0.0;
   for (int c = threadIdx.x; c < N; c += 256)
+    if (src.has(t0 + c))
 #pragma unroll
-    for (int q = 0; q < R; ++q) a[q] += tmp[(R * corner + q) * Tu + t0 + c];
+      for (int q = 0; q < R; ++q) a[q] += src.at(t0 + c, corner, q);
 #pragma unroll
   for (int q = 0; q < R; ++q) {
 #pragma unroll
@@ -833,11 +851,11 @@ __device__ inline void mwss_pole_reduce(const cmm_mwss_mesh& M, const double* __
 // blocks 0 .. ceil((Nu - 2) / 256) - 1: the interior vertices; the last two
 // blocks: the poles
 template <int NC>
-__global__ __launch_bounds__(256) void k_mwss_assemble_adj_gather(cmm_mwss_mesh M, const double* __restrict__ tmp,
+__global__ __launch_bounds__(256) void k_mwss_assemble_adj_gather(cmm_mwss_mesh M, CornerSrc<NC> src,
                                                                   double* __restrict__ vb) {
   const int nbi = (int)((M.Nu - 2 + 255) / 256);
   if ((int)blockIdx.x >= nbi) {
-    mwss_pole_reduce<NC>(M, tmp, vb, (int)blockIdx.x - nbi);
+    mwss_pole_reduce<NC>(M, src, vb, (int)blockIdx.x - nbi);
     return;
   }
   const int64_t v = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -868,14 +886,208 @@ __global__ __launch_bounds__(256) void k_mwss_assemble_adj_gather(cmm_mwss_mesh 
   for (int q = 0; q < R; ++q) acc[q] = 0.0;
 #pragma unroll
   for (int e = 0; e < 6; ++e) {  // fixed trip count: every load of the vertex in flight together
-    if (e < k) {
+    if (e < k && src.has(ts[e])) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) acc[q] += tmp[(R * cs[e] + q) * Tu + ts[e]];
+      for (int q = 0; q < R; ++q) acc[q] += src.at(ts[e], cs[e], q);
     }
   }
   const int64_t Nu = M.Nu;
 #pragma unroll
   for (int q = 0; q < R; ++q) vb[q * Nu + v] += acc[q];
+}
+
+// (row, column, half) of MWSS triangle t (< 2^31): N cap triangles, 2 N per
+// band, N cap triangles
+__device__ inline void mwss_tri_cell(const cmm_mwss_mesh& M, int t, int& row, int& col, int& half) {
+  const int N = M.N, Tu = (int)M.Tu;
+  if (t < N) { row = 0; col = t; half = 0; }
+  else if (t >= Tu - N) { row = M.M - 2; col = t - (Tu - N); half = 0; }
+  else {
+    const int u = t - N;
+    half = u & 1;
+    const int k = u >> 1;
+    row = 1 + k / N;
+    col = k - (row - 1) * N;
+  }
+}
+
+// Sparse transpose of the velocity assembly for the stencil-feet adjoint.
+// The records (value records, 18 sums, class = RK stage, sub = sub-triangle)
+// are grouped by triangle, with the lists of the triangles that have records
+// (at most kCornerSplit of them: one thread per (triangle, level); more: one
+// wave).  For triangle t and level o, the stage-weighted sums of its records
+// through the transposed sub-triangle packing and ps_coeffs_adj give the jets
+// adjoints of the 3 corners, added into corners[o] (layout of CornerSrc with
+// touched flags; stored on the first touch).  Each (t, o) has one writer.
+constexpr uint32_t kCornerSplit = 8;
+
+struct CornerBars {
+  double* c[3];
+  uint8_t* touched[3];
+  double W[4][3];
+  int nout;
+};
+
+// level o's arrays and stage weights, selected without indexing the kernel
+// argument block (a dynamic index would copy it to scratch)
+struct CornerLevel {
+  double* c;
+  uint8_t* touched;
+  double w[4];
+  __device__ __forceinline__ double weight(int cls) const {
+    return cls == 0 ? w[0] : (cls == 1 ? w[1] : (cls == 2 ? w[2] : w[3]));
+  }
+};
+#define CORNER_LEVEL(out, o, L)                                                                      \
+  CornerLevel L;                                                                                     \
+  L.c = (o) == 0 ? (out).c[0] : ((o) == 1 ? (out).c[1] : (out).c[2]);                                \
+  L.touched = (o) == 0 ? (out).touched[0] : ((o) == 1 ? (out).touched[1] : (out).touched[2]);        \
+  for (int c_ = 0; c_ < 4; ++c_)                                                                     \
+    L.w[c_] = (o) == 0 ? (out).W[c_][0] : ((o) == 1 ? (out).W[c_][1] : (out).W[c_][2]);
+
+// one record into the coefficient adjoints: every sub-triangle's 18 terms
+// with the weight zeroed unless it is the record's (branch-free, constant
+// indices: a switch over the sub-triangle is merged by the compiler into an
+// indexed access that puts cb in scratch)
+__device__ __forceinline__ void corner_accumulate(const ScatterRecV& rv, const CornerLevel& lv, double (&cb)[3][19]) {
+  const double wt = lv.weight(rv.cls);
+  const int sub = rv.sub;
+  double v[18];
+#pragma unroll
+  for (int e = 0; e < 18; ++e) v[e] = rv.v[e];
+#pragma unroll
+  for (int S = 0; S < 6; ++S) {
+    const double m = sub == S ? wt : 0.0;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) cb[cc][cs_table(S, j)] += m * v[6 * cc + j];
+  }
+}
+
+// the row geometry (unrotated) and column rotation of triangle t
+struct CornerGeom {
+  PSGeom P;
+  double c, sn;
+  __device__ __forceinline__ CornerGeom(const cmm_mwss_mesh& M, uint32_t t) {
+    int row, col, half;
+    mwss_tri_cell(M, (int)t, row, col, half);
+    ps_unpack(M.rowgeom + ((int64_t)row * 2 + half) * kPSGeomLen, 1, P);
+    c = M.cp[col];
+    sn = M.sp[col];
+  }
+};
+
+// component cc: coefficient adjoints cb -> its 12 corner jets adjoints
+// res[12 i + row].  ps_coeffs_adj runs on the unrotated row geometry and the
+// gradient adjoints are rotated to the column afterwards (H enters them
+// linearly).
+__device__ __forceinline__ void corner_jets(const CornerGeom& G, int cc, const double (&cb)[19], double (&res)[36]) {
+  double fb[3] = {0.0, 0.0, 0.0};
+  d3 gb[3] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
+  ps_coeffs_adj(G.P, cb, fb, gb);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    res[12 * i + cc * 4] = fb[i];                            // f
+    res[12 * i + 1 + cc] = G.c * gb[i].x - G.sn * gb[i].y;   // d/dx
+    res[12 * i + 5 + cc] = G.sn * gb[i].x + G.c * gb[i].y;   // d/dy
+    res[12 * i + 9 + cc] = gb[i].z;                          // d/dz
+  }
+}
+
+// short groups: a fixed grid strides over the blocks (list chunk k / nout,
+// level k % nout: the levels of a chunk run together and share its records
+// in L2)
+__global__ __launch_bounds__(128) void k_mwss_corner_reduce(cmm_mwss_mesh M, ScatterWork w, CornerBars out) {
+  const uint32_t nlist = w.meta[0];
+  const uint32_t nblk = (nlist + 127) / 128 * (uint32_t)out.nout;
+  for (uint32_t k = blockIdx.x; k < nblk; k += gridDim.x) {
+    const int o = (int)(k % out.nout);
+    const uint32_t j = (k / out.nout) * 128 + threadIdx.x;
+    if (j >= nlist) continue;
+    const uint32_t t = w.keys_u[j];
+    const uint32_t a = w.offs[t], b = w.offs[t + 1];
+    CORNER_LEVEL(out, o, lv)
+    double cb[3][19];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int q = 0; q < 19; ++q) cb[cc][q] = 0.0;
+    for (uint32_t r = a; r < b; ++r) corner_accumulate(w.recv[w.idx_s[r]], lv, cb);
+    const CornerGeom G(M, t);
+    double res[36];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) corner_jets(G, cc, cb[cc], res);
+    double2* dst = reinterpret_cast<double2*>(lv.c + (int64_t)t * 36);
+    if (lv.touched[t] != 0) {
+      double2 old[18];
+#pragma unroll
+      for (int q = 0; q < 18; ++q) old[q] = dst[q];
+#pragma unroll
+      for (int q = 0; q < 18; ++q) {
+        res[2 * q] += old[q].x;
+        res[2 * q + 1] += old[q].y;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 18; ++q) dst[q] = double2{res[2 * q], res[2 * q + 1]};
+    lv.touched[t] = 1;
+  }
+}
+
+// long groups: 16-lane segments (4 per wave) of a fixed grid stride over
+// (list entry g / nout, level g % nout); the segment's lanes stride over the
+// records, a butterfly within the segment sums them, segment lanes 0..2
+// store the 3 components.  Segments past the end take part in the
+// butterflies with zeros and store nothing.
+__global__ __launch_bounds__(256) void k_mwss_corner_reduce_long(cmm_mwss_mesh M, ScatterWork w, CornerBars out) {
+  const int lane = (int)(threadIdx.x & 63), sl = lane & 15;
+  const uint32_t ng = w.meta[1] * (uint32_t)out.nout;
+  const uint32_t nseg = gridDim.x * (blockDim.x / 16);
+  const uint32_t seg0 = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) / 16;  // first segment of this wave
+  for (uint32_t gw = seg0; gw < ng; gw += nseg) {  // wave-uniform
+    const uint32_t g = gw + (uint32_t)(lane >> 4);
+    const bool live = g < ng;
+    const int o = live ? (int)(g % out.nout) : 0;
+    const uint32_t t = live ? w.keys_l[g / out.nout] : 0u;
+    const uint32_t a = live ? w.offs[t] : 0u, b = live ? w.offs[t + 1] : 0u;
+    CORNER_LEVEL(out, o, lv)
+    double cb[3][19];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int q = 0; q < 19; ++q) cb[cc][q] = 0.0;
+    for (uint32_t r = a + sl; r < b; r += 16) corner_accumulate(w.recv[w.idx_s[r]], lv, cb);
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        double v = cb[cc][q];
+#pragma unroll
+        for (int m = 8; m > 0; m >>= 1) v += __shfl_xor(v, m);
+        cb[cc][q] = v;
+      }
+    const bool fresh = live && lv.touched[t] == 0;
+    if (live && sl < 3) {
+      const CornerGeom G(M, t);
+      double res[36];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+        if (sl == cc) corner_jets(G, cc, cb[cc], res);
+      double* dst = lv.c + (int64_t)t * 36;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+        if (sl == cc)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const int rr[4] = {12 * i + cc * 4, 12 * i + 1 + cc, 12 * i + 5 + cc, 12 * i + 9 + cc};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dst[rr[q]] = fresh ? res[rr[q]] : dst[rr[q]] + res[rr[q]];
+          }
+    }
+    // every lane of the segment has read the flag above
+    if (live && sl == 0) lv.touched[t] = 1;
+  }
 }
 
 // located velocity-spline point: triangle, sub-triangle, sub-triangle
@@ -918,8 +1130,10 @@ __device__ __forceinline__ VLoc vel_locate(const cmm_mwss_mesh& M, d3 x) {
 // summed over the 4 lanes of a quad first when the whole quad sits in one
 // sub-triangle: the four stencil feet of a vertex are 1e-5 apart) and
 // returns xbar = (dv/dx)^T vbar.  Every lane of the quad must call it.
+// Records are keyed by sub-triangle (tri_keys false) or by triangle (true,
+// the sub-triangle travels in the record).
 __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, double* const* lb,
-                                      const ScatterWork* sw, int64_t slot) {
+                                      const ScatterWork* sw, int64_t slot, bool tri_keys) {
   const VLoc r = vel_locate(V.mesh, x);
   const LevelSet& ls = V.ls[stage];
   const int lane = threadIdx.x & 63;
@@ -951,7 +1165,7 @@ __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, d
     // weights are applied there); slot < 0: a lane past the end
     if (slot >= 0) {
       if (issue)
-        scatter_put_values(*sw, slot, (uint32_t)(r.tri * 6 + r.s), cv, stage);
+        scatter_put_values(*sw, slot, (uint32_t)(tri_keys ? r.tri : r.tri * 6 + r.s), cv, stage, r.s);
       else
         scatter_empty(*sw, slot);
     }
@@ -991,7 +1205,7 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
                                                             const double* __restrict__ xs, int64_t xs_row, int64_t nv,
                                                             const double* __restrict__ yb, int64_t yb_row,
                                                             LevelBars LB, double* __restrict__ xb, int64_t xb_row,
-                                                            ScatterWork swk, int rec) {
+                                                            ScatterWork swk, int rec, int tri_keys) {
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // the last quad may reach past the end: clamp (its lanes then repeat a
   // valid point and add nothing)
@@ -1016,7 +1230,7 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
     const d3 v1 = vel_eval(V, 0, xn);
     const d3 z = sub(xn, scl(dt, v1));
     const d3 zb = unit_adj(z, ybar);
-    xbar = add(zb, vel_adj(V, 0, xn, scl(-dt, zb), lb, sw, live ? s0 : -1));
+    xbar = add(zb, vel_adj(V, 0, xn, scl(-dt, zb), lb, sw, live ? s0 : -1, tri_keys != 0));
   } else if (integ == CMM_INT_IMPROVED_EULER) {
     const d3 v1 = vel_eval(V, 0, xn);
     const d3 z1 = sub(xn, scl(dt, v1));
@@ -1026,10 +1240,10 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
     const d3 zb = unit_adj(z, ybar);
     const d3 v2b = scl(-dt / 2, zb);
     d3 v1b = v2b;
-    const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, live ? s0 + 1 : -1);
+    const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, live ? s0 + 1 : -1, tri_keys != 0);
     const d3 z1b = unit_adj(z1, s1b);
     v1b = add(v1b, scl(-dt, z1b));
-    xbar = add(add(zb, z1b), vel_adj(V, 0, xn, v1b, lb, sw, live ? s0 : -1));
+    xbar = add(add(zb, z1b), vel_adj(V, 0, xn, v1b, lb, sw, live ? s0 : -1, tri_keys != 0));
   } else {
   const d3 v1 = vel_eval(V, 0, xn);
   const d3 z1 = d3{xn.x - dt * v1.x / 2, xn.y - dt * v1.y / 2, xn.z - dt * v1.z / 2};
@@ -1047,16 +1261,16 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
   const d3 ab = scl(-dt / 6, unit_adj(z4, ybar));
   d3 v1b = ab, v2b = scl(2, ab), v3b = scl(2, ab);
   const d3 v4b = ab;
-  const d3 s3b = vel_adj(V, 3, s3, v4b, lb, sw, live ? s0 + 3 : -1);
+  const d3 s3b = vel_adj(V, 3, s3, v4b, lb, sw, live ? s0 + 3 : -1, tri_keys != 0);
   v3b = add(v3b, scl(-dt, unit_adj(z3, s3b)));
-  const d3 s2b = vel_adj(V, 2, s2, v3b, lb, sw, live ? s0 + 2 : -1);
+  const d3 s2b = vel_adj(V, 2, s2, v3b, lb, sw, live ? s0 + 2 : -1, tri_keys != 0);
   v2b = add(v2b, scl(-dt / 2, unit_adj(z2, s2b)));
-  const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, live ? s0 + 1 : -1);
+  const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, live ? s0 + 1 : -1, tri_keys != 0);
   const d3 z1b = unit_adj(z1, s1b);
   v1b = add(v1b, scl(-dt / 2, z1b));
   // z4 = xn - ..., z3 = xn - ..., z2 = xn - ..., z1 = xn - ... all carry xn
   const d3 z4b = unit_adj(z4, ybar), z3b = unit_adj(z3, s3b), z2b = unit_adj(z2, s2b);
-  xbar = add(add(add(z4b, z3b), add(z2b, z1b)), vel_adj(V, 0, xn, v1b, lb, sw, live ? s0 : -1));
+  xbar = add(add(add(z4b, z3b), add(z2b, z1b)), vel_adj(V, 0, xn, v1b, lb, sw, live ? s0 : -1, tri_keys != 0));
   }
   if (xb && live) {
     double* dst = xb + (int64_t)k * nv + v;
@@ -1312,12 +1526,12 @@ extern "C" int cmm_mwss_assemble_adjoint(const cmm_mwss_mesh* m, const double* p
     const int64_t ni = m->Nu - 2;
     if (ncomp == 3) {
       hipLaunchKernelGGL(k_mwss_assemble_adj_corners<3>, dim3(grid_for(m->Tu, 64)), dim3(192), 0, st, *m, packed_bar, tmp);
-      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<3>, dim3(grid_for(ni, 256) + 2), dim3(256), 0, st, *m, tmp,
-                         vals_bar);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<3>, dim3(grid_for(ni, 256) + 2), dim3(256), 0, st, *m,
+                         CornerSrc<3>{tmp, nullptr, m->Tu}, vals_bar);
     } else {
       hipLaunchKernelGGL(k_mwss_assemble_adj_corners<1>, dim3(grid_for(m->Tu, 64)), dim3(64), 0, st, *m, packed_bar, tmp);
-      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<1>, dim3(grid_for(ni, 256) + 2), dim3(256), 0, st, *m, tmp,
-                         vals_bar);
+      hipLaunchKernelGGL(k_mwss_assemble_adj_gather<1>, dim3(grid_for(ni, 256) + 2), dim3(256), 0, st, *m,
+                         CornerSrc<1>{tmp, nullptr, m->Tu}, vals_bar);
     }
     CMM_LAUNCH_CHECK("cmm_mwss_assemble_adjoint");
     return CMM_OK;
@@ -1356,8 +1570,12 @@ extern "C" int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integ, do
     if (rc) return rc;
   }
   hipStream_t s = as_stream(stream);
+  if (work) {
+    rc = scatter_begin(sw, s);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_advect_stencil_adj, dim3(grid_for(4 * nv, 128)), dim3(128), 0, s, V, integ, dt, st, st_row, nv,
-                     yst_bar, yb_row, LB, st_bar, sb_row, sw, work ? 1 : 0);
+                     yst_bar, yb_row, LB, st_bar, sb_row, sw, work ? 1 : 0, 0);
   CMM_LAUNCH_CHECK("cmm_advect_stencil_adjoint");
   if (work) {
     ScatterTargets t;
@@ -1370,6 +1588,70 @@ extern "C" int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integ, do
     }
     return scatter_reduce(sw, nrec, t, s);
   }
+  return CMM_OK;
+}
+
+extern "C" int cmm_advect_stencil_adjoint_corners(const cmm_velocity* vel, int integ, double dt, const double* st,
+                                                  int64_t st_row, int64_t nv, const double* yst_bar, int64_t yb_row,
+                                                  double* const* level_corners, uint8_t* const* level_touched,
+                                                  double* st_bar, int64_t sb_row, void* work, int64_t work_bytes,
+                                                  void* stream) {
+  if (!vel || !st || !yst_bar || !level_corners || !level_touched || !work || nv < 1 || st_row < 4 * nv ||
+      yb_row < 4 * nv || (st_bar && sb_row < 4 * nv) ||
+      (integ != CMM_INT_EULER && integ != CMM_INT_IMPROVED_EULER && integ != CMM_INT_RK4))
+    return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint_corners: bad arguments");
+  if (vel->kind != CMM_VEL_SPLINE)
+    return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint_corners: needs a spline velocity");
+  VelDev V;
+  int rc = velocity_device(vel, "cmm_advect_stencil_adjoint_corners", V);
+  if (rc) return rc;
+  CornerBars out;
+  out.nout = vel->nlev;
+  for (int k = 0; k < 3; ++k) {
+    out.c[k] = k < vel->nlev ? level_corners[k] : nullptr;
+    out.touched[k] = k < vel->nlev ? level_touched[k] : nullptr;
+    if (k < vel->nlev && (!out.c[k] || !out.touched[k]))
+      return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint_corners: null level adjoint");
+    for (int g = 0; g < 4; ++g) out.W[g][k] = k < vel->nlev ? vel->stage_w[g][k] : 0.0;
+  }
+  LevelBars LB;
+  for (int k = 0; k < 3; ++k) LB.p[k] = nullptr;
+  ScatterWork sw;
+  const int64_t nrec = 16 * nv;  // 4 stages per stencil foot
+  if (vel->mesh->Tu > (int64_t)0x7fffffff) return fail(CMM_E_RANGE, "cmm_advect_stencil_adjoint_corners: mesh too large");
+  rc = scatter_layout(work, work_bytes, nrec, vel->mesh->Tu, sw);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  rc = scatter_begin(sw, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_advect_stencil_adj, dim3(grid_for(4 * nv, 128)), dim3(128), 0, s, V, integ, dt, st, st_row, nv,
+                     yst_bar, yb_row, LB, st_bar, sb_row, sw, 1, 1);
+  CMM_LAUNCH_CHECK("cmm_advect_stencil_adjoint_corners");
+  rc = scatter_group(sw, nrec, s, kCornerSplit);
+  if (rc) return rc;
+  // at most min(records, Tu) short and min(records / (split + 1), Tu) long
+  // groups
+  // fixed grids that stride over the lists (their lengths are on the
+  // device): up to 8 blocks of 128 / 2 of 256 threads per CU
+  const int64_t Tu = vel->mesh->Tu;
+  const int64_t nshort = grid_for(std::min(nrec, Tu), 128) * vel->nlev;
+  hipLaunchKernelGGL(k_mwss_corner_reduce, dim3((unsigned)std::min<int64_t>(nshort, 2048)), dim3(128), 0, s, *vel->mesh,
+                     sw, out);
+  const int64_t nlong = std::min(nrec / (kCornerSplit + 1), Tu) * vel->nlev;  // 16-lane segments
+  if (nlong > 0)
+    hipLaunchKernelGGL(k_mwss_corner_reduce_long, dim3((unsigned)std::min<int64_t>(grid_for(nlong, 16), 512)),
+                       dim3(256), 0, s, *vel->mesh, sw, out);
+  CMM_LAUNCH_CHECK("cmm_advect_stencil_adjoint_corners: reduce");
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_corner_jets_adjoint(const cmm_mwss_mesh* m, const double* corners, const uint8_t* touched,
+                                            double* vals_bar, void* stream) {
+  if (!m || !corners || !touched || !vals_bar) return fail(CMM_E_ARG, "cmm_mwss_corner_jets_adjoint: bad arguments");
+  const int64_t ni = m->Nu - 2;
+  hipLaunchKernelGGL(k_mwss_assemble_adj_gather<3>, dim3(grid_for(ni, 256) + 2), dim3(256), 0, as_stream(stream), *m,
+                     CornerSrc<3>{corners, touched, m->Tu}, vals_bar);
+  CMM_LAUNCH_CHECK("cmm_mwss_corner_jets_adjoint");
   return CMM_OK;
 }
 
@@ -1387,6 +1669,10 @@ extern "C" int cmm_sample_vorticity_adjoint(const cmm_mwss_mesh* wmesh, const do
     if (rc) return rc;
   }
   hipStream_t s = as_stream(stream);
+  if (rec) {
+    int rc = scatter_begin(sw, s);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_sample_vorticity_adj, dim3(grid_for(n, 256)), dim3(256), 0, s, *wmesh, wpacked, X, n, sp, sc,
                      wbar, rot_rate, xbar, bsp, bsc, wpacked_bar, sw, rec ? 1 : 0);
   CMM_LAUNCH_CHECK("cmm_sample_vorticity_adjoint");

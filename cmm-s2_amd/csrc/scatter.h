@@ -9,6 +9,11 @@
 // counting sort (int32 histogram, one scan, placement by counter) groups
 // equal keys and one thread per group adds it with plain stores (one writer
 // per target block, no FP64 atomics).
+//
+// Sequence: scatter_layout, scatter_begin (zero the counters), the producer
+// kernel (scatter_put* / scatter_empty: each record counts its key as it is
+// written), then scatter_reduce (scan, placement, sums) or scatter_group
+// (scan, placement only) and a caller-specific reduction.
 #pragma once
 #include "common.h"
 
@@ -21,7 +26,8 @@ struct alignas(64) ScatterRec {
   double a[3];
   double cb[3];
   int32_t cls;
-  int32_t pad[3];
+  int32_t sub;  // sub-triangle of the record's target block
+  int32_t pad[2];
 };
 
 // a record that carries its 18 sums (NC = 3) directly: partial sums formed
@@ -29,17 +35,24 @@ struct alignas(64) ScatterRec {
 struct alignas(16) ScatterRecV {
   double v[18];
   int32_t cls;
-  int32_t pad[3];
+  int32_t sub;
+  int32_t pad[2];
 };
 constexpr int64_t kScatterRecBytes = sizeof(ScatterRecV);  // slot size (both kinds)
 
 struct ScatterWork {
   uint32_t* keys = nullptr;    // (n) written by the producer; `empty` = unused slot
-  uint32_t* idx = nullptr;     // (n) slot index (written by the producer)
-  uint32_t* keys_s = nullptr;  // grouped by key (empty slots last)
-  uint32_t* idx_s = nullptr;
-  uint32_t* count = nullptr;   // (empty + 1) records per key, then placement counters
-  uint32_t* offs = nullptr;    // (empty + 1) first grouped position of each key
+  uint32_t* keys_s = nullptr;  // grouped by key: positions [0, offs[empty]) hold the records
+  uint32_t* idx_s = nullptr;   // slot of the record at each grouped position
+  uint32_t* count = nullptr;   // (empty + 1) records per key (counted by the producer), then
+                               // placement counters, left at zero
+  uint32_t* offs = nullptr;    // (empty + 1) first grouped position of each key; offs[empty]
+                               // = the number of records
+  uint32_t* keys_u = nullptr;  // (n) with key lists: the keys with 1 .. split records, in
+                               // increasing order, meta[0] of them
+  uint32_t* keys_l = nullptr;  // (n) with key lists: the keys with more than split records,
+                               // in increasing order, meta[1] of them
+  uint32_t* meta = nullptr;    // (2) list lengths
   ScatterRec* rec = nullptr;   // (n) slots of kScatterRecBytes
   ScatterRecV* recv = nullptr; // the same slots as value records
   void* temp = nullptr;        // scan temporary storage
@@ -67,12 +80,23 @@ int64_t scatter_bytes(int64_t n, int64_t key_range);
 // carve a caller workspace for n records with keys < key_range; fails if it
 // is too small
 int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range, ScatterWork& w);
+// zero the per-key counters (before the producer kernel)
+int scatter_begin(const ScatterWork& w, hipStream_t st);
+// group the n records by key (after the producer kernel); with split > 0,
+// also list the keys with records: keys_u (at most split records) and
+// keys_l (more)
+int scatter_group(const ScatterWork& w, int64_t n, hipStream_t st, uint32_t split = 0);
 // group the n records by key and add every group into its target
 int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hipStream_t st);
 
-__device__ inline void scatter_put(const ScatterWork& w, int64_t slot, uint32_t key, d3 a, d3 cb, int cls) {
+__device__ inline void scatter_count(const ScatterWork& w, uint32_t key) {
+  __hip_atomic_fetch_add(w.count + key, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void scatter_put(const ScatterWork& w, int64_t slot, uint32_t key, d3 a, d3 cb, int cls,
+                                   int sub = 0) {
   w.keys[slot] = key;
-  w.idx[slot] = (uint32_t)slot;
+  scatter_count(w, key);
   ScatterRec r;
   r.a[0] = a.x;
   r.a[1] = a.y;
@@ -81,25 +105,24 @@ __device__ inline void scatter_put(const ScatterWork& w, int64_t slot, uint32_t 
   r.cb[1] = cb.y;
   r.cb[2] = cb.z;
   r.cls = cls;
-  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  r.sub = sub;
+  r.pad[0] = r.pad[1] = 0;
   w.rec[slot] = r;
 }
 
 __device__ inline void scatter_put_values(const ScatterWork& w, int64_t slot, uint32_t key, const double v[18],
-                                          int cls) {
+                                          int cls, int sub = 0) {
   w.keys[slot] = key;
-  w.idx[slot] = (uint32_t)slot;
+  scatter_count(w, key);
   ScatterRecV r;
 #pragma unroll
   for (int e = 0; e < 18; ++e) r.v[e] = v[e];
   r.cls = cls;
-  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  r.sub = sub;
+  r.pad[0] = r.pad[1] = 0;
   w.recv[slot] = r;
 }
 
-__device__ inline void scatter_empty(const ScatterWork& w, int64_t slot) {
-  w.keys[slot] = w.empty;
-  w.idx[slot] = (uint32_t)slot;
-}
+__device__ inline void scatter_empty(const ScatterWork& w, int64_t slot) { w.keys[slot] = w.empty; }
 
 }  // namespace cmm

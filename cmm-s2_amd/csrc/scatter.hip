@@ -10,11 +10,38 @@ namespace cmm {
 namespace {
 inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
+// key k has 1 .. split records / more than split records
+struct ShortGroup {
+  const uint32_t* offs;
+  uint32_t split;
+  __host__ __device__ bool operator()(const uint32_t& k) const {
+    const uint32_t d = offs[k + 1] - offs[k];
+    return d != 0u && d <= split;
+  }
+};
+struct LongGroup {
+  const uint32_t* offs;
+  uint32_t split;
+  __host__ __device__ bool operator()(const uint32_t& k) const { return offs[k + 1] - offs[k] > split; }
+};
+
+hipError_t list_keys(void* temp, size_t& bytes, const uint32_t* offs, uint32_t split, uint32_t* keys_u,
+                     uint32_t* keys_l, uint32_t* meta, int nkeys, hipStream_t st) {
+  return hipcub::DevicePartition::If(temp, bytes, hipcub::CountingInputIterator<uint32_t>(0), keys_u, keys_l,
+                                     hipcub::DiscardOutputIterator<>(), meta, nkeys, ShortGroup{offs, split},
+                                     LongGroup{offs, split}, st);
+}
+
+// temporary storage of the scan over the bins and of the key lists (the
+// larger of the two)
 size_t scan_temp_bytes(int64_t bins) {
-  size_t bytes = 0;
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+  size_t scan = 0, sel = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                                   (int)bins, (hipStream_t)0);
-  return e == hipSuccess ? bytes : 0;
+  if (e != hipSuccess) return 0;
+  e = list_keys(nullptr, sel, nullptr, 1, nullptr, nullptr, nullptr, (int)(bins - 1), (hipStream_t)0);
+  if (e != hipSuccess) return 0;
+  return std::max(scan, sel);
 }
 }  // namespace
 
@@ -23,7 +50,7 @@ int64_t scatter_bytes(int64_t n, int64_t key_range) {
   const int64_t bins = key_range + 1;
   const size_t tb = scan_temp_bytes(bins);
   if (!tb) return 0;
-  return 4 * align256(4 * n) + 2 * align256(4 * bins) + align256(kScatterRecBytes * n) + align256((int64_t)tb);
+  return 5 * align256(4 * n) + 2 * align256(4 * bins) + 256 + align256(kScatterRecBytes * n) + align256((int64_t)tb);
 }
 
 int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range, ScatterWork& w) {
@@ -42,11 +69,13 @@ int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range,
   const int64_t bins = key_range + 1;
   w.n = n;
   w.keys = reinterpret_cast<uint32_t*>(take(4 * n));
-  w.idx = reinterpret_cast<uint32_t*>(take(4 * n));
   w.keys_s = reinterpret_cast<uint32_t*>(take(4 * n));
   w.idx_s = reinterpret_cast<uint32_t*>(take(4 * n));
   w.count = reinterpret_cast<uint32_t*>(take(4 * bins));
   w.offs = reinterpret_cast<uint32_t*>(take(4 * bins));
+  w.keys_u = reinterpret_cast<uint32_t*>(take(4 * n));
+  w.keys_l = reinterpret_cast<uint32_t*>(take(4 * n));
+  w.meta = reinterpret_cast<uint32_t*>(take(8));
   char* recs = take(kScatterRecBytes * n);
   w.rec = reinterpret_cast<ScatterRec*>(recs);
   w.recv = reinterpret_cast<ScatterRecV*>(recs);
@@ -56,28 +85,16 @@ int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range,
   return CMM_OK;
 }
 
-// zero the per-key counters, mark every grouped position empty
-__global__ __launch_bounds__(256) void k_scatter_init(uint32_t* __restrict__ count, int64_t bins,
-                                                      uint32_t* __restrict__ keys_s, int64_t n, uint32_t empty) {
+// zero the per-key counters (the producer counts its records into them)
+__global__ __launch_bounds__(256) void k_scatter_init(uint32_t* __restrict__ count, int64_t bins) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < bins) count[i] = 0u;
-  if (i < n) keys_s[i] = empty;
 }
 
-// records per key; empty slots (and keys above the empty key, none are
-// written) are not counted: they stay at the end, marked by k_scatter_init
-__global__ __launch_bounds__(256) void k_scatter_count(const uint32_t* __restrict__ keys, int64_t n, uint32_t empty,
-                                                       uint32_t* __restrict__ count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = keys[i];
-  if (k < empty) __hip_atomic_fetch_add(count + k, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// record i goes to the next free position of its key's range (the counters
-// hold the group sizes on entry and are left at zero)
-__global__ __launch_bounds__(256) void k_scatter_place(const uint32_t* __restrict__ keys,
-                                                       const uint32_t* __restrict__ idx, int64_t n, uint32_t empty,
+// slot i goes to the next free position of its key's range (the counters
+// hold the group sizes on entry and are left at zero); empty slots are not
+// placed: positions from offs[empty] on are never read
+__global__ __launch_bounds__(256) void k_scatter_place(const uint32_t* __restrict__ keys, int64_t n, uint32_t empty,
                                                        const uint32_t* __restrict__ offs,
                                                        uint32_t* __restrict__ count, uint32_t* __restrict__ keys_s,
                                                        uint32_t* __restrict__ idx_s) {
@@ -88,7 +105,7 @@ __global__ __launch_bounds__(256) void k_scatter_place(const uint32_t* __restric
   const uint32_t pos = offs[k] + __hip_atomic_fetch_add(count + k, 0xffffffffu, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT) - 1u;
   keys_s[pos] = k;
-  idx_s[pos] = idx[i];
+  idx_s[pos] = (uint32_t)i;
 }
 
 // Records per reducing thread: a group (records of one key) is cut into
@@ -120,9 +137,9 @@ __device__ inline void add_block(double* base, const double* acc, bool atomic) {
 template <int NC, int MODE>
 __global__ __launch_bounds__(256) void k_scatter_reduce(ScatterWork w, int64_t n, ScatterTargets t) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  const int64_t total = w.offs[w.empty];  // records placed
+  if (j >= total) return;
   const uint32_t key = w.keys_s[j];
-  if (key >= w.empty) return;
   const bool starts_group = j == 0 || w.keys_s[j - 1] != key;
   if (!starts_group && j % kChunk != 0) return;  // inside another thread's chunk
   constexpr int NO = MODE == 0 ? 1 : 3;
@@ -133,7 +150,7 @@ __global__ __launch_bounds__(256) void k_scatter_reduce(ScatterWork w, int64_t n
     for (int e = 0; e < NC * 6; ++e) acc[o][e] = 0.0;
   const int64_t stop = (j / kChunk + 1) * kChunk;  // next chunk boundary
   int64_t r = j;
-  for (; r < n && r < stop && w.keys_s[r] == key; ++r) {
+  for (; r < total && r < stop && w.keys_s[r] == key; ++r) {
     if (MODE == 2) {
       const ScatterRecV& rv = w.recv[w.idx_s[r]];
       const int cl = rv.cls;
@@ -171,7 +188,7 @@ __global__ __launch_bounds__(256) void k_scatter_reduce(ScatterWork w, int64_t n
       }
     }
   }
-  const bool ends_group = r >= n || w.keys_s[r] != key;
+  const bool ends_group = r >= total || w.keys_s[r] != key;
   const bool atomic = !(starts_group && ends_group);
   if (MODE == 0) {
     const int cl = (int)(key / (uint64_t)t.span);
@@ -184,21 +201,36 @@ __global__ __launch_bounds__(256) void k_scatter_reduce(ScatterWork w, int64_t n
   }
 }
 
-int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hipStream_t st) {
+int scatter_begin(const ScatterWork& w, hipStream_t st) {
+  const int64_t bins = (int64_t)w.empty + 1;
+  hipLaunchKernelGGL(k_scatter_init, dim3(grid_for(bins, 256)), dim3(256), 0, st, w.count, bins);
+  CMM_LAUNCH_CHECK("scatter: init");
+  return CMM_OK;
+}
+
+int scatter_group(const ScatterWork& w, int64_t n, hipStream_t st, uint32_t split) {
   if (n < 1) return CMM_OK;
   if (n > w.n) return fail(CMM_E_ARG, "scatter: more records than the workspace holds");
   const int64_t bins = (int64_t)w.empty + 1;
-  const int grid = grid_for(n, 256);
-  hipLaunchKernelGGL(k_scatter_init, dim3(grid_for(std::max(bins, n), 256)), dim3(256), 0, st, w.count, bins,
-                     w.keys_s, n, w.empty);
-  hipLaunchKernelGGL(k_scatter_count, dim3(grid), dim3(256), 0, st, w.keys, n, w.empty, w.count);
-  CMM_LAUNCH_CHECK("scatter: count");
   size_t tb = w.temp_bytes;
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(w.temp, tb, w.count, w.offs, (int)bins, st);
   if (e != hipSuccess) return hip_check(e, "scatter: scan");
-  hipLaunchKernelGGL(k_scatter_place, dim3(grid), dim3(256), 0, st, w.keys, w.idx, n, w.empty, w.offs, w.count,
+  if (split) {
+    tb = w.temp_bytes;
+    e = list_keys(w.temp, tb, w.offs, split, w.keys_u, w.keys_l, w.meta, (int)w.empty, st);
+    if (e != hipSuccess) return hip_check(e, "scatter: key lists");
+  }
+  hipLaunchKernelGGL(k_scatter_place, dim3(grid_for(n, 256)), dim3(256), 0, st, w.keys, n, w.empty, w.offs, w.count,
                      w.keys_s, w.idx_s);
   CMM_LAUNCH_CHECK("scatter: place");
+  return CMM_OK;
+}
+
+int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hipStream_t st) {
+  if (n < 1) return CMM_OK;
+  int rc = scatter_group(w, n, st);
+  if (rc) return rc;
+  const int grid = grid_for(n, 256);
   if (t.mode == 0) {
     if (t.nc == 3)
       hipLaunchKernelGGL((k_scatter_reduce<3, 0>), dim3(grid), dim3(256), 0, st, w, n, t);

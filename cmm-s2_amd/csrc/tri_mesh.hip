@@ -1119,6 +1119,10 @@ extern "C" int cmm_tri_compose_adjoint(const cmm_tri_mesh* m, const double* cons
     if (rc) return rc;
   }
   hipStream_t st = as_stream(stream);
+  if (work) {
+    int rc = scatter_begin(sw, st);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_tri_compose_adj, dim3(grid_for(n, 256)), dim3(256), 0, st, *m, lv, lb, depth, q, n, sp, sc,
                      out_bar, q_bar, sw, work ? 1 : 0);
   CMM_LAUNCH_CHECK("cmm_tri_compose_adjoint");
@@ -1165,6 +1169,10 @@ extern "C" int cmm_tri_stencil_jets_adjoint(const cmm_tri_mesh* m, const double*
     if (rc) return rc;
   }
   hipStream_t st = as_stream(stream);
+  if (rec) {
+    int rc = scatter_begin(sw, st);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_tri_stencil_jets_adj, dim3(grid_for(m->nv, 128)), dim3(128), 0, st, *m, cur, yv, yrow, sf,
                      srow, tanv, eps, jets_bar, sf_bar, sbrow, cur ? cur_bar : nullptr, sw, rec ? 1 : 0);
   CMM_LAUNCH_CHECK("cmm_tri_stencil_jets_adjoint");

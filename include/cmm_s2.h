@@ -345,6 +345,24 @@ int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integrator, double d
                                int64_t st_row, int64_t nv, const double* yst_bar, int64_t yst_bar_row,
                                double* const* level_bars, double* st_bar, int64_t st_bar_row, void* work,
                                int64_t work_bytes, void* stream);
+/* cmm_advect_stencil_adjoint with the velocity-level adjoints kept sparse,
+ * as corner jets: level_corners[k] (Tu, 3, 12) holds, for every triangle t
+ * with level_touched[k][t] != 0, the adjoints of the 12 jets rows
+ * (cmm_mwss_velocity_jets order) at its 3 corners; triangles with a zero flag
+ * hold no adjoint and are not read.  Adds into the levels (a level's first
+ * touch of a triangle stores and sets its flag; zero the flags of a new
+ * level).  Equals cmm_advect_stencil_adjoint followed by the per-triangle
+ * pass of cmm_mwss_assemble_adjoint, without the (Tu, 108) packed adjoints.
+ * work (required): cmm_scatter_workspace_bytes(16 nv, Tu) bytes. */
+int cmm_advect_stencil_adjoint_corners(const cmm_velocity* vel, int integrator, double dt, const double* st,
+                                       int64_t st_row, int64_t nv, const double* yst_bar, int64_t yst_bar_row,
+                                       double* const* level_corners, uint8_t* const* level_touched,
+                                       double* st_bar, int64_t st_bar_row, void* work, int64_t work_bytes,
+                                       void* stream);
+/* second pass of the transposed velocity assembly from the sparse corner jets
+ * of cmm_advect_stencil_adjoint_corners: vals_bar (12, Nu) accumulates. */
+int cmm_mwss_corner_jets_adjoint(const cmm_mwss_mesh* mesh, const double* corners, const uint8_t* touched,
+                                 double* vals_bar, void* stream);
 /* adjoint of cmm_tri_stencil_jets: jets_bar (3,4,Nv) -> stencil-feet
  * adjoints sf_bar (3, 4 Nv) and, when cur != NULL, the current map's packed
  * adjoints cur_bar (accumulates). */

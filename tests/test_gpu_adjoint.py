@@ -248,18 +248,22 @@ def _spline_velocity(L, seed):
     return mu, lv, vals
 
 
-@pytest.mark.parametrize("scatter", ["atomics", "grouped"])
+@pytest.mark.parametrize("scatter", ["atomics", "grouped", "corners", "corners_dense"])
 @pytest.mark.parametrize("integ", ["euler", "improved_euler", "rk4"])
 def test_advect_stencil_adjoint_fd(integ, scatter):
+    """"corners": the sparse corner-jet form, also against the packed
+    level adjoints of the grouped form through the assembly transpose;
+    "corners_dense": the same with ico-5 feet on an L = 16 velocity mesh
+    (~40 records per triangle: groups of both reduction kernels)."""
     from cmm_s2 import _native as N
     from cmm_s2.core._backend import stream
     from cmm_s2.core.evolution_functions import _STAGES
     from cmm_s2.core.spherical_spline import lagrange_weights
     kind = {"euler": N.INT_EULER, "improved_euler": N.INT_IMPROVED_EULER, "rk4": N.INT_RK4}[integ]
     nvs = {"euler": 1, "improved_euler": 2, "rk4": 3}[integ]
-    L = 32
+    L, level = (16, 5) if scatter == "corners_dense" else (32, 3)
     mu, lv, vals = _spline_velocity(L, 70)
-    mesh = _ico(3)
+    mesh = _ico(level)
     nv = mesh.nv
     dt, t = 0.05, 0.1
     xs = mesh.d_spts
@@ -286,17 +290,59 @@ def test_advect_stencil_adjoint_fd(integ, scatter):
                ys.data_ptr(), ys.stride(0), stream())
         return _dot(ys, yb)
 
-    lb = [torch.zeros_like(p) for p in lv]
-    xb = torch.empty(3, 4 * nv, dtype=torch.float64, device="cuda")
-    wp, wsz = _work(None if scatter == "atomics" else 16 * nv, 6 * mu.ntri)
-    N.call("cmm_advect_stencil_adjoint", desc(lv), kind, dt, xs.data_ptr(), xs.stride(0), nv, yb.data_ptr(),
-           yb.stride(0), N.ptr_array([p.data_ptr() for p in lb]), xb.data_ptr(), xb.stride(0), wp, wsz, stream())
+    def packed_form(sc):
+        lb = [torch.zeros_like(p) for p in lv]
+        xb = torch.empty(3, 4 * nv, dtype=torch.float64, device="cuda")
+        wp, wsz = _work(None if sc == "atomics" else 16 * nv, 6 * mu.ntri)
+        N.call("cmm_advect_stencil_adjoint", desc(lv), kind, dt, xs.data_ptr(), xs.stride(0), nv, yb.data_ptr(),
+               yb.stride(0), N.ptr_array([p.data_ptr() for p in lb]), xb.data_ptr(), xb.stride(0), wp, wsz,
+               stream())
+        vbs = []
+        for k in range(nvs):
+            vb = torch.zeros_like(vals[k])
+            N.call("cmm_mwss_assemble_adjoint", mu._h, lb[k].data_ptr(), 3, vb.data_ptr(), None, 0, stream())
+            vbs.append(vb)
+        return vbs, xb
+
+    def corner_form(calls):
+        # the levels start NaN-filled with clear flags; `calls` adjoint calls
+        # accumulate into them
+        lc = [torch.full((mu.ntri, 36), float("nan"), dtype=torch.float64, device="cuda") for _ in range(nvs)]
+        lt = [torch.zeros(mu.ntri, dtype=torch.uint8, device="cuda") for _ in range(nvs)]
+        xb = torch.empty(3, 4 * nv, dtype=torch.float64, device="cuda")
+        wp, wsz = _work(16 * nv, mu.ntri)
+        for _ in range(calls):
+            N.call("cmm_advect_stencil_adjoint_corners", desc(lv), kind, dt, xs.data_ptr(), xs.stride(0), nv,
+                   yb.data_ptr(), yb.stride(0), N.ptr_array([c.data_ptr() for c in lc]),
+                   N.ptr_array([f.data_ptr() for f in lt]), xb.data_ptr(), xb.stride(0), wp, wsz, stream())
+        vbs = []
+        for k in range(nvs):
+            vb = torch.zeros_like(vals[k])
+            N.call("cmm_mwss_corner_jets_adjoint", mu._h, lc[k].data_ptr(), lt[k].data_ptr(), vb.data_ptr(),
+                   stream())
+            vbs.append(vb)
+        return vbs, xb
+
+    if scatter.startswith("corners"):
+        vbs, xb = corner_form(1)
+        ref, xref = packed_form("grouped")
+        assert torch.equal(xb, xref)
+        for a, b in zip(vbs, ref):
+            assert bool(torch.isfinite(a).all())
+            assert float((a - b).abs().max()) <= 1e-12 * float(b.abs().max())
+        twice, _ = corner_form(2)  # accumulation: two calls give twice one
+        for a, b in zip(twice, vbs):
+            assert float((a - 2 * b).abs().max()) <= 1e-12 * float(b.abs().max())
+        if scatter == "corners_dense":
+            # 160k feet on a 1024-triangle mesh cross the C1 seams of the
+            # spline within the difference step: the central difference
+            # carries an O(h) error there, the form is checked against the
+            # packed one above
+            return
+    else:
+        vbs, xb = packed_form(scatter)
     dv = [_rand(12, mu.nv, seed=80 + k) for k in range(3)]
-    ad = 0.0
-    for k in range(nvs):
-        vb = torch.zeros_like(vals[k])
-        N.call("cmm_mwss_assemble_adjoint", mu._h, lb[k].data_ptr(), 3, vb.data_ptr(), None, 0, stream())
-        ad += _dot(vb, dv[k])
+    ad = sum(_dot(vbs[k], dv[k]) for k in range(nvs))
 
     def perturbed(h):
         out = []
