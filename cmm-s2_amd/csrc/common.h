@@ -241,7 +241,8 @@ __host__ __device__ __forceinline__ void ps_unpack(const double* __restrict__ p,
 
 // Write the packed evaluation layout (Ntri, 6, NC, 6) of a block of up to
 // 256 triangles through LDS in rounds of 64 triangles so that the global
-// stores are contiguous.  cf[c][k] are this thread's coefficients.
+// stores are contiguous.  cf[c][k] are this thread's coefficients; lds and
+// packed are 16-byte aligned.
 template <int NC>
 __device__ inline void store_packed_block(const double (&cf)[NC][19], int64_t t0, int64_t ntri,
                                           double* __restrict__ packed, double* lds) {
@@ -261,9 +262,13 @@ __device__ inline void store_packed_block(const double (&cf)[NC][19], int64_t t0
     __syncthreads();
     const int64_t tr = t0 + 64 * r;
     if (tr < ntri) {
+      // 16-byte non-temporal stores: the block is written once and read by
+      // a later kernel, keeping it out of the caches costs nothing
+      typedef double dbl2 __attribute__((ext_vector_type(2)));
       const int nvalid = (int)((ntri - tr) < 64 ? (ntri - tr) : 64);
-      double* out = packed + tr * PER;
-      for (int e = tid; e < nvalid * PER; e += blockDim.x) out[e] = lds[e];
+      dbl2* out = reinterpret_cast<dbl2*>(packed + tr * PER);
+      const dbl2* src = reinterpret_cast<const dbl2*>(lds);
+      for (int e = tid; e < nvalid * (PER / 2); e += blockDim.x) __builtin_nontemporal_store(src[e], out + e);
     }
     __syncthreads();
   }

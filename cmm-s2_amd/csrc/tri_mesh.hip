@@ -583,7 +583,7 @@ __global__ __launch_bounds__(256) void k_tri_compose_grad(cmm_tri_mesh M, LevelP
 template <int NC>
 __global__ __launch_bounds__(256) void k_tri_assemble(cmm_tri_mesh M, const double* __restrict__ vals,
                                                       double* __restrict__ coeffs, double* __restrict__ packed) {
-  __shared__ double lds[64 * 6 * NC * 6];
+  __shared__ __align__(16) double lds[64 * 6 * NC * 6];
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x;
   const int64_t t = t0 + threadIdx.x;
   double cf[NC][19];
