@@ -80,7 +80,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the composition and the "
-              "CMM step shard by points / vertices with no collectives (see DESIGN.md)", file=sys.stderr)
+              "CMM step shard by points / vertices with no collectives", file=sys.stderr)
         sys.exit(2)
     if a.ensemble:
         return ensemble_bench(a)
