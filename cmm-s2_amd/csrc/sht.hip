@@ -17,6 +17,7 @@
 // Spectral layout [lm][f] complex with lm = m L - m(m-1)/2 + (l - m).
 #include <algorithm>
 #include <cmath>
+#include <queue>
 #include <vector>
 
 #include "common.h"
@@ -67,10 +68,16 @@ struct cmm_sht_plan {
   double* msc_r = nullptr;   // (L / pi) * (1 for m = 0, else 2)  (L)
   double* K2 = nullptr;      // (2, jpad, tcols) K transposed for the adjoint resampling: K2[p][j][t] = K[p][t][j]
   int jpad = 0, tcols = 0;
+  // synthesis schedule: wave w of k_leg_inverse runs the (order, ring block)
+  // tasks inv_tasks[inv_off[w] .. inv_off[w + 1])
+  int inv_waves = 0;
+  int2* inv_tasks = nullptr;
+  int* inv_off = nullptr;
   std::vector<double> hx_g, hw_g;
 };
 
 static int build_restart(cmm_sht_plan* p, const std::vector<int2>& units, const std::vector<int>& unit0);
+static int build_inverse_tasks(cmm_sht_plan* p);
 
 // ---------------------------------------------------------------------------
 // host setup
@@ -147,7 +154,7 @@ extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
   (void)hipFree(p->K); (void)hipFree(p->tw); (void)hipFree(p->dinv); (void)hipFree(p->rec);
   (void)hipFree(p->units); (void)hipFree(p->unit0); (void)hipFree(p->st_p); (void)hipFree(p->st_s);
   (void)hipFree(p->st_pr); (void)hipFree(p->st_sr); (void)hipFree(p->w_r); (void)hipFree(p->msc_r);
-  (void)hipFree(p->K2);
+  (void)hipFree(p->K2); (void)hipFree(p->inv_tasks); (void)hipFree(p->inv_off);
   delete p;
   return CMM_OK;
 }
@@ -240,7 +247,7 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
       for (int j = 0; j < L; ++j)
         K2[((size_t)par * p->jpad + j) * p->tcols + t] = K[(size_t)par * L * tpad + (size_t)t * L + j];
   if ((rc = upload(&p->w_r, ones.data(), ones.size())) || (rc = upload(&p->msc_r, msc.data(), msc.size())) ||
-      (rc = upload(&p->K2, K2.data(), K2.size()))) {
+      (rc = upload(&p->K2, K2.data(), K2.size())) || (rc = build_inverse_tasks(p))) {
     cmm_sht_plan_destroy(p);
     return rc;
   }
@@ -511,18 +518,21 @@ typedef float fx4 __attribute__((ext_vector_type(4)));
 constexpr int kMixedFlush = 8;
 
 template <int NT, bool MIXED>
-__global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, int nfp, int nrb,
-                                                       const double* __restrict__ x_t, const double* __restrict__ dP,
-                                                       const int* __restrict__ dS, const double2* __restrict__ rec,
+__global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, int nfp, const int2* __restrict__ tasks,
+                                                       const int* __restrict__ toff, const double* __restrict__ x_t,
+                                                       const double* __restrict__ dP, const int* __restrict__ dS,
+                                                       const double2* __restrict__ rec,
                                                        const double* __restrict__ flm, double* __restrict__ F) {
   __shared__ double Pt[kInvDeg][kInvRings + 2];
   extern __shared__ double2 rsh[];  // recurrence pairs of this order, degrees m .. L-1 (+ padding)
-  const int m = (int)blockIdx.x / nrb, rb = (int)blockIdx.x % nrb;
+  const int k1 = toff[blockIdx.x + 1];
+  for (int k = toff[blockIdx.x]; k < k1; ++k) {
+  __syncthreads();  // the previous task's reads of rsh and Pt are done
+  const int m = tasks[k].x, rb = tasks[k].y;
   const int lane = threadIdx.x;
   const int nh = L / 2 + 1;
   const int tfirst = m == 0 ? 0 : 1;
   const int nr = nh - tfirst;
-  if (rb * kInvRings >= nr) return;
   if (m > 0 && rb == 0 && lane < nfp) {
     const double2 z = {0.0, 0.0};
     double2* F2 = reinterpret_cast<double2*>(F);
@@ -663,6 +673,58 @@ __global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, in
         if (L - tr != tr) F[(((int64_t)(f0 + f) * (L + 1) + (L - tr)) * L + m) * 2 + (col & 1)] = ev - od;
       }
     }
+  }  // task
+}
+
+// Static schedule of k_leg_inverse: the (order, 64-ring block) tasks,
+// longest first, each to the least loaded of W waves (load: the task's
+// chunks of 8 degrees, padded to pairs, plus two for the staging and the
+// stores), W = the blocks that are resident at once (at most 2 per SIMD,
+// fewer where the staged recurrence pairs fill the LDS).  Orders have L - m
+// degrees: with one wave per task, SIMDs that start with two long orders
+// carry twice the mean load (12-field synthesis at L=512: 90 -> 87 us; the
+// matrix cores stay busy ~35% of the kernel, the rest is the recurrence and
+// LDS latency of each wave's chunk chain).
+static int build_inverse_tasks(cmm_sht_plan* p) {
+  const int L = p->L, nh = L / 2 + 1;
+  std::vector<std::pair<int, int2>> tasks;  // (load, (m, rb))
+  for (int m = 0; m < L; ++m) {
+    const int nr = nh - (m == 0 ? 0 : 1);
+    const int nch = (L - m + kInvDeg - 1) / kInvDeg;
+    const int load = 2 * ((nch + 1) / 2) + 2;
+    for (int rb = 0; rb * kInvRings < nr; ++rb) tasks.push_back({load, int2{m, rb}});
+  }
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+    ncu = 256;
+  const size_t lds = (size_t)((L + kInvDeg - 1) / kInvDeg + 2) * kInvDeg * sizeof(double2) +
+                     sizeof(double) * kInvDeg * (kInvRings + 2);
+  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / lds));
+  const int W = std::max(1, std::min<int>(ncu * per_cu, (int)tasks.size()));
+  std::stable_sort(tasks.begin(), tasks.end(), [](const std::pair<int, int2>& a, const std::pair<int, int2>& b) {
+    return a.first > b.first;
+  });
+  typedef std::pair<int64_t, int> Slot;  // (load, wave)
+  std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+  for (int w = 0; w < W; ++w) heap.push(Slot{0, w});
+  std::vector<std::vector<int2>> per(W);
+  for (const auto& t : tasks) {
+    const Slot sl = heap.top();
+    heap.pop();
+    per[sl.second].push_back(t.second);
+    heap.push(Slot{sl.first + t.first, sl.second});
+  }
+  std::vector<int2> flat;
+  std::vector<int> off(W + 1, 0);
+  for (int w = 0; w < W; ++w) {
+    off[w + 1] = off[w] + (int)per[w].size();
+    flat.insert(flat.end(), per[w].begin(), per[w].end());
+  }
+  p->inv_waves = W;
+  int rc = upload(&p->inv_tasks, flat.data(), flat.size());
+  if (!rc) rc = upload(&p->inv_off, off.data(), off.size());
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1592,9 +1654,7 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   const int L = p->L;
   double2* F = reinterpret_cast<double2*>(work);
   hipStream_t st = as_stream(stream);
-  const int nh = L / 2 + 1;
-  const int nrb = (nh + kInvRings - 1) / kInvRings;
-  const dim3 grid((unsigned)(L * nrb)), block(64);
+  const dim3 grid((unsigned)p->inv_waves), block(64);
   // staged recurrence pairs: at most ceil(L / 8) + 2 chunks of 8 degrees
   const size_t lds_rec = (size_t)((L + kInvDeg - 1) / kInvDeg + 2) * kInvDeg * sizeof(double2);
   if (p->legendre >= CMM_LEGENDRE_BF16X3) {
@@ -1610,8 +1670,8 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
       const int nfp = std::min(16, nf - f0);
       auto kern = p->legendre == CMM_LEGENDRE_FP32 ? (nfp > 8 ? k_leg_inverse<2, true> : k_leg_inverse<1, true>)
                                                    : (nfp > 8 ? k_leg_inverse<2, false> : k_leg_inverse<1, false>);
-      hipLaunchKernelGGL(kern, grid, block, lds_rec, st, L, nf, f0, nfp, nrb, p->x_t, p->dP_t, p->dS_t, p->rec, flm,
-                         reinterpret_cast<double*>(F));
+      hipLaunchKernelGGL(kern, grid, block, lds_rec, st, L, nf, f0, nfp, p->inv_tasks, p->inv_off, p->x_t, p->dP_t,
+                         p->dS_t, p->rec, flm, reinterpret_cast<double*>(F));
       CMM_LAUNCH_CHECK("cmm_sht_inverse: legendre");
       f0 += nfp;
     }
