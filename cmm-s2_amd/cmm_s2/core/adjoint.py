@@ -403,6 +403,7 @@ class MisfitProblem(object):
         final_maps = {k: self.maps[k] for k in self.submaps}
         self.lam_M, self.lam_U = {}, {}
         self.lam_w = B.zeros(self.w0.packed.shape[0], self.w0.packed.shape[1])
+        self._wdefer = []
         tape = []
         self._objective(tape)
         self._reverse(tape)
@@ -433,11 +434,13 @@ class MisfitProblem(object):
             self.maps = dict(final_maps)
             self.maps.update(maps)
             self._reverse(tape)
+            self._flush_w()
             del seg, tape, U, maps
             if ready is not None:
                 main.wait_event(ready)
             elif k > 0 and nxt is None:
                 nxt = self._record(k - 1, final_maps)
+        self._flush_w()
         g = self._omega0_adjoint()
         torch.cuda.synchronize()
         self.timings["reverse_s"] = time.perf_counter() - t0
@@ -469,10 +472,13 @@ class MisfitProblem(object):
     def _sample_and_compose_adj(self, chain, X, wbar, rot):
         n = self.n
         xbar = B.empty(n, 3) if chain else None
-        # atomics (wave-summed at the pole rings): cheaper than a sort here
-        N.call("cmm_sample_vorticity_adjoint", self.mesh_w._h, self.w0.packed.data_ptr(), X.data_ptr(), n,
-               X.stride(0), X.stride(1), wbar.data_ptr(), float(rot), B.ptr(xbar), 3, 1, self.lam_w.data_ptr(),
-               None, 0, B.stream())
+        # the position adjoint now; the initial-vorticity coefficient adjoint
+        # of (X, wbar) later, grouped with the other steps' (_flush_w)
+        if chain:
+            N.call("cmm_sample_vorticity_adjoint", self.mesh_w._h, self.w0.packed.data_ptr(), X.data_ptr(), n,
+                   X.stride(0), X.stride(1), wbar.data_ptr(), float(rot), xbar.data_ptr(), 3, 1, None, None, 0,
+                   B.stream())
+        self._wdefer.append((X, wbar))
         if chain:
             lv = N.ptr_array([self.maps[i].data_ptr() for i in chain])
             lb = N.ptr_array([self._lam_map(i).data_ptr() for i in chain])
@@ -480,6 +486,21 @@ class MisfitProblem(object):
             wp, wb = self._scatter(len(chain) * n, 6 * len(chain) * self.mesh.ntri)
             N.call("cmm_tri_compose_adjoint", self.mesh._h, lv, lb, len(chain), q.data_ptr(), n, q.stride(0),
                    q.stride(1), xbar.data_ptr(), None, wp, wb, B.stream())
+
+    def _flush_w(self):
+        """coefficient adjoint of the deferred vorticity samples: one grouped
+        scatter over all their points (the per-step alternative, FP64
+        atomics, cost ~50 us a step at L=256; a grouped scatter per step
+        would scan the 6 Tu_w bins of the vorticity mesh every step)"""
+        if not self._wdefer:
+            return
+        X = torch.cat([x.reshape(-1, 3) for x, _ in self._wdefer], dim=0)
+        w = torch.cat([b.reshape(-1) for _, b in self._wdefer])
+        self._wdefer = []
+        n = X.shape[0]
+        wp, wb = self._scatter(n, 6 * self.mesh_w.ntri)
+        N.call("cmm_sample_vorticity_adjoint", self.mesh_w._h, self.w0.packed.data_ptr(), X.data_ptr(), n,
+               X.stride(0), X.stride(1), w.data_ptr(), 0.0, None, 3, 1, self.lam_w.data_ptr(), wp, wb, B.stream())
 
     def _reverse(self, tape):
         plan, L = self.plan, self.L
@@ -526,10 +547,12 @@ class MisfitProblem(object):
                 sb = B.empty(3, 4 * nv)
                 cur = None if src is None else self.maps[src].data_ptr()
                 curb = None if src is None else self._lam_map(src).data_ptr()
-                # atomics: the four feet of a vertex are summed per thread first
+                # one value record per vertex (its four feet share the block),
+                # grouped: no FP64 atomics
+                wp, wb = self._scatter(nv, 6 * m.ntri) if cur is not None else (None, 0)
                 N.call("cmm_tri_stencil_jets_adjoint", m._h, cur, yv.data_ptr(), yv.stride(0), ys.data_ptr(),
                        ys.stride(0), m.d_tan.data_ptr(), self.EPS, jb.data_ptr(), sb.data_ptr(), sb.stride(0),
-                       curb, None, 0, B.stream())
+                       curb, wp, wb, B.stream())
                 desc = self._desc(V, ikind, t)
                 lams = [self._lam_vel(V.ids[k]) for k in range(min(V.nVs, 3))]
                 lcs = N.ptr_array([c.data_ptr() for c, _ in lams])

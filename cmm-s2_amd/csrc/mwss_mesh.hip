@@ -1125,6 +1125,11 @@ __device__ __forceinline__ VLoc vel_locate(const cmm_mwss_mesh& M, d3 x) {
   return r;
 }
 
+// packed adjoints of the velocity levels (atomic form)
+struct LevelBars {
+  double* p[3];
+};
+
 // Adjoint of one velocity-spline evaluation v = sum_k w_k PS_k(x) of stage
 // `stage`: adds w_k vbar_c basis_j into the level adjoints lb[k] (atomics,
 // summed over the 4 lanes of a quad first when the whole quad sits in one
@@ -1132,8 +1137,8 @@ __device__ __forceinline__ VLoc vel_locate(const cmm_mwss_mesh& M, d3 x) {
 // returns xbar = (dv/dx)^T vbar.  Every lane of the quad must call it.
 // Records are keyed by sub-triangle (tri_keys false) or by triangle (true,
 // the sub-triangle travels in the record).
-__device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, double* const* lb,
-                                      const ScatterWork* sw, int64_t slot, bool tri_keys) {
+__device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, const LevelBars& lb,
+                                      const ScatterWork& sw, bool rec, int64_t slot, bool tri_keys) {
   const VLoc r = vel_locate(V.mesh, x);
   const LevelSet& ls = V.ls[stage];
   const int lane = threadIdx.x & 63;
@@ -1160,14 +1165,14 @@ __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, d
     cv[e] = same ? t : cv[e];
   }
   const bool issue = r.ok && (!same || (lane & 3) == 0);
-  if (sw) {
+  if (rec) {
     // value record for the sorted scatter-add (class = stage; the stage
     // weights are applied there); slot < 0: a lane past the end
     if (slot >= 0) {
       if (issue)
-        scatter_put_values(*sw, slot, (uint32_t)(tri_keys ? r.tri : r.tri * 6 + r.s), cv, stage, r.s);
+        scatter_put_values(sw, slot, (uint32_t)(tri_keys ? r.tri : r.tri * 6 + r.s), cv, stage, r.s);
       else
-        scatter_empty(*sw, slot);
+        scatter_empty(sw, slot);
     }
   }
   d3 cbar = d3{0, 0, 0};
@@ -1179,8 +1184,8 @@ __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, d
         const double* p = ls.p[k] + (r.tri * 6 + r.s) * 18;
 #pragma unroll
         for (int c = 0; c < 3; ++c) cbar = add(cbar, scl(w * vv[c], bb6_db(r.cb, p + 6 * c)));
-        if (!sw && issue && w != 0.0) {
-          double* dst = lb[k] + (r.tri * 6 + r.s) * 18;
+        if (!rec && issue && w != 0.0) {
+          double* dst = lb.p[k] + (r.tri * 6 + r.s) * 18;
 #pragma unroll
           for (int e = 0; e < 18; ++e) unsafeAtomicAdd(dst + e, w * cv[e]);
         }
@@ -1192,9 +1197,6 @@ __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, d
   return add(add(scl(bbar.x, r.w[0]), scl(bbar.y, r.w[1])), scl(bbar.z, r.w[2]));
 }
 
-struct LevelBars {
-  double* p[3];
-};
 
 // Adjoint of advect_point for the stencil feet (integrators Euler, improved
 // Euler, RK4): recompute the stages, then reverse them.  Point i = (vertex
@@ -1217,9 +1219,9 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
   const d3 xn = d3{src[0], src[xs_row], src[2 * xs_row]};
   const double* bsrc = yb + (int64_t)k * nv + v;
   d3 ybar = live ? d3{bsrc[0], bsrc[yb_row], bsrc[2 * yb_row]} : d3{0, 0, 0};
-  double* const* lb = LB.p;
+  const LevelBars& lb = LB;
+  const ScatterWork& sw = swk;
   // record slots 4 i + stage (unused stages marked empty)
-  const ScatterWork* sw = rec ? &swk : nullptr;
   const int64_t s0 = 4 * i0;
   if (rec && live) {
     const int used = integ == CMM_INT_EULER ? 1 : (integ == CMM_INT_IMPROVED_EULER ? 2 : 4);
@@ -1230,7 +1232,7 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
     const d3 v1 = vel_eval(V, 0, xn);
     const d3 z = sub(xn, scl(dt, v1));
     const d3 zb = unit_adj(z, ybar);
-    xbar = add(zb, vel_adj(V, 0, xn, scl(-dt, zb), lb, sw, live ? s0 : -1, tri_keys != 0));
+    xbar = add(zb, vel_adj(V, 0, xn, scl(-dt, zb), lb, sw, rec != 0, live ? s0 : -1, tri_keys != 0));
   } else if (integ == CMM_INT_IMPROVED_EULER) {
     const d3 v1 = vel_eval(V, 0, xn);
     const d3 z1 = sub(xn, scl(dt, v1));
@@ -1240,10 +1242,10 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
     const d3 zb = unit_adj(z, ybar);
     const d3 v2b = scl(-dt / 2, zb);
     d3 v1b = v2b;
-    const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, live ? s0 + 1 : -1, tri_keys != 0);
+    const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, rec != 0, live ? s0 + 1 : -1, tri_keys != 0);
     const d3 z1b = unit_adj(z1, s1b);
     v1b = add(v1b, scl(-dt, z1b));
-    xbar = add(add(zb, z1b), vel_adj(V, 0, xn, v1b, lb, sw, live ? s0 : -1, tri_keys != 0));
+    xbar = add(add(zb, z1b), vel_adj(V, 0, xn, v1b, lb, sw, rec != 0, live ? s0 : -1, tri_keys != 0));
   } else {
   const d3 v1 = vel_eval(V, 0, xn);
   const d3 z1 = d3{xn.x - dt * v1.x / 2, xn.y - dt * v1.y / 2, xn.z - dt * v1.z / 2};
@@ -1261,16 +1263,16 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
   const d3 ab = scl(-dt / 6, unit_adj(z4, ybar));
   d3 v1b = ab, v2b = scl(2, ab), v3b = scl(2, ab);
   const d3 v4b = ab;
-  const d3 s3b = vel_adj(V, 3, s3, v4b, lb, sw, live ? s0 + 3 : -1, tri_keys != 0);
+  const d3 s3b = vel_adj(V, 3, s3, v4b, lb, sw, rec != 0, live ? s0 + 3 : -1, tri_keys != 0);
   v3b = add(v3b, scl(-dt, unit_adj(z3, s3b)));
-  const d3 s2b = vel_adj(V, 2, s2, v3b, lb, sw, live ? s0 + 2 : -1, tri_keys != 0);
+  const d3 s2b = vel_adj(V, 2, s2, v3b, lb, sw, rec != 0, live ? s0 + 2 : -1, tri_keys != 0);
   v2b = add(v2b, scl(-dt / 2, unit_adj(z2, s2b)));
-  const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, live ? s0 + 1 : -1, tri_keys != 0);
+  const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, rec != 0, live ? s0 + 1 : -1, tri_keys != 0);
   const d3 z1b = unit_adj(z1, s1b);
   v1b = add(v1b, scl(-dt / 2, z1b));
   // z4 = xn - ..., z3 = xn - ..., z2 = xn - ..., z1 = xn - ... all carry xn
   const d3 z4b = unit_adj(z4, ybar), z3b = unit_adj(z3, s3b), z2b = unit_adj(z2, s2b);
-  xbar = add(add(add(z4b, z3b), add(z2b, z1b)), vel_adj(V, 0, xn, v1b, lb, sw, live ? s0 : -1, tri_keys != 0));
+  xbar = add(add(add(z4b, z3b), add(z2b, z1b)), vel_adj(V, 0, xn, v1b, lb, sw, rec != 0, live ? s0 : -1, tri_keys != 0));
   }
   if (xb && live) {
     double* dst = xb + (int64_t)k * nv + v;

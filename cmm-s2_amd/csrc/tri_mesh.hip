@@ -855,30 +855,35 @@ __global__ __launch_bounds__(192) void k_tri_assemble_adj_corners(cmm_tri_mesh M
   }
 }
 
+// thread (row r, vertex v), r-major: a wave covers 64 consecutive vertices
+// of one jets row (12 x more threads than one per vertex: the gather is
+// latency-bound)
 __global__ __launch_bounds__(256) void k_tri_assemble_adj_gather(cmm_tri_mesh M, const double* __restrict__ tmp,
                                                                  double* __restrict__ vb) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= M.nv) return;
-  double acc[12];
-#pragma unroll
-  for (int r = 0; r < 12; ++r) acc[r] = 0.0;
+  const int64_t Nv = M.nv;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 12 * Nv) return;
+  const int r = (int)(idx / Nv);
+  const int64_t v = idx - r * Nv;
   const int64_t nt = M.ntri;
   const int b = M.vstart[v], e = M.vstart[v + 1];
-  for (int k0 = b; k0 < e; k0 += 6) {  // 6 incident corners per pass, their loads in flight together
+  double acc = 0.0;
+  for (int k0 = b; k0 < e; k0 += 8) {  // 8 incident corners per pass, their loads in flight together
+    double x[8];
 #pragma unroll
-    for (int k = k0; k < k0 + 6; ++k) {
-      if (k < e) {
-        const int tc = M.vinc[k];
-        const int64_t t = tc / 3;
-        const int i = tc - 3 * (int)t;
-#pragma unroll
-        for (int r = 0; r < 12; ++r) acc[r] += tmp[(12 * i + r) * nt + t];
+    for (int k = 0; k < 8; ++k) {
+      x[k] = 0.0;
+      if (k0 + k < e) {
+        const int tc = M.vinc[k0 + k];
+        const int t = tc / 3;
+        const int i = tc - 3 * t;
+        x[k] = tmp[(12 * i + r) * nt + t];
       }
     }
-  }
-  const int64_t Nv = M.nv;
 #pragma unroll
-  for (int r = 0; r < 12; ++r) vb[r * Nv + v] += acc[r];
+    for (int k = 0; k < 8; ++k) acc += x[k];
+  }
+  vb[r * Nv + v] += acc;
 }
 
 // adjoint of k_tri_stencil_jets: jet adjoints (3,4,Nv) -> stencil-feet
@@ -917,8 +922,7 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, co
     }
   }
   d3 Sb[4] = {d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}, d3{0, 0, 0}};
-  if (rec && !(ok && cur))
-    for (int k = 0; k < 4; ++k) scatter_empty(sw, 4 * v + k);
+  if (rec && !(ok && cur)) scatter_empty(sw, v);
   if (ok) {
     const d3 mu = d3{(S[0].x + S[1].x + S[2].x + S[3].x) / 4, (S[0].y + S[1].y + S[2].y + S[3].y) / 4,
                      (S[0].z + S[1].z + S[2].z + S[3].z) / 4};
@@ -948,7 +952,6 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, co
       for (int e = 0; e < 18; ++e) acc[e] = 0.0;
       const double* p = cur + ((int64_t)L.tri * 6 + s) * 18;
       for (int k = 0; k < 4; ++k) {
-        if (rec) scatter_put(sw, 4 * v + k, (uint32_t)((int64_t)L.tri * 6 + s), Sb[k], cbk[k], 0);
         double bas[6];
         bb6_basis(cbk[k], bas);
         const double rv[3] = {Sb[k].x, Sb[k].y, Sb[k].z};
@@ -960,7 +963,11 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, co
         const d3 bb = sub_bary_T(s, cbar, L.s12, L.s23, L.s31, L.s4);
         Sb[k] = add(add(scl(bb.x, L.w0), scl(bb.y, L.w1)), scl(bb.z, L.w2));
       }
-      if (curb && !rec) {
+      // the four feet share the block: one value record (or 18 atomics)
+      // per vertex
+      if (rec) {
+        scatter_put_values(sw, v, (uint32_t)((int64_t)L.tri * 6 + s), acc, 0);
+      } else if (curb) {
         double* dst = curb + ((int64_t)L.tri * 6 + s) * 18;
         for (int e = 0; e < 18; ++e) unsafeAtomicAdd(dst + e, acc[e]);
       }
@@ -1150,7 +1157,7 @@ extern "C" int cmm_tri_assemble_adjoint(const cmm_tri_mesh* m, const double* pac
   double* tmp = static_cast<double*>(work);
   hipLaunchKernelGGL(k_tri_assemble_adj_corners, dim3(grid_for(m->ntri, 64)), dim3(192), 0, st, *m, packed_bar, tmp);
   CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
-  hipLaunchKernelGGL(k_tri_assemble_adj_gather, dim3(grid_for(m->nv, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
+  hipLaunchKernelGGL(k_tri_assemble_adj_gather, dim3(grid_for(12 * m->nv, 256)), dim3(256), 0, st, *m, tmp, vals_bar);
   CMM_LAUNCH_CHECK("cmm_tri_assemble_adjoint");
   return CMM_OK;
 }
@@ -1165,7 +1172,7 @@ extern "C" int cmm_tri_stencil_jets_adjoint(const cmm_tri_mesh* m, const double*
   ScatterWork sw;
   const bool rec = work && cur && cur_bar;
   if (rec) {
-    int rc = scatter_layout(work, work_bytes, 4 * m->nv, m->ntri * 6, sw);
+    int rc = scatter_layout(work, work_bytes, m->nv, m->ntri * 6, sw);
     if (rc) return rc;
   }
   hipStream_t st = as_stream(stream);
@@ -1177,12 +1184,13 @@ extern "C" int cmm_tri_stencil_jets_adjoint(const cmm_tri_mesh* m, const double*
                      srow, tanv, eps, jets_bar, sf_bar, sbrow, cur ? cur_bar : nullptr, sw, rec ? 1 : 0);
   CMM_LAUNCH_CHECK("cmm_tri_stencil_jets_adjoint");
   if (rec) {
+    // value records of the per-vertex sums (class 0, weight 1)
     ScatterTargets t;
-    t.mode = 1;
+    t.mode = 2;
     t.nc = 3;
     t.nout = 1;
     t.T[0] = cur_bar;
-    return scatter_reduce(sw, 4 * m->nv, t, st);
+    return scatter_reduce(sw, m->nv, t, st);
   }
   return CMM_OK;
 }

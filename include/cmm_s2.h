@@ -302,7 +302,7 @@ int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const double* wpa
  * scatter-add (counting sort by target block, one writer per block) in a
  * caller workspace of cmm_scatter_workspace_bytes(records, key_range) bytes
  * (records / key_range: depth * n / depth * 6 Ntri for the composition,
- * 16 Nv / 6 Tu of the velocity mesh for the RK feet, 4 Nv / 6 Ntri for the
+ * 16 Nv / 6 Tu of the velocity mesh for the RK feet, Nv / 6 Ntri for the
  * stencil projection, n / 6 Tu of the vorticity mesh for the sampling):
  * several times faster than the FP64 atomics. */
 int cmm_scatter_workspace_bytes(int64_t records, int64_t key_range, int64_t* bytes);

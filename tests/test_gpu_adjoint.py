@@ -398,7 +398,7 @@ def test_stencil_jets_adjoint_fd(identity, scatter):
     pk = _packed(mesh, cur_j)
     sb = torch.empty_like(ys)
     curb = torch.zeros_like(pk)
-    wp, wsz = _work(None if scatter == "atomics" else 4 * nv, 6 * mesh.ntri)
+    wp, wsz = _work(None if scatter == "atomics" else nv, 6 * mesh.ntri)
     N.call("cmm_tri_stencil_jets_adjoint", mesh._h, None if identity else pk.data_ptr(), yv.data_ptr(), yv.stride(0),
            ys.data_ptr(), ys.stride(0), mesh.d_tan.data_ptr(), eps, jb.data_ptr(), sb.data_ptr(), sb.stride(0),
            curb.data_ptr(), wp, wsz, stream())
