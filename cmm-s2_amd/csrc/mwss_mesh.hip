@@ -1300,8 +1300,22 @@ __global__ __launch_bounds__(256) void k_sample_vorticity_adj(cmm_mwss_mesh W, c
   const VLoc r = vel_locate(W, q);
   d3 gq = d3{0, 0, 0};
   if (rec) {
-    if (r.ok)
-      scatter_put(sw, i, (uint32_t)(r.tri * 6 + r.s), d3{wb, 0.0, 0.0}, r.cb, 0);
+    // a full wave at one point (the 2L samples of a pole ring) writes one
+    // record with the wave's sum: one key is otherwise counted and placed
+    // 2L times per ring and call
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t key = r.ok ? r.tri * 6 + r.s : -1 - (int64_t)lane;
+    const bool full = __ballot(1) == ~0ull;
+    const int64_t k0 = __shfl(key, 0);
+    const double c0x = __shfl(r.cb.x, 0), c0y = __shfl(r.cb.y, 0), c0z = __shfl(r.cb.z, 0);
+    const bool same = full && __all(key == k0 && r.cb.x == c0x && r.cb.y == c0y && r.cb.z == c0z);
+    double a = wb;
+    if (same) {
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) a += __shfl_xor(a, o);
+    }
+    if (r.ok && (!same || lane == 0))
+      scatter_put(sw, i, (uint32_t)(r.tri * 6 + r.s), d3{a, 0.0, 0.0}, r.cb, 0);
     else
       scatter_empty(sw, i);
   }
