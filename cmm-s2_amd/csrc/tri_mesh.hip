@@ -710,6 +710,27 @@ __device__ inline d3 eval_level3_adj(const double* __restrict__ packed, double* 
   return add(add(scl(bb.x, L.w0), scl(bb.y, L.w1)), scl(bb.z, L.w2));
 }
 
+// the 18 coefficients of sub-triangle s of triangle t (9 16-byte loads)
+__device__ inline void load_block18(const double* __restrict__ packed, int t, int s, double (&buf)[18]) {
+  const double2* p = reinterpret_cast<const double2*>(packed + ((int64_t)t * 6 + s) * 18);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const double2 v = p[i];
+    buf[2 * i] = v.x;
+    buf[2 * i + 1] = v.y;
+  }
+}
+
+// eval_level3_adj on a block already in registers (no atomics)
+__device__ inline d3 eval_level3_adj_buf(const double (&buf)[18], const Located& L, d3 cb, int s, d3 rb) {
+  const double rv[3] = {rb.x, rb.y, rb.z};
+  d3 cbar = d3{0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) cbar = add(cbar, scl(rv[c], bb6_db(cb, buf + 6 * c)));
+  const d3 bb = sub_bary_T(s, cbar, L.s12, L.s23, L.s31, L.s4);
+  return add(add(scl(bb.x, L.w0), scl(bb.y, L.w1)), scl(bb.z, L.w2));
+}
+
 // adjoint of k_tri_compose: xbar of the output (n,3) -> coefficient adjoints
 // of every level (null entries skipped) and, if qb, the input adjoint
 // With rec (a workspace) the coefficient adjoints become records
@@ -753,15 +774,20 @@ __global__ __launch_bounds__(256) void k_tri_compose_adj(cmm_tri_mesh M, LevelPt
         located_at(M.geom, ts[l], xs[l], L);
         const int s = ss[l];
         const d3 cb = sub_bary(s, L.b0, L.b1, L.b2, L.s12, L.s23, L.s31, L.s4);
-        const d3 r = eval_level3(lv.p[l], L, d3{L.b0, L.b1, L.b2}, s);
+        // the block once for the value (unit_adj) and the position adjoint
+        double buf[18];
+        load_block18(lv.p[l], L.tri, s, buf);
+        const d3 r = d3{bb6(cb, buf), bb6(cb, buf + 6), bb6(cb, buf + 12)};
         const d3 rb = unit_adj(r, xb);
         if (rec) {
           if (lb.p[l])
             scatter_put(sw, (int64_t)l * n + i, (uint32_t)(l * span + (int64_t)L.tri * 6 + s), rb, cb, l);
           else
             scatter_empty(sw, (int64_t)l * n + i);
+          xb = eval_level3_adj_buf(buf, L, cb, s, rb);
+        } else {
+          xb = eval_level3_adj(lv.p[l], lb.p[l], L, cb, s, rb);
         }
-        xb = eval_level3_adj(lv.p[l], rec ? nullptr : lb.p[l], L, cb, s, rb);
       } else if (rec) {
         scatter_empty(sw, (int64_t)l * n + i);
       }
