@@ -219,9 +219,10 @@ struct LevelSet {
 
 // sum_k w_k * PS_k(q) for ncomp components
 __device__ __forceinline__ bool mwss_eval_levels(const cmm_mwss_mesh& M, const LevelSet& ls, int ncomp, d3 q,
-                                        double* r) {
+                                        double* r, int64_t* tri_out = nullptr) {
   MLoc o;
   if (!mwss_locate(M, q, o)) return false;
+  if (tri_out) *tri_out = o.tri;
   int s = split_index(o.b.x, o.b.y, o.b.z);
   d3 cb = sub_bary_v(s, o.b, o.v);
   // compile-time trip counts with guards: runtime-indexed arrays would be
@@ -313,6 +314,19 @@ __device__ __forceinline__ d3 vel_eval(const VelDev& V, int stage, d3 x) {
     case CMM_VEL_DEFORM_ROT: return vel_deform_rot(V.stage_t[stage], x, V.params[1]);
     default: return vel_rh4(V.stage_t[stage], x, V.params[0], V.params[1], V.params[2]);
   }
+}
+
+// spline velocity at x with the located triangle (-1: not located), for an
+// adjoint that revisits x
+__device__ __forceinline__ d3 vel_eval_t(const VelDev& V, int stage, d3 x, int64_t& tri) {
+  double r[3];
+  tri = -1;
+  if (!mwss_eval_levels(V.mesh, V.ls[stage], 3, x, r, &tri)) {
+    const double nan = __builtin_nan("");
+    tri = -1;
+    return d3{nan, nan, nan};
+  }
+  return d3{r[0], r[1], r[2]};
 }
 
 // evolution_functions.div_norm (82-85): x * (1 / |x|)
@@ -1130,6 +1144,31 @@ struct LevelBars {
   double* p[3];
 };
 
+// vel_locate for a point already located in triangle tri (the closed-form
+// vertices of tri are those mwss_locate builds: same barycentrics, bits
+// included, without the angles and cell search)
+__device__ __forceinline__ VLoc vel_locate_at(const cmm_mwss_mesh& M, d3 x, int64_t tri) {
+  VLoc r;
+  MLoc o;
+  o.tri = tri;
+  mwss_tri_vertices(M, tri, o.vi, o.v);
+  o.b = bary_fast(o.v[0], o.v[1], o.v[2], x);
+  r.ok = true;
+  r.tri = o.tri;
+  r.s = split_index(o.b.x, o.b.y, o.b.z);
+  const d3 s3 = add(add(o.v[0], o.v[1]), o.v[2]);
+  r.s4 = norm3(s3);
+  if (r.s <= 1) r.se = norm3(add(o.v[1], o.v[2]));
+  else if (r.s <= 3) r.se = norm3(add(o.v[2], o.v[0]));
+  else r.se = norm3(add(o.v[0], o.v[1]));
+  r.cb = sub_bary(r.s, o.b.x, o.b.y, o.b.z, r.se, r.se, r.se, r.s4);
+  const double D = det3(o.v[0], o.v[1], o.v[2]);
+  r.w[0] = scl(1 / D, cross(o.v[1], o.v[2]));
+  r.w[1] = scl(1 / D, cross(o.v[2], o.v[0]));
+  r.w[2] = scl(1 / D, cross(o.v[0], o.v[1]));
+  return r;
+}
+
 // Adjoint of one velocity-spline evaluation v = sum_k w_k PS_k(x) of stage
 // `stage`: adds w_k vbar_c basis_j into the level adjoints lb[k] (atomics,
 // summed over the 4 lanes of a quad first when the whole quad sits in one
@@ -1138,8 +1177,9 @@ struct LevelBars {
 // Records are keyed by sub-triangle (tri_keys false) or by triangle (true,
 // the sub-triangle travels in the record).
 __device__ __forceinline__ d3 vel_adj(const VelDev& V, int stage, d3 x, d3 vb, const LevelBars& lb,
-                                      const ScatterWork& sw, bool rec, int64_t slot, bool tri_keys) {
-  const VLoc r = vel_locate(V.mesh, x);
+                                      const ScatterWork& sw, bool rec, int64_t slot, bool tri_keys,
+                                      int64_t tri_known) {
+  const VLoc r = tri_known >= 0 ? vel_locate_at(V.mesh, x, tri_known) : vel_locate(V.mesh, x);
   const LevelSet& ls = V.ls[stage];
   const int lane = threadIdx.x & 63;
   const int64_t key = r.ok ? r.tri * 6 + r.s : -1 - (int64_t)lane;
@@ -1228,51 +1268,54 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
     for (int st = used; st < 4; ++st) scatter_empty(swk, s0 + st);
   }
   d3 xbar;
+  // the stage points are located once: the adjoint of each evaluation
+  // revisits its triangle (t0 .. t3)
+  int64_t t0, t1, t2, t3;
   if (integ == CMM_INT_EULER) {
-    const d3 v1 = vel_eval(V, 0, xn);
+    const d3 v1 = vel_eval_t(V, 0, xn, t0);
     const d3 z = sub(xn, scl(dt, v1));
     const d3 zb = unit_adj(z, ybar);
-    xbar = add(zb, vel_adj(V, 0, xn, scl(-dt, zb), lb, sw, rec != 0, live ? s0 : -1, tri_keys != 0));
+    xbar = add(zb, vel_adj(V, 0, xn, scl(-dt, zb), lb, sw, rec != 0, live ? s0 : -1, tri_keys != 0, t0));
   } else if (integ == CMM_INT_IMPROVED_EULER) {
-    const d3 v1 = vel_eval(V, 0, xn);
+    const d3 v1 = vel_eval_t(V, 0, xn, t0);
     const d3 z1 = sub(xn, scl(dt, v1));
     const d3 s1 = proj_norm(z1);
-    const d3 v2 = vel_eval(V, 1, s1);
+    const d3 v2 = vel_eval_t(V, 1, s1, t1);
     const d3 z = sub(xn, scl(dt / 2, add(v1, v2)));
     const d3 zb = unit_adj(z, ybar);
     const d3 v2b = scl(-dt / 2, zb);
     d3 v1b = v2b;
-    const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, rec != 0, live ? s0 + 1 : -1, tri_keys != 0);
+    const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, rec != 0, live ? s0 + 1 : -1, tri_keys != 0, t1);
     const d3 z1b = unit_adj(z1, s1b);
     v1b = add(v1b, scl(-dt, z1b));
-    xbar = add(add(zb, z1b), vel_adj(V, 0, xn, v1b, lb, sw, rec != 0, live ? s0 : -1, tri_keys != 0));
+    xbar = add(add(zb, z1b), vel_adj(V, 0, xn, v1b, lb, sw, rec != 0, live ? s0 : -1, tri_keys != 0, t0));
   } else {
-  const d3 v1 = vel_eval(V, 0, xn);
+  const d3 v1 = vel_eval_t(V, 0, xn, t0);
   const d3 z1 = d3{xn.x - dt * v1.x / 2, xn.y - dt * v1.y / 2, xn.z - dt * v1.z / 2};
   const d3 s1 = proj_norm(z1);
-  const d3 v2 = vel_eval(V, 1, s1);
+  const d3 v2 = vel_eval_t(V, 1, s1, t1);
   const d3 z2 = sub(xn, scl(dt / 2, v2));
   const d3 s2 = proj_norm(z2);
-  const d3 v3 = vel_eval(V, 2, s2);
+  const d3 v3 = vel_eval_t(V, 2, s2, t2);
   const d3 z3 = sub(xn, scl(dt, v3));
   const d3 s3 = proj_norm(z3);
-  const d3 v4 = vel_eval(V, 3, s3);
+  const d3 v4 = vel_eval_t(V, 3, s3, t3);
   const d3 acc = d3{v1.x + 2 * v2.x + 2 * v3.x + v4.x, v1.y + 2 * v2.y + 2 * v3.y + v4.y,
                     v1.z + 2 * v2.z + 2 * v3.z + v4.z};
   const d3 z4 = sub(xn, scl(dt / 6, acc));
   const d3 ab = scl(-dt / 6, unit_adj(z4, ybar));
   d3 v1b = ab, v2b = scl(2, ab), v3b = scl(2, ab);
   const d3 v4b = ab;
-  const d3 s3b = vel_adj(V, 3, s3, v4b, lb, sw, rec != 0, live ? s0 + 3 : -1, tri_keys != 0);
+  const d3 s3b = vel_adj(V, 3, s3, v4b, lb, sw, rec != 0, live ? s0 + 3 : -1, tri_keys != 0, t3);
   v3b = add(v3b, scl(-dt, unit_adj(z3, s3b)));
-  const d3 s2b = vel_adj(V, 2, s2, v3b, lb, sw, rec != 0, live ? s0 + 2 : -1, tri_keys != 0);
+  const d3 s2b = vel_adj(V, 2, s2, v3b, lb, sw, rec != 0, live ? s0 + 2 : -1, tri_keys != 0, t2);
   v2b = add(v2b, scl(-dt / 2, unit_adj(z2, s2b)));
-  const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, rec != 0, live ? s0 + 1 : -1, tri_keys != 0);
+  const d3 s1b = vel_adj(V, 1, s1, v2b, lb, sw, rec != 0, live ? s0 + 1 : -1, tri_keys != 0, t1);
   const d3 z1b = unit_adj(z1, s1b);
   v1b = add(v1b, scl(-dt / 2, z1b));
   // z4 = xn - ..., z3 = xn - ..., z2 = xn - ..., z1 = xn - ... all carry xn
   const d3 z4b = unit_adj(z4, ybar), z3b = unit_adj(z3, s3b), z2b = unit_adj(z2, s2b);
-  xbar = add(add(add(z4b, z3b), add(z2b, z1b)), vel_adj(V, 0, xn, v1b, lb, sw, rec != 0, live ? s0 : -1, tri_keys != 0));
+  xbar = add(add(add(z4b, z3b), add(z2b, z1b)), vel_adj(V, 0, xn, v1b, lb, sw, rec != 0, live ? s0 : -1, tri_keys != 0, t0));
   }
   if (xb && live) {
     double* dst = xb + (int64_t)k * nv + v;
