@@ -253,8 +253,10 @@ int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hip
 
 extern "C" int cmm_scatter_workspace_bytes(int64_t records, int64_t key_range, int64_t* bytes) {
   if (!bytes || records < 1 || key_range < 1) return cmm::fail(CMM_E_ARG, "cmm_scatter_workspace_bytes: bad arguments");
+  if (records > (int64_t)0x7fffffff || key_range >= (int64_t)cmm::kScatterEmpty)
+    return cmm::fail(CMM_E_RANGE, "cmm_scatter_workspace_bytes: record count or key range out of range");
   const int64_t b = cmm::scatter_bytes(records, key_range);
-  if (!b) return cmm::fail(CMM_E_RANGE, "cmm_scatter_workspace_bytes: record count or key range out of range");
+  if (!b) return cmm::fail(CMM_E_HIP, "cmm_scatter_workspace_bytes: hipCUB temporary storage query failed");
   *bytes = b;
   return CMM_OK;
 }

@@ -46,6 +46,17 @@ def test_argument_errors_without_gpu():
     vel = N.Velocity()
     vel.kind = 9
     assert lib.cmm_advect_points(ctypes.byref(vel), 2, 0.1, None, 0, None, None) == -1
+    # grouped-scatter sizes and the sparse velocity adjoint: checked before
+    # any device work
+    nb = ctypes.c_int64(0)
+    assert lib.cmm_scatter_workspace_bytes(0, 10, ctypes.byref(nb)) == -1
+    assert lib.cmm_scatter_workspace_bytes(1 << 31, 10, ctypes.byref(nb)) == -3
+    assert lib.cmm_scatter_workspace_bytes(10, 1 << 32, ctypes.byref(nb)) == -3
+    vel.kind = N.VEL_SPLINE
+    assert lib.cmm_advect_stencil_adjoint_corners(ctypes.byref(vel), N.INT_RK4, 0.1, None, 0, 0, None, 0, None,
+                                                  None, None, 0, None, 0, None) == -1
+    assert b"cmm_advect_stencil_adjoint_corners" in lib.cmm_last_error()
+    assert lib.cmm_mwss_corner_jets_adjoint(None, None, None, None, None) == -1
     with pytest.raises(N.CmmError):
         N.check(-1, "x")
 
