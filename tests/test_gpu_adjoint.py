@@ -369,6 +369,33 @@ def test_advect_stencil_adjoint_fd(integ, scatter):
     _close(_dot(xb, dx), (Jx((xs + h * dx).contiguous()) - Jx((xs - h * dx).contiguous())) / (2 * h), 1e-6)
 
 
+def test_corner_gather_reads_flagged_triangles_only():
+    """Sparse corner jets: triangles with a clear flag are never read (NaN
+    there leaves the jets adjoint exactly zero); one flagged triangle adds
+    the 12 rows of each of its corners into that corner's vertex."""
+    from cmm_s2 import _native as N
+    from cmm_s2.core._backend import stream
+    from cmm_s2.core import mesh_functions as mf
+    mu = mf.structure_spherical_triangulation(16)
+    Tu, Nu = mu.ntri, mu.nv
+    corners = torch.full((Tu, 36), float("nan"), dtype=torch.float64, device="cuda")
+    flags = torch.zeros(Tu, dtype=torch.uint8, device="cuda")
+    vb = torch.zeros(12, Nu, dtype=torch.float64, device="cuda")
+    N.call("cmm_mwss_corner_jets_adjoint", mu._h, corners.data_ptr(), flags.data_ptr(), vb.data_ptr(), stream())
+    torch.cuda.synchronize()
+    assert float(vb.abs().max()) == 0.0
+    t = Tu // 2 + 3  # a band triangle: three distinct non-pole vertices
+    corners[t] = _rand(36, seed=9)
+    flags[t] = 1
+    N.call("cmm_mwss_corner_jets_adjoint", mu._h, corners.data_ptr(), flags.data_ptr(), vb.data_ptr(), stream())
+    torch.cuda.synchronize()
+    got = vb.cpu().numpy()
+    c = corners[t].cpu().numpy().reshape(3, 12)
+    touched = np.nonzero(np.abs(got).sum(axis=0))[0]
+    assert len(touched) == 3
+    assert sorted(tuple(got[:, v]) for v in touched) == sorted(tuple(c[i]) for i in range(3))
+
+
 @pytest.mark.parametrize("scatter", ["atomics", "grouped"])
 @pytest.mark.parametrize("identity", [True, False])
 def test_stencil_jets_adjoint_fd(identity, scatter):
