@@ -53,6 +53,7 @@ SIGNATURES = {
     "cmm_mwss_mesh_create": (C.c_int, [C.c_int, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_vp]),
     "cmm_mwss_mesh_destroy": (C.c_int, [vp]),
     "cmm_mwss_locate": (C.c_int, [vp, vp, i64, i64, i64, vp, vp, vp]),
+    "cmm_mwss_split_locate": (C.c_int, [vp, vp, i64, i64, i64, vp, vp, vp, vp, vp, vp]),
     "cmm_mwss_assemble": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
     "cmm_mwss_eval": (C.c_int, [vp, P_vp, P_dbl, C.c_int, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp]),
     "cmm_advect_points": (C.c_int, [C.POINTER(Velocity), C.c_int, dbl, vp, i64, vp, vp]),

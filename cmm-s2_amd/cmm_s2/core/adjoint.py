@@ -356,9 +356,36 @@ class MisfitProblem(object):
         self._cps = []
         self._run(p, checkpoints=self._cps, keep_bytes=keep_bytes)
         J = self._objective(None)
+        self._last = self._tape_state()
         torch.cuda.synchronize()
         self.timings = {"forward_s": time.perf_counter() - t0}
         return J
+
+    def _tape_state(self):
+        """Everything ``backward`` reads of one forward run: the remap
+        checkpoints, the kept segment tapes, the final maps and loop
+        counters, the velocity levels and the initial-vorticity spline.  The
+        tensors are referenced, not copied (they are written once), so
+        holding several states lets several forward runs be reversed later,
+        in any order."""
+        return dict(cps=list(self._cps), kept=dict(self._kept), maps=dict(self.maps),
+                    submaps=list(self.submaps), curr=self.curr, w0=self.w0, sampler=self.sampler,
+                    V=self.V.copy(), U=dict(self.U), count=self.count, next_vid=self.next_vid,
+                    next_mid=self.next_mid)
+
+    def last_state(self):
+        """The tape state of the last ``forward`` (pass it to ``backward``)."""
+        if getattr(self, "_last", None) is None:
+            raise N.CmmError("backward() needs a forward() run first")
+        return self._last
+
+    def _load_state(self, st):
+        self._cps = list(st["cps"])
+        self._kept = dict(st["kept"])
+        self.maps, self.submaps, self.curr = dict(st["maps"]), list(st["submaps"]), st["curr"]
+        self.w0, self.sampler = st["w0"], st["sampler"]
+        self.V, self.U = st["V"].copy(), dict(st["U"])
+        self.count, self.next_vid, self.next_mid = st["count"], st["next_vid"], st["next_mid"]
 
     def _record(self, k, final_maps):
         """Recompute remap segment k from its checkpoint with the tape on:
@@ -389,14 +416,16 @@ class MisfitProblem(object):
                 if isinstance(x, torch.Tensor):
                     x.record_stream(stream)
 
-    def backward(self, overlap=True):
-        """dJ/dp at the point of the last ``forward``: recompute each remap
+    def backward(self, overlap=True, state=None):
+        """dJ/dp at the point of the ``forward`` run whose ``last_state()``
+        is ``state`` (default: the last forward): recompute each remap
         segment from its checkpoint with the tape on and walk it backwards.
         With ``overlap`` the recomputation of segment k-1 runs on a second
         HIP stream (default priority) while segment k is reversed on the
         current one: the two are independent, and the reverse of segment k-1
-        waits for its recording only."""
+        waits for its recording only.  ``state`` is not consumed."""
         import time
+        self._load_state(self.last_state() if state is None else state)
         cps = self._cps
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -445,7 +474,7 @@ class MisfitProblem(object):
         torch.cuda.synchronize()
         self.timings["reverse_s"] = time.perf_counter() - t0
         self.maps, self.U, self.lam_M, self.lam_U = {}, {}, {}, {}
-        self._cps = None
+        self._cps, self._kept = None, {}
         return g
 
     def value_and_grad(self, p, keep_bytes=None, overlap=True):
@@ -580,17 +609,25 @@ class MisfitProblem(object):
 
 
 class VorticityMisfit(torch.autograd.Function):
-    """J(p) as an autograd function: forward runs the loop (keeping the remap
-    checkpoints), backward the recomputing reverse sweep."""
+    """J(p) as an autograd function: forward runs the loop and keeps its
+    tape state (remap checkpoints, kept tapes, final maps) on ``ctx``,
+    backward the recomputing reverse sweep from that state.  Several calls
+    on one problem before ``.backward()`` (a batch of fields summed into one
+    loss) each reverse their own run."""
 
     @staticmethod
     def forward(ctx, p, problem):
         ctx.problem = problem
-        return torch.tensor(problem.forward(p.detach()), dtype=torch.float64, device=p.device)
+        J = problem.forward(p.detach())
+        ctx.state = problem.last_state()
+        problem._last = None
+        return torch.tensor(J, dtype=torch.float64, device=p.device)
 
     @staticmethod
     def backward(ctx, gout):
-        return gout * ctx.problem.backward(), None
+        g = ctx.problem.backward(state=ctx.state)
+        ctx.state = None
+        return gout * g, None
 
 
 def misfit(p, problem):

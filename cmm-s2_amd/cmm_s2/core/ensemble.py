@@ -1,11 +1,14 @@
 """Ensembles of barotropic CMM runs on one GPU.
 
 Members may differ in bandlimit, map mesh and initial vorticity.  Each member
-owns its buffers and issues its kernels on its own HIP stream (default
-priority), so the work of small members overlaps on the chip while the
-host walks the members in lockstep; members share read-only tables (map
-meshes, SHT plans of equal bandlimit).  A member's results do not depend on
-the others: running it alone gives the same bits.
+owns its buffers and issues its kernels on its own HIP stream (created with
+hipStreamCreate, default priority; torch's stream pool would hand 64 members
+32 shared streams), so the work of small members overlaps on the chip while
+the host walks the members in lockstep; members share read-only tables (map
+meshes, SHT plans of equal bandlimit).  The runtime multiplexes the streams
+over the process's hardware queues (GPU_MAX_HW_QUEUES, 4 by default), which
+bounds how many members run concurrently.  A member's results do not depend
+on the others: running it alone gives the same bits.
 
 ``capture()`` records one whole run of every member (bootstrap and all
 steps, forked over the member streams) as a single HIP graph; ``replay()``
@@ -14,28 +17,78 @@ the host's per-step work.  Every kernel argument is fixed at capture (buffer
 addresses, times, depths), so a replay writes the bits of the eager run; the
 initial vorticity buffers may be rewritten in place between replays.
 """
+import ctypes
 import time
 
 import numpy as np
 import torch
 
+from .. import _native as N
 from .spherical_simulations import BarotropicCMM
+
+_hip = None
+
+
+def _hip_lib():
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        _hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    return _hip
+
+
+class _OwnStream(object):
+    """A HIP stream of its own (hipStreamCreate: default priority, not one of
+    torch's pooled streams), usable wherever torch takes a stream."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        rc = _hip_lib().hipStreamCreate(ctypes.byref(h))
+        if rc != 0:
+            raise N.CmmError(f"hipStreamCreate failed ({rc})")
+        self.handle = h.value
+        self.stream = torch.cuda.ExternalStream(self.handle)
+
+    def destroy(self):
+        if self.handle:
+            _hip_lib().hipStreamDestroy(ctypes.c_void_p(self.handle))
+            self.handle = None
 
 
 class Ensemble(object):
     def __init__(self, builders, streams=True):
         """``builders``: callables returning a BarotropicCMM; each is called
         with its member's stream current, so the member's buffers belong to
-        that stream."""
+        that stream.  ``streams`` False runs every member on one stream."""
         n = len(builders)
         self.own_streams = bool(streams)
-        self.streams = [torch.cuda.Stream() for _ in range(n)] if streams else \
-            [torch.cuda.current_stream()] * n
+        self._own = [_OwnStream() for _ in range(n if streams else 1)]
+        # the capture stream: used by no member, waited on by all
+        self._cap = _OwnStream()
+        self.streams = [o.stream for o in self._own] if streams else [self._own[0].stream] * n
+        self.graph = None
         self.members = []
         for b, s in zip(builders, self.streams):
             with torch.cuda.stream(s):
                 self.members.append(b())
         torch.cuda.synchronize()
+
+    def close(self):
+        """Release the graph and the streams (after the device is idle)."""
+        torch.cuda.synchronize()
+        self.graph = None
+        self.members = []
+        for o in self._own + [self._cap]:
+            o.destroy()
+        self._own = []
+
+    def __del__(self):
+        try:
+            if self._own:
+                self.close()
+        except Exception:
+            pass
 
     def bootstrap(self):
         for m, s in zip(self.members, self.streams):
@@ -68,23 +121,22 @@ class Ensemble(object):
             m.reset()
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
-        cap = torch.cuda.Stream()
-        eager = self.streams
-        if not self.own_streams:  # one stream: record on the capture stream itself
-            self.streams = [cap] * len(self.members)
-        try:
-            with torch.cuda.graph(g, stream=cap):
-                for s in set(self.streams):
+        # one stream: record on the members' stream itself (its workspaces
+        # exist from the eager run); member streams: fork from a capture
+        # stream that no member uses
+        cap = self._cap.stream if self.own_streams else self.streams[0]
+        with torch.cuda.graph(g, stream=cap):
+            for s in set(self.streams):
+                if s is not cap:
                     s.wait_stream(cap)
-                for m, s in zip(self.members, self.streams):
-                    with torch.cuda.stream(s):
-                        m.bootstrap(tune=False)
-                while self.step():
-                    pass
-                for s in set(self.streams):
+            for m, s in zip(self.members, self.streams):
+                with torch.cuda.stream(s):
+                    m.bootstrap(tune=False)
+            while self.step():
+                pass
+            for s in set(self.streams):
+                if s is not cap:
                     cap.wait_stream(s)
-        finally:
-            self.streams = eager
         torch.cuda.synchronize()
         self.graph = g
         return g

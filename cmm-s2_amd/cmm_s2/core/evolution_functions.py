@@ -111,7 +111,8 @@ def _work(mesh):
     w = ws.get(key)
     if w is None:
         w = _Work(mesh)
-        ws[key] = w
+        if not torch.cuda.is_current_stream_capturing():  # graph-pool buffers stay with their graph
+            ws[key] = w
     return w
 
 

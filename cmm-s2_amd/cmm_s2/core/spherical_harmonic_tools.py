@@ -87,14 +87,17 @@ class SHT(object):
 
     def work(self, nf):
         """Device workspace for nf fields on the current stream (transforms
-        on different streams share the plan tables, never the workspace)."""
+        on different streams share the plan tables, never the workspace).
+        A buffer first needed while the stream is being captured into a HIP
+        graph belongs to that graph's memory pool and is not cached."""
         key = (nf, B.stream())
         t = self._work.get(key)
         if t is None:
             b = np.zeros(1, dtype=np.int64)
             N.call("cmm_sht_workspace_bytes", self._h, int(nf), N.lptr(b))
             t = B.empty(int(b[0]) // 8 + 1)
-            self._work[key] = t
+            if not torch.cuda.is_current_stream_capturing():
+                self._work[key] = t
         return t
 
     def inverse(self, flm, nf=None, out=None):

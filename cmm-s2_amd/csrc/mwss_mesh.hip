@@ -1144,9 +1144,11 @@ struct LevelBars {
   double* p[3];
 };
 
-// vel_locate for a point already located in triangle tri (the closed-form
-// vertices of tri are those mwss_locate builds: same barycentrics, bits
-// included, without the angles and cell search)
+// vel_locate for a point already located in triangle tri: the closed-form
+// vertices of tri are those mwss_locate builds, bit for bit, without the
+// angles and cell search; the barycentrics then agree with vel_locate's to
+// rounding (the compiler may fuse the Cramer products differently in the two
+// inlined contexts)
 __device__ __forceinline__ VLoc vel_locate_at(const cmm_mwss_mesh& M, d3 x, int64_t tri) {
   VLoc r;
   MLoc o;
@@ -1167,6 +1169,35 @@ __device__ __forceinline__ VLoc vel_locate_at(const cmm_mwss_mesh& M, d3 x, int6
   r.w[1] = scl(1 / D, cross(o.v[2], o.v[0]));
   r.w[2] = scl(1 / D, cross(o.v[0], o.v[1]));
   return r;
+}
+
+// split location of points on the velocity mesh: the full search, or from a
+// known triangle (the form the RK-feet adjoint uses for its stage points)
+__global__ __launch_bounds__(256) void k_mwss_split_locate(cmm_mwss_mesh M, const double* __restrict__ q, int64_t n,
+                                                           int64_t sp, int64_t sc, const int32_t* __restrict__ known,
+                                                           int32_t* __restrict__ tri, int32_t* __restrict__ sub,
+                                                           double* __restrict__ cb, double* __restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const d3 x = load_pt(q, i, sp, sc);
+  const int64_t k = known ? (int64_t)known[i] : -1;
+  const VLoc r = known ? (k >= 0 && k < M.Tu ? vel_locate_at(M, x, k) : VLoc{false, -1, 0}) : vel_locate(M, x);
+  tri[i] = r.ok ? (int32_t)r.tri : -1;
+  if (sub) sub[i] = r.ok ? r.s : -1;
+  const double nan = __builtin_nan("");
+  if (cb) {
+    cb[3 * i] = r.ok ? r.cb.x : nan;
+    cb[3 * i + 1] = r.ok ? r.cb.y : nan;
+    cb[3 * i + 2] = r.ok ? r.cb.z : nan;
+  }
+  if (w) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      w[9 * i + 3 * j] = r.ok ? r.w[j].x : nan;
+      w[9 * i + 3 * j + 1] = r.ok ? r.w[j].y : nan;
+      w[9 * i + 3 * j + 2] = r.ok ? r.w[j].z : nan;
+    }
+  }
 }
 
 // Adjoint of one velocity-spline evaluation v = sum_k w_k PS_k(x) of stage
@@ -1410,6 +1441,17 @@ extern "C" int cmm_mwss_locate(const cmm_mwss_mesh* m, const double* q, int64_t 
   hipLaunchKernelGGL(k_mwss_locate, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, q, n,
                      sp, sc, tri, bcc);
   CMM_LAUNCH_CHECK("cmm_mwss_locate");
+  return CMM_OK;
+}
+
+extern "C" int cmm_mwss_split_locate(const cmm_mwss_mesh* m, const double* q, int64_t n, int64_t sp, int64_t sc,
+                                     const int32_t* known, int32_t* tri, int32_t* sub, double* cb, double* w,
+                                     void* stream) {
+  if (!m || n < 0 || (n > 0 && (!q || !tri))) return fail(CMM_E_ARG, "cmm_mwss_split_locate: bad arguments");
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_mwss_split_locate, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, q, n, sp,
+                     sc, known, tri, sub, cb, w);
+  CMM_LAUNCH_CHECK("cmm_mwss_split_locate");
   return CMM_OK;
 }
 

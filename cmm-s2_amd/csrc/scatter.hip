@@ -10,6 +10,10 @@ namespace cmm {
 namespace {
 inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
+// hipCUB takes the bin count (key_range + 1) as an int: the largest key
+// range whose bin count is still a positive int
+constexpr int64_t kMaxKeyRange = (int64_t)0x7fffffff - 1;
+
 // key k has 1 .. split records / more than split records
 struct ShortGroup {
   const uint32_t* offs;
@@ -46,7 +50,7 @@ size_t scan_temp_bytes(int64_t bins) {
 }  // namespace
 
 int64_t scatter_bytes(int64_t n, int64_t key_range) {
-  if (n < 1 || n > (int64_t)0x7fffffff || key_range < 1 || key_range >= (int64_t)kScatterEmpty) return 0;
+  if (n < 1 || n > (int64_t)0x7fffffff || key_range < 1 || key_range > kMaxKeyRange) return 0;
   const int64_t bins = key_range + 1;
   const size_t tb = scan_temp_bytes(bins);
   if (!tb) return 0;
@@ -54,8 +58,8 @@ int64_t scatter_bytes(int64_t n, int64_t key_range) {
 }
 
 int scatter_layout(void* work, int64_t work_bytes, int64_t n, int64_t key_range, ScatterWork& w) {
-  if (key_range < 1 || key_range >= (int64_t)kScatterEmpty)
-    return fail(CMM_E_RANGE, "scatter workspace: keys do not fit 32 bits");
+  if (key_range < 1 || key_range > kMaxKeyRange)
+    return fail(CMM_E_RANGE, "scatter workspace: key range above 2^31 - 2");
   const int64_t need = scatter_bytes(n, key_range);
   if (!need) return fail(CMM_E_RANGE, "scatter workspace: bad record count");
   if (!work || work_bytes < need)
@@ -253,7 +257,7 @@ int scatter_reduce(const ScatterWork& w, int64_t n, const ScatterTargets& t, hip
 
 extern "C" int cmm_scatter_workspace_bytes(int64_t records, int64_t key_range, int64_t* bytes) {
   if (!bytes || records < 1 || key_range < 1) return cmm::fail(CMM_E_ARG, "cmm_scatter_workspace_bytes: bad arguments");
-  if (records > (int64_t)0x7fffffff || key_range >= (int64_t)cmm::kScatterEmpty)
+  if (records > (int64_t)0x7fffffff || key_range > cmm::kMaxKeyRange)
     return cmm::fail(CMM_E_RANGE, "cmm_scatter_workspace_bytes: record count or key range out of range");
   const int64_t b = cmm::scatter_bytes(records, key_range);
   if (!b) return cmm::fail(CMM_E_HIP, "cmm_scatter_workspace_bytes: hipCUB temporary storage query failed");

@@ -152,6 +152,20 @@ int cmm_mwss_locate(const cmm_mwss_mesh* mesh, const double* q, int64_t n,
                     int64_t q_stride_pt, int64_t q_stride_comp, int32_t* tri,
                     double* bcc, void* stream);
 
+/* The location step of spline_interp_velocity.__call__ (spherical_spline.py
+ * :570-615 with bary_minmax / new_vpoints, 660-697): tri (n) int32 (-1 on a
+ * miss), and where not NULL sub (n) int32 the split sub-triangle, cb (n,3)
+ * its barycentrics, w (n,9) the triangle's dual basis w_0, w_1, w_2
+ * (homogeneous barycentrics b_i = q . w_i).  known != NULL (device int32
+ * (n)): point i is taken to lie in triangle known[i] and its vertices are
+ * built in closed form without the angle / cell search (what the RK-feet
+ * adjoint does for the stage points it recomputes); when known[i] is the
+ * triangle the full search finds, the triangle and sub-triangle are the
+ * search's and cb, w agree with it to rounding. */
+int cmm_mwss_split_locate(const cmm_mwss_mesh* mesh, const double* q, int64_t n, int64_t q_stride_pt,
+                          int64_t q_stride_comp, const int32_t* known, int32_t* tri, int32_t* sub,
+                          double* cb, double* w, void* stream);
+
 /* assemble_coefficients on the MWSS mesh; jets (ncomp,4,Nu). */
 int cmm_mwss_assemble(const cmm_mwss_mesh* mesh, const double* vals, int ncomp,
                       double* coeffs, double* packed, void* stream);

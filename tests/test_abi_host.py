@@ -52,6 +52,8 @@ def test_argument_errors_without_gpu():
     assert lib.cmm_scatter_workspace_bytes(0, 10, ctypes.byref(nb)) == -1
     assert lib.cmm_scatter_workspace_bytes(1 << 31, 10, ctypes.byref(nb)) == -3
     assert lib.cmm_scatter_workspace_bytes(10, 1 << 32, ctypes.byref(nb)) == -3
+    # hipCUB counts key_range + 1 bins in an int: 2^31 - 1 keys would wrap it
+    assert lib.cmm_scatter_workspace_bytes(10, (1 << 31) - 1, ctypes.byref(nb)) == -3
     vel.kind = N.VEL_SPLINE
     assert lib.cmm_advect_stencil_adjoint_corners(ctypes.byref(vel), N.INT_RK4, 0.1, None, 0, 0, None, 0, None,
                                                   None, None, 0, None, 0, None) == -1

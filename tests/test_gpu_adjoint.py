@@ -527,8 +527,27 @@ def test_autograd_function():
     J = misfit(p, prob)
     (3.0 * J).backward()
     J2, g2 = prob.value_and_grad(p.detach())
-    assert abs(float(J) - J2) <= 1e-14 * J2
+    assert abs(float(J.detach()) - J2) <= 1e-14 * J2
     assert torch.allclose(p.grad, 3.0 * g2, rtol=1e-10, atol=0)
+
+
+def test_autograd_function_batch():
+    """Two misfit() calls on one problem before a single .backward() (a batch
+    of initial fields summed into one loss): each call reverses its own run,
+    so the gradients equal the separate value_and_grad gradients, whichever
+    forward ran last."""
+    from cmm_s2.core import dynamical_fields as df
+    from cmm_s2.core.adjoint import misfit
+    prob = _small_problem()
+    ps = [torch.from_numpy(prob.params.from_coeffs(df.random_sh_coefficients(s, 8, 32))).cuda().requires_grad_()
+          for s in (501, 502)]
+    loss = misfit(ps[0], prob) + 2.0 * misfit(ps[1], prob)
+    loss.backward()
+    refs = [prob.value_and_grad(p.detach()) for p in ps]
+    assert abs(float(loss.detach()) - (refs[0][0] + 2.0 * refs[1][0])) <= 1e-13 * float(loss.detach())
+    assert float((refs[0][1] - refs[1][1]).norm()) > 1e-3 * float(refs[0][1].norm())  # distinct gradients
+    assert torch.allclose(ps[0].grad, refs[0][1], rtol=1e-10, atol=0)
+    assert torch.allclose(ps[1].grad, 2.0 * refs[1][1], rtol=1e-10, atol=0)
 
 
 def test_gradient_matches_torch_autograd():

@@ -56,6 +56,66 @@ def test_mwss_locate(L):
     assert np.abs(b[same] - bo[same]).max() < 1e-12
 
 
+def test_split_locate_from_known_triangle():
+    """The RK-feet adjoint rebuilds a stage point's triangle in closed form
+    from the triangle the forward recomputation found (no angle / cell
+    search).  Through the C ABI, on stencil quads (eps = 1e-5) of seeded
+    points next to rings and meridians and in both polar caps, at L = 32:
+    the split location from the known triangle gives the full search's
+    triangle and sub-triangle, and its sub-triangle barycentrics and dual
+    basis to rounding (the two inlined contexts may fuse the Cramer products
+    differently)."""
+    from cmm_s2 import _native as N
+    from cmm_s2.core import _backend as B, mesh_functions as mf
+    L = 32
+    pm = mf.structure_spherical_triangulation(L)
+    rng = np.random.default_rng(32)
+    base = _grid_and_ring_points(L, 7)[::4]
+    caps = np.concatenate([rng.uniform(0, np.pi / L, 600), np.pi - rng.uniform(0, np.pi / L, 600)])
+    ph = rng.uniform(0, 2 * np.pi, caps.size)
+    base = np.concatenate([base, np.stack([np.sin(caps) * np.cos(ph), np.sin(caps) * np.sin(ph), np.cos(caps)], 1)])
+    a1 = np.cross(base, [0.3, -0.5, 0.8])
+    a1 /= np.linalg.norm(a1, axis=1)[:, None]
+    a2 = np.cross(base, a1)
+    eps = 1e-5
+    q = np.concatenate([base + sa * eps * a1 + sb * eps * a2 for sa in (-1, 1) for sb in (-1, 1)])
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    n = q.shape[0]
+    qd = torch.from_numpy(q).cuda()
+
+    def run(known):
+        tri = torch.empty(n, dtype=torch.int32, device="cuda")
+        sub = torch.empty(n, dtype=torch.int32, device="cuda")
+        cb, w = B.empty(n, 3), B.empty(n, 9)
+        N.call("cmm_mwss_split_locate", pm._h, qd.data_ptr(), n, 3, 1, None if known is None else known.data_ptr(),
+               tri.data_ptr(), sub.data_ptr(), cb.data_ptr(), w.data_ptr(), B.stream())
+        return tri, sub, cb, w
+
+    t0, s0, c0, w0 = run(None)
+    t1, s1, c1, w1 = run(t0)
+    tl, _ = pm.locate(qd, "N3")
+    assert torch.equal(t0, tl.to(torch.int32)) and int((t0 < 0).sum()) == 0
+    tri = t0.cpu().numpy()
+    Tu = 4 * L * (L - 1)
+    assert (tri < 2 * L).sum() > 100 and (tri >= Tu - 2 * L).sum() > 100  # both polar caps
+    assert ((tri >= 2 * L) & (tri < Tu - 2 * L)).sum() > 1000
+    assert len(set(s0.cpu().numpy().tolist())) == 6
+    assert torch.equal(t0, t1)
+    # the sub-triangle is (argmin, argmax) of the barycentrics: it may only
+    # differ where two of them tie to rounding
+    c0h, c1h = c0.cpu().numpy(), c1.cpu().numpy()
+    diff = (s0 != s1).cpu().numpy()
+    if diff.any():
+        b, _, _ = pm.query(q[diff])
+        bs = np.sort(b, axis=1)
+        assert (np.minimum(bs[:, 1] - bs[:, 0], bs[:, 2] - bs[:, 1]) < 1e-13).all()
+    same = ~diff
+    assert diff.sum() <= 8
+    assert np.abs(c0h[same] - c1h[same]).max() < 1e-13
+    w0h, w1h = w0.cpu().numpy(), w1.cpu().numpy()
+    assert np.abs(w0h - w1h).max() < 1e-13 * np.abs(w0h).max()
+
+
 def test_velocity_spline_eval():
     from cmm_s2.core import mesh_functions as mf
     from cmm_s2.core.spherical_spline import spline_interp_velocity
@@ -118,6 +178,33 @@ def test_sht_inverse_forward(L):
                 i = m * L - m * (m - 1) // 2 + (l - m)
                 ref_p[i] = [fo[fi, l, m].real, fo[fi, l, m].imag]
         assert rel_err(fg[:, fi], ref_p) < 1e-12
+
+
+@pytest.mark.parametrize("L", [16, 24, 48])
+def test_sht_forward_non_band_limited(L):
+    """The device forward transform on input that is not band-limited: pure
+    theta-Nyquist rings cos(L theta) cos(m phi), odd orders sin(k theta)
+    cos(m phi) and steep smooth fields, against the exact projection of
+    their split-Nyquist trigonometric interpolant computed independently
+    (numpy FFTs, 4L-node Gauss-Legendre quadrature, scipy sph_harm_y)."""
+    from cmm_s2.core.spherical_harmonic_tools import SHT, nlm
+    from helpers import interpolant_projection, mwss_samples
+    plan = SHT.get(L)
+    funs = [lambda T, P: np.cos(L * T) * np.cos(2 * P), lambda T, P: np.cos(L * T),
+            lambda T, P: np.sin((L - 1) * T) * np.cos(3 * P),
+            lambda T, P: np.tanh(8 * (np.cos(T) - 0.3)) * (1 + 0.2 * np.sin(T) * np.cos(3 * P)),
+            lambda T, P: np.exp(-((np.sin(T) * np.cos(P) - 1) ** 2 + (np.sin(T) * np.sin(P)) ** 2
+                                  + np.cos(T) ** 2) * 16)]
+    f = np.stack([mwss_samples(fn, L) for fn in funs])
+    got = plan.forward(torch.from_numpy(f).cuda()).cpu().numpy()
+    for fi in range(len(funs)):
+        ref = interpolant_projection(f[fi], L)
+        ref_p = np.zeros((nlm(L), 2))
+        for m in range(L):
+            for l in range(m, L):
+                i = m * L - m * (m - 1) // 2 + (l - m)
+                ref_p[i] = [ref[l, m].real, ref[l, m].imag]
+        assert np.abs(got[:, fi] - ref_p).max() < 1e-12 * np.abs(ref_p).max(), fi
 
 
 def test_project_stream_matches_oracle():

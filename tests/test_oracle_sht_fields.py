@@ -211,3 +211,101 @@ def test_enstrophy_energy_closed_forms():
     f = osht.inverse_lm(flm_all, L)
     assert abs(osht.enstrophy(f, L) - tot_e) < 1e-13 * tot_e
     assert abs(osht.energy(f, L) - tot_n) < 1e-13 * tot_n
+
+
+# ----------------------------------------------------------------------------
+# forward transform on NON-band-limited input: the Nyquist term and the theta
+# extension, pinned against the definition computed independently (numpy
+# FFTs, 4L-node Gauss-Legendre quadrature, scipy's sph_harm_y) and against
+# closed forms
+# ----------------------------------------------------------------------------
+def _lm_of(fun, L):
+    from helpers import mwss_samples
+    return osh.forward_lm(mwss_samples(fun, L), L)
+
+
+def _projection_of_function(fun_theta, m, L, nodes):
+    """2 pi int_0^pi fun_theta(theta) Pbar_lm(cos theta) sin theta dtheta for
+    l = m..L-1, by Gauss-Legendre quadrature in cos theta (exact for the
+    polynomial integrands used below)."""
+    x, w = np.polynomial.legendre.leggauss(nodes)
+    th = np.arccos(x)
+    ls = np.arange(m, L)
+    P = sph_harm_y(ls[:, None], m, th[None, :], 0.0).real
+    return 2 * np.pi * (P * (w * fun_theta(th))[None, :]).sum(axis=1)
+
+
+@pytest.mark.parametrize("L", [16, 24])
+def test_forward_nyquist_term_closed_form(L):
+    """cos(L theta) cos(m phi) (even m) samples to (-1)^t cos(m phi): pure
+    Nyquist in theta.  Split as cos(L theta) it is reproduced exactly, so
+    the transform must return the exact projection of cos(L theta) cos(m phi)
+    onto l < L (T_L(x) Pbar_lm(x) has degree <= 2L - 1: quadrature exact).
+    A dropped, doubled or one-sided Nyquist term fails."""
+    for m in (0, 2, 6, L - 2):
+        a = _lm_of(lambda T, P: np.cos(L * T) * np.cos(m * P), L)
+        ref = _projection_of_function(lambda th: np.cos(L * th), m, L, 4 * L) * (1.0 if m == 0 else 0.5)
+        assert m > 6 or np.abs(ref).max() > 0.05
+        assert np.abs(a[m:, m] - ref).max() < 1e-13, m
+        others = np.delete(a, m, axis=1)
+        assert np.abs(others).max() < 1e-13
+
+
+@pytest.mark.parametrize("L", [16, 24])
+def test_forward_odd_order_extension_closed_form(L):
+    """sin(k theta) cos(m phi) with odd m: the odd extension
+    F_m(2 pi - theta) = -F_m(theta) reproduces sin(k theta) (k < L) exactly;
+    for m >= 3 the field is not band-limited (it vanishes like theta, not
+    theta^m, at the poles).  The transform must return its exact projection;
+    an even extension for odd m fails."""
+    for m in (1, 3, 5):
+        for k in (L - 1, L // 2 + 1):
+            a = _lm_of(lambda T, P: np.sin(k * T) * np.cos(m * P), L)
+            ref = 0.5 * _projection_of_function(lambda th: np.sin(k * th), m, L, 4 * L)
+            assert np.abs(ref).max() > 1e-3
+            assert np.abs(a[m:, m] - ref).max() < 1e-13, (m, k)
+
+
+def _smooth_fields():
+    return {"exp": lambda T, P: np.exp(np.sin(T) * np.cos(P) + 0.5 * np.sin(T) * np.sin(P) - 0.3 * np.cos(T)),
+            "bump": lambda T, P: 4 * np.pi * np.exp(-((np.sin(T) * np.cos(P) - 1) ** 2 + (np.sin(T) * np.sin(P)) ** 2
+                                                     + np.cos(T) ** 2) / (1 / 16)),
+            "cap": lambda T, P: np.tanh(8 * (np.cos(T) - 0.3)) * (1 + 0.2 * np.sin(T) * np.cos(3 * P))}
+
+
+@pytest.mark.parametrize("L", [16, 24, 32])
+def test_forward_non_band_limited_matches_definition(L):
+    """Smooth fields that are not band-limited (functions on the sphere: pole
+    rings constant): forward_lm equals the exact projection of the field's
+    trigonometric interpolant with the split Nyquist term, computed
+    independently; the same interpolant with a one-sided Nyquist term
+    differs by far more than the tolerance (the check discriminates)."""
+    from helpers import interpolant_projection, mwss_samples
+    sens = {}
+    for name, fun in _smooth_fields().items():
+        f = mwss_samples(fun, L)
+        a = osh.forward_lm(f, L)
+        ref = interpolant_projection(f, L)
+        scale = np.abs(ref).max()
+        assert np.abs(a - ref).max() < 1e-12 * scale, name
+        wrong = interpolant_projection(f, L, split_nyquist=False)
+        sens[name] = np.abs(wrong - ref).max() / scale
+    # the entire function "exp" has no Nyquist content at these L; the
+    # steep fields do
+    assert sens["cap"] > 1e-6 and sens["bump"] > 1e-9, sens
+
+
+def test_theta_interpolant_split_matches_scipy_resample():
+    """The split-Nyquist interpolant is the band-limited resampling of
+    scipy.signal.resample (which splits an even-length signal's Nyquist
+    component when upsampling): both evaluated on an 8L-point circle."""
+    from scipy.signal import resample
+    from helpers import mwss_samples, theta_interpolant
+    L = 16
+    f = mwss_samples(_smooth_fields()["cap"], L)
+    th = 2 * np.pi * np.arange(8 * L) / (8 * L)
+    G, s = theta_interpolant(f, L, th)
+    R = resample(s, 8 * L, axis=0)
+    assert np.abs(G - R).max() < 1e-13 * np.abs(R).max()
+    G1, _ = theta_interpolant(f, L, th, split_nyquist=False)
+    assert np.abs(G1 - R).max() > 1e-6 * np.abs(R).max()
