@@ -20,11 +20,14 @@ P_dbl = C.POINTER(C.c_double)
 P_i64 = C.POINTER(C.c_int64)
 
 VEL_SPLINE, VEL_SBR, VEL_DEFORM_ROT, VEL_RH4 = 0, 1, 2, 3
+VEL_DEFORM, VEL_STATIC_VORTEX, VEL_ROTATING_VORTEX, VEL_EQUATOR_VORTEX, VEL_DIV = 4, 5, 6, 7, 8
 INT_EULER, INT_IMPROVED_EULER, INT_RK4, INT_RK2, INT_EVAL = 0, 1, 2, 3, 4
 LEGENDRE_FP64, LEGENDRE_FP32 = 0, 1
 LEGENDRE_BF16X3, LEGENDRE_BF16X4, LEGENDRE_BF16X5, LEGENDRE_BF16X6, LEGENDRE_BF16X7 = 2, 3, 4, 5, 6
 VORT_SPLINE, VORT_ROSSBY, VORT_ZONAL_JET, VORT_GAUSSIAN = 0, 1, 2, 3
 VORT_PERTURBED_JET, VORT_MULTI_JET, VORT_PERTURBED_ROSSBY = 4, 5, 6
+VORT_COSINE_BELL, VORT_SLOTTED_CYLINDER, VORT_STATIC_VORTEX_TRACER, VORT_ROTATING_VORTEX_TRACER = 7, 8, 9, 10
+VORT_SH_SERIES = 11
 
 
 class Velocity(C.Structure):

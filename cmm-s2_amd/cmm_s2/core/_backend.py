@@ -57,3 +57,29 @@ def point_strides(t, layout):
     if layout == "N3":
         return t.stride(0), t.stride(1)
     return t.stride(1), t.stride(0)
+
+
+_host_only = set()
+
+
+def call_plugin(fn, *args):
+    """fn(*args) for a caller-provided field (tensor arguments on the
+    device): called on the device tensors first; a callable that fails there
+    or returns no tensor (the reference's fields use np.* functions) is
+    evaluated from then on with host NumPy copies of the tensor arguments
+    (warned once), the result moved back to the device as float64."""
+    dev = next(a.device for a in args if isinstance(a, torch.Tensor))
+    if id(fn) not in _host_only:
+        try:
+            r = fn(*args)
+        except Exception:
+            r = None
+        if isinstance(r, torch.Tensor):
+            return r.to(device=dev, dtype=torch.float64)
+        import warnings
+        warnings.warn(f"{getattr(fn, '__name__', fn)!r} does not take device tensors: it is evaluated on the "
+                      "host (NumPy) with a device round trip per call", RuntimeWarning, stacklevel=3)
+        _host_only.add(id(fn))
+    host = [a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else a for a in args]
+    r = np.asarray(fn(*host), dtype=np.float64)
+    return torch.from_numpy(np.ascontiguousarray(r)).to(dev)

@@ -31,7 +31,12 @@ class DeviceVelocity(object):
 
 
 u_sbr = DeviceVelocity("u_sbr", N.VEL_SBR, [1.0, 0.0])                 # dynamical_fields.py:18-22
+u_deform = DeviceVelocity("u_deform", N.VEL_DEFORM, [1.0])              # 24-30
 u_deform_rot = DeviceVelocity("u_deform_rot", N.VEL_DEFORM_ROT, [1.0, 0.0])  # 32-42
+u_static_vortex = DeviceVelocity("u_static_vortex", N.VEL_STATIC_VORTEX, [])      # 57-66
+u_rotating_vortex = DeviceVelocity("u_rotating_vortex", N.VEL_ROTATING_VORTEX, [1.0])  # 68-78
+equator_vortex = DeviceVelocity("equator_vortex", N.VEL_EQUATOR_VORTEX, [])      # 80-85
+u_div = DeviceVelocity("u_div", N.VEL_DIV, [1.0])                       # 99-108
 u_rh4 = DeviceVelocity("u_rh4", N.VEL_RH4, [RH4_OMEGA, RH4_K, 2 * pi])
 
 
@@ -41,17 +46,32 @@ def rh4_nu(w=RH4_OMEGA, R=4, Omega=2 * pi):
 
 
 class DeviceVorticity(object):
-    def __init__(self, name, kind, params=(0.0, 0.0, 0.0, 0.0)):
+    """An analytic field w(phi, theta) evaluated by a HIP kernel (kind id +
+    4 parameters; a spherical-harmonic series also carries its coefficient
+    table)."""
+
+    def __init__(self, name, kind, params=(0.0, 0.0, 0.0, 0.0), table=None):
         self.__name__ = name
         self._cmm_kind = kind
         self._cmm_params = list(params) + [0.0] * (4 - len(params))
+        self._table_host = table
+        self._table = None
+
+    @property
+    def _cmm_table(self):
+        """device coefficient table (series kinds), uploaded on first use"""
+        if self._table_host is None:
+            return None
+        if self._table is None or self._table.device != B.device():
+            self._table = B.to_dev(self._table_host)
+        return self._table
 
     def at(self, **kw):
         """Copy with parameters (e.g. rossby_wave.at(t=1.0))."""
         p = list(self._cmm_params)
         if "t" in kw:
             p[0] = kw["t"]
-        return DeviceVorticity(self.__name__, self._cmm_kind, p)
+        return DeviceVorticity(self.__name__, self._cmm_kind, p, self._table_host)
 
     def sample_xyz(self, X, out=None):
         """w0 at device points X (N,3) (no rotation term)."""
@@ -59,7 +79,7 @@ class DeviceVorticity(object):
         if out is None:
             out = B.empty(n)
         zero = B.zeros(1)
-        N.call("cmm_sample_vorticity", self._cmm_kind, None, None, N.dbl_array(self._cmm_params),
+        N.call("cmm_sample_vorticity", self._cmm_kind, None, B.ptr(self._cmm_table), N.dbl_array(self._cmm_params),
                X.data_ptr(), n, X.stride(0), X.stride(1), zero.data_ptr(), n, 0.0, out.data_ptr(), B.stream())
         return out
 
@@ -76,6 +96,78 @@ perturbed_zonal_jet = DeviceVorticity("perturbed_zonal_jet", N.VORT_PERTURBED_JE
 multi_jet = DeviceVorticity("multi_jet", N.VORT_MULTI_JET)            # 195-198
 gaussian_vortex = DeviceVorticity("gaussian_vortex", N.VORT_GAUSSIAN) # 200-203
 perturbed_rossby_wave = DeviceVorticity("perturbed_rossby_wave", N.VORT_PERTURBED_ROSSBY)  # 213-214
+
+
+cosine_bell = DeviceVorticity("cosine_bell", N.VORT_COSINE_BELL)      # dynamical_fields.py:123-138
+IC_SC = DeviceVorticity("IC_SC", N.VORT_SLOTTED_CYLINDER)             # 140-156
+
+
+class DeviceTracer(object):
+    """Tracer field f(t, xyz (3,N)) -> (N,) of dynamical_fields.py (exact
+    solutions of the vortex flows), evaluated on the GPU; ``at(t)`` is the
+    field at time t as a sampler of composed-map positions."""
+
+    def __init__(self, name, kind):
+        self.__name__ = name
+        self._kind = kind
+
+    def at(self, t):
+        return DeviceVorticity(self.__name__, self._kind, [float(t)])
+
+    def __call__(self, t, xyz):
+        """xyz (3, N) as in the reference -> (N,)."""
+        X = B.to_dev(xyz).reshape(3, -1).t().contiguous()
+        return B.like(self.at(t).sample_xyz(X), xyz)
+
+
+static_vort_IC = DeviceTracer("static_vort_IC", N.VORT_STATIC_VORTEX_TRACER)        # 87-90
+rotating_vort_IC = DeviceTracer("rotating_vort_IC", N.VORT_ROTATING_VORTEX_TRACER)  # 92-97
+
+
+def random_ic_table():
+    """dynamical_fields.py:159-171 as real-field coefficients: np.random.seed(42)
+    draws c = uniform(-1, 1, 33^2) consumed in the order l = 0..32,
+    m = -l..l-1 (the reference's range stops before m = l) as
+    f = sum c_lm Re Y_l^m; with Re Y_l^-m = (-1)^m Re Y_lm and
+    Re Y_lm = (Y_lm + conj Y_lm) / 2 that is a_l0 = c_l0 and
+    a_lm = (c_lm + (-1)^m c_l,-m) / 2 for m > 0 ([m][l] complex (33, 33))."""
+    c = np.random.RandomState(42).uniform(-1, 1, 33 ** 2)
+    d = np.zeros((33, 33))  # [l, m], coefficient of Re Y_lm (m >= 0)
+    lm = 0
+    for l in range(33):
+        for m in range(-l, l):
+            if m >= 0:
+                d[l, m] += c[lm]
+            else:
+                d[l, -m] += (-1.0) ** (-m) * c[lm]
+            lm += 1
+    tab = np.zeros((33, 33, 2))
+    for m in range(33):
+        tab[m, :, 0] = d[:, m] * (1.0 if m == 0 else 0.5)
+    return tab
+
+
+class _SeriesField(DeviceVorticity):
+    def __call__(self, xyz):
+        """xyz (3, N) as in the reference -> (N,)."""
+        X = B.to_dev(xyz).reshape(3, -1).t().contiguous()
+        return B.like(self.sample_xyz(X), xyz)
+
+
+random_initial_condition = _SeriesField("random_initial_condition", N.VORT_SH_SERIES, [33.0],
+                                        table=random_ic_table())
+
+
+def sh_series(a, name="sh_series"):
+    """A real field from its coefficients a [l, m >= 0] complex (Ls, Ls)
+    (the layout of random_sh_coefficients), evaluated exactly at any point."""
+    a = np.asarray(a)
+    Ls = a.shape[0]
+    tab = np.zeros((Ls, Ls, 2))
+    for m in range(Ls):
+        tab[m, :, 0] = a[:, m].real
+        tab[m, :, 1] = a[:, m].imag
+    return DeviceVorticity(name, N.VORT_SH_SERIES, [float(Ls)], table=tab)
 
 
 def random_sh_coefficients(seed, lmax, L):

@@ -5,8 +5,10 @@ The integrators keep the reference signature ``integrator(t, dt, xn, V)``
 (evolution_functions.py:19-80).  When V is a ``spline_interp_velocity`` or
 one of the analytic fields of ``dynamical_fields`` the whole multi-stage step
 (every stage's velocity location + time-interpolated evaluation +
-projection) is ONE kernel launch per point set; other callables are not
-supported on the device path and raise.
+projection) is ONE kernel launch per point set.  Any other callable
+V(t, dt, X (3,N)) -> (3,N) (a caller's own field) is integrated stage by
+stage on the device with the same formulas, V called on device tensors or,
+when it only takes NumPy, on host copies.
 """
 import numpy as np
 import torch
@@ -49,11 +51,46 @@ def velocity_descriptor(V, stage_times, dt):
     return d
 
 
+def is_device_velocity(V):
+    return isinstance(V, spline_interp_velocity) or hasattr(V, "_cmm_kind")
+
+
+def _norm(y):
+    """div_norm (evolution_functions.py:82-85) on a (3, M) device tensor."""
+    return (1 / torch.sqrt(y[0] ** 2 + y[1] ** 2 + y[2] ** 2))[None, :] * y
+
+
+def integrate_plugin(kind, t, dt, x, V):
+    """evolution_functions.py:19-80 for a caller's velocity callable V(t, dt,
+    X (3,M)): the stages of integrator ``kind`` on device tensors."""
+    f = lambda tt, y: B.call_plugin(V, tt, dt, y).reshape(3, -1)
+    if kind == N.INT_EVAL:
+        return f(t, x)
+    if kind == N.INT_EULER:
+        return _norm(x - dt * f(t, x))
+    if kind == N.INT_IMPROVED_EULER:
+        v1 = f(t, x)
+        v2 = f(t + dt, _norm(x - dt * v1))
+        return _norm(x - (dt / 2) * (v1 + v2))
+    if kind == N.INT_RK2:
+        v1 = f(t + dt, x)
+        v2 = f(t + dt / 2, _norm(x - (dt / 2) * v1))
+        return _norm(x - dt * v2)
+    v1 = f(t + dt, x)
+    v2 = f(t + dt / 2, _norm(x - dt * v1 / 2))
+    v3 = f(t + 0.5 * dt, _norm(x - (dt / 2) * v2))
+    v4 = f(t, _norm(x - dt * v3))
+    return _norm(x - (dt / 6) * (v1 + 2 * v2 + 2 * v3 + v4))
+
+
 def integrate(kind, t, dt, x, V, out=None):
     """One backward step of integrator ``kind`` for points x (3,M) on device."""
     m = x.shape[1]
     if out is None:
         out = B.empty(3, m)
+    if not is_device_velocity(V):
+        out.copy_(integrate_plugin(kind, t, dt, x, V))
+        return out
     desc = velocity_descriptor(V, _STAGES[kind](t, dt), dt)
     N.call("cmm_advect_points", desc, kind, float(dt), x.data_ptr(), m, out.data_ptr(), B.stream())
     return out
@@ -118,7 +155,12 @@ def _work(mesh):
 
 def _feet(interp, integrator, t, dt, V, identity, w):
     mesh = interp.mesh
-    if integrator in _KIND:
+    if integrator in _KIND and not is_device_velocity(V):
+        kind = _KIND[integrator]
+        if not identity:
+            w.yv.copy_(integrate_plugin(kind, t, dt, mesh.d_verts3, V))
+        w.ys.copy_(integrate_plugin(kind, t, dt, mesh.d_spts, V))
+    elif integrator in _KIND:
         kind = _KIND[integrator]
         desc = velocity_descriptor(V, _STAGES[kind](t, dt), dt)
         xv, xs = mesh.d_verts3, mesh.d_spts

@@ -45,11 +45,17 @@ class VorticitySampler(object):
             self.params = [0.0] * 4
             self._keep = vorticity
         elif hasattr(vorticity, "_cmm_kind") and hasattr(vorticity, "sample_xyz"):
-            self.kind, self.wmesh, self.wpacked = vorticity._cmm_kind, None, None
+            self.kind, self.wmesh = vorticity._cmm_kind, None
+            self.wpacked = getattr(vorticity, "_cmm_table", None)  # series kinds: coefficient table
             self.params = list(vorticity._cmm_params)
+            self._keep = vorticity
+        elif callable(vorticity):
+            # a caller's own w(phi, theta) (dynamical_fields.py:176-214 style)
+            self.kind, self.wmesh, self.wpacked, self.params = None, None, None, [0.0] * 4
+            self._fn = vorticity
         else:
-            raise N.CmmError("vorticity must be a spherical_spline on an MWSS mesh or a "
-                             "cmm_s2.core.dynamical_fields vorticity")
+            raise N.CmmError("vorticity must be a spherical_spline on an MWSS mesh, a "
+                             "cmm_s2.core.dynamical_fields vorticity or a callable w(phi, theta)")
 
     def __call__(self, X, out=None, rot=None, cos_grid=None, ring_len=None):
         n = X.shape[0]
@@ -57,9 +63,24 @@ class VorticitySampler(object):
             out = B.empty(n)
         cg = self.grid.d_cos_theta if cos_grid is None else cos_grid
         rl = 2 * self.grid.L if ring_len is None else ring_len
+        rot = float(self.rot if rot is None else rot)
+        if self.kind is None:
+            return self._call_plugin(X, out, rot, cg, int(rl))
         N.call("cmm_sample_vorticity", self.kind, self.wmesh, B.ptr(self.wpacked),
                N.dbl_array(self.params), X.data_ptr(), n, X.stride(0), X.stride(1), cg.data_ptr(),
-               int(rl), float(self.rot if rot is None else rot), out.data_ptr(), B.stream())
+               int(rl), rot, out.data_ptr(), B.stream())
+        return out
+
+    def _call_plugin(self, X, out, rot, cos_grid, ring_len):
+        """w(phi, theta) + 2 rot (cos theta(X) - cos theta_ring) for a caller's
+        callable: phi, theta of X on the device (utils.cart2sphere), w called
+        on device tensors, or on host arrays when it only takes NumPy (the
+        reference's fields use np.* functions)."""
+        from . import utils
+        phi, theta = utils.cart2sphere(X.t())
+        w = B.call_plugin(self._fn, phi, theta)
+        ring = torch.arange(X.shape[0], device=X.device) // ring_len
+        out.copy_(w.reshape(-1) + 2 * rot * torch.cos(theta) - 2 * rot * cos_grid[ring])
         return out
 
 

@@ -292,6 +292,74 @@ __device__ inline d3 vel_rh4(double t, d3 X, double w, double K, double Om) {
   return d3{y * pz - z * py, z * px - x * pz, x * py - y * px};
 }
 
+// utils.cart2sphere (utils.py:18-23)
+__device__ inline void cart2sphere(d3 X, double& phi, double& theta) {
+  phi = npy_mod(atan2(X.y, X.x) + 2 * M_PI, 2 * M_PI);
+  theta = atan2(sqrt(X.y * X.y + X.x * X.x), X.z);
+}
+
+// dynamical_fields.u_deform (dynamical_fields.py:24-30)
+__device__ inline d3 vel_deform(double t, d3 X, double T) {
+  const double c1 = 2 * 2 * cos(M_PI * t / T);
+  return cross(X, d3{0.0, c1 * X.y, 0.0});
+}
+
+// dynamical_fields.omega_r (dynamical_fields.py:48-55), rho_0 = 3
+__device__ inline double vortex_omega(double theta) {
+  if (theta == 0.0) return 0.0;
+  const double r = 3.0 * sin(theta), ch = cosh(r);
+  return (2 * M_PI) * sqrt(3.0) * 1.5 * (1 / r) * tanh(r) * (1 / (ch * ch));
+}
+
+// dynamical_fields.u_static_vortex (dynamical_fields.py:57-66)
+__device__ inline d3 vel_static_vortex(d3 X) {
+  double phi, theta;
+  cart2sphere(X, phi, theta);
+  const double w = vortex_omega(theta);
+  return d3{-X.y * w, X.x * w, 0.0};
+}
+
+// dynamical_fields.u_rotating_vortex (dynamical_fields.py:68-78): the static
+// vortex in the frame Rot_x(pi/2) Rot_z(tau)^T, tau = 2 pi t / T, plus the
+// solid-body rotation
+__device__ inline d3 vel_rotating_vortex(double t, d3 X, double T) {
+  const double tau = 2 * M_PI * t / T, ct = cos(tau), st = sin(tau);
+  const double c = cos(M_PI / 2), s = sin(M_PI / 2);
+  const double Rx[3][3] = {{1, 0, 0}, {0, c, -s}, {0, s, c}};
+  const double RzT[3][3] = {{ct, st, 0}, {-st, ct, 0}, {0, 0, 1}};
+  const double Rz[3][3] = {{ct, -st, 0}, {st, ct, 0}, {0, 0, 1}};
+  const double RxT[3][3] = {{1, 0, 0}, {0, c, s}, {0, -s, c}};
+  double A[3][3], Bm[3][3];
+  matmul3(Rx, RzT, A);
+  matmul3(Rz, RxT, Bm);
+  const d3 up = vel_static_vortex(rot_apply(A, X));
+  return add(rot_apply(Bm, up), vel_sbr(X, 1.0, 0.0));
+}
+
+// dynamical_fields.equator_vortex (dynamical_fields.py:80-85)
+__device__ inline d3 vel_equator_vortex(d3 X) {
+  const double c = cos(M_PI / 2), s = sin(M_PI / 2);
+  const double Rx[3][3] = {{1, 0, 0}, {0, c, -s}, {0, s, c}};
+  const double RxT[3][3] = {{1, 0, 0}, {0, c, s}, {0, -s, c}};
+  return rot_apply(RxT, vel_static_vortex(rot_apply(Rx, X)));
+}
+
+// dynamical_fields.u_div (dynamical_fields.py:99-108)
+__device__ inline d3 vel_div(double t, d3 X, double T) {
+  double phi, theta;
+  cart2sphere(X, phi, theta);
+  const double sp2 = sin(phi / 2), st = sin(theta), ct = cos(theta), cpt = cos(M_PI * t / T);
+  const double u = -(sp2 * sp2) * sin(2 * theta) * st * cpt;
+  const double v = (-0.5) * sin(phi) * (st * st * st) * cpt;
+  const double a0 = ct * cos(phi) * v - st * sin(phi) * u;
+  const double a1 = ct * sin(phi) * v + st * cos(phi) * u;
+  const double a2 = -st * v;
+  // utils.tan_proj (utils.py:29-34)
+  return d3{(1 - X.x * X.x) * a0 + (-X.x * X.y) * a1 + (-X.x * X.z) * a2,
+            (-X.x * X.y) * a0 + (1 - X.y * X.y) * a1 + (-X.y * X.z) * a2,
+            (-X.x * X.z) * a0 + (-X.y * X.z) * a1 + (1 - X.z * X.z) * a2};
+}
+
 struct VelDev {
   int kind;
   cmm_mwss_mesh mesh;
@@ -312,7 +380,12 @@ __device__ __forceinline__ d3 vel_eval(const VelDev& V, int stage, d3 x) {
     }
     case CMM_VEL_SBR: return vel_sbr(x, V.params[0], V.params[1]);
     case CMM_VEL_DEFORM_ROT: return vel_deform_rot(V.stage_t[stage], x, V.params[1]);
-    default: return vel_rh4(V.stage_t[stage], x, V.params[0], V.params[1], V.params[2]);
+    case CMM_VEL_RH4: return vel_rh4(V.stage_t[stage], x, V.params[0], V.params[1], V.params[2]);
+    case CMM_VEL_DEFORM: return vel_deform(V.stage_t[stage], x, V.params[0]);
+    case CMM_VEL_STATIC_VORTEX: return vel_static_vortex(x);
+    case CMM_VEL_ROTATING_VORTEX: return vel_rotating_vortex(V.stage_t[stage], x, V.params[0]);
+    case CMM_VEL_EQUATOR_VORTEX: return vel_equator_vortex(x);
+    default: return vel_div(V.stage_t[stage], x, V.params[0]);
   }
 }
 
@@ -593,11 +666,75 @@ __device__ inline double vort_analytic(int kind, double phi, double theta, const
       double x = sin(theta) * cos(phi), y = sin(theta) * sin(phi), z = cos(theta);
       return 4 * pi * exp(-((x - 1) * (x - 1) + y * y + z * z) / (1.0 / 16));
     }
-    default: {  // perturbed rossby wave
+    case CMM_VORT_PERTURBED_ROSSBY: {
       double s = sin(theta);
       return 30 * cos(theta) * cos(4 * phi) * (s * s * s * s) + 0.1 * sin(theta) * cos(30 * theta);
     }
+    case CMM_VORT_COSINE_BELL:
+    case CMM_VORT_SLOTTED_CYLINDER: {
+      // dynamical_fields.py:110-156: two bells / slotted cylinders of radius
+      // R = 1/2 centred at (pi +- pi/6, pi/2)
+      const double R = 0.5, h0 = 1.0;
+      const double pc[2] = {pi + pi / 6, pi - pi / 6}, tc = pi / 2;
+      double o[2];
+      for (int k = 0; k < 2; ++k) {
+        const double arg = sin(tc - pi / 2) * sin(theta - pi / 2) +
+                           cos(tc - pi / 2) * cos(theta - pi / 2) * cos(phi - pc[k]);
+        const double r = acos(arg);
+        if (kind == CMM_VORT_COSINE_BELL) {
+          o[k] = r < R ? (h0 / 2) * (1 + cos(r * pi / R)) : 0.0;
+        } else {
+          const bool in = r <= R, slot = fabs(phi - pc[k]) < R / 6;
+          const bool keep = k == 0 ? theta < (-5 * R / 12 + pi / 2) : theta > (5 * R / 12 + pi / 2);
+          o[k] = (in && (!slot || keep)) ? 0.9 : 0.0;
+        }
+      }
+      return kind == CMM_VORT_COSINE_BELL ? 0.9 * (o[0] + o[1]) + 0.1 : o[0] + o[1] + 0.1;
+    }
+    case CMM_VORT_STATIC_VORTEX_TRACER:
+    case CMM_VORT_ROTATING_VORTEX_TRACER: {
+      // dynamical_fields.py:87-97: 1 - tanh(rho/5 sin(phi - omega_r(theta) t)),
+      // rho = 3 sin(theta); the rotating form in the frame Rot_x(pi/2)
+      double ph = phi, th = theta;
+      if (kind == CMM_VORT_ROTATING_VORTEX_TRACER) {
+        const double c = cos(M_PI / 2), s = sin(M_PI / 2);
+        const double Rx[3][3] = {{1, 0, 0}, {0, c, -s}, {0, s, c}};
+        cart2sphere(rot_apply(Rx, d3{sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)}), ph, th);
+      }
+      const double rho = 3.0 * sin(th);
+      return 1 - tanh((rho / 5) * sin(ph - vortex_omega(th) * prm[0]));
+    }
+    default:
+      return __builtin_nan("");
   }
+}
+
+// real field sum_l sum_{m >= 0} w_m Re(a_lm Y_lm(theta, phi)), w_0 = 1,
+// w_m = 2 (a_l,-m = (-1)^m conj a_lm), coefficients a [m][l] complex (Ls, Ls)
+__device__ inline double sh_series(const double* __restrict__ a, int Ls, double phi, double theta) {
+  const double x = cos(theta), st = sin(theta);
+  double pmm = 1.0 / sqrt(4 * M_PI), sum = 0.0;
+  for (int m = 0; m < Ls; ++m) {
+    if (m > 0) pmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m)) * st;
+    const double cm = cos(m * phi), sm = sin(m * phi);
+    const double* am = a + 2 * (int64_t)m * Ls;
+    double p0 = pmm, acc = am[2 * m] * cm - am[2 * m + 1] * sm;
+    acc *= p0;
+    if (m + 1 < Ls) {
+      double p1 = sqrt(2.0 * m + 3.0) * x * p0;
+      acc += p1 * (am[2 * (m + 1)] * cm - am[2 * (m + 1) + 1] * sm);
+      for (int l = m + 2; l < Ls; ++l) {
+        const double al = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+        const double bl = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+        const double p2 = al * (x * p1 - bl * p0);
+        acc += p2 * (am[2 * l] * cm - am[2 * l + 1] * sm);
+        p0 = p1;
+        p1 = p2;
+      }
+    }
+    sum += (m == 0 ? 1.0 : 2.0) * acc;
+  }
+  return sum;
 }
 
 __global__ __launch_bounds__(256) void k_sample_vorticity(VortArgs A, cmm_mwss_mesh W, const double* __restrict__ wpacked,
@@ -610,7 +747,9 @@ __global__ __launch_bounds__(256) void k_sample_vorticity(VortArgs A, cmm_mwss_m
   double phi = npy_mod(atan2(x.y, x.x) + 2 * M_PI, 2 * M_PI);
   double theta = atan2(sqrt(x.y * x.y + x.x * x.x), x.z);
   double w0;
-  if (A.kind == CMM_VORT_SPLINE) {
+  if (A.kind == CMM_VORT_SH_SERIES) {
+    w0 = sh_series(wpacked, (int)A.params[0], phi, theta);
+  } else if (A.kind == CMM_VORT_SPLINE) {
     // spherical_spline.__call__ (spherical_spline.py:42-70): back to xyz
     d3 q = d3{sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
     LevelSet ls;
@@ -1505,7 +1644,7 @@ static int velocity_device(const cmm_velocity* vel, const char* who, VelDev& V) 
       for (int k = 0; k < vel->nlev; ++k)
         if (!vel->levels[k]) return fail(CMM_E_ARG, std::string(who) + ": null velocity level");
     }
-  } else if (vel->kind < 0 || vel->kind > CMM_VEL_RH4) {
+  } else if (vel->kind < 0 || vel->kind > CMM_VEL_DIV) {
     return fail(CMM_E_ARG, std::string(who) + ": unknown velocity kind");
   }
   return CMM_OK;
@@ -1576,11 +1715,13 @@ extern "C" int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const 
                                     const double* params, const double* X, int64_t n, int64_t sp,
                                     int64_t sc, const double* cosg, int ring_len, double rot_rate,
                                     double* out, void* stream) {
-  if (kind < 0 || kind > CMM_VORT_PERTURBED_ROSSBY || n < 0 || (n > 0 && (!X || !out || !cosg)) ||
-      ring_len < 1)
+  if (kind < 0 || kind > CMM_VORT_SH_SERIES || n < 0 || (n > 0 && (!X || !out || !cosg)) || ring_len < 1)
     return fail(CMM_E_ARG, "cmm_sample_vorticity: bad arguments");
   if (kind == CMM_VORT_SPLINE && (!wmesh || !wpacked))
     return fail(CMM_E_ARG, "cmm_sample_vorticity: spline vorticity needs a mesh and coefficients");
+  if (kind == CMM_VORT_SH_SERIES && (!wpacked || !params || !(params[0] >= 1 && params[0] <= 1024) ||
+                                     params[0] != (double)(int)params[0]))
+    return fail(CMM_E_ARG, "cmm_sample_vorticity: a spherical-harmonic series needs its table and 1 <= Ls <= 1024");
   VortArgs A;
   A.kind = kind;
   for (int k = 0; k < 4; ++k) A.params[k] = params ? params[k] : 0.0;

@@ -190,6 +190,11 @@ int cmm_mwss_eval(const cmm_mwss_mesh* mesh, const double* const* levels, const 
 #define CMM_VEL_SBR 1        /* dynamical_fields.u_sbr: params[0]=T, [1]=alpha */
 #define CMM_VEL_DEFORM_ROT 2 /* dynamical_fields.u_deform_rot: params[1]=alpha */
 #define CMM_VEL_RH4 3        /* Rossby-Haurwitz-4: params[0]=w, [1]=K, [2]=Omega */
+#define CMM_VEL_DEFORM 4     /* dynamical_fields.u_deform: params[0]=T */
+#define CMM_VEL_STATIC_VORTEX 5   /* dynamical_fields.u_static_vortex */
+#define CMM_VEL_ROTATING_VORTEX 6 /* dynamical_fields.u_rotating_vortex: params[0]=T */
+#define CMM_VEL_EQUATOR_VORTEX 7  /* dynamical_fields.equator_vortex */
+#define CMM_VEL_DIV 8        /* dynamical_fields.u_div: params[0]=T */
 
 #define CMM_INT_EULER 0      /* Euler_step_proj */
 #define CMM_INT_IMPROVED_EULER 1 /* improved_Euler_proj */
@@ -285,8 +290,9 @@ int cmm_mwss_scalar_jets(const cmm_mwss_mesh* mesh, const double* fields, double
 /* Vorticity sampling (spherical_simulations.py:268-273):
  * out_t,p = w0(X) + 2 rot (cos theta(X) - cos theta_t) at the MWSS grid of
  * bandlimit L (theta_t from cos_t, device (L+1)); X (n,3) with n = (L+1)*2L.
- * w0 is a scalar PS spline on `wmesh` (packed) when kind = 0, else an
- * analytic field CMM_VORT_*.  */
+ * w0 is a scalar PS spline on `wmesh` (packed) when kind = 0, a
+ * spherical-harmonic series (table in wpacked) for CMM_VORT_SH_SERIES, else
+ * an analytic field CMM_VORT_*.  rot_rate 0 samples w0 alone (tracers). */
 #define CMM_VORT_SPLINE 0
 #define CMM_VORT_ROSSBY 1        /* dynamical_fields.rossby_wave, params[0]=t */
 #define CMM_VORT_ZONAL_JET 2
@@ -294,6 +300,16 @@ int cmm_mwss_scalar_jets(const cmm_mwss_mesh* mesh, const double* fields, double
 #define CMM_VORT_PERTURBED_JET 4
 #define CMM_VORT_MULTI_JET 5
 #define CMM_VORT_PERTURBED_ROSSBY 6
+/* tracer initial conditions (dynamical_fields.py:87-156) */
+#define CMM_VORT_COSINE_BELL 7
+#define CMM_VORT_SLOTTED_CYLINDER 8        /* IC_SC */
+#define CMM_VORT_STATIC_VORTEX_TRACER 9    /* static_vort_IC(t, .), params[0]=t */
+#define CMM_VORT_ROTATING_VORTEX_TRACER 10 /* rotating_vort_IC(t, .), params[0]=t */
+/* a band-limited real field from its coefficients: wpacked is then a device
+ * table [m][l] complex (Ls, Ls) of a_lm (m >= 0, conjugate-symmetric
+ * field), params[0] = Ls; e.g. dynamical_fields.random_initial_condition
+ * (dynamical_fields.py:159-171) */
+#define CMM_VORT_SH_SERIES 11
 int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const double* wpacked,
                          const double* params, const double* X, int64_t n, int64_t x_stride_pt,
                          int64_t x_stride_comp, const double* cos_grid_theta, int ring_len,

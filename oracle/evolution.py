@@ -120,3 +120,19 @@ def compose_maps(remaps, evals, current=[]):
     for i in range(len(rev_maps)):
         evals_c[:] = rev_maps[i](evals_c).T
     return evals_c
+
+
+def transport(mesh, V, T, Nt, integrator=None):
+    """The linear-transport loop of tests/test_spherical_transport.py:60-117
+    without remapping (its default): identity jets, then Nt steps of
+    advect_project_sphere over tspan = linspace(0, T, Nt, endpoint=False).
+    Returns the final map."""
+    integrator = RK4_proj if integrator is None else integrator
+    mapping = Map(mesh, identity_jets(mesh.vertices))
+    tspan = np.linspace(0, T, Nt, endpoint=False)
+    dt = tspan[1] - tspan[0]
+    identity = True
+    for t in tspan:
+        mapping = advect_project_sphere(mapping, integrator, t, dt, V, identity)
+        identity = False
+    return mapping

@@ -55,6 +55,144 @@ def u_deform_rot(t, dt, X, alpha=0.0):
     return np.array(u_sr) + u_sbr(t, dt, X)
 
 
+def cart2sphere(X):
+    """utils.py:18-23: phi in [0, 2 pi), theta the colatitude."""
+    phi = (np.arctan2(X[1], X[0]) + 2 * pi) % (2 * pi)
+    theta = np.arctan2(np.sqrt(X[1] ** 2 + X[0] ** 2), X[2])
+    return phi, theta
+
+
+def _Rx(a):
+    return np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+
+
+RHO_0 = 3.0  # dynamical_fields.py:44
+
+
+def omega_r(theta):
+    """dynamical_fields.py:48-55: angular speed of the static vortex."""
+    theta = np.asarray(theta, dtype=np.float64)
+    s = np.sin(theta)
+    out = np.zeros_like(theta)
+    nz = theta != 0.0
+    r = RHO_0 * s[nz]
+    out[nz] = (2 * pi) * np.sqrt(3) * (3 / 2) * (1 / r) * np.tanh(r) * (1 / np.cosh(r)) ** 2
+    return out
+
+
+def u_static_vortex(t, dt, X):
+    """dynamical_fields.py:57-66."""
+    phi, theta = cart2sphere(X)
+    omg = omega_r(theta)
+    return np.array([-X[1] * omg, X[0] * omg, 0 * X[2]])
+
+
+def u_rotating_vortex(t, dt, X, T=1.0):
+    """dynamical_fields.py:68-78."""
+    tau = 2 * pi * t / T
+    X_r = _rotate(np.matmul(_Rx(pi / 2), _Rz(tau).T), X)
+    u_p = u_static_vortex(t, dt, np.array(X_r))
+    u_sr = _rotate(np.matmul(_Rz(tau), _Rx(pi / 2).T), u_p)
+    return np.array(u_sr) + u_sbr(t, dt, X)
+
+
+def equator_vortex(t, dt, X):
+    """dynamical_fields.py:80-85."""
+    X_r = _rotate(_Rx(pi / 2), X)
+    u_p = u_static_vortex(t, dt, np.array(X_r))
+    return np.array(_rotate(_Rx(pi / 2).T, u_p))
+
+
+def u_div(t, dt, X, T=1.0):
+    """dynamical_fields.py:99-108: (dphi/dt, dtheta/dt) = (u, v) mapped to
+    Cartesian components, then projected to the tangent plane."""
+    phi, theta = cart2sphere(X)
+    u = -(np.sin(phi / 2) ** 2) * np.sin(2 * theta) * np.sin(theta) * np.cos(pi * t / T)
+    v = (-1 / 2) * np.sin(phi) * (np.sin(theta) ** 3) * np.cos(pi * t / T)
+    u_x = np.cos(theta) * np.cos(phi) * v - np.sin(theta) * np.sin(phi) * u
+    u_y = np.cos(theta) * np.sin(phi) * v + np.sin(theta) * np.cos(phi) * u
+    u_z = -np.sin(theta) * v
+    return np.array(g.tan_proj(X, [u_x, u_y, u_z]))
+
+
+def static_vort_IC(t, xyz):
+    """dynamical_fields.py:87-90: the tracer the static vortex carries,
+    exact at time t."""
+    phi, theta = cart2sphere(xyz)
+    rho = RHO_0 * np.sin(theta)
+    return 1 - np.tanh((rho / 5) * np.sin(phi - omega_r(theta) * t))
+
+
+def rotating_vort_IC(t, xyz):
+    """dynamical_fields.py:92-97 (its frame angle tau is 0)."""
+    X_r = _rotate(np.matmul(_Rx(pi / 2), _Rz(0.0).T), xyz)
+    phi, theta = cart2sphere(np.array(X_r))
+    rho = RHO_0 * np.sin(theta)
+    return 1 - np.tanh((rho / 5) * np.sin(phi - omega_r(theta) * t))
+
+
+# tracer initial conditions (dynamical_fields.py:110-156)
+H_0 = 1.0
+PHI_C1, THE_C1 = pi + pi / 6, pi / 2
+PHI_C2, THE_C2 = pi - pi / 6, pi / 2
+R_BELL = 1 / 2
+
+
+def _bell_dist(phi, theta, phc, thc):
+    arg = np.sin(thc - pi / 2) * np.sin(theta - pi / 2) + np.cos(thc - pi / 2) * np.cos(theta - pi / 2) * \
+        np.cos(phi - phc)
+    return np.arccos(arg)
+
+
+def cosine_bell(phi, theta):
+    """dynamical_fields.py:123-138."""
+    out = np.zeros_like(np.asarray(theta, dtype=np.float64))
+    out2 = np.zeros_like(out)
+    r1 = _bell_dist(phi, theta, PHI_C1, THE_C1)
+    r2 = _bell_dist(phi, theta, PHI_C2, THE_C2)
+    i1, i2 = r1 < R_BELL, r2 < R_BELL
+    out[i1] = (H_0 / 2) * (1 + np.cos(r1[i1] * pi / R_BELL))
+    out2[i2] = (H_0 / 2) * (1 + np.cos(r2[i2] * pi / R_BELL))
+    return 0.9 * (out + out2) + 0.1
+
+
+def IC_SC(phi, theta):
+    """dynamical_fields.py:140-156 (slotted cylinders)."""
+    phi = np.asarray(phi, dtype=np.float64)
+    theta = np.asarray(theta, dtype=np.float64)
+    r1 = _bell_dist(phi, theta, PHI_C1, THE_C1)
+    r2 = _bell_dist(phi, theta, PHI_C2, THE_C2)
+    R = R_BELL
+    out = np.zeros_like(theta)
+    out[(r1 <= R) & (np.abs(phi - PHI_C1) >= R / 6)] = 0.9
+    out[(r1 <= R) & (np.abs(phi - PHI_C1) < R / 6) & (theta < (-5 * R / 12 + pi / 2))] = 0.9
+    out2 = np.zeros_like(theta)
+    out2[(r2 <= R) & (np.abs(phi - PHI_C2) >= R / 6)] = 0.9
+    out2[(r2 <= R) & (np.abs(phi - PHI_C2) < R / 6) & (theta > (5 * R / 12 + pi / 2))] = 0.9
+    return out + out2 + 0.1
+
+
+def random_ic_coeffs():
+    """dynamical_fields.py:159-160: np.random.seed(42); uniform(-1, 1, 33^2)."""
+    return np.random.RandomState(42).uniform(-1, 1, 33 ** 2)
+
+
+def random_initial_condition(xyz):
+    """dynamical_fields.py:161-171: sum over l <= 32 and m = -l .. l-1 (the
+    reference's range stops before m = l) of c Re Y_l^m, coefficients in
+    that order."""
+    from scipy.special import sph_harm_y
+    phi, theta = cart2sphere(xyz)
+    c = random_ic_coeffs()
+    out = 0.0
+    lm = 0
+    for l in range(33):
+        for m in range(-l, l):
+            out = out + c[lm] * sph_harm_y(l, m, theta, phi).real
+            lm += 1
+    return out
+
+
 RH4_OMEGA = 7.848e-6
 RH4_K = 7.848e-6
 RH4_R = 4
