@@ -69,6 +69,7 @@ SIGNATURES = {
     "cmm_sht_workspace_bytes": (C.c_int, [vp, C.c_int, P_i64]),
     "cmm_sht_inverse": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
     "cmm_sht_forward": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
+    "cmm_sht_forward_noalloc": (C.c_int, [vp, vp, C.c_int, vp, vp]),
     "cmm_sht_stream_operators": (C.c_int, [vp, vp, C.c_int, vp, vp]),
     "cmm_sht_scalar_operators": (C.c_int, [vp, vp, vp, vp]),
     "cmm_mwss_velocity_jets": (C.c_int, [vp, vp, vp, vp]),

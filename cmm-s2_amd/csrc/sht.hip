@@ -32,8 +32,17 @@ constexpr int kOzJ = 16;          // split-BF16 synthesis: degrees of one parity
 constexpr int kOzMaxS = 7;        // split-BF16 synthesis: most digits per operand
 }
 
+// Scratch of cmm_sht_forward_noalloc: a fixed array in the code object, so
+// every GPU the library runs on holds its own copy (at its own address)
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel (its host-side name is a stand-in, not a device address): the plan
+// keeps the address hipGetSymbolAddress returns on the plan's device.
+constexpr int64_t kNoAllocScratchBytes = (int64_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 struct cmm_sht_plan {
   int L = 0, N = 0;  // N = 2L
+  void* scratch = nullptr;   // device address of `scratch` on the plan's device
   int nrad = 0;
   int radix[24];
   // device tables
@@ -260,6 +269,12 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
   if ((rc = build_restart(p, units, unit0))) {
     cmm_sht_plan_destroy(p);
     return rc;
+  }
+  // the no-allocation forward's scratch on this (current) device
+  hipError_t e = hipGetSymbolAddress(&p->scratch, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    cmm_sht_plan_destroy(p);
+    return hip_check(e, "cmm_sht_plan_create: hipGetSymbolAddress(scratch)");
   }
   *out = p;
   return CMM_OK;
@@ -1683,9 +1698,33 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   return CMM_OK;
 }
 
+// analysis workspace: F (nf, L+1, L) then G (nf, L, L), complex
+static int64_t sht_forward_bytes(const cmm_sht_plan* p, int nf) {
+  return sht_f_bytes(p, nf) + (int64_t)nf * p->L * p->L * 16;
+}
+
+static int sht_forward(const cmm_sht_plan* p, const double* f, int nf, double* flm, void* work, void* stream);
+
 extern "C" int cmm_sht_forward(const cmm_sht_plan* p, const double* f, int nf, double* flm, void* work,
                                void* stream) {
   if (!p || !flm || !f || !work || nf < 1) return fail(CMM_E_ARG, "cmm_sht_forward: bad arguments");
+  return sht_forward(p, f, nf, flm, work, stream);
+}
+
+extern "C" int cmm_sht_forward_noalloc(const cmm_sht_plan* p, const double* f, int nf, double* flm, void* stream) {
+  if (!p || !flm || !f || nf < 1) return fail(CMM_E_ARG, "cmm_sht_forward_noalloc: bad arguments");
+  if (!p->scratch) return fail(CMM_E_ARG, "cmm_sht_forward_noalloc: plan has no scratch address");
+  const int64_t need = sht_forward_bytes(p, nf);
+  if (need > kNoAllocScratchBytes)
+    return fail(CMM_E_RANGE, "cmm_sht_forward_noalloc: needs " + std::to_string(need) + " bytes of scratch, " +
+                                 "the limit is " + std::to_string(kNoAllocScratchBytes) +
+                                 " (16 nf L (2L + 1) bytes: L = 1024 with nf = 1, L = 512 with nf <= 7)");
+  return sht_forward(p, f, nf, flm, p->scratch, stream);
+}
+
+// FFT -> resampling -> Gauss-Legendre analysis; `work` holds F and G (every
+// element is written before it is read, so no clearing)
+static int sht_forward(const cmm_sht_plan* p, const double* f, int nf, double* flm, void* work, void* stream) {
   const int L = p->L;
   double2* F = reinterpret_cast<double2*>(work);
   double2* G = F + (int64_t)nf * (L + 1) * L;

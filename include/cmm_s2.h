@@ -268,6 +268,15 @@ int cmm_sht_inverse(const cmm_sht_plan* plan, const double* flm, int nf, double*
 int cmm_sht_forward(const cmm_sht_plan* plan, const double* f, int nf, double* flm,
                     void* work, void* stream);
 
+/* cmm_sht_forward with its workspace in a fixed device array of the library
+ * (64 MiB per GPU, held until the process ends) instead of `work`: calls no
+ * allocator.  No counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Needs 16 nf L (2L + 1) bytes <= 64 MiB (L = 1024 with
+ * nf = 1, L = 512 with nf <= 7), else CMM_E_RANGE.  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (other streams, other threads) race. */
+int cmm_sht_forward_noalloc(const cmm_sht_plan* plan, const double* f, int nf, double* flm, void* stream);
+
 /* inv_Laplacian scaling + angular_momentum twice (spherical_harmonic_tools.py
  * :14-19, 49-62, 79-91): from omega_lm [lm] (one field) writes out [lm][12]:
  * fields 0..2 = (r x grad) psi, field 3+3i+k = (r x grad)_k of field i, with
