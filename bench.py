@@ -27,11 +27,26 @@ for _p in (ROOT, os.path.join(ROOT, "cmm-s2_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_tri_compose for the default workload (ico-7 map mesh,
-# 992 cells per cube-face edge, depth 5, L=2048 grid, tile rows) from two
-# rocprofv3 counter passes, FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a
-# 256 MiB device copy): profiles/compose_L2048_depth5_ico7_pmc.txt
-MEASURED_TRAFFIC = {(7, 2048, 5, 992, 8): 2.584e9, (7, 2048, 5, 992, 4): 2.394e9}
+# rocprofv3 counter summary of the composition kernel: its machine-readable
+# lines "traffic_bytes[ico=..,Lf=..,depth=..,cells=..,tile_rows=..] = B" give
+# the DRAM bytes per launch (FETCH_SIZE x read factor + WRITE_SIZE x write
+# factor, the factors read off a device copy of known size in the same runs)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "compose_L2048_depth5_ico7_pmc.txt")
+# the oracle's CPU step time at the default workload (bench.py --cpu-steps 5 on
+# the GPU box's host): a line "step_s_median = S"
+CPU_STEP_PROFILE = os.path.join(ROOT, "profiles", "cpu_oracle_step_L512_ico7.txt")
+
+
+def _profile_value(path, key):
+    """The number after "key = " in a committed profile summary, or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key + " = "):
+                    return float(line.split("=", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -52,9 +67,12 @@ def parse():
     ap.add_argument("--lmax", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cells", type=int, default=0, help="cube-map cells per face edge of the location grid (0: auto)")
-    ap.add_argument("--cpu-sample", type=int, default=2097152,
-                    help="points of the CPU-baseline sample (every k-th grid point)")
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="processes of the CPU-baseline composition (0: the host CPUs this process may use, "
+                         "at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="instead: time K steady steps of the CPU oracle's loop at the workload (minutes per step)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-Legendre comparison run")
     ap.add_argument("--ensemble", type=int, default=0,
                     help="instead: M seeded members (L in {128..512}, ico 5-7, 50 steps, remap every 10, "
@@ -82,6 +100,8 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the composition and the "
               "CMM step shard by points / vertices with no collectives", file=sys.stderr)
         sys.exit(2)
+    if a.cpu_steps:
+        return cpu_step_bench(a)
     if a.ensemble:
         return ensemble_bench(a)
     if a.synthesis:
@@ -96,9 +116,16 @@ def main():
     from cmm_s2.core import dynamical_fields as df
     from cmm_s2.core.spherical_simulations import BarotropicCMM
 
+    ico = mf.icosahedral_mesh(a.ico)
+    pool = None
+    if not a.no_cpu:
+        # the CPU baseline's processes start before this one touches the GPU
+        # and build their Qhull triangulations while the GPU runs
+        from oracle.parallel import ComposePool
+        n_w = a.cpu_workers or min(16, len(os.sched_getaffinity(0)))
+        pool = ComposePool(ico.points, n_w, qhull_options=None if a.ico <= 5 else "QJ")
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
-    ico = mf.icosahedral_mesh(a.ico)
     mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices, cells_per_face=a.cells)
     log(f"mesh ico-{a.ico}: {mesh.nv} vertices, {mesh.ntri} triangles, "
         f"setup {time.perf_counter() - t0:.1f}s, {mesh.stats()}")
@@ -167,7 +194,8 @@ def main():
     ms_per = wall / a.steps * 1e3
     mpts = n / (ms_per * 1e-3) / 1e6
     alg_bytes = (48 + 216 * depth) * n
-    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    comp = compulsory_bytes(mesh, levels, q, gshape)
+    achieved = comp["bytes"] / (k_ms * 1e-3) / 1e9
     # vorticity on the fine grid (reported separately)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -191,10 +219,11 @@ def main():
         mixed = {mode: mixed_run(a, mesh, w0, wf, grid, s_step, sim, mode) for mode in _MIXED}
         mixed["fp64_initial_field_perturbed_2^-50"] = sensitivity_run(a, mesh, w0, wf, grid)
     cells = int(mesh.stats()["cells_per_face"])
-    traffic = MEASURED_TRAFFIC.get((a.ico, a.Lf, depth, cells, tile_rows))
+    traffic = _profile_value(TRAFFIC_PROFILE, f"traffic_bytes[ico={a.ico},Lf={a.Lf},depth={depth},cells={cells},"
+                                              f"tile_rows={tile_rows}]")
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(mesh, sim, q, depth, a.cpu_sample)
+        cpu = cpu_baseline(pool, sim, q, depth, ms_per * 1e-3, s_step)
 
     rec = {
         "metric": "composed-map Mpts/s + s/step at L=512 ico-7; mixed vs FP64 at 1e-10 err",
@@ -211,10 +240,19 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_tri_compose", "kernel_ms": round(k_ms, 4),
-                     "algorithmic_bytes": alg_bytes,
-                     "note": "algorithmic gather bytes (48 + 216 x depth per point) over the event-timed kernel; "
-                             "they exceed the DRAM bytes ('traffic', rocprofv3 counters) because neighbouring "
-                             "points share triangles in L2 / MALL; the kernel is bound by its load-address rate"},
+                     "compulsory_bytes": comp["bytes"], "compulsory": comp,
+                     "traffic_GBs": None if traffic is None else round(traffic / (k_ms * 1e-3) / 1e9, 1),
+                     "traffic_source": None if traffic is None else os.path.relpath(TRAFFIC_PROFILE, ROOT),
+                     "gather_model_bytes": alg_bytes,
+                     "gather_model_GBs": round(alg_bytes / (k_ms * 1e-3) / 1e9, 1),
+                     "note": "achieved = compulsory HBM bytes per launch (points in and out, 48 B each, plus for "
+                             "every level the 144-B coefficient block of each (triangle, sub-triangle) some point "
+                             "lands in and the 128-B geometry line of each triangle some point lands in; the "
+                             "location cells are left out, so it is a lower bound) over the HIP-event kernel time; "
+                             "traffic = DRAM bytes per launch from rocprofv3 counters (traffic_source); the gather "
+                             "model (48 + 216 x depth B per point) counts every lane's loads and exceeds the HBM "
+                             "peak because neighbouring points share blocks in L2 / MALL; the kernel is bound by "
+                             "its load-address rate"},
         "cpu_baseline": cpu,
         "mixed": mixed,
         "area_distortion": area,
@@ -500,27 +538,36 @@ def mixed_run(a, mesh, w0, wf64, grid, s64, sim64, mode):
             "target_rel_err": 1e-10, "meets_target": bool(err <= 1e-10 and s32 < s64)}
 
 
-def cpu_baseline(mesh, sim, q, depth, nsample):
-    """The oracle's composition (scipy find_simplex + NumPy PS evaluation,
-    single thread) on the first `nsample` grid points with the same submaps."""
-    import numpy as np
-    from oracle import meshes as om, splines as osp
-    t0 = time.perf_counter()
-    opts = None if mesh.ntri <= 20480 else "QJ"
-    oms = om.SphericalTriangulation(mesh.vertices, qhull_options=opts)
-    vals = sim.submaps.vals[:depth].cpu().numpy()
-    levels = [osp.assemble(vals[i], oms.simplices, oms.ps_split) for i in range(depth)]
-    setup = time.perf_counter() - t0
-    stride = max(1, q.shape[0] // nsample)
-    qd = q[::stride][:nsample].contiguous()
-    nsample = qd.shape[0]
-    qs = qd.cpu().numpy()
-    t0 = time.perf_counter()
-    ref = osp.compose_eval(oms.query, levels, qs)
-    dt = time.perf_counter() - t0
-    got = sim.submaps(qd).cpu().numpy()
-    err = float(np.abs(got - ref).max())
-    log(f"cpu oracle: setup {setup:.1f}s (not timed), {nsample} pts in {dt:.2f}s, max |gpu-cpu| {err:.2e}")
+def compulsory_bytes(mesh, levels, q, grid):
+    """Lower bound of the composition's HBM bytes per launch: every point is
+    read and written once (48 B), and for every level each coefficient block
+    (144 B: 3 components x 6 Bernstein-Bezier coefficients of one split
+    sub-triangle) and each triangle geometry line (128 B: dual basis and edge
+    norms) that some point lands in is read at least once.  The levels'
+    input points come from the composition's intermediates; their triangle
+    and sub-triangle from the device locator (first-occurrence argmin /
+    argmax of the barycentrics, the split rule)."""
+    import torch
+    from cmm_s2.core import _backend as B
+    from cmm_s2.core.spherical_spline import compose_packed
+    n, depth = q.shape[0], len(levels)
+    inter = B.empty(depth, n, 3)
+    compose_packed(mesh, levels, q, "N3", B.empty(n, 3), inter=inter, grid=grid)
+    blocks, tris = [], []
+    for i in range(depth):
+        x = q if i == 0 else inter[i - 1]
+        tri, bcc = mesh.locate(x, "N3")
+        mn, mx = torch.argmin(bcc, dim=1), torch.argmax(bcc, dim=1)
+        mx = torch.where(mn == mx, (mn + 1) % 3, mx)
+        sub = 2 * mn + mx - (mx > mn).long()
+        blocks.append(int(torch.unique(tri.long() * 6 + sub).numel()))
+        tris.append(int(torch.unique(tri).numel()))
+    del inter
+    total = 48 * n + sum(144 * b + 128 * t for b, t in zip(blocks, tris))
+    return {"bytes": total, "points": n, "blocks_per_level": blocks, "triangles_per_level": tris}
+
+
+def _host_info():
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -530,12 +577,86 @@ def cpu_baseline(mesh, sim, q, depth, nsample):
                     break
     except OSError:
         pass
-    return {"value": round(nsample / dt / 1e6, 4), "unit": "Mpts/s", "cores": 1, "kind": "port",
-            "sample": f"{nsample} points (every {stride}th grid point, grid order) of the same "
-                      f"composition, depth {depth}, scipy find_simplex + NumPy, single thread; "
-                      f"Qhull set-up {setup:.1f} s not timed",
-            "host_cpus": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": cpu_model,
-            "max_abs_diff_vs_gpu": err}
+    return {"host_cpus": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": cpu_model}
+
+
+def cpu_baseline(pool, sim, q, depth, gpu_compose_s, gpu_step_s):
+    """The oracle's composition (scipy find_simplex + NumPy PS evaluation) of
+    the same submaps on the whole fine grid, split over a spawned process
+    pool as compose_maps_parallel (evolution_functions.py:158-165) splits it
+    over its pool; the oracle's step time from the committed --cpu-steps
+    record."""
+    import numpy as np
+    n_w = pool.n
+    t0 = time.perf_counter()
+    pool.set_levels(sim.submaps.vals[:depth].cpu().numpy())
+    setup = time.perf_counter() - t0
+    qs = q.cpu().numpy()
+    log(f"cpu oracle: {n_w} spawned workers hold the levels after {setup:.1f} s more (assembly, not timed)")
+    t0 = time.perf_counter()
+    ref = pool(qs)
+    dt = time.perf_counter() - t0
+    pool.close()
+    got = sim.submaps(q[::97].contiguous()).cpu().numpy()
+    err = float(np.abs(got - ref[::97]).max())
+    log(f"cpu oracle: {len(qs)} pts in {dt:.2f}s on {n_w} processes ({len(qs) / dt / 1e6:.3f} Mpts/s), "
+        f"max |gpu-cpu| on every 97th point {err:.2e}")
+    step = _profile_value(CPU_STEP_PROFILE, "step_s_median")
+    out = {"value": round(len(qs) / dt / 1e6, 4), "unit": "Mpts/s", "cores": n_w, "kind": "port",
+           "sample": f"the whole composition: {len(qs)} points of the L-grid in grid order through the {depth} "
+                     f"submaps, split into {n_w} contiguous chunks over a spawned process pool "
+                     "(compose_maps_parallel), scipy find_simplex + NumPy per process; set-up not timed",
+           "seconds": round(dt, 3), "max_abs_diff_vs_gpu": err, **_host_info(),
+           "step_s": step,
+           "step_source": None if step is None else os.path.relpath(CPU_STEP_PROFILE, ROOT)}
+    if step is not None and gpu_step_s:
+        out["speedup_step_plus_composition"] = round((step + dt) / (gpu_step_s + gpu_compose_s), 1)
+    return out
+
+
+def cpu_step_bench(a):
+    """The oracle's loop (oracle/driver.py, the reference's algorithm in NumPy /
+    SciPy, single process; SHTs with numpy FFTs) at the default workload:
+    set-up and bootstrap untimed, then the wall time of K steady steps
+    (step_hook timestamps).  Minutes per step: run once and keep the record
+    under profiles/."""
+    import numpy as np
+    from oracle import meshes as om, sht as osh, driver as odr
+    from cmm_s2.core import dynamical_fields as df
+    t0 = time.perf_counter()
+    say = lambda msg: print(f"[{time.perf_counter() - t0:7.1f} s] {msg}", flush=True)
+    pts = om.icosahedral_points(a.ico)
+    oms = om.SphericalTriangulation(pts, qhull_options=None if a.ico <= 5 else "QJ")
+    say(f"ico-{a.ico} triangulation (Qhull)")
+    coeffs = df.random_sh_coefficients(a.seed, a.lmax, a.Lw)
+    mw = om.StructuredTriangulation(a.Lw)
+    say(f"L={a.Lw} MWSS mesh")
+    w0 = osh.project_onto_S12(osh.lm_to_full(coeffs, a.Lw), a.Lw, mw)
+    say("initial vorticity spline")
+    stamps = []
+
+    class Stop(Exception):
+        pass
+
+    def hook(count, cm, U):
+        stamps.append(time.perf_counter())
+        say(f"step {count}" + (f": {stamps[-1] - stamps[-2]:.2f} s" if len(stamps) > 1 else ""))
+        if len(stamps) > a.cpu_steps:
+            raise Stop()
+
+    try:
+        odr.euler_simulation_sphere_remapping(a.L, a.Nt, a.Nt / a.L, oms, w0, 2 * np.pi, a.n_maps,
+                                              step_hook=hook, log=say)
+    except Stop:
+        pass
+    steps = np.diff(stamps)
+    rec = {"metric": "CPU oracle seconds per steady step", "value": round(float(np.median(steps)), 3), "unit": "s",
+           "higher_is_better": False, "steps": [round(float(x), 3) for x in steps], "kind": "port", "cores": 1,
+           "config": {"workload": f"barotropic CMM L={a.L} ico-{a.ico}, dt=1/{a.L}, remap every {a.n_maps}, "
+                                  f"initial vorticity spline on L={a.Lw}", "steps_timed": len(steps),
+                      "first_timed_step": 4},
+           **_host_info()}
+    print(json.dumps(rec), flush=True)
 
 
 if __name__ == "__main__":

@@ -38,9 +38,11 @@ class Composite:
 
 
 def euler_simulation_sphere_remapping(L, Nt, T, mesh, vorticity, rot_rate=2 * np.pi,
-                                      n_maps=10, step_hook=None):
+                                      n_maps=10, step_hook=None, log=None):
     """spherical_simulations.py:144-297.  ``step_hook(count, curr_map, U)`` is
-    an optional test probe called after every loop step."""
+    an optional test probe called after every loop step; ``log(msg)`` an
+    optional progress callback for the set-up and bootstrap stages."""
+    log = log or (lambda msg: None)
     ns = [i for i in range(1, Nt) if i % n_maps == 0]
     ns.append(Nt)
     jet_vals = np.zeros([len(ns), 3, 4, len(mesh.x)])
@@ -69,19 +71,23 @@ def euler_simulation_sphere_remapping(L, Nt, T, mesh, vorticity, rot_rate=2 * np
     psi_lms = sht.inv_Laplacian(psi0, L, D_inv)
     temp = 0 * psi_lms[:]
     sht.project_stream(psi_lms, L, L_plus, L_minus, L_z, U, 0, temp)
+    log("bootstrap: initial velocity")
 
     t = 0
     int0 = ev.advect_project_sphere(map0, ev.Euler_step_proj, t, dt, U, identity=True)
     psi_lms1 = sht.inv_Laplacian(omega(int0(sample_pts)), L, D_inv)
     sht.project_stream(psi_lms1, L, L_plus, L_minus, L_z, U, 1, temp)
+    log("bootstrap: Euler step")
     U.nVs = 2
     int1 = ev.advect_project_sphere(map0, ev.improved_Euler_proj, t, dt, U, identity=True)
     psi_lms2 = sht.inv_Laplacian(omega(int1(sample_pts)), L, D_inv)
     sht.project_stream(psi_lms2, L, L_plus, L_minus, L_z, U, 1, temp)
+    log("bootstrap: improved Euler step")
     int2 = ev.advect_project_sphere(int1, ev.RK4_proj, t + dt, dt, U)
     psi_lms3 = sht.inv_Laplacian(omega(int2(sample_pts)), L, D_inv)
     U.nVs = 3
     sht.project_stream(psi_lms3, L, L_plus, L_minus, L_z, U, 2, temp)
+    log("bootstrap: RK4 step")
     curr_map = int2
 
     U.stepping = True
