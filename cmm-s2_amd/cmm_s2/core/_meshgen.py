@@ -25,6 +25,20 @@ def hull_triangles(pts):
     return orient(pts, ConvexHull(pts).simplices)
 
 
+def delaunay_triangles(pts, qhull_options=None):
+    """The reference's triangulation and row order (mesh_functions.py:41-56):
+    Qhull's 3-D Delaunay tessellation of the origin and the points, the
+    origin dropped from every tetrahedron (the remaining three indices kept
+    in their order), clockwise rows fixed by swapping their last two
+    vertices.  Returns (simplices, the Delaunay object)."""
+    from scipy.spatial import Delaunay
+    pp = np.vstack((np.zeros((1, 3)), pts))
+    d = Delaunay(pp) if qhull_options is None else Delaunay(pp, qhull_options=qhull_options)
+    s0 = d.simplices - 1
+    s = np.array([t[t != -1] for t in s0], dtype=np.int64)
+    return orient(pts, s), d
+
+
 def refine(vertices, simplices, levels):
     """Midpoint bisection: every triangle (a,b,c) -> (a,ab,ca), (ab,b,bc),
     (ca,bc,c), (ab,bc,ca) with normalised edge midpoints appended after the

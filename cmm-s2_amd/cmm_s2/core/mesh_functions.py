@@ -74,20 +74,33 @@ def stencil_points(verts, eps=1e-5):
 class spherical_triangulation(object):
     """Spherical triangulation of points ``pts`` (N,3) with optional uniform
     midpoint refinement (mesh_functions.py:15-66).  ``simplices`` may be
-    given to skip the convex-hull triangulation (e.g. from
-    ``icosahedral_mesh``)."""
+    given to skip the triangulation (e.g. from ``icosahedral_mesh``).
+    Otherwise ``triangulation`` picks how it is built: "hull" (default) takes
+    the convex hull (seconds at any size; the same triangles as the
+    reference, in Qhull's hull order), "delaunay" the reference's own call,
+    Qhull's Delaunay tessellation of the origin and the points with its row
+    order (mesh_functions.py:41-56; ``qhull_options`` as scipy takes them,
+    None = the reference's defaults, which take minutes from ~40,000 points
+    on), so that ``query()[1]`` equals the reference's triangle indices; its
+    Delaunay object is kept as ``mesh_d``."""
 
-    def __init__(self, pts, refinement_levels=0, simplices=None, cells_per_face=0):
+    def __init__(self, pts, refinement_levels=0, simplices=None, cells_per_face=0, triangulation="hull",
+                 qhull_options=None):
         pts = np.asarray(pts, dtype=np.float64)
-        if simplices is None:
+        mesh_d = None
+        if simplices is not None:
+            simplices = G.orient(pts, simplices)
+        elif triangulation == "delaunay":
+            simplices, mesh_d = G.delaunay_triangles(pts, qhull_options)
+        elif triangulation == "hull":
             simplices = G.hull_triangles(pts)
         else:
-            simplices = G.orient(pts, simplices)
+            raise N.CmmError('triangulation must be "hull" or "delaunay"')
         V, S = G.refine(pts, simplices, refinement_levels)
         self.vertices = np.ascontiguousarray(V)
         self.simplices = np.ascontiguousarray(S, dtype=np.int64)
         self.x, self.y, self.z = V[:, 0], V[:, 1], V[:, 2]
-        self.mesh_d = None
+        self.mesh_d = mesh_d if refinement_levels == 0 else None
         self._ps = None
         self.stencil_pts()
         h = N.vp()

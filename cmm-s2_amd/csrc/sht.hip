@@ -26,7 +26,7 @@ using namespace cmm;
 
 namespace {
 constexpr int kScaleExp = 600;
-constexpr int kMaxFFT = 2048;
+constexpr int kMaxFFT = 4096;  // ring length 2L: L <= 2048 (the FFT's two LDS buffers take 32 N bytes)
 constexpr int kFwdSegHost = 128;  // degrees per forward work unit (kFwdSeg)
 constexpr int kOzJ = 16;          // split-BF16 synthesis: degrees of one parity per chunk
 constexpr int kOzMaxS = 7;        // split-BF16 synthesis: most digits per operand
@@ -170,18 +170,19 @@ extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
 
 extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin_t, cmm_sht_plan** out) {
   if (L < 2 || !cos_t || !sin_t || !out) return fail(CMM_E_ARG, "cmm_sht_plan_create: bad arguments");
-  if (2 * L > kMaxFFT) return fail(CMM_E_RANGE, "cmm_sht_plan_create: L > 1024");
+  if (2 * L > kMaxFFT) return fail(CMM_E_RANGE, "cmm_sht_plan_create: L > 2048");
   cmm_sht_plan* p = new cmm_sht_plan();
   p->L = L;
   p->N = 2 * L;
+  // ring FFT passes: radix 4, 2 and 3 butterflies, then every remaining prime
+  // factor as a direct DFT pass (any L: 2L = 2000 for L = 1000 takes three
+  // radix-5 passes, 2L = 2 p for a prime L one pass of length L)
   int n = p->N;
   while (n % 4 == 0) { p->radix[p->nrad++] = 4; n /= 4; }
   while (n % 2 == 0) { p->radix[p->nrad++] = 2; n /= 2; }
   while (n % 3 == 0) { p->radix[p->nrad++] = 3; n /= 3; }
-  if (n != 1) {
-    delete p;
-    return fail(CMM_E_RANGE, "cmm_sht_plan_create: 2L must be 2^a 3^b");
-  }
+  for (int f = 5; n > 1; f += 2)
+    while (n % f == 0) { p->radix[p->nrad++] = f; n /= f; }
   std::vector<double> xt(cos_t, cos_t + L + 1), st(sin_t, sin_t + L + 1);
   std::vector<double> xg, wg;
   gauss_legendre(L, xg, wg);
@@ -313,7 +314,8 @@ extern "C" int cmm_sht_workspace_bytes(const cmm_sht_plan* p, int nf, int64_t* b
 }
 
 // ---------------------------------------------------------------------------
-// FFT (Stockham, radix 4 / 2 / 3, in LDS; one ring pair per block)
+// FFT (Stockham, radix 4 / 2 / 3 butterflies and direct-DFT passes for the
+// other prime factors, in LDS; one ring pair per block)
 // ---------------------------------------------------------------------------
 struct FFTPlanDev {
   int N, nrad;
@@ -372,6 +374,33 @@ __device__ inline void fft_pass(const double2* __restrict__ tw, const double2* i
   }
 }
 
+// Stockham pass of any radix R as a direct DFT: output q of butterfly group
+// j (k = j mod Ns) is sum_r in[j + r N/R] W^(r (k + q Ns) N / (Ns R)), W =
+// exp(sign 2 pi i / N) -- the group twiddle and the R-point DFT root in one
+// table lookup; one thread per output, R complex products each.
+__device__ inline void fft_pass_dft(const double2* __restrict__ tw, const double2* in, double2* out, int N, int Ns,
+                                    int R, double sign) {
+  const int NR = N / R;
+  const int unit = N / (Ns * R);
+  for (int o = threadIdx.x; o < N; o += blockDim.x) {
+    const int j = o % NR, q = o / NR;
+    const int k = j % Ns;
+    const int e1 = ((k + q * Ns) * unit) % N;  // exponent step per r
+    double2 acc = {0.0, 0.0};
+    int e = 0;
+    for (int r = 0; r < R; ++r) {
+      double2 w = tw[e];
+      if (sign > 0) w.y = -w.y;
+      const double2 a = in[j + r * NR];
+      acc.x = fma(a.x, w.x, fma(-a.y, w.y, acc.x));
+      acc.y = fma(a.x, w.y, fma(a.y, w.x, acc.y));
+      e += e1;
+      if (e >= N) e -= N;
+    }
+    out[(j / Ns) * Ns * R + k + q * Ns] = acc;
+  }
+}
+
 // FFT of length N held in buf0 (ping-pong with buf1); sign -1 forward,
 // +1 inverse (unnormalised).  Returns the buffer holding the result.
 __device__ double2* fft_lds(const FFTPlanDev& P, const double2* __restrict__ tw, double2* buf0,
@@ -384,7 +413,8 @@ __device__ double2* fft_lds(const FFTPlanDev& P, const double2* __restrict__ tw,
     const int R = P.radix[pass];
     if (R == 4) fft_pass<4>(tw, in, out, N, Ns, sign);
     else if (R == 2) fft_pass<2>(tw, in, out, N, Ns, sign);
-    else fft_pass<3>(tw, in, out, N, Ns, sign);
+    else if (R == 3) fft_pass<3>(tw, in, out, N, Ns, sign);
+    else fft_pass_dft(tw, in, out, N, Ns, R, sign);
     __syncthreads();
     double2* t = in;
     in = out;
@@ -1739,7 +1769,7 @@ static int sht_forward(const cmm_sht_plan* p, const double* f, int nf, double* f
   CMM_LAUNCH_CHECK("cmm_sht_forward: resample");
   const int nh = (L + 1) / 2;
   const int T = std::max(64, ((nh + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
-  if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_forward: L > 1024");
+  if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_forward: L > 2048");
   size_t lds2 = (size_t)kFwdChunk * (T + 4) * sizeof(double2);
   hipLaunchKernelGGL(k_leg_forward<kFwdChunk>, dim3(p->nunits, nf), dim3(T), lds2, st, L, L, nf, p->units, p->x_g,
                      p->w_g, p->st_p, p->st_s, p->rec, G, reinterpret_cast<double2*>(flm), nullptr, 0);
@@ -1925,7 +1955,7 @@ extern "C" int cmm_sht_inverse_adjoint(const cmm_sht_plan* p, const double* fbar
     return CMM_OK;
   }
   const int T = std::max(64, ((nhr + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
-  if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_inverse_adjoint: L > 1024");
+  if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_inverse_adjoint: L > 2046");
   const size_t lds2 = (size_t)kFwdChunk * (T + 4) * sizeof(double2);
   hipLaunchKernelGGL(k_leg_forward<kFwdChunk>, dim3(p->nunits, nf), dim3(T), lds2, st, L, L + 1, nf, p->units,
                      p->x_t, p->w_r, p->st_pr, p->st_sr, p->rec, F, reinterpret_cast<double2*>(flm_bar), p->msc_r, 1);

@@ -40,9 +40,11 @@ def test_argument_errors_without_gpu():
     th = np.zeros(3)
     assert lib.cmm_mwss_mesh_create(1, N.dptr(th), N.dptr(th), N.dptr(th), N.dptr(th), N.dptr(th),
                                     N.dptr(th), ctypes.byref(h)) == -1
-    ct = np.zeros(2050)
+    ct = np.zeros(4098)
+    # any L up to 2048 (ring FFTs of any length 2L); beyond it the two LDS
+    # buffers of a ring FFT no longer fit
     assert lib.cmm_sht_plan_create(2049, N.dptr(ct), N.dptr(ct), ctypes.byref(h)) == -3
-    assert lib.cmm_sht_plan_create(1025, N.dptr(ct), N.dptr(ct), ctypes.byref(h)) == -3
+    assert lib.cmm_sht_plan_create(4097, N.dptr(ct), N.dptr(ct), ctypes.byref(h)) == -3
     vel = N.Velocity()
     vel.kind = 9
     assert lib.cmm_advect_points(ctypes.byref(vel), 2, 0.1, None, 0, None, None) == -1

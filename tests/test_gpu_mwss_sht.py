@@ -207,6 +207,48 @@ def test_sht_forward_non_band_limited(L):
         assert np.abs(got[:, fi] - ref_p).max() < 1e-12 * np.abs(ref_p).max(), fi
 
 
+@pytest.mark.parametrize("L", [37, 1000, 2048])
+def test_sht_any_bandlimit(L):
+    """Bandlimits whose ring length 2L has prime factors other than 2 and 3
+    (2 x 37; 2^4 x 5^3, the reference's upsampling grids Lf = 1000) and the
+    largest plan (L = 2048, the reference's Poisson test up to 2048):
+    synthesis of single Y_lm against scipy's sph_harm_y on the grid, the
+    Rossby-Haurwitz (5,4) Poisson closed form of the reference's test
+    (tests/test_spherical_harmonic_tools.py:48-56), and analysis of random
+    band-limited spectra returning them."""
+    from scipy.special import sph_harm_y
+    from cmm_s2.core.spherical_harmonic_tools import SHT, nlm
+    from helpers import mwss_samples
+    plan = SHT.get(L)
+    th = np.pi * np.arange(L + 1) / L
+    ph = np.pi * np.arange(2 * L) / L
+    picks = [(0, 0), (5, 4), (L - 1, 1), (L - 1, L - 1), (L // 2 + 3, L // 3)]
+    packed = np.zeros((nlm(L), len(picks), 2))
+    for f, (l, m) in enumerate(picks):
+        packed[m * L - m * (m - 1) // 2 + (l - m), f] = [1.0, 0.5]
+    g = plan.inverse(torch.from_numpy(packed).cuda()).cpu().numpy()
+    rows = np.arange(0, L + 1, max(1, L // 64))
+    for f, (l, m) in enumerate(picks):
+        Y = sph_harm_y(l, m, th[rows][:, None], ph[None, :])
+        ref = ((1.0 + 0.5j) * Y).real * (1 if m == 0 else 2)
+        assert np.abs(g[f, rows] - ref).max() < 1e-11 * max(1.0, np.abs(ref).max()), (l, m)
+    # Delta psi = w for the RH (5,4) wave: psi = -z((1-z^2)^2 - 8(1-z^2)x^2 + 8x^4)
+    w = mwss_samples(lambda T, P: 30 * np.cos(T) * np.cos(4 * P) * np.sin(T) ** 4, L)
+    w_lm = plan.forward(torch.from_numpy(w).cuda().reshape(1, L + 1, 2 * L))
+    dinv = np.zeros(L)
+    dinv[1:] = -1.0 / (np.arange(1, L) * (np.arange(1, L) + 1.0))
+    psi = plan.inverse(w_lm * torch.from_numpy(dinv[plan.l_of]).cuda()[:, None, None])[0].cpu().numpy()
+    x, y, z = [mwss_samples(f, L) for f in (lambda T, P: np.sin(T) * np.cos(P), lambda T, P: np.sin(T) * np.sin(P),
+                                               lambda T, P: np.cos(T) + 0 * P)]
+    sol = -z * ((1 - z ** 2) ** 2 - 8 * (1 - z ** 2) * x ** 2 + 8 * x ** 4)
+    assert rel_err(psi, sol) < 1e-11
+    rng = np.random.default_rng(L)
+    a = rng.standard_normal((nlm(L), 1, 2)) * (1.0 + plan.l_of[:, None, None]) ** -1.0
+    a[plan.m_of == 0, :, 1] = 0.0
+    back = plan.forward(plan.inverse(torch.from_numpy(a).cuda())).cpu().numpy()
+    assert rel_err(back, a) < 1e-11
+
+
 def test_project_stream_matches_oracle():
     from cmm_s2.core import mesh_functions as mf
     from cmm_s2.core.spherical_spline import spline_interp_velocity

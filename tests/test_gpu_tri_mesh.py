@@ -51,6 +51,27 @@ def test_locate_bit_exact_outside_ties(meshes):
     assert np.abs(rec - q).max() < 1e-14
 
 
+def test_delaunay_row_order_is_the_references():
+    """triangulation="delaunay" builds the mesh as the reference does
+    (Delaunay of the origin and the points, mesh_functions.py:41-56): the
+    simplices equal the oracle's row for row, and query()[1] returns the
+    reference's triangle indices directly (no vertex-triple remap)."""
+    from cmm_s2.core import mesh_functions as mf
+    from oracle import meshes as om
+    ico = mf.icosahedral_mesh(3)
+    pm = mf.spherical_triangulation(ico.points, triangulation="delaunay")
+    oms = om.SphericalTriangulation(ico.points)
+    assert np.array_equal(pm.simplices, oms.simplices)
+    assert pm.mesh_d is not None
+    q = random_unit(20000, 17)
+    bo, to, _ = oms.query(q)
+    b, t, _ = pm.query(torch.from_numpy(q).cuda())
+    t = t.cpu().numpy()
+    clear = om.orientation_margin(pm.vertices, pm.simplices, to, q) > T_TIE
+    assert (t[clear] == to[clear]).all()
+    assert np.abs(b.cpu().numpy()[clear] - bo[clear]).max() < 1e-12
+
+
 def test_assemble_and_eval(meshes):
     from oracle import splines as osp
     from cmm_s2.core.spherical_spline import sphere_diffeomorphism
