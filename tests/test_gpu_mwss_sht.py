@@ -222,7 +222,9 @@ def test_sht_any_bandlimit(L):
     plan = SHT.get(L)
     th = np.pi * np.arange(L + 1) / L
     ph = np.pi * np.arange(2 * L) / L
-    picks = [(0, 0), (5, 4), (L - 1, 1), (L - 1, L - 1), (L // 2 + 3, L // 3)]
+    # scipy's sph_harm_y overflows to NaN for low orders at degrees beyond
+    # ~900; those picks stay below it
+    picks = [(0, 0), (5, 4), (min(L - 1, 600), 1), (L - 1, L - 1), (min(L - 1, 503), min(L - 2, 333))]
     packed = np.zeros((nlm(L), len(picks), 2))
     for f, (l, m) in enumerate(picks):
         packed[m * L - m * (m - 1) // 2 + (l - m), f] = [1.0, 0.5]
@@ -230,6 +232,7 @@ def test_sht_any_bandlimit(L):
     rows = np.arange(0, L + 1, max(1, L // 64))
     for f, (l, m) in enumerate(picks):
         Y = sph_harm_y(l, m, th[rows][:, None], ph[None, :])
+        assert np.isfinite(Y).all(), (l, m)
         ref = ((1.0 + 0.5j) * Y).real * (1 if m == 0 else 2)
         assert np.abs(g[f, rows] - ref).max() < 1e-11 * max(1.0, np.abs(ref).max()), (l, m)
     # Delta psi = w for the RH (5,4) wave: psi = -z((1-z^2)^2 - 8(1-z^2)x^2 + 8x^4)
