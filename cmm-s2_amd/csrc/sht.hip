@@ -27,7 +27,7 @@ using namespace cmm;
 namespace {
 constexpr int kScaleExp = 600;
 constexpr int kMaxFFT = 4096;  // ring length 2L: L <= 2048 (the FFT's two LDS buffers take 32 N bytes)
-constexpr int kFwdSegHost = 128;  // degrees per forward work unit (kFwdSeg)
+constexpr int kFwdSegHost = 64;   // degrees per forward work unit (kFwdSeg)
 constexpr int kOzJ = 16;          // split-BF16 synthesis: degrees of one parity per chunk
 constexpr int kOzMaxS = 7;        // split-BF16 synthesis: most digits per operand
 }
@@ -1154,8 +1154,9 @@ __global__ __launch_bounds__(64) void k_resample(int L, int nmg, int tpad, const
 // pairs of the next chunk are staged through LDS during the current one.
 // Fixed summation order: deterministic.
 // ---------------------------------------------------------------------------
-constexpr int kFwdSeg = 128;
+constexpr int kFwdSeg = 64;
 constexpr int kFwdChunk = 16;
+constexpr int kFwdChunkA = 8;  // degrees per LDS chunk of the analysis kernel
 constexpr int kFwdNpt = 2;
 
 // state (P_{l-2}, P_{l-1}, s) before the chunk at every segment start
@@ -1770,8 +1771,12 @@ static int sht_forward(const cmm_sht_plan* p, const double* f, int nf, double* f
   const int nh = (L + 1) / 2;
   const int T = std::max(64, ((nh + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
   if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_forward: L > 2048");
-  size_t lds2 = (size_t)kFwdChunk * (T + 4) * sizeof(double2);
-  hipLaunchKernelGGL(k_leg_forward<kFwdChunk>, dim3(p->nunits, nf), dim3(T), lds2, st, L, L, nf, p->units, p->x_g,
+  // chunks of 8 degrees: half the LDS tile of the 16-degree chunks the
+  // restart table is built on (values and scale states agree: rescaling is
+  // exact and the restart points are whole chunks of either), so more
+  // blocks stay resident on this latency-bound kernel
+  size_t lds2 = (size_t)kFwdChunkA * (T + 4) * sizeof(double2);
+  hipLaunchKernelGGL(k_leg_forward<kFwdChunkA>, dim3(p->nunits, nf), dim3(T), lds2, st, L, L, nf, p->units, p->x_g,
                      p->w_g, p->st_p, p->st_s, p->rec, G, reinterpret_cast<double2*>(flm), nullptr, 0);
   CMM_LAUNCH_CHECK("cmm_sht_forward: legendre");
   return CMM_OK;

@@ -589,8 +589,10 @@ __global__ __launch_bounds__(256) void k_tri_assemble(cmm_tri_mesh M, const doub
   double cf[NC][19];
   if (t < M.ntri) {
     int vi[3] = {M.tris[3 * t], M.tris[3 * t + 1], M.tris[3 * t + 2]};
+    // Powell-Sabin geometry recomputed from the three vertices (72 bytes of
+    // gathers instead of the 600-byte stored table row)
     PSGeom P;
-    ps_unpack(M.psgeom + t, M.ntri, P);
+    ps_geometry(load_pt(M.verts, vi[0], 3, 1), load_pt(M.verts, vi[1], 3, 1), load_pt(M.verts, vi[2], 3, 1), P);
     const int64_t Nv = M.nv;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
