@@ -246,9 +246,9 @@ def main():
                      "gather_model_bytes": alg_bytes,
                      "gather_model_GBs": round(alg_bytes / (k_ms * 1e-3) / 1e9, 1),
                      "note": "achieved = compulsory HBM bytes per launch (points in and out, 48 B each, plus for "
-                             "every level the 144-B coefficient block of each (triangle, sub-triangle) some point "
-                             "lands in and the 128-B geometry line of each triangle some point lands in; the "
-                             "location cells are left out, so it is a lower bound) over the HIP-event kernel time; "
+                             "every level the 16-B location cell, the 128-B geometry line of each triangle and the "
+                             "144-B coefficient block of each (triangle, sub-triangle) some point lands in; a lower "
+                             "bound) over the HIP-event kernel time; "
                              "traffic = DRAM bytes per launch from rocprofv3 counters (traffic_source); the gather "
                              "model (48 + 216 x depth B per point) counts every lane's loads and exceeds the HBM "
                              "peak because neighbouring points share blocks in L2 / MALL; the kernel is bound by "
@@ -456,8 +456,8 @@ def area_distortion(sim, q):
     a = torch.cross(q, torch.tensor([0.3, -0.5, 0.8], dtype=q.dtype, device=q.device).expand_as(q), dim=1)
     t1 = a / a.norm(dim=1, keepdim=True)
     t2 = torch.cross(q, t1, dim=1)
-    u = torch.einsum("cdn,nd->nc", J, t1)
-    v = torch.einsum("cdn,nd->nc", J, t2)
+    u = (J * t1.t()[None]).sum(dim=1).t()  # J t1 per point, elementwise (no batched GEMM)
+    v = (J * t2.t()[None]).sum(dim=1).t()
     f = torch.cross(u, v, dim=1).norm(dim=1)
     dev = (f - 1).abs()
     out = {"points": int(q.shape[0]), "max_abs_dev": float(dev.max()), "rms_dev": float((dev ** 2).mean().sqrt())}
@@ -538,22 +538,43 @@ def mixed_run(a, mesh, w0, wf64, grid, s64, sim64, mode):
             "target_rel_err": 1e-10, "meets_target": bool(err <= 1e-10 and s32 < s64)}
 
 
+def _cube_cells(x, R):
+    """Cube-map location cell of every point (the rule of the device
+    locator's cube_cell: dominant axis and sign, then an R x R grid of the
+    face's tangent-plane coordinates)."""
+    import torch
+    a = x.abs()
+    k = torch.argmax(a, dim=1)  # first maximum, as the locator's strict comparisons
+    m = a.gather(1, k[:, None])[:, 0]
+    c = x.gather(1, k[:, None])[:, 0]
+    sg = (c < 0).long()
+    u = x.gather(1, ((k + 1) % 3)[:, None])[:, 0] / m
+    v = x.gather(1, ((k + 2) % 3)[:, None])[:, 0] / m
+    i = torch.clamp(torch.floor((u + 1.0) * 0.5 * R), 0, R - 1).long()
+    j = torch.clamp(torch.floor((v + 1.0) * 0.5 * R), 0, R - 1).long()
+    return ((2 * k + sg) * R + i) * R + j
+
+
 def compulsory_bytes(mesh, levels, q, grid):
     """Lower bound of the composition's HBM bytes per launch: every point is
-    read and written once (48 B), and for every level each coefficient block
-    (144 B: 3 components x 6 Bernstein-Bezier coefficients of one split
-    sub-triangle) and each triangle geometry line (128 B: dual basis and edge
-    norms) that some point lands in is read at least once.  The levels'
-    input points come from the composition's intermediates; their triangle
-    and sub-triangle from the device locator (first-occurrence argmin /
-    argmax of the barycentrics, the split rule)."""
+    read and written once (48 B), and for every level each location cell
+    (16 B: candidate count and the first three candidates), each triangle
+    geometry line (128 B: dual basis and edge norms) and each coefficient
+    block (144 B: 3 components x 6 Bernstein-Bezier coefficients of one
+    split sub-triangle) that some point lands in is read at least once
+    (further candidates of a cell and the DRAM line granularity are not
+    counted).  The levels' input points come from the composition's
+    intermediates; their cell from the locator's cube-map rule, their
+    triangle and sub-triangle from the device locator (first-occurrence
+    argmin / argmax of the barycentrics, the split rule)."""
     import torch
     from cmm_s2.core import _backend as B
     from cmm_s2.core.spherical_spline import compose_packed
     n, depth = q.shape[0], len(levels)
+    R = int(mesh.stats()["cells_per_face"])
     inter = B.empty(depth, n, 3)
     compose_packed(mesh, levels, q, "N3", B.empty(n, 3), inter=inter, grid=grid)
-    blocks, tris = [], []
+    blocks, tris, cells = [], [], []
     for i in range(depth):
         x = q if i == 0 else inter[i - 1]
         tri, bcc = mesh.locate(x, "N3")
@@ -562,9 +583,11 @@ def compulsory_bytes(mesh, levels, q, grid):
         sub = 2 * mn + mx - (mx > mn).long()
         blocks.append(int(torch.unique(tri.long() * 6 + sub).numel()))
         tris.append(int(torch.unique(tri).numel()))
+        cells.append(int(torch.unique(_cube_cells(x, R)).numel()))
     del inter
-    total = 48 * n + sum(144 * b + 128 * t for b, t in zip(blocks, tris))
-    return {"bytes": total, "points": n, "blocks_per_level": blocks, "triangles_per_level": tris}
+    total = 48 * n + sum(16 * c + 128 * t + 144 * b for c, t, b in zip(cells, tris, blocks))
+    return {"bytes": total, "points": n, "cells_per_level": cells, "triangles_per_level": tris,
+            "blocks_per_level": blocks}
 
 
 def _host_info():

@@ -123,8 +123,8 @@ def test_full_size_properties(run):
     a = torch.cross(q, torch.tensor([0.3, -0.5, 0.8], dtype=q.dtype, device=q.device).expand_as(q), dim=1)
     t1 = a / a.norm(dim=1, keepdim=True)
     t2 = torch.cross(q, t1, dim=1)
-    u = torch.einsum("cdn,nd->nc", J, t1)
-    v = torch.einsum("cdn,nd->nc", J, t2)
+    u = (J * t1.t()[None]).sum(dim=1).t()
+    v = (J * t2.t()[None]).sum(dim=1).t()
     f = torch.cross(u, v, dim=1).norm(dim=1)
     dev = (f - 1).abs()
     assert float(dev.max()) < 5e-3 and float((dev ** 2).mean().sqrt()) < 2e-4
