@@ -43,7 +43,7 @@ def _profile_value(path, key):
         with open(path) as f:
             for line in f:
                 if line.startswith(key + " = "):
-                    return float(line.split("=", 1)[1].split()[0])
+                    return float(line[len(key) + 3:].split()[0])
     except OSError:
         pass
     return None

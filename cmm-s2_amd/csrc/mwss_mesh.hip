@@ -368,16 +368,20 @@ struct VelDev {
   double params[8];
 };
 
+// SPLINE: the kernels are instantiated twice, for the spline velocity alone
+// (the CMM loop: no registers spent on the analytic fields) and for the
+// analytic fields
+template <bool SPLINE>
 __device__ __forceinline__ d3 vel_eval(const VelDev& V, int stage, d3 x) {
-  switch (V.kind) {
-    case CMM_VEL_SPLINE: {
-      double r[3];
-      if (!mwss_eval_levels(V.mesh, V.ls[stage], 3, x, r)) {
-        double nan = __builtin_nan("");
-        return d3{nan, nan, nan};
-      }
-      return d3{r[0], r[1], r[2]};
+  if constexpr (SPLINE) {
+    double r[3];
+    if (!mwss_eval_levels(V.mesh, V.ls[stage], 3, x, r)) {
+      double nan = __builtin_nan("");
+      return d3{nan, nan, nan};
     }
+    return d3{r[0], r[1], r[2]};
+  }
+  switch (V.kind) {
     case CMM_VEL_SBR: return vel_sbr(x, V.params[0], V.params[1]);
     case CMM_VEL_DEFORM_ROT: return vel_deform_rot(V.stage_t[stage], x, V.params[1]);
     case CMM_VEL_RH4: return vel_rh4(V.stage_t[stage], x, V.params[0], V.params[1], V.params[2]);
@@ -409,7 +413,9 @@ __device__ inline d3 proj_norm(d3 x) {
 }
 
 // one backward step of integrator `integ` for the point xn
+template <bool SPLINE>
 __device__ __forceinline__ d3 advect_point(const VelDev& V, int integ, double dt, d3 xn) {
+  auto vel_eval = [&](const VelDev& W, int stage, d3 x) { return ::vel_eval<SPLINE>(W, stage, x); };
   if (integ == CMM_INT_EVAL) return vel_eval(V, 0, xn);
   if (integ == CMM_INT_EULER) {
     d3 v1 = vel_eval(V, 0, xn);
@@ -439,11 +445,12 @@ __device__ __forceinline__ d3 advect_point(const VelDev& V, int integ, double dt
   return proj_norm(sub(xn, scl(dt / 6, acc)));
 }
 
+template <bool SPLINE>
 __global__ __launch_bounds__(128) void k_advect(VelDev V, int integ, double dt, const double* __restrict__ x, int64_t m,
                                                 double* __restrict__ y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const d3 out = advect_point(V, integ, dt, d3{x[i], x[m + i], x[2 * m + i]});
+  const d3 out = advect_point<SPLINE>(V, integ, dt, d3{x[i], x[m + i], x[2 * m + i]});
   y[i] = out.x;
   y[m + i] = out.y;
   y[2 * m + i] = out.z;
@@ -453,7 +460,7 @@ __global__ __launch_bounds__(128) void k_advect(VelDev V, int integ, double dt, 
 // (vertex v = i / P, slot k = i % P), slot 0 the vertex itself when P = 5,
 // the other slots its four stencil points, so the feet of one vertex sit in
 // neighbouring lanes and their velocity gathers share cache lines
-template <int P>
+template <int P, bool SPLINE>
 __global__ __launch_bounds__(128) void k_advect_stencil(VelDev V, int integ, double dt, const double* __restrict__ xv,
                                                         int64_t xv_row, const double* __restrict__ xs, int64_t xs_row,
                                                         int64_t nv, double* __restrict__ yv, int64_t yv_row,
@@ -464,7 +471,7 @@ __global__ __launch_bounds__(128) void k_advect_stencil(VelDev V, int integ, dou
   const int k = (int)(i - v * P) - (P - 4);  // stencil index, -1 = vertex
   const double* src = k < 0 ? xv + v : xs + (int64_t)k * nv + v;
   const int64_t row = k < 0 ? xv_row : xs_row;
-  const d3 out = advect_point(V, integ, dt, d3{src[0], src[row], src[2 * row]});
+  const d3 out = advect_point<SPLINE>(V, integ, dt, d3{src[0], src[row], src[2 * row]});
   double* dst = k < 0 ? yv + v : ys + (int64_t)k * nv + v;
   const int64_t orow = k < 0 ? yv_row : ys_row;
   dst[0] = out.x;
@@ -1658,7 +1665,10 @@ extern "C" int cmm_advect_points(const cmm_velocity* vel, int integ, double dt, 
   int rc = velocity_device(vel, "cmm_advect_points", V);
   if (rc) return rc;
   if (m == 0) return CMM_OK;
-  hipLaunchKernelGGL(k_advect, dim3(grid_for(m, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, x, m, y);
+  if (V.kind == CMM_VEL_SPLINE)
+    hipLaunchKernelGGL(k_advect<true>, dim3(grid_for(m, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, x, m, y);
+  else
+    hipLaunchKernelGGL(k_advect<false>, dim3(grid_for(m, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, x, m, y);
   CMM_LAUNCH_CHECK("cmm_advect_points");
   return CMM_OK;
 }
@@ -1673,12 +1683,12 @@ extern "C" int cmm_advect_stencil(const cmm_velocity* vel, int integ, double dt,
   int rc = velocity_device(vel, "cmm_advect_stencil", V);
   if (rc) return rc;
   if (nv == 0) return CMM_OK;
-  if (verts)
-    hipLaunchKernelGGL(k_advect_stencil<5>, dim3(grid_for(5 * nv, 128)), dim3(128), 0, as_stream(stream), V, integ,
-                       dt, verts, verts_row, st, st_row, nv, yv, yv_row, yst, yst_row);
-  else
-    hipLaunchKernelGGL(k_advect_stencil<4>, dim3(grid_for(4 * nv, 128)), dim3(128), 0, as_stream(stream), V, integ,
-                       dt, verts, verts_row, st, st_row, nv, yv, yv_row, yst, yst_row);
+  const bool sp = V.kind == CMM_VEL_SPLINE;
+  auto kern = verts ? (sp ? k_advect_stencil<5, true> : k_advect_stencil<5, false>)
+                    : (sp ? k_advect_stencil<4, true> : k_advect_stencil<4, false>);
+  const int P = verts ? 5 : 4;
+  hipLaunchKernelGGL(kern, dim3(grid_for(P * nv, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, verts,
+                     verts_row, st, st_row, nv, yv, yv_row, yst, yst_row);
   CMM_LAUNCH_CHECK("cmm_advect_stencil");
   return CMM_OK;
 }
