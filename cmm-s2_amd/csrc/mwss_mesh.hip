@@ -750,25 +750,31 @@ __global__ __launch_bounds__(256) void k_sample_vorticity(VortArgs A, cmm_mwss_m
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   d3 x = load_pt(X, i, sp, sc);
-  // utils.cart2sphere (utils.py:18-23)
-  double phi = npy_mod(atan2(x.y, x.x) + 2 * M_PI, 2 * M_PI);
-  double theta = atan2(sqrt(x.y * x.y + x.x * x.x), x.z);
-  double w0;
-  if (A.kind == CMM_VORT_SH_SERIES) {
-    w0 = sh_series(wpacked, (int)A.params[0], phi, theta);
-  } else if (A.kind == CMM_VORT_SPLINE) {
-    // spherical_spline.__call__ (spherical_spline.py:42-70): back to xyz
-    d3 q = d3{sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
+  double w0, cth;
+  if (A.kind == CMM_VORT_SPLINE) {
+    // spherical_spline.__call__ (spherical_spline.py:42-70) takes the angles
+    // of X (cart2sphere, utils.py:18-23) back to xyz and locates that point:
+    // the round trip is X / |X| (to rounding), located here directly, and
+    // cos theta(X) = z / |X| (differences of a few ulp, no transcendental
+    // functions before the location's own angles)
+    const double nn = sqrt(x.x * x.x + x.y * x.y + x.z * x.z);
+    const d3 q = d3{x.x / nn, x.y / nn, x.z / nn};
     LevelSet ls;
     ls.p[0] = wpacked; ls.w[0] = 1.0; ls.nlev = 1;
     double r[1];
     if (!mwss_eval_levels(W, ls, 1, q, r)) r[0] = __builtin_nan("");
     w0 = r[0];
+    cth = q.z;
   } else {
-    w0 = vort_analytic(A.kind, phi, theta, A.params);
+    // utils.cart2sphere (utils.py:18-23)
+    const double phi = npy_mod(atan2(x.y, x.x) + 2 * M_PI, 2 * M_PI);
+    const double theta = atan2(sqrt(x.y * x.y + x.x * x.x), x.z);
+    w0 = A.kind == CMM_VORT_SH_SERIES ? sh_series(wpacked, (int)A.params[0], phi, theta)
+                                      : vort_analytic(A.kind, phi, theta, A.params);
+    cth = cos(theta);
   }
   int ring = (int)(i / A.ring_len);
-  out[i] = w0 + 2 * A.rot * cos(theta) - 2 * A.rot * cosg[ring];
+  out[i] = w0 + 2 * A.rot * cth - 2 * A.rot * cosg[ring];
 }
 
 // ---------------------------------------------------------------------------
@@ -1504,7 +1510,8 @@ __global__ __launch_bounds__(128) void k_advect_stencil_adj(VelDev V, int integ,
 
 // Adjoint of k_sample_vorticity for a spline w0: xbar (overwrite, may be
 // null) and the adjoint of the w0 coefficients (accumulate, may be null).
-// q = x / |x| and cos theta(x) = z / |x| as in the forward sampling.
+// The spline is located at q = x / |x| and cos theta(x) = z / |x|, as in the
+// forward sampling.
 __global__ __launch_bounds__(256) void k_sample_vorticity_adj(cmm_mwss_mesh W, const double* __restrict__ wpacked,
                                                               const double* __restrict__ X, int64_t n, int64_t sp,
                                                               int64_t sc, const double* __restrict__ wbar, double rot,
@@ -1513,9 +1520,8 @@ __global__ __launch_bounds__(256) void k_sample_vorticity_adj(cmm_mwss_mesh W, c
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const d3 x = load_pt(X, i, sp, sc);
-  const double phi = npy_mod(atan2(x.y, x.x) + 2 * M_PI, 2 * M_PI);
-  const double theta = atan2(sqrt(x.y * x.y + x.x * x.x), x.z);
-  const d3 q = d3{sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
+  const double nq = sqrt(x.x * x.x + x.y * x.y + x.z * x.z);
+  const d3 q = d3{x.x / nq, x.y / nq, x.z / nq};
   const double wb = wbar[i];
   const VLoc r = vel_locate(W, q);
   d3 gq = d3{0, 0, 0};

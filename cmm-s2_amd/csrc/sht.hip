@@ -1327,8 +1327,10 @@ __device__ inline double Lm(int l, int m) {
 
 // angular_momentum at (l, m): returns [i Lx a, i Ly a, i Lz a]
 template <class F>
-__device__ inline void angmom(int l, int m, F get, Cplx out[3]) {
-  Cplx am1 = coef_any(l, m - 1, get), ap1 = coef_any(l, m + 1, get), a0 = coef_any(l, m, get);
+__device__ inline void angmom(int l, int m, F get, Cplx out[3]);
+
+// angmom from the three coefficients it reads: (l, m-1), (l, m+1), (l, m)
+__device__ inline void angmom_from(int l, int m, Cplx am1, Cplx ap1, Cplx a0, Cplx out[3]) {
   double lp = Lp(l, m), lmn = Lm(l, m);
   Cplx P = Cplx{lp * am1.re, lp * am1.im};
   Cplx Mi = Cplx{lmn * ap1.re, lmn * ap1.im};
@@ -1339,6 +1341,11 @@ __device__ inline void angmom(int l, int m, F get, Cplx out[3]) {
   out[0] = Cplx{0.0 * ox.re - 1.0 * ox.im, 0.0 * ox.im + 1.0 * ox.re};
   out[1] = Cplx{0.0 * oy.re - 1.0 * oy.im, 0.0 * oy.im + 1.0 * oy.re};
   out[2] = Cplx{0.0 * az.re - 1.0 * az.im, 0.0 * az.im + 1.0 * az.re};
+}
+
+template <class F>
+__device__ inline void angmom(int l, int m, F get, Cplx out[3]) {
+  angmom_from(l, m, coef_any(l, m - 1, get), coef_any(l, m + 1, get), coef_any(l, m, get), out);
 }
 
 __global__ __launch_bounds__(128) void k_stream_ops(int L, const double2* __restrict__ w, const double* __restrict__ dinv,
@@ -1362,17 +1369,26 @@ __global__ __launch_bounds__(128) void k_stream_ops(int L, const double2* __rest
     double dd = apply_dinv ? dinv[ll] : 1.0;
     return apply_dinv ? Cplx{v.x * dd, v.y * dd} : Cplx{v.x, v.y};
   };
-  Cplx u[3];
+  // u = (r x grad) psi at (l, m - 1), (l, m), (l, m + 1) once each, with the
+  // conjugate symmetry of a real field (coef_any) for m - 1 = -1; then the
+  // second application to each component reuses them
+  Cplx u[3], up[3], um[3];
   angmom(l, m, psi, u);
+  const Cplx z = cz();
+  if (m + 1 <= l) angmom(l, m + 1, psi, up);
+  else up[0] = up[1] = up[2] = z;
+  if (m >= 1) {
+    angmom(l, m - 1, psi, um);
+  } else if (l >= 1) {  // (l, -1) from (l, 1): (-1)^1 conj
+    angmom(l, 1, psi, um);
+    for (int i = 0; i < 3; ++i) um[i] = Cplx{-um[i].re, um[i].im};
+  } else {
+    um[0] = um[1] = um[2] = z;
+  }
   for (int i = 0; i < 3; ++i) out[idx * 12 + i] = double2{u[i].re, u[i].im};
   for (int i = 0; i < 3; ++i) {
-    auto ui = [&](int ll, int mm) -> Cplx {
-      Cplx r[3];
-      angmom(ll, mm, psi, r);
-      return r[i];
-    };
     Cplx g[3];
-    angmom(l, m, ui, g);
+    angmom_from(l, m, um[i], up[i], u[i], g);
     for (int k = 0; k < 3; ++k) out[idx * 12 + 3 + 3 * i + k] = double2{g[k].re, g[k].im};
   }
 }
