@@ -1824,7 +1824,7 @@ extern "C" int cmm_sht_scalar_operators(const cmm_sht_plan* p, const double* w, 
 // S_t = G_t + G_{L-t} for even l - m, D_t = G_t - G_{L-t} for odd.  One block
 // per (order m, chunk c of 32 degrees; blockIdx.x = offJ[m] + c) and group of
 // <= 16 fields; wave w takes northern rings 64 w .. 64 w + 63 (lane = ring),
-// restarts the recurrence from the analysis restart state of its 128-degree
+// restarts the recurrence from the analysis restart state of its 64-degree
 // unit, writes the 16 even-degree values of the chunk into an LDS tile
 // [degree][ring] and multiplies (v_mfma_f64_16x16x4f64: A = 16 degrees x 4
 // rings from the tile, B = 4 rings x 16 columns of S from G), then the odd
