@@ -162,6 +162,9 @@ def main():
         s_step = statistics.median(step_t[1:] if len(step_t) > 2 else step_t)
         log(f"bootstrap {t_boot:.3f}s, steady step median {s_step * 1e3:.3f} ms, remap steps "
             f"{[round(x * 1e3, 3) for x in remap_t]} ms, total loop {sum(step_t) + sum(remap_t):.3f}s")
+        if sim.U.sparse:
+            log("velocity levels: blocks assembled on "
+                f"{[round(sim.U.valid_fraction(i) * 100, 1) for i in range(3)]} % of the triangles")
         if a.save_maps:
             torch.save(sim.submaps.vals[:sim.submaps.Nc].cpu(), a.save_maps)
 
@@ -518,6 +521,8 @@ def mixed_run(a, mesh, w0, wf64, grid, s64, sim64, mode):
     w32 = sim.vorticity_on_grid(a.Lf, grid=grid)
     err = float((w32 - wf64).abs().max() / wf64.abs().max())
     rel = lambda x, y: float((x - y).abs().max() / y.abs().max())
+    sim.U.complete()
+    sim64.U.complete()
     dinv = torch.tensor([0.0] + [-1.0 / (l * (l + 1)) for l in range(1, a.L)], dtype=torch.float64,
                         device="cuda")[torch.from_numpy(sim.plan.l_of).cuda()]
     per = {"psi_lm": rel(sim.w_lm[:, 0] * dinv[:, None], sim64.w_lm[:, 0] * dinv[:, None]),

@@ -24,13 +24,22 @@ CFLAGS = [
 ]
 
 
+# Per-file flags.  mwss_mesh.hip evaluates the velocity spline in several
+# kernel instantiations and rebuilds missing velocity blocks outside the
+# assembly kernel; with contraction decided per source expression
+# (-ffp-contract=on: a * b + c fused as written, never across statements)
+# every copy of a formula rounds the same way, so all of them give the same
+# bits.
+FILE_FLAGS = {"mwss_mesh.hip": ["-ffp-contract=on"]}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *CFLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

@@ -33,7 +33,8 @@ VORT_SH_SERIES = 11
 class Velocity(C.Structure):
     """struct cmm_velocity."""
     _fields_ = [("kind", C.c_int), ("nlev", C.c_int), ("mesh", vp), ("levels", vp * 3),
-                ("stage_w", (dbl * 3) * 4), ("stage_t", dbl * 4), ("params", dbl * 8)]
+                ("stage_w", (dbl * 3) * 4), ("stage_t", dbl * 4), ("params", dbl * 8),
+                ("vbits", vp), ("vslot", C.c_int * 3), ("jets", vp * 3), ("mark", vp), ("mark_every", C.c_int)]
 
 
 # name -> (restype, argtypes)
@@ -58,6 +59,7 @@ SIGNATURES = {
     "cmm_mwss_locate": (C.c_int, [vp, vp, i64, i64, i64, vp, vp, vp]),
     "cmm_mwss_split_locate": (C.c_int, [vp, vp, i64, i64, i64, vp, vp, vp, vp, vp, vp]),
     "cmm_mwss_assemble": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
+    "cmm_mwss_assemble_sparse": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp]),
     "cmm_mwss_eval": (C.c_int, [vp, P_vp, P_dbl, C.c_int, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp]),
     "cmm_advect_points": (C.c_int, [C.POINTER(Velocity), C.c_int, dbl, vp, i64, vp, vp]),
     "cmm_advect_stencil": (C.c_int, [C.POINTER(Velocity), C.c_int, dbl, vp, i64, vp, i64, i64, vp, i64, vp, i64,

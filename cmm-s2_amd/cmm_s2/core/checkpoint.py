@@ -100,7 +100,7 @@ def load_state(cls, path, mesh, vorticity=None):
     uv = B.to_dev(tensors["U_vals"])
     for k in range(3):
         U.level_vals(k).copy_(uv[k])
-        U.init_coeffs(k)
+        U.init_coeffs(k, complete=True)
     U.ts = np.array(meta["ts"], dtype=np.float64)
     U.nVs = int(meta["nVs"])
     U.stepping = bool(meta["stepping"])

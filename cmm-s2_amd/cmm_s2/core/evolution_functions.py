@@ -35,6 +35,13 @@ def velocity_descriptor(V, stage_times, dt):
         d.mesh = V.mesh._h
         for k in range(nl):
             d.levels[k] = V.level_packed(k).data_ptr()
+            if V.sparse:
+                d.vslot[k] = V.level_slot(k)
+                d.jets[k] = V.level_vals(k).data_ptr()
+        if V.sparse:
+            d.vbits = V._vbits.data_ptr()
+            d.mark = V._mark.data_ptr()
+            d.mark_every = int(V.mark_every)
         for s, ts in enumerate(stage_times):
             w = V.weights(ts, dt)
             for k in range(3):

@@ -87,7 +87,10 @@ class VorticitySampler(object):
 class BarotropicCMM(object):
     """State of euler_simulation_sphere_remapping (spherical_simulations.py:144-297)."""
 
-    def __init__(self, L, Nt, T, mesh, vorticity, rot_rate=2 * np.pi, n_maps=10, legendre="fp64"):
+    RESEED = 16
+
+    def __init__(self, L, Nt, T, mesh, vorticity, rot_rate=2 * np.pi, n_maps=10, legendre="fp64",
+                 sparse_velocity=True):
         self.L, self.Nt, self.T, self.mesh = L, Nt, T, mesh
         self.legendre = legendre
         self.rot = rot_rate
@@ -102,6 +105,9 @@ class BarotropicCMM(object):
         self.mesh_u = structure_spherical_triangulation(L)
         self.U = spline_interp_velocity(self.mesh_u, B.zeros(3, 3, 4, self.mesh_u.nv), None,
                                         ts=self.tspan[0:3])
+        # velocity levels assembled where the map's feet fall (the same bits
+        # as full levels; see spline_interp_velocity)
+        self.U.sparse = bool(sparse_velocity)
         self.plan = SHT.get(L, legendre)
         self.sampler = VorticitySampler(vorticity, self.mesh_u, rot_rate)
         self.samples = self.mesh_u.sample_points()
@@ -140,6 +146,7 @@ class BarotropicCMM(object):
         """spherical_simulations.py:216-261.  ``tune`` times the composition
         tile shapes (host synchronisation; pass False under stream capture)."""
         U, dt = self.U, self.dt
+        U.mark_all()
         self._velocity_from(self.samples, 0, rot=0.0)
         int0 = evol.advect_project_sphere(self.map0, evol.Euler_step_proj, 0, dt, U, identity=True)
         int0.eval(self.samples, "N3", out=self.X)
@@ -172,6 +179,10 @@ class BarotropicCMM(object):
         """One loop iteration (spherical_simulations.py:263-294).  Returns True
         when this step ended with a remap."""
         t = self.tspan[self.count]
+        # sparse velocity levels: every RESEED-th step rebuilds the assembled
+        # set from all of the step's evaluations; in between it grows by the
+        # triangles some evaluation missed
+        self.U.mark_every = self.count % self.RESEED == 0
         self.curr = evol.advect_project_sphere(self.curr, evol.RK4_proj, t, self.dt, self.U,
                                                identity=self.identity)
         self.submaps.compose(self.samples, "N3", out=self.X, pre=[self.curr.packed], grid=(self.L + 1, 2 * self.L))

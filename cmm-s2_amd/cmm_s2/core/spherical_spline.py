@@ -296,7 +296,18 @@ class spline_interp_velocity(object):
     """2+1-D velocity spline (spherical_spline.py:545-655) on an MWSS mesh.
     vals (3 levels, 3, 4, Nu); coeffs optional; ts the time grid.  Time levels
     live in a 3-slot ring: ``shift()`` rotates slots instead of copying the
-    1.4 GB of level data (spherical_harmonic_tools.py:73-76 copies)."""
+    1.4 GB of level data (spherical_harmonic_tools.py:73-76 copies).
+
+    ``sparse = True`` (the CMM loop sets it): a level's packed blocks are
+    written only on a set of triangles and the rows next to the poles
+    (``cmm_mwss_assemble_sparse``), with a validity bit per slot and
+    triangle; an evaluation elsewhere builds its block from the level's jets
+    with the same arithmetic, so results do not depend on the set.  With
+    ``mark_every`` the set is the triangles the evaluations since the
+    previous level located; otherwise it is the previous level's set plus
+    the triangles whose block some evaluation missed.  The feet of a fixed
+    set of points move little from one step to the next, so most evaluations
+    find their block assembled."""
 
     def __init__(self, mesh, vals, coeffs, ts):
         self.mesh = mesh
@@ -307,6 +318,18 @@ class spline_interp_velocity(object):
         self.ts = np.array(ts, dtype=np.float64)
         self.stepping = False
         self.nVs = 1
+        self.sparse = False
+        self._vbits = torch.full((mesh.ntri,), 7, dtype=torch.uint8, device=self._packed.device)
+        self._mark = torch.ones(mesh.ntri, dtype=torch.uint8, device=self._packed.device)
+        self.mark_every = True
+        self._newest = -1  # physical slot of the last sparse assembly
+
+    def _assemble_sparse(self, i, mark):
+        inherit = -1 if (mark is None or self.mark_every or self._newest == self._order[i]) else self._newest
+        N.call("cmm_mwss_assemble_sparse", self.mesh._h, self.level_vals(i).data_ptr(), self.level_packed(i).data_ptr(),
+               self._vbits.data_ptr(), self._order[i], inherit, B.ptr(mark), B.stream())
+        if mark is not None:
+            self._newest = self._order[i]
 
     # logical level access
     def level_vals(self, i):
@@ -314,6 +337,29 @@ class spline_interp_velocity(object):
 
     def level_packed(self, i):
         return self._packed[self._order[i]]
+
+    def level_slot(self, i):
+        return self._order[i]
+
+    def valid_fraction(self, i):
+        """Fraction of triangles whose packed blocks of level i are current."""
+        return int(((self._vbits >> self._order[i]) & 1).sum()) / self.mesh.ntri
+
+    def mark_all(self):
+        """Make the next sparse assembly write every triangle and the
+        evaluations mark every triangle they locate (a torch fill kernel: no
+        memset, so it may run under stream capture)."""
+        self._mark.fill_(1)
+        self.mark_every = True
+        self._newest = -1
+
+    def complete(self):
+        """Assemble the triangles every level still lacks (before callers
+        that read the packed blocks directly)."""
+        if not self.sparse:
+            return
+        for i in range(3):
+            self._assemble_sparse(i, None)
 
     def shift(self):
         """Logical levels (0,1,2) <- (1,2,stale); level 2 is rewritten next."""
@@ -330,12 +376,17 @@ class spline_interp_velocity(object):
             _assemble(self.mesh, self.level_vals(i), 3, c[i], None)
         return c
 
-    def init_coeffs(self, i):
-        """spherical_spline.py:565-568."""
+    def init_coeffs(self, i, complete=False):
+        """spherical_spline.py:565-568.  With ``sparse`` (and not
+        ``complete``) only the triangles marked since the last call."""
         dev = None
         if self._coeffs_buf is not None:
             dev, host = _canonical_out(self._coeffs_buf[i], (3, 19, self.mesh.ntri))
+        if self.sparse and dev is None and not complete:
+            self._assemble_sparse(i, self._mark)
+            return
         _assemble(self.mesh, self.level_vals(i), 3, dev, self.level_packed(i))
+        self._vbits.bitwise_or_(1 << self._order[i])
         if dev is not None:
             _write_back(dev, host)
 
@@ -346,6 +397,7 @@ class spline_interp_velocity(object):
         return min(self.nVs, 3) if self.nVs in (1, 2, 3) else 3
 
     def eval(self, t, dt, q, layout="3N", out=None):
+        self.complete()
         n = q.shape[0] if layout == "N3" else q.shape[1]
         if out is None:
             out = B.empty(n, 3) if layout == "N3" else B.empty(3, n)

@@ -97,10 +97,23 @@ __host__ __device__ inline int split_index(double b0, double b1, double b2) {
 // Coefficient indices of the 6 split sub-triangles, by split_index order
 // (argmin, argmax) = (0,1),(0,2),(1,0),(1,2),(2,0),(2,1): rows of the Cs table
 // of spherical_spline.py:660-662.
-__host__ __device__ inline int cs_table(int s, int j) {
-  const int T[6][6] = {{18, 1, 13, 7, 6, 16}, {13, 2, 18, 11, 10, 16}, {18, 14, 0, 17, 5, 4},
-                       {14, 18, 2, 17, 10, 9}, {0, 12, 18, 3, 15, 4},  {1, 18, 12, 7, 15, 8}};
+__host__ __device__ constexpr int cs_table(int s, int j) {
+  constexpr int T[6][6] = {{18, 1, 13, 7, 6, 16}, {13, 2, 18, 11, 10, 16}, {18, 14, 0, 17, 5, 4},
+                           {14, 18, 2, 17, 10, 9}, {0, 12, 18, 3, 15, 4},  {1, 18, 12, 7, 15, 8}};
   return T[s][j];
+}
+
+// row s of the Cs table as six 5-bit fields (entry j at bit 5 j), selected
+// without an indexed local array
+__host__ __device__ constexpr uint32_t cs_row_bits(int s) {
+  uint32_t c = 0;
+  for (int j = 0; j < 6; ++j) c |= (uint32_t)cs_table(s, j) << (5 * j);
+  return c;
+}
+__host__ __device__ inline uint32_t cs_code(int s) {
+  constexpr uint32_t c0 = cs_row_bits(0), c1 = cs_row_bits(1), c2 = cs_row_bits(2), c3 = cs_row_bits(3),
+                     c4 = cs_row_bits(4), c5 = cs_row_bits(5);
+  return s == 0 ? c0 : s == 1 ? c1 : s == 2 ? c2 : s == 3 ? c3 : s == 4 ? c4 : c5;
 }
 
 // Barycentrics of q in split sub-triangle s from the macro barycentrics b and

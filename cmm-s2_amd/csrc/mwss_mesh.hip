@@ -217,14 +217,146 @@ struct LevelSet {
   int nlev;
 };
 
-// sum_k w_k * PS_k(q) for ncomp components
+// Velocity levels assembled on a subset of triangles (cmm_mwss_assemble_sparse):
+// bit slot[k] of vbits[t] is set where level k's packed blocks of triangle t
+// are current; jets[k] (3, 4, Nu) is the level's source; mark (may be NULL)
+// collects every triangle an evaluation locates.
+struct SparseLv {
+  const uint8_t* vbits;
+  int slot[3];
+  const double* jets[3];
+  uint8_t* mark;
+  int mark_every;  // 0: mark only the triangles with a missing block
+};
+
+// (row, half) of the per-row geometry table and the column whose angle
+// rotates it, for MWSS triangle t (the ordering of mwss_tri_vertices)
+__device__ inline void mwss_tri_rowcol(const cmm_mwss_mesh& M, int64_t t, int& row, int& half, int& col) {
+  const int N = M.N;
+  if (t < N) { row = 0; col = (int)t; half = 0; }
+  else if (t >= M.Tu - N) { row = M.M - 2; col = (int)(t - (M.Tu - N)); half = 0; }
+  else {
+    const int64_t u = t - N;
+    half = (int)(u % 2);
+    const int64_t k = u / 2;
+    row = 1 + (int)(k / N);
+    col = (int)(k % N);
+  }
+}
+
+// Coefficient kk < 12 of component cc of triangle t's Powell-Sabin spline
+// from the jets J (3, 4, Nu): vertex values and the c_{3+k} formulas of
+// ps_coeffs with the column-rotated row geometry g (ps_pack layout: H at 0,
+// G at 27, E at 57, 1/G at 66), written exactly as k_mwss_assemble
+// evaluates them so that both give the same bits.
+__device__ inline double mwss_coeff_direct(const cmm_mwss_mesh& M, const double* __restrict__ J,
+                                           const double* __restrict__ g, double c, double sn, int64_t v0,
+                                           int64_t v1, int64_t v2, int cc, int kk) {
+  const int64_t Nu = M.Nu;
+  if (kk < 3) return J[(int64_t)(cc * 4) * Nu + (kk == 0 ? v0 : (kk == 1 ? v1 : v2))];
+  const int k9 = kk - 3, i = k9 / 3;
+  const int r = (int)((0x385172460ull >> (4 * k9)) & 0xF);  // ps_div_row(k9)
+  const int64_t vi = i == 0 ? v0 : (i == 1 ? v1 : v2);
+  const d3 h0 = d3{g[3 * r], g[3 * r + 1], g[3 * r + 2]};
+  const d3 h = d3{c * h0.x - sn * h0.y, sn * h0.x + c * h0.y, h0.z};
+  const double f = J[(int64_t)(cc * 4) * Nu + vi];
+  const d3 gr = d3{J[(int64_t)(0 * 4 + 1 + cc) * Nu + vi], J[(int64_t)(1 * 4 + 1 + cc) * Nu + vi],
+                   J[(int64_t)(2 * 4 + 1 + cc) * Nu + vi]};
+  return g[66 + k9] * (dot(h, gr) / 2 - g[27 + 3 * r] * f);
+}
+
+// Coefficients 12..18 from the direct ones x[0..11] (ps_coeffs)
+__device__ inline double mwss_coeff_derived(const double* __restrict__ g, const double* x, int kk) {
+  switch (kk) {
+    case 12: return g[57] * x[3] + g[58] * x[8];
+    case 13: return g[61] * x[6] + g[62] * x[11];
+    case 14: return g[65] * x[9] + g[63] * x[5];
+    case 15: return g[57] * x[4] + g[58] * x[7];
+    case 16: return g[61] * x[7] + g[62] * x[10];
+    case 17: return g[63] * x[4] + g[65] * x[10];
+    default: return g[54] * x[4] + g[55] * x[7] + g[56] * x[10];
+  }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Level K's values for the lanes of this wave whose triangle's block of that
+// level is not assembled (have == false): while such a lane remains, the
+// wave takes the lowest one's triangle, computes the 57 coefficients of
+// level K there together (one or two per lane, through LDS), and every
+// lane with that triangle evaluates its sub-triangle from them.  Each round
+// serves at least the leading lane, so the loop ends after at most 64
+// rounds; it is entered only when a lane misses.  Blocks of at most 256
+// threads.
+template <int K>
+__device__ __forceinline__ void mwss_fill_missing(const cmm_mwss_mesh& M, const SparseLv& sv, int64_t tri, int s,
+                                                  d3 cb, bool have, double (&bk)[3]) {
+  __shared__ double fb_lds[4][64];
+  double* X = fb_lds[threadIdx.x >> 6];
+  uint64_t pend = __ballot(!have);
+  while (pend) {
+    // the leading lane's triangle as a wave-uniform (scalar) value: the
+    // geometry, vertex ids and jets addresses below stay in scalar registers
+    const int leader = __ffsll((unsigned long long)pend) - 1;
+    const int lt = __builtin_amdgcn_readlane((int)tri, leader);
+    int row, half, col;
+    mwss_tri_rowcol(M, lt, row, half, col);
+    int64_t vi[3];
+    d3 vv[3];
+    mwss_tri_vertices(M, lt, vi, vv);
+    const double* g = M.rowgeom + ((int64_t)row * 2 + half) * kPSGeomLen;
+    const double c = M.cp[col], sn = M.sp[col];
+    const uint64_t act = __ballot(1);
+    const int nact = __popcll(act);
+    const int rank = __popcll(act & ((1ull << __lane_id()) - 1));
+    wave_lds_sync();
+    for (int j = rank; j < 36; j += nact)
+      X[(j / 12) * 19 + j % 12] = mwss_coeff_direct(M, sv.jets[K], g, c, sn, vi[0], vi[1], vi[2], j / 12, j % 12);
+    wave_lds_sync();
+    for (int j = rank; j < 21; j += nact) X[(j / 7) * 19 + 12 + j % 7] = mwss_coeff_derived(g, X + (j / 7) * 19, 12 + j % 7);
+    wave_lds_sync();
+    if (!have && tri == lt) {
+      const uint32_t code = cs_code(s);
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        double a[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) a[j] = X[cc * 19 + ((code >> (5 * j)) & 31u)];
+        bk[cc] = bb6(cb, a);
+      }
+      have = true;
+    }
+    pend = __ballot(!have);
+  }
+}
+
+// sum_k w_k * PS_k(q) for ncomp components; sv (ncomp 3 only): levels kept
+// on a subset of triangles.  Every level's block is loaded together with its
+// validity byte (a stale block costs nothing when it is not used), so the
+// common case keeps the loads of complete levels in flight at once; a wave
+// in which some lane's block is missing sums again, level by level, with the
+// missing blocks built from the jets.  Levels are added in the same order
+// either way, so both give the bits of complete levels.
 __device__ __forceinline__ bool mwss_eval_levels(const cmm_mwss_mesh& M, const LevelSet& ls, int ncomp, d3 q,
-                                        double* r, int64_t* tri_out = nullptr) {
+                                        double* r, int64_t* tri_out = nullptr, const SparseLv* sv = nullptr) {
   MLoc o;
   if (!mwss_locate(M, q, o)) return false;
   if (tri_out) *tri_out = o.tri;
   int s = split_index(o.b.x, o.b.y, o.b.z);
   d3 cb = sub_bary_v(s, o.b, o.v);
+  unsigned have = 7;
+  if (sv) {
+    const unsigned vb = sv->vbits[o.tri];
+    have = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (k >= ls.nlev || ((vb >> sv->slot[k]) & 1u)) have |= 1u << k;
+    if (sv->mark && sv->mark_every) sv->mark[o.tri] = 1;
+  }
   // compile-time trip counts with guards: runtime-indexed arrays would be
   // placed in scratch memory
 #pragma unroll
@@ -238,6 +370,29 @@ __device__ __forceinline__ bool mwss_eval_levels(const cmm_mwss_mesh& M, const L
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         if (c < ncomp) r[c] += w * bb6(cb, p + 6 * c);
+    }
+  }
+  if (sv && __ballot(have != 7)) {
+    if (sv->mark && have != 7) sv->mark[o.tri] = 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (k < ls.nlev) {
+        const bool hk = (have >> k) & 1u;
+        double bk[3] = {0, 0, 0};
+        if (hk) {
+          const double* p = ls.p[k] + (o.tri * 6 + s) * 18;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) bk[c] = bb6(cb, p + 6 * c);
+        }
+        if (k == 0) mwss_fill_missing<0>(M, *sv, o.tri, s, cb, hk, bk);
+        if (k == 1) mwss_fill_missing<1>(M, *sv, o.tri, s, cb, hk, bk);
+        if (k == 2) mwss_fill_missing<2>(M, *sv, o.tri, s, cb, hk, bk);
+        const double w = ls.w[k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] += w * bk[c];
+      }
     }
   }
   return true;
@@ -366,16 +521,20 @@ struct VelDev {
   LevelSet ls[4];
   double stage_t[4];
   double params[8];
+  SparseLv sv;  // sv.vbits == NULL: every level complete
 };
 
-// SPLINE: the kernels are instantiated twice, for the spline velocity alone
-// (the CMM loop: no registers spent on the analytic fields) and for the
-// analytic fields
-template <bool SPLINE>
+// VK: the kernels are instantiated for the analytic fields (kVelAnalytic),
+// the spline velocity with complete levels (kVelSpline: the CMM loop spends
+// no registers on the analytic fields or on missing blocks) and with levels
+// kept on a subset of triangles (kVelSparse)
+constexpr int kVelAnalytic = 0, kVelSpline = 1, kVelSparse = 2;
+
+template <int VK>
 __device__ __forceinline__ d3 vel_eval(const VelDev& V, int stage, d3 x) {
-  if constexpr (SPLINE) {
+  if constexpr (VK != kVelAnalytic) {
     double r[3];
-    if (!mwss_eval_levels(V.mesh, V.ls[stage], 3, x, r)) {
+    if (!mwss_eval_levels(V.mesh, V.ls[stage], 3, x, r, nullptr, VK == kVelSparse ? &V.sv : nullptr)) {
       double nan = __builtin_nan("");
       return d3{nan, nan, nan};
     }
@@ -413,9 +572,9 @@ __device__ inline d3 proj_norm(d3 x) {
 }
 
 // one backward step of integrator `integ` for the point xn
-template <bool SPLINE>
+template <int VK>
 __device__ __forceinline__ d3 advect_point(const VelDev& V, int integ, double dt, d3 xn) {
-  auto vel_eval = [&](const VelDev& W, int stage, d3 x) { return ::vel_eval<SPLINE>(W, stage, x); };
+  auto vel_eval = [&](const VelDev& W, int stage, d3 x) { return ::vel_eval<VK>(W, stage, x); };
   if (integ == CMM_INT_EVAL) return vel_eval(V, 0, xn);
   if (integ == CMM_INT_EULER) {
     d3 v1 = vel_eval(V, 0, xn);
@@ -445,12 +604,12 @@ __device__ __forceinline__ d3 advect_point(const VelDev& V, int integ, double dt
   return proj_norm(sub(xn, scl(dt / 6, acc)));
 }
 
-template <bool SPLINE>
+template <int VK>
 __global__ __launch_bounds__(128) void k_advect(VelDev V, int integ, double dt, const double* __restrict__ x, int64_t m,
                                                 double* __restrict__ y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const d3 out = advect_point<SPLINE>(V, integ, dt, d3{x[i], x[m + i], x[2 * m + i]});
+  const d3 out = advect_point<VK>(V, integ, dt, d3{x[i], x[m + i], x[2 * m + i]});
   y[i] = out.x;
   y[m + i] = out.y;
   y[2 * m + i] = out.z;
@@ -460,7 +619,7 @@ __global__ __launch_bounds__(128) void k_advect(VelDev V, int integ, double dt, 
 // (vertex v = i / P, slot k = i % P), slot 0 the vertex itself when P = 5,
 // the other slots its four stencil points, so the feet of one vertex sit in
 // neighbouring lanes and their velocity gathers share cache lines
-template <int P, bool SPLINE>
+template <int P, int VK>
 __global__ __launch_bounds__(128) void k_advect_stencil(VelDev V, int integ, double dt, const double* __restrict__ xv,
                                                         int64_t xv_row, const double* __restrict__ xs, int64_t xs_row,
                                                         int64_t nv, double* __restrict__ yv, int64_t yv_row,
@@ -471,7 +630,7 @@ __global__ __launch_bounds__(128) void k_advect_stencil(VelDev V, int integ, dou
   const int k = (int)(i - v * P) - (P - 4);  // stencil index, -1 = vertex
   const double* src = k < 0 ? xv + v : xs + (int64_t)k * nv + v;
   const int64_t row = k < 0 ? xv_row : xs_row;
-  const d3 out = advect_point<SPLINE>(V, integ, dt, d3{src[0], src[row], src[2 * row]});
+  const d3 out = advect_point<VK>(V, integ, dt, d3{src[0], src[row], src[2 * row]});
   double* dst = k < 0 ? yv + v : ys + (int64_t)k * nv + v;
   const int64_t orow = k < 0 ? yv_row : ys_row;
   dst[0] = out.x;
@@ -537,6 +696,44 @@ __global__ __launch_bounds__(256) void k_mwss_eval_grad(cmm_mwss_mesh M, const d
   out[2 * n + i] = g.z - gq * x.z;
 }
 
+// the 19 coefficients per component of MWSS triangle t from the jets
+// (NC, 4, Nu): the per-row geometry rotated to the triangle's column
+template <int NC>
+__device__ __forceinline__ void mwss_tri_coeffs(const cmm_mwss_mesh& M, const double* __restrict__ vals, int64_t t,
+                                                double (&cf)[NC][19]) {
+  int row, col, half;
+  mwss_tri_rowcol(M, t, row, half, col);
+  int64_t vi[3];
+  d3 v[3];
+  mwss_tri_vertices(M, t, vi, v);
+  PSGeom P;
+  ps_unpack(M.rowgeom + ((int64_t)row * 2 + half) * kPSGeomLen, 1, P);
+  const double c = M.cp[col], sn = M.sp[col];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    d3 h = P.H[i];
+    P.H[i] = d3{c * h.x - sn * h.y, sn * h.x + c * h.y, h.z};
+  }
+  const int64_t Nu = M.Nu;
+#pragma unroll
+  for (int cc = 0; cc < NC; ++cc) {
+    double f[3];
+    d3 gr[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (NC == 1) {
+        f[i] = vals[vi[i]];
+        gr[i] = d3{vals[Nu + vi[i]], vals[2 * Nu + vi[i]], vals[3 * Nu + vi[i]]};
+      } else {
+        f[i] = vals[(cc * 4) * Nu + vi[i]];
+        gr[i] = d3{vals[(0 * 4 + 1 + cc) * Nu + vi[i]], vals[(1 * 4 + 1 + cc) * Nu + vi[i]],
+                   vals[(2 * 4 + 1 + cc) * Nu + vi[i]]};
+      }
+    }
+    ps_coeffs(P, f, gr, cf[cc]);
+  }
+}
+
 template <int NC>
 __global__ __launch_bounds__(256) void k_mwss_assemble(cmm_mwss_mesh M, const double* __restrict__ vals,
                                                        double* __restrict__ coeffs, double* __restrict__ packed) {
@@ -545,50 +742,100 @@ __global__ __launch_bounds__(256) void k_mwss_assemble(cmm_mwss_mesh M, const do
   const int64_t t = t0 + threadIdx.x;
   double cf[NC][19];
   if (t < M.Tu) {
-    const int N = M.N, Mr = M.M;
-    int row, col, half;
-    if (t < N) { row = 0; col = (int)t; half = 0; }
-    else if (t >= M.Tu - N) { row = Mr - 2; col = (int)(t - (M.Tu - N)); half = 0; }
-    else {
-      int64_t u = t - N;
-      half = (int)(u % 2);
-      int64_t k = u / 2;
-      row = 1 + (int)(k / N);
-      col = (int)(k % N);
-    }
-    int64_t vi[3];
-    d3 v[3];
-    mwss_tri_vertices(M, t, vi, v);
-    PSGeom P;
-    ps_unpack(M.rowgeom + ((int64_t)row * 2 + half) * kPSGeomLen, 1, P);
-    const double c = M.cp[col], sn = M.sp[col];
+    mwss_tri_coeffs<NC>(M, vals, t, cf);
+    if (coeffs)
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-      d3 h = P.H[i];
-      P.H[i] = d3{c * h.x - sn * h.y, sn * h.x + c * h.y, h.z};
-    }
-    const int64_t Nu = M.Nu;
-#pragma unroll
-    for (int cc = 0; cc < NC; ++cc) {
-      double f[3];
-      d3 gr[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        if (NC == 1) {
-          f[i] = vals[vi[i]];
-          gr[i] = d3{vals[Nu + vi[i]], vals[2 * Nu + vi[i]], vals[3 * Nu + vi[i]]};
-        } else {
-          f[i] = vals[(cc * 4) * Nu + vi[i]];
-          gr[i] = d3{vals[(0 * 4 + 1 + cc) * Nu + vi[i]], vals[(1 * 4 + 1 + cc) * Nu + vi[i]],
-                     vals[(2 * 4 + 1 + cc) * Nu + vi[i]]};
-        }
-      }
-      ps_coeffs(P, f, gr, cf[cc]);
-      if (coeffs)
+      for (int cc = 0; cc < NC; ++cc)
         for (int k = 0; k < 19; ++k) coeffs[((int64_t)cc * 19 + k) * M.Tu + t] = cf[cc][k];
-    }
   }
   if (packed) store_packed_block<NC>(cf, t0, M.Tu, packed, lds);
+}
+
+// Rows of MWSS triangles next to either pole that a sparse velocity level
+// always holds: a foot near a pole crosses many of their thin triangles from
+// one step to the next, and every missing block costs its wave one round of
+// the fallback in mwss_fill_missing.
+constexpr int kPolarRows = 4;
+
+// cmm_mwss_assemble_sparse: one wave per chunk of triangles decides which
+// to assemble (mark != NULL: the marked ones and the polar rows, bit `slot`
+// of vbits set for them and cleared for the others, mark cleared; mark ==
+// NULL: those whose bit is clear, bit set), compacts their ids in LDS and
+// assembles them a lane each, so the waves that run the 212-register
+// assembly are about as many as the selected triangles fill.  The polar rows
+// (always selected) come in chunks of 64, the rest in chunks of 256, so no
+// wave carries more than one round of the assembly on average.
+__global__ __launch_bounds__(64) void k_mwss_assemble_sparse(cmm_mwss_mesh M, const double* __restrict__ vals,
+                                                             double* __restrict__ packed, uint8_t* __restrict__ vbits,
+                                                             int slot, int inherit, uint8_t* __restrict__ mark,
+                                                             int64_t npolar, int64_t polar_chunks) {
+  __shared__ int32_t ids[256];
+  __shared__ __align__(16) double stage[16 * 108];
+  const int lane = threadIdx.x;
+  const unsigned bit = 1u << slot;
+  const int64_t Tu = M.Tu;
+  const int64_t b = blockIdx.x;
+  int64_t c0, c1;
+  if (b < polar_chunks) {  // [0, npolar) then [Tu - npolar, Tu), 64 at a time
+    const int64_t half = (polar_chunks + 1) / 2;
+    c0 = b < half ? 64 * b : Tu - npolar + 64 * (b - half);
+    c1 = b < half ? min(c0 + 64, npolar) : min(c0 + 64, Tu);
+  } else {
+    c0 = npolar + 256 * (b - polar_chunks);
+    c1 = min(c0 + 256, Tu - npolar);
+  }
+  int nsel = 0;
+  for (int64_t base = c0; base < c1; base += 64) {
+    const int64_t t = base + lane;
+    bool sel = false;
+    if (t < c1) {
+      const unsigned vb = vbits[t];
+      if (mark) {
+        const bool polar = t < npolar || t >= Tu - npolar;
+        const bool mk = mark[t] != 0;
+        sel = mk || polar || (inherit >= 0 && ((vb >> inherit) & 1u));
+        if (mk) mark[t] = 0;
+        const unsigned nb = sel ? (vb | bit) : (vb & ~bit);
+        if (nb != vb) vbits[t] = (uint8_t)nb;
+      } else {
+        sel = (vb & bit) == 0;
+        if (sel) vbits[t] = (uint8_t)(vb | bit);
+      }
+    }
+    const uint64_t bl = __ballot(sel);
+    if (sel) ids[nsel + __popcll(bl & ((1ull << lane) - 1))] = (int32_t)t;
+    nsel += __popcll(bl);
+  }
+  __syncthreads();
+  // 64 selected triangles at a time a lane each, their blocks written 16 at
+  // a time through LDS so that every store instruction covers whole blocks
+  typedef double dbl2 __attribute__((ext_vector_type(2)));
+  for (int i0 = 0; i0 < nsel; i0 += 64) {
+    double cf[3][19];
+    const bool mine = i0 + lane < nsel;
+    if (mine) mwss_tri_coeffs<3>(M, vals, ids[i0 + lane], cf);
+    const int cnt = min(64, nsel - i0);
+    for (int r0 = 0; r0 < cnt; r0 += 16) {
+      if (mine && lane >= r0 && lane < r0 + 16) {
+        double* dst = stage + (lane - r0) * 108;
+#pragma unroll
+        for (int ss = 0; ss < 6; ++ss)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) dst[(ss * 3 + c) * 6 + j] = cf[c][cs_table(ss, j)];
+      }
+      __syncthreads();
+      const int nb = min(16, cnt - r0);
+      const dbl2* src = reinterpret_cast<const dbl2*>(stage);
+      for (int e = lane; e < nb * 54; e += 64) {
+        const int tl = e / 54;
+        dbl2* out = reinterpret_cast<dbl2*>(packed + (int64_t)ids[i0 + r0 + tl] * 108);
+        __builtin_nontemporal_store(src[e], out + (e - tl * 54));
+      }
+      __syncthreads();
+    }
+  }
 }
 
 // grid index (ring, column) of mesh vertex v: poles take the phi = 0 sample
@@ -1621,6 +1868,20 @@ extern "C" int cmm_mwss_assemble(const cmm_mwss_mesh* m, const double* vals, int
   return CMM_OK;
 }
 
+extern "C" int cmm_mwss_assemble_sparse(const cmm_mwss_mesh* m, const double* vals, double* packed, uint8_t* vbits,
+                                        int slot, int inherit, uint8_t* mark, void* stream) {
+  if (!m || !vals || !packed || !vbits || slot < 0 || slot > 7 || inherit < -1 || inherit > 7 || inherit == slot)
+    return fail(CMM_E_ARG, "cmm_mwss_assemble_sparse: bad arguments");
+  // the rows of triangles next to either pole: a cap of N, then 2N a row
+  const int64_t npolar = std::min<int64_t>(m->N + (int64_t)2 * m->N * (kPolarRows - 1), m->Tu / 2);
+  const int64_t pc = 2 * grid_for(npolar, 64);
+  const int64_t inner = grid_for(m->Tu - 2 * npolar, 256);
+  hipLaunchKernelGGL(k_mwss_assemble_sparse, dim3((unsigned)(pc + inner)), dim3(64), 0, as_stream(stream), *m, vals,
+                     packed, vbits, slot, inherit, mark, npolar, pc);
+  CMM_LAUNCH_CHECK("cmm_mwss_assemble_sparse");
+  return CMM_OK;
+}
+
 extern "C" int cmm_mwss_eval(const cmm_mwss_mesh* m, const double* const* levels, const double* w,
                              int nlev, int ncomp, const double* q, int64_t n, int64_t sp, int64_t sc,
                              double* out, int64_t osp, int64_t osc, void* stream) {
@@ -1657,6 +1918,22 @@ static int velocity_device(const cmm_velocity* vel, const char* who, VelDev& V) 
       for (int k = 0; k < vel->nlev; ++k)
         if (!vel->levels[k]) return fail(CMM_E_ARG, std::string(who) + ": null velocity level");
     }
+    V.sv = SparseLv{nullptr, {0, 0, 0}, {nullptr, nullptr, nullptr}, nullptr, 1};
+    if (vel->vbits) {
+      V.sv.vbits = vel->vbits;
+      for (int k = 0; k < vel->nlev; ++k) {
+        if (!vel->jets[k] || vel->vslot[k] < 0 || vel->vslot[k] > 7)
+          return fail(CMM_E_ARG, std::string(who) + ": a sparse velocity needs jets and a bit for every level");
+        V.sv.slot[k] = vel->vslot[k];
+        V.sv.jets[k] = vel->jets[k];
+      }
+      for (int k = vel->nlev; k < 3; ++k) {  // never read: point at level 0
+        V.sv.slot[k] = V.sv.slot[0];
+        V.sv.jets[k] = vel->jets[0];
+      }
+      V.sv.mark = vel->mark;
+      V.sv.mark_every = vel->mark_every;
+    }
   } else if (vel->kind < 0 || vel->kind > CMM_VEL_DIV) {
     return fail(CMM_E_ARG, std::string(who) + ": unknown velocity kind");
   }
@@ -1671,10 +1948,9 @@ extern "C" int cmm_advect_points(const cmm_velocity* vel, int integ, double dt, 
   int rc = velocity_device(vel, "cmm_advect_points", V);
   if (rc) return rc;
   if (m == 0) return CMM_OK;
-  if (V.kind == CMM_VEL_SPLINE)
-    hipLaunchKernelGGL(k_advect<true>, dim3(grid_for(m, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, x, m, y);
-  else
-    hipLaunchKernelGGL(k_advect<false>, dim3(grid_for(m, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, x, m, y);
+  auto kern = V.kind != CMM_VEL_SPLINE ? k_advect<kVelAnalytic>
+              : (V.sv.vbits ? k_advect<kVelSparse> : k_advect<kVelSpline>);
+  hipLaunchKernelGGL(kern, dim3(grid_for(m, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, x, m, y);
   CMM_LAUNCH_CHECK("cmm_advect_points");
   return CMM_OK;
 }
@@ -1689,9 +1965,11 @@ extern "C" int cmm_advect_stencil(const cmm_velocity* vel, int integ, double dt,
   int rc = velocity_device(vel, "cmm_advect_stencil", V);
   if (rc) return rc;
   if (nv == 0) return CMM_OK;
-  const bool sp = V.kind == CMM_VEL_SPLINE;
-  auto kern = verts ? (sp ? k_advect_stencil<5, true> : k_advect_stencil<5, false>)
-                    : (sp ? k_advect_stencil<4, true> : k_advect_stencil<4, false>);
+  const int vk = V.kind != CMM_VEL_SPLINE ? kVelAnalytic : (V.sv.vbits ? kVelSparse : kVelSpline);
+  auto kern = verts ? (vk == kVelAnalytic ? k_advect_stencil<5, kVelAnalytic>
+                       : vk == kVelSpline ? k_advect_stencil<5, kVelSpline> : k_advect_stencil<5, kVelSparse>)
+                    : (vk == kVelAnalytic ? k_advect_stencil<4, kVelAnalytic>
+                       : vk == kVelSpline ? k_advect_stencil<4, kVelSpline> : k_advect_stencil<4, kVelSparse>);
   const int P = verts ? 5 : 4;
   hipLaunchKernelGGL(kern, dim3(grid_for(P * nv, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, verts,
                      verts_row, st, st_row, nv, yv, yv_row, yst, yst_row);
@@ -1813,6 +2091,8 @@ extern "C" int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integ, do
       (integ != CMM_INT_EULER && integ != CMM_INT_IMPROVED_EULER && integ != CMM_INT_RK4))
     return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint: bad arguments");
   if (vel->kind != CMM_VEL_SPLINE) return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint: needs a spline velocity");
+  if (vel->vbits)
+    return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint: sparse velocity levels (complete them first)");
   VelDev V;
   int rc = velocity_device(vel, "cmm_advect_stencil_adjoint", V);
   if (rc) return rc;
@@ -1860,6 +2140,8 @@ extern "C" int cmm_advect_stencil_adjoint_corners(const cmm_velocity* vel, int i
     return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint_corners: bad arguments");
   if (vel->kind != CMM_VEL_SPLINE)
     return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint_corners: needs a spline velocity");
+  if (vel->vbits)
+    return fail(CMM_E_ARG, "cmm_advect_stencil_adjoint_corners: sparse velocity levels (complete them first)");
   VelDev V;
   int rc = velocity_device(vel, "cmm_advect_stencil_adjoint_corners", V);
   if (rc) return rc;

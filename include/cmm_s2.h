@@ -170,6 +170,24 @@ int cmm_mwss_split_locate(const cmm_mwss_mesh* mesh, const double* q, int64_t n,
 int cmm_mwss_assemble(const cmm_mwss_mesh* mesh, const double* vals, int ncomp,
                       double* coeffs, double* packed, void* stream);
 
+/* cmm_mwss_assemble of a velocity level (ncomp 3, packed only) on a subset
+ * of the Tu triangles.  Bit `slot` (0..7) of vbits[t] says where the level's
+ * packed blocks are current (the rest is never written):
+ *   mark != NULL: the triangles t with mark[t] != 0, those of the 4 rows of
+ *     triangles next to either pole and, if inherit >= 0, those with bit
+ *     `inherit` set (the set of an earlier level) are assembled, their bit
+ *     set and the bit of every other triangle cleared, and mark is cleared
+ *     (mark is what the evaluations of cmm_velocity.mark flagged since the
+ *     last call: the triangles the next steps' feet will fall in);
+ *   mark == NULL: the triangles whose bit is clear are assembled and their
+ *     bit set (the level becomes complete).
+ * Replaces the full-mesh assembly of spline_interp_velocity.init_coeffs
+ * (spherical_spline.py:565-568) in the loop; evaluations outside the
+ * assembled set compute their block from the jets (cmm_velocity.vbits), with
+ * the same arithmetic. */
+int cmm_mwss_assemble_sparse(const cmm_mwss_mesh* mesh, const double* vals, double* packed,
+                             uint8_t* vbits, int slot, int inherit, uint8_t* mark, void* stream);
+
 /* spherical_spline.gradient (spherical_spline.py:73-111) for a scalar spline
  * (packed, ncomp 1) on the MWSS mesh: tangential gradient, out (3, n). */
 int cmm_mwss_eval_grad(const cmm_mwss_mesh* mesh, const double* packed, const double* q, int64_t n,
@@ -210,6 +228,19 @@ typedef struct cmm_velocity {
   double stage_w[4][3];           /* spline: time weights for each stage */
   double stage_t[4];              /* analytic: time of each stage */
   double params[8];
+  /* spline, optional (vbits NULL: every level complete).  Levels assembled
+   * on a subset of triangles (cmm_mwss_assemble_sparse): where bit vslot[k]
+   * of vbits[t] is clear an evaluation builds level k's block of triangle t
+   * from jets[k] (3, 4, Nu) (the wave computes the 57 coefficients together;
+   * same arithmetic as the assembly).  mark != NULL: the triangles an
+   * evaluation locates are set to 1 there, every one (mark_every != 0) or
+   * only those whose block of some level was missing.  Not accepted by the
+   * adjoint entry points. */
+  const uint8_t* vbits;
+  int vslot[3];
+  const double* jets[3];
+  uint8_t* mark;
+  int mark_every;
 } cmm_velocity;
 
 /* x, y: (3, m) component-major; y may alias x. */

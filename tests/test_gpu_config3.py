@@ -38,6 +38,7 @@ def runs():
 
 
 def _outputs(sim, w):
+    sim.U.complete()
     dinv = torch.tensor([0.0] + [-1.0 / (l * (l + 1)) for l in range(1, L)], dtype=torch.float64,
                         device="cuda")[torch.from_numpy(sim.plan.l_of).cuda()]
     out = {"psi_lm": sim.w_lm[:, 0] * dinv[:, None], "u_grid": sim.fields[0:3], "grad_u_grid": sim.fields[3:12],

@@ -38,6 +38,8 @@ def test_64_member_ensemble_matches_solo_runs():
         nc = m.submaps.Nc
         assert torch.equal(m.submaps.packed[:nc], solo.submaps.packed[:nc]), k
         assert torch.equal(m.X, solo.X), k
+        m.U.complete()
+        solo.U.complete()
         assert torch.equal(m.U.level_packed(2), solo.U.level_packed(2)), k
         if q is None:
             g = torch.Generator().manual_seed(5)

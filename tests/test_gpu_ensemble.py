@@ -48,7 +48,9 @@ def _meshes():
 
 
 def _state(m):
-    """Every buffer a run leaves results in."""
+    """Every buffer a run leaves results in (velocity levels completed: the
+    loop assembles them where the feet fall)."""
+    m.U.complete()
     nc = m.submaps.Nc
     return [m.submaps.packed[:nc].clone(), m.submaps.vals[:nc].clone(), m.curr.packed.clone(),
             m.U._packed.clone(), m.U._vals.clone(), m.X.clone(), m.w.clone(), m.fields.clone()]
@@ -73,6 +75,7 @@ def test_captured_run_replays_the_eager_bits(own_streams):
     mk = lambda L, i, Nt, w: (lambda: BarotropicCMM(L, Nt, Nt / L, meshes[i], w, 2 * np.pi, n_maps=4))
     ens = Ensemble([mk(*s) for s in specs], streams=own_streams)
     ens.run()
+    torch.cuda.synchronize()
     ref = [_state(m) for m in ens.members]
     ens.capture()
     for rep in range(3):
