@@ -26,11 +26,12 @@ CFLAGS = [
 
 # Per-file flags.  mwss_mesh.hip evaluates the velocity spline in several
 # kernel instantiations and rebuilds missing velocity blocks outside the
-# assembly kernel; with contraction decided per source expression
-# (-ffp-contract=on: a * b + c fused as written, never across statements)
-# every copy of a formula rounds the same way, so all of them give the same
-# bits.
-FILE_FLAGS = {"mwss_mesh.hip": ["-ffp-contract=on"]}
+# assembly kernel; tri_mesh.hip locates and evaluates maps both point by
+# point and along runs of points that reuse a located triangle.  With
+# contraction decided per source expression (-ffp-contract=on: a * b + c
+# fused as written, never across statements) every copy of a formula rounds
+# the same way, so all of them give the same bits.
+FILE_FLAGS = {"mwss_mesh.hip": ["-ffp-contract=on"], "tri_mesh.hip": ["-ffp-contract=on"]}
 
 
 def sources():

@@ -164,9 +164,11 @@ extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int
     if (simplices[t] < 0 || simplices[t] >= nv)
       return fail(CMM_E_ARG, "cmm_tri_mesh_create: simplex index out of range");
   int R = cells_per_face;
-  // default: about 9 cells per triangle (fewer cells straddle an edge, so
-  // fewer candidate tests per query; 94 MB of cells at 327,680 triangles)
-  if (R <= 0) R = std::max(4, (int)std::ceil(std::sqrt((double)ntri * 3.0)));
+  // default: about 36 cells per triangle (fewer cells straddle an edge, so
+  // the waves of a composition run fewer candidate tests: 0.853 -> 0.801 ms
+  // for the depth-5 composition of 8.4 M points on ico-7, whose 23.6 M cells
+  // take 378 MB; 144 cells per triangle measured slower, 0.911 ms)
+  if (R <= 0) R = 2 * std::max(2, (int)std::ceil(std::sqrt((double)ntri * 3.0)));
   if ((int64_t)6 * R * R > ((int64_t)1 << 31) - 1) return fail(CMM_E_RANGE, "cmm_tri_mesh_create: too many cells");
 
   // geometry records

@@ -172,18 +172,33 @@ def test_locate_large_mesh_against_kdtree():
 
 def test_tiled_grid_composition_is_identical():
     """cmm_tri_compose_grid walks a ring-major grid in tiles: same bits as the
-    point-order composition, including ragged edges (ring count and length
-    not multiples of the tile)."""
+    point-order composition for every tile shape, including ragged edges
+    (ring count and length not multiples of the tile), the intermediates,
+    NaN points, and waves whose points are spread over many triangles
+    (ico-6 mesh, L=24 grid)."""
     from cmm_s2.core import mesh_functions as mf
-    from cmm_s2.core.spherical_spline import sphere_diffeomorphism, compose_packed
+    from cmm_s2.core import spherical_spline as ss
     from helpers import perturbed_identity_jets
-    ico = mf.icosahedral_mesh(3)
-    pm = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
-    maps = [sphere_diffeomorphism(pm, torch.from_numpy(perturbed_identity_jets(pm.vertices, 40 + i)).cuda())
-            for i in range(3)]
-    for L in (24, 37):
-        grid = mf.structure_spherical_triangulation(L)
-        q = grid.sample_points()
-        a = compose_packed(pm, [m.packed for m in maps], q, "N3")
-        b = compose_packed(pm, [m.packed for m in maps], q, "N3", grid=(L + 1, 2 * L))
-        assert torch.equal(a, b)
+    for ico_level, Ls in ((3, (24, 37, 96)), (6, (24,))):
+        ico = mf.icosahedral_mesh(ico_level)
+        pm = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
+        maps = [ss.sphere_diffeomorphism(pm, torch.from_numpy(perturbed_identity_jets(pm.vertices, 40 + i)).cuda())
+                for i in range(3)]
+        levels = [m.packed for m in maps]
+        for L in Ls:
+            grid = mf.structure_spherical_triangulation(L)
+            q = grid.sample_points().clone()
+            q[5] = float("nan")
+            q[2 * L + 7] = float("nan")
+            n = q.shape[0]
+            inter_a = torch.empty(3, n, 3, dtype=torch.float64, device="cuda")
+            a = ss.compose_packed(pm, levels, q, "N3", inter=inter_a)
+            assert torch.isnan(a[5]).all() and torch.isfinite(a[6]).all()
+            key = (pm._h, L + 1, 2 * L, len(levels))
+            for rows in (1, 2, 4, 8, 16):
+                ss._tiles[key] = rows
+                inter_b = torch.full_like(inter_a, 7.0)
+                b = ss.compose_packed(pm, levels, q, "N3", inter=inter_b, grid=(L + 1, 2 * L))
+                assert torch.equal(a.nan_to_num(9.0), b.nan_to_num(9.0)), (ico_level, L, rows)
+                assert torch.equal(inter_a.nan_to_num(9.0), inter_b.nan_to_num(9.0)), (ico_level, L, rows)
+            ss._tiles.pop(key)
