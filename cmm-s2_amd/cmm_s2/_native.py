@@ -24,6 +24,7 @@ VEL_DEFORM, VEL_STATIC_VORTEX, VEL_ROTATING_VORTEX, VEL_EQUATOR_VORTEX, VEL_DIV 
 INT_EULER, INT_IMPROVED_EULER, INT_RK4, INT_RK2, INT_EVAL = 0, 1, 2, 3, 4
 LEGENDRE_FP64, LEGENDRE_FP32 = 0, 1
 LEGENDRE_BF16X3, LEGENDRE_BF16X4, LEGENDRE_BF16X5, LEGENDRE_BF16X6, LEGENDRE_BF16X7 = 2, 3, 4, 5, 6
+LEGENDRE_I8X4, LEGENDRE_I8X5, LEGENDRE_I8X6, LEGENDRE_I8X7, LEGENDRE_I8X8 = 7, 8, 9, 10, 11
 VORT_SPLINE, VORT_ROSSBY, VORT_ZONAL_JET, VORT_GAUSSIAN = 0, 1, 2, 3
 VORT_PERTURBED_JET, VORT_MULTI_JET, VORT_PERTURBED_ROSSBY = 4, 5, 6
 VORT_COSINE_BELL, VORT_SLOTTED_CYLINDER, VORT_STATIC_VORTEX_TRACER, VORT_ROTATING_VORTEX_TRACER = 7, 8, 9, 10

@@ -33,11 +33,15 @@ def _lm_tables(L):
     return np.array(ls), np.array(ms)
 
 
-# arithmetic of the Legendre synthesis: "fp64" (reference precision) or
-# "fp32" (FP32 matrix cores, FP64 recurrence, FP64-compensated partial sums)
+# arithmetic of the Legendre synthesis: "fp64" (reference precision),
+# "fp32" (FP32 matrix cores, FP64 recurrence, FP64-compensated partial sums),
+# "bf16xS" (S fixed-point bf16 digits per operand, FP32 accumulators) or
+# "i8xS" (S int8 digits per operand, the Legendre digits tabulated once per
+# plan, exact int32 accumulators)
 _LEGENDRE = {"fp64": N.LEGENDRE_FP64, "fp32": N.LEGENDRE_FP32, "bf16x3": N.LEGENDRE_BF16X3,
              "bf16x4": N.LEGENDRE_BF16X4, "bf16x5": N.LEGENDRE_BF16X5, "bf16x6": N.LEGENDRE_BF16X6,
-             "bf16x7": N.LEGENDRE_BF16X7}
+             "bf16x7": N.LEGENDRE_BF16X7, "i8x4": N.LEGENDRE_I8X4, "i8x5": N.LEGENDRE_I8X5,
+             "i8x6": N.LEGENDRE_I8X6, "i8x7": N.LEGENDRE_I8X7, "i8x8": N.LEGENDRE_I8X8}
 
 
 class SHT(object):

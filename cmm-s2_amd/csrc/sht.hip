@@ -30,6 +30,9 @@ constexpr int kMaxFFT = 4096;  // ring length 2L: L <= 2048 (the FFT's two LDS b
 constexpr int kFwdSegHost = 64;   // degrees per forward work unit (kFwdSeg)
 constexpr int kOzJ = 16;          // split-BF16 synthesis: degrees of one parity per chunk
 constexpr int kOzMaxS = 7;        // split-BF16 synthesis: most digits per operand
+constexpr int kI8MaxS = 8;        // int8 digit-table synthesis: most digits per operand
+constexpr int kI8Rings = 32;      // int8 synthesis: rings per tile (M of the 32x32x32 instruction)
+constexpr int kI8Deg = 32;        // int8 synthesis: degrees of one parity per chunk (its K)
 }
 
 // Scratch of cmm_sht_forward_noalloc: a fixed array in the code object, so
@@ -82,6 +85,20 @@ struct cmm_sht_plan {
   int inv_waves = 0;
   int2* inv_tasks = nullptr;
   int* inv_off = nullptr;
+  // int8 digit-table synthesis: nrt tiles of 32 northern rings (t = 1 .. L/2);
+  // per (order m, parity p) nk(m, p) = ceil(ceil((L - m - p) / 2) / 32) chunks
+  // of 32 degrees; the coefficient digits of chunk (m, p, kc) are chunk block
+  // i8_boff[2 m + p] + kc of the workspace (i8_nb blocks in all).  The table
+  // (i8S digits) holds, per (m, p, tile), the chunks from the first one with
+  // a nonzero digit on: i8_tdesc = (that chunk, its block in i8_tab, the
+  // tile's exponent).
+  int nrt = 0;
+  int i8_nb = 0;
+  int* i8_boff = nullptr;
+  int i8S = 0;
+  int64_t i8_blocks = 0;
+  int4* i8_tdesc = nullptr;
+  unsigned char* i8_tab = nullptr;
   std::vector<double> hx_g, hw_g;
 };
 
@@ -164,6 +181,7 @@ extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
   (void)hipFree(p->units); (void)hipFree(p->unit0); (void)hipFree(p->st_p); (void)hipFree(p->st_s);
   (void)hipFree(p->st_pr); (void)hipFree(p->st_sr); (void)hipFree(p->w_r); (void)hipFree(p->msc_r);
   (void)hipFree(p->K2); (void)hipFree(p->inv_tasks); (void)hipFree(p->inv_off);
+  (void)hipFree(p->i8_boff); (void)hipFree(p->i8_tdesc); (void)hipFree(p->i8_tab);
   delete p;
   return CMM_OK;
 }
@@ -246,6 +264,19 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
     cmm_sht_plan_destroy(p);
     return rc;
   }
+  // int8 synthesis: coefficient chunk blocks per (order, parity)
+  p->nrt = (L / 2 + kI8Rings - 1) / kI8Rings;
+  std::vector<int> boff(2 * L + 1, 0);
+  for (int m = 0; m < L; ++m)
+    for (int par = 0; par < 2; ++par) {
+      const int nl = (L - m - par + 1) / 2;
+      boff[2 * m + par + 1] = boff[2 * m + par] + (nl + kI8Deg - 1) / kI8Deg;
+    }
+  p->i8_nb = boff[2 * L];
+  if ((rc = upload(&p->i8_boff, boff.data(), boff.size()))) {
+    cmm_sht_plan_destroy(p);
+    return rc;
+  }
   std::vector<double> ones(L + 1, 1.0), msc(L);
   for (int m = 0; m < L; ++m) msc[m] = (m == 0 ? 1.0 : 2.0) * L / M_PI;
   // transposed resampling matrices, zero rows j >= L and columns t > L
@@ -281,9 +312,15 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
   return CMM_OK;
 }
 
+static int build_i8_table(cmm_sht_plan* p, int S);
+
 extern "C" int cmm_sht_set_legendre(cmm_sht_plan* p, int mode) {
-  if (!p || mode < CMM_LEGENDRE_FP64 || mode > CMM_LEGENDRE_BF16X7)
+  if (!p || mode < CMM_LEGENDRE_FP64 || mode > CMM_LEGENDRE_I8X8)
     return fail(CMM_E_ARG, "cmm_sht_set_legendre: bad arguments");
+  if (mode >= CMM_LEGENDRE_I8X4) {
+    const int rc = build_i8_table(p, 4 + (mode - CMM_LEGENDRE_I8X4));
+    if (rc) return rc;
+  }
   p->legendre = mode;
   return CMM_OK;
 }
@@ -300,11 +337,15 @@ extern "C" int cmm_sht_plan_gl(const cmm_sht_plan* p, double* x, double* w) {
 // kOzMaxS digits whatever the plan's mode) and their exponents (L, 32).
 static int64_t sht_f_bytes(const cmm_sht_plan* p, int nf) { return (int64_t)nf * (p->L + 1) * p->L * 16; }
 static int64_t sht_oz_dstride(const cmm_sht_plan* p, int nfp) { return p->nJtot * 2 * (2 * nfp) * kOzJ; }
+// int8 synthesis: the coefficient digits of one group of fields (room for
+// kI8MaxS digits), then their exponents (L, 32)
+static int64_t sht_i8_dbytes(const cmm_sht_plan* p) { return ((int64_t)kI8MaxS * p->i8_nb * 1024 + 255) / 256 * 256; }
 static int64_t sht_work_bytes(const cmm_sht_plan* p, int nf) {
   const int64_t L = p->L;
   const int64_t g = (int64_t)nf * L * L * 16;
   const int64_t oz = kOzMaxS * sht_oz_dstride(p, std::min(nf, 16)) * 2 + 256 + L * 32 * 4;
-  return sht_f_bytes(p, nf) + std::max(g, oz);
+  const int64_t i8 = sht_i8_dbytes(p) + L * 32 * 4;
+  return sht_f_bytes(p, nf) + std::max(g, std::max(oz, i8));
 }
 
 extern "C" int cmm_sht_workspace_bytes(const cmm_sht_plan* p, int nf, int64_t* bytes) {
@@ -451,6 +492,13 @@ __global__ __launch_bounds__(256) void k_fft_r2c(FFTPlanDev P, const double2* __
 }
 
 // half spectrum F[f][t][m] -> real rings (irfft * 2L)
+// RF (rings fastest): the coefficients come as G[f][m][t] (the int8
+// synthesis writes whole rings of one order contiguously) instead of
+// F[f][t][m]; element m of ring t is then G[(f L + m)(L + 1) + t].  Blocks
+// are mapped so that four consecutive ring pairs (the eight rings of a
+// 128-byte line of G) run on one XCD: block b runs on XCD b % 8 and its k-th
+// block there (k = b / 8) is pair 4 (8 (k / 4) + b % 8) + k % 4.
+template <bool RF>
 __global__ __launch_bounds__(256) void k_fft_c2r(FFTPlanDev P, const double2* __restrict__ tw, const double2* __restrict__ F,
                           int L, double* __restrict__ f) {
   extern __shared__ double2 lds[];
@@ -458,27 +506,39 @@ __global__ __launch_bounds__(256) void k_fft_c2r(FFTPlanDev P, const double2* __
   double2* b1 = lds + P.N;
   const int N = P.N;
   int pairs = (L + 2) / 2;
-  int fi = blockIdx.x / pairs, pr = blockIdx.x % pairs;
+  int fi, pr;
+  if constexpr (RF) {
+    const int b = (int)blockIdx.x, k = b >> 3;
+    fi = blockIdx.y;
+    pr = 4 * (8 * (k >> 2) + (b & 7)) + (k & 3);
+    if (pr >= pairs) return;
+  } else {
+    fi = blockIdx.x / pairs;
+    pr = blockIdx.x % pairs;
+  }
   int ta = 2 * pr, tb = 2 * pr + 1;
   bool hasb = tb <= L;
-  const double2* Fa = F + ((int64_t)fi * (L + 1) + ta) * L;
-  const double2* Fb = F + ((int64_t)fi * (L + 1) + (hasb ? tb : ta)) * L;
+  // element m of ring a / b: Fa[m * st], Fb[m * st]
+  const int64_t st = RF ? (int64_t)L + 1 : 1;
+  const double2* Fa = RF ? F + (int64_t)fi * L * (L + 1) + ta : F + ((int64_t)fi * (L + 1) + ta) * L;
+  const double2* Fb = RF ? F + (int64_t)fi * L * (L + 1) + (hasb ? tb : ta)
+                         : F + ((int64_t)fi * (L + 1) + (hasb ? tb : ta)) * L;
   for (int n = threadIdx.x; n < N; n += blockDim.x) {
     double2 xa, xb;
     if (n == 0) {
       xa = double2{Fa[0].x, 0.0};
       xb = double2{hasb ? Fb[0].x : 0.0, 0.0};
     } else if (n < L) {
-      xa = Fa[n];
-      xb = hasb ? Fb[n] : double2{0.0, 0.0};
+      xa = Fa[n * st];
+      xb = hasb ? Fb[n * st] : double2{0.0, 0.0};
     } else if (n == L) {
       xa = double2{0.0, 0.0};
       xb = double2{0.0, 0.0};
     } else {
-      double2 a = Fa[N - n];
+      double2 a = Fa[(N - n) * st];
       xa = double2{a.x, -a.y};
       if (hasb) {
-        double2 b = Fb[N - n];
+        double2 b = Fb[(N - n) * st];
         xb = double2{b.x, -b.y};
       } else {
         xb = double2{0.0, 0.0};
@@ -1064,6 +1124,461 @@ __global__ __launch_bounds__(128, S >= 6 ? 1 : 2) void k_leg_inverse_oz(int L, i
     }
     __syncthreads();
     if ((c + 1) % kOzFlush == 0 || c + 1 == nJ) flush(c < kOzFlush);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Int8 digit-table synthesis: the Ozaki scheme on the integer matrix cores
+// (v_mfma_i32_32x32x32_i8), with the Legendre operand precomputed.
+//   Pbar_lm(x_t) = 2^eT sum_{i<S} d_i 2^(-8 (i + 1)),
+//   flm[col]     = 2^eB sum_{j<S} c_j 2^(-8 (j + 1)),   d_i, c_j in [-128, 127],
+// balanced base-256 digits (round to nearest; a digit of +128 becomes -128
+// with one carried into the digit above; the exponents leave |value| 2^-e
+// <= 126/256, so the leading digit stays in range after a carry).  eT is one
+// exponent per (order m, parity p of l - m, tile of 32 northern rings), eB
+// one per (order, column).  Products d_i c_j are exact integers below 2^14
+// and are summed over the degrees in exact int32 accumulators, one per
+// weight k = i + j < S (at most (k + 1) 2^14 L/2 < 2^31 for L <= 2048), so
+// the only errors are the digit truncation of both operands and the left-out
+// products with i + j >= S, each ~2^(-8 S) of the operands' scales; the
+// accumulators become FP64 once per parity.  The table holds the digits of
+// every (m, p, tile, chunk of 32 degrees of parity p) in the instruction's A
+// layout, chunks before the first one with a nonzero digit left out (their
+// rings lie beyond the turning point of the order, where Pbar_lm is below
+// 2^(-8 S - 1) of the tile's scale: ~18 % of the table at L = 512): S x 1 KB
+// per chunk, ~150 MB at L = 512 with 5 digits, so a 12-field synthesis
+// streams the table once instead of running the FP64 recurrence.  Rings
+// t = 1 .. L/2 and their mirrors come from the table; the pole rings (only
+// m = 0 is nonzero there) are summed in FP64 with Pbar_l0(+-1) =
+// (+-1)^l sqrt((2 l + 1) / 4 pi).
+// Operand layouts (v_mfma_i32_32x32x32_i8, measured): A lane l holds
+// A[l % 32][16 (l / 32) + j], B lane l holds B[16 (l / 32) + j][l % 32]
+// (j < 16, one byte each), D lane l element r holds
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// ---------------------------------------------------------------------------
+typedef int ix4 __attribute__((ext_vector_type(4)));
+typedef int ix16 __attribute__((ext_vector_type(16)));
+
+// the S balanced base-256 digits of V, |V| <= 126/256
+template <int S>
+__device__ inline void i8_digits(double V, int (&d)[S]) {
+  double r = V;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const double t = r * 256.0;
+    const double q = rint(t);
+    d[i] = (int)q;
+    r = t - q;  // exact, |r| <= 1/2
+  }
+#pragma unroll
+  for (int i = S - 1; i > 0; --i)
+    if (d[i] == 128) {
+      d[i] = -128;
+      d[i - 1] += 1;
+    }
+}
+
+// exponent e with |a| 2^-e <= 126/256 (0 for a = 0)
+__host__ __device__ inline int i8_exponent(double a) {
+  if (!(a > 0.0)) return 0;
+  int e = ilogb(a) + 2;
+  if (ldexp(a, -e) > 126.0 / 256.0) ++e;
+  return e;
+}
+
+// Table set-up, pass 1: max |Pbar_lm| over each (order, parity, tile, chunk)
+// -> cmax[((m 2 + p) nrt + rt) nkmax + kc].  Block (pair of tiles, order m);
+// lane = ring t = 1 + 32 rt + lane % 32 of tile rt = 2 blockIdx.x + lane / 32,
+// running the FP64 recurrence of the synthesis over all degrees.
+__global__ __launch_bounds__(64) void k_i8_chunk_max(int L, int nrt, int nkmax, const double* __restrict__ x_t,
+                                                     const double* __restrict__ dP, const int* __restrict__ dS,
+                                                     const double2* __restrict__ rec, double* __restrict__ cmax) {
+  const int m = blockIdx.y, lane = threadIdx.x;
+  const int rt = 2 * blockIdx.x + (lane >> 5), lc = lane & 31;
+  const int t = 1 + kI8Rings * rt + lc;
+  const bool valid = rt < nrt && t <= L / 2;
+  const int tt = valid ? t : 1;
+  const double x = x_t[tt];
+  double pp = 0.0, pc = dP[(int64_t)m * (L + 1) + tt];
+  int s = dS[(int64_t)m * (L + 1) + tt];
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  for (int kc = 0; 2 * kI8Deg * kc < L - m; ++kc) {
+    double mx0 = 0.0, mx1 = 0.0;
+    for (int i = 0; i < 2 * kI8Deg; ++i) {
+      const int l = m + 2 * kI8Deg * kc + i;
+      if (l >= L) break;
+      if (l > m) {
+        const double2 ab = rec[lm0 + (l - m)];
+        leg_step(ab.x, ab.y, x, pp, pc);
+      }
+      leg_rescale(pp, pc, s);
+      const double v = ((s == 0) & valid) ? fabs(pc) : 0.0;
+      if (i & 1) mx1 = fmax(mx1, v);
+      else mx0 = fmax(mx0, v);
+    }
+    for (int o = 16; o > 0; o >>= 1) {
+      mx0 = fmax(mx0, __shfl_xor(mx0, o));
+      mx1 = fmax(mx1, __shfl_xor(mx1, o));
+    }
+    if (lc == 0 && rt < nrt) {
+      cmax[(((int64_t)m * 2 + 0) * nrt + rt) * nkmax + kc] = mx0;
+      cmax[(((int64_t)m * 2 + 1) * nrt + rt) * nkmax + kc] = mx1;
+    }
+  }
+}
+
+// Table set-up, pass 2: the digits.  Same lanes and recurrence as pass 1;
+// lane (ring r of tile rt) writes bytes 16 h .. 16 h + 15 of the A operand
+// row r for half h of each stored chunk and digit, the padded rings
+// (t > L/2) and degrees (l >= L) as zeros.
+template <int S>
+__global__ __launch_bounds__(64) void k_i8_table(int L, int nrt, const double* __restrict__ x_t,
+                                                 const double* __restrict__ dP, const int* __restrict__ dS,
+                                                 const double2* __restrict__ rec, const int4* __restrict__ tdesc,
+                                                 unsigned char* __restrict__ tab) {
+  const int m = blockIdx.y, lane = threadIdx.x;
+  const int rt = 2 * blockIdx.x + (lane >> 5), lc = lane & 31;
+  if (rt >= nrt) return;  // second tile of an odd last pair
+  const int t = 1 + kI8Rings * rt + lc;
+  const bool valid = t <= L / 2;
+  const int tt = valid ? t : 1;
+  const double x = x_t[tt];
+  double pp = 0.0, pc = dP[(int64_t)m * (L + 1) + tt];
+  int s = dS[(int64_t)m * (L + 1) + tt];
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  const int4 td[2] = {tdesc[((int64_t)m * 2 + 0) * nrt + rt], tdesc[((int64_t)m * 2 + 1) * nrt + rt]};
+  const int nk[2] = {((L - m + 1) / 2 + kI8Deg - 1) / kI8Deg, ((L - m) / 2 + kI8Deg - 1) / kI8Deg};
+  for (int kc = 0; 2 * kI8Deg * kc < L - m; ++kc) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint32_t w[2][S][4];
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) w[p][i][q] = 0u;
+#pragma unroll
+      for (int b = 0; b < 16; ++b)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const int l = m + 2 * (kI8Deg * kc + 16 * h + b) + p;
+          if (l < L) {
+            if (l > m) {
+              const double2 ab = rec[lm0 + (l - m)];
+              leg_step(ab.x, ab.y, x, pp, pc);
+            }
+            leg_rescale(pp, pc, s);
+            const double v = ((s == 0) & valid) ? pc : 0.0;
+            int d[S];
+            i8_digits<S>(ldexp(v, -td[p].z), d);
+#pragma unroll
+            for (int i = 0; i < S; ++i) w[p][i][b >> 2] |= (uint32_t)(d[i] & 0xff) << (8 * (b & 3));
+          }
+        }
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        if (kc < td[p].x || kc >= nk[p]) continue;
+        unsigned char* dst = tab + ((int64_t)(td[p].y + kc - td[p].x) * S) * 1024 + (lc + 32 * h) * 16;
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+          *reinterpret_cast<uint4*>(dst + i * 1024) = uint4{w[p][i][0], w[p][i][1], w[p][i][2], w[p][i][3]};
+      }
+    }
+  }
+}
+
+// Coefficient digits of order m = blockIdx.x for the columns col < ncols
+// (field f0 + col / 2, real / imaginary part) and the pole rings.  Phase 1
+// (thread = column x one of 16 degree groups): eB[m][col] from max_l |flm|;
+// for m = 0 the FP64 pole sums, written to F[f][0][0] and F[f][L][0] (zeros
+// for m > 0).  Phase 2 (wave = chunk): the B operand of chunk (m, p, kc),
+// byte 16 lane + j of digit i at chunk block boff[2 m + p] + kc = degree
+// l = m + p + 2 (32 kc + 16 (lane / 32) + j), column lane % 32 (zero for
+// l >= L and col >= ncols).
+template <int S>
+__global__ __launch_bounds__(512) void k_i8_coeffs(int L, int nf, int f0, int ncols, const int* __restrict__ boff,
+                                                   const double* __restrict__ flm, int* __restrict__ eB,
+                                                   unsigned char* __restrict__ Bd, double* __restrict__ F) {
+  __shared__ double red[3][16][33];
+  __shared__ int esh[32];
+  const int m = blockIdx.x;
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  const int64_t st = (int64_t)nf * 2;
+  {
+    const int col = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int cc = col < ncols ? col : 0;
+    const double* base = flm + (lm0 * nf + f0 + (cc >> 1)) * 2 + (cc & 1);
+    double amax = 0.0, pn = 0.0, ps = 0.0;
+    for (int l0 = m + g; l0 < L; l0 += 16 * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {  // clamped addresses: loads without branches, all in flight
+        const double x = base[(min(l0 + 16 * u, L - 1) - m) * st];
+        v[u] = l0 + 16 * u < L ? x : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        amax = fmax(amax, fabs(v[u]));
+        if (m == 0) {
+          const int l = l0 + 16 * u;
+          const double c = sqrt((2.0 * l + 1.0) / (4.0 * M_PI)) * v[u];
+          pn += c;
+          ps += (l & 1) ? -c : c;
+        }
+      }
+    }
+    red[0][g][col] = amax;
+    red[1][g][col] = pn;
+    red[2][g][col] = ps;
+    __syncthreads();
+    if (g == 0) {
+      double a = 0.0, sn = 0.0, ss = 0.0;
+      for (int i = 0; i < 16; ++i) {
+        a = fmax(a, red[0][i][col]);
+        sn += red[1][i][col];
+        ss += red[2][i][col];
+      }
+      const int e = i8_exponent(a);
+      esh[col] = e;
+      if (col < ncols) {
+        eB[(int64_t)m * 32 + col] = e;
+        const int64_t gb = ((int64_t)(f0 + (col >> 1)) * L + m) * (L + 1);
+        F[(gb + 0) * 2 + (col & 1)] = sn;
+        F[(gb + L) * 2 + (col & 1)] = ss;
+      }
+    }
+    __syncthreads();
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int bc = lane & 31, h = lane >> 5;
+  const int nk0 = ((L - m + 1) / 2 + kI8Deg - 1) / kI8Deg, nk1 = ((L - m) / 2 + kI8Deg - 1) / kI8Deg;
+  const double sc = ldexp(1.0, -esh[bc]);
+  const double* bb = flm + (lm0 * nf + f0 + (min(bc, ncols - 1) >> 1)) * 2 + (bc & 1);
+  const int bo0 = boff[2 * m], bo1 = boff[2 * m + 1];
+  for (int ci = w; ci < nk0 + nk1; ci += 8) {
+    const int p = ci < nk0 ? 0 : 1, kc = ci < nk0 ? ci : ci - nk0;
+    uint32_t word[S][4];
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) word[i][q] = 0u;
+    const int l0 = m + p + 2 * (kI8Deg * kc + 16 * h);
+    double x[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = bb[(min(l0 + 2 * j, L - 1) - m) * st];
+    __builtin_amdgcn_sched_barrier(0);  // the 16 loads in flight together
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const double v = (l0 + 2 * j < L && bc < ncols) ? x[j] * sc : 0.0;
+      int d[S];
+      i8_digits<S>(v, d);
+#pragma unroll
+      for (int i = 0; i < S; ++i) word[i][j >> 2] |= (uint32_t)(d[i] & 0xff) << (8 * (j & 3));
+    }
+    unsigned char* dst = Bd + ((int64_t)((p ? bo1 : bo0) + kc) * S) * 1024 + lane * 16;
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+      *reinterpret_cast<uint4*>(dst + i * 1024) = uint4{word[i][0], word[i][1], word[i][2], word[i][3]};
+  }
+}
+
+// Synthesis: one block of two waves per (order m, tile rt); wave w sums the
+// degrees with l - m = w (mod 2).  Blocks are mapped so that groups of eight
+// consecutive orders run on one XCD, the eight orders of a tile back to back:
+// their outputs F[f][t][m .. m + 7] share cache lines, which that XCD's L2
+// then completes before writing them back (with the orders spread over the
+// XCDs the 8-byte stores went out as partial lines: 69 against 30 us at
+// L = 512), and its L2 holds the group's coefficient digits for all tiles;
+// groups ascending (longest first).  Per chunk: the operands of the next chunk are
+// issued before the S (S + 1) / 2 products of this one (buffer loads, fenced
+// from the products for the scheduler) into the S int32 accumulators.  The
+// parity sums meet in LDS: E + O and E - O scaled by 2^(eT + eB - 16) give
+// ring t (wave 0) and L - t (wave 1).
+template <int S>
+__global__ __launch_bounds__(128, S >= 7 ? 1 : 2) void k_leg_inverse_i8(int L, int nf, int f0, int nfp, int nrt,
+                                                                       const int4* __restrict__ tdesc,
+                                                                       const int* __restrict__ boff,
+                                                                       const ix4* __restrict__ tab,
+                                                                       const ix4* __restrict__ Bd,
+                                                                       const int* __restrict__ eB,
+                                                                       double* __restrict__ F) {
+  __shared__ double xch[2][16][66];  // (64 lanes; or a 32 x 33 tile)
+  // block b runs on XCD b % 8; its k-th block there (k = b / 8) is order
+  // 8 (8 (k / (8 nrt)) + b % 8) + k % 8 of tile (k / 8) % nrt
+  const int b = (int)blockIdx.x;
+  const int k = b >> 3;
+  const int m = 8 * (8 * (k / (8 * nrt)) + (b & 7)) + (k & 7), rt = (k >> 3) % nrt;
+  if (m >= L) return;
+  const int p = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, hi = lane >> 5, lc = lane & 31;
+  const int nk = ((L - m - p + 1) / 2 + kI8Deg - 1) / kI8Deg;
+  const int4 td = tdesc[((int64_t)m * 2 + p) * nrt + rt];
+  const int nc = nk - td.x;
+  ix16 acc[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) acc[k] = ix16{};
+  if (nc > 0) {
+    const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(tab + (int64_t)td.y * S * 64), (short)0, nc * S * 1024, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(Bd + ((int64_t)boff[2 * m + p] + td.x) * S * 64), (short)0, nc * S * 1024, 0x00020000);
+    const int lo = lane * 16;
+    // two operand sets, each loaded one chunk ahead of its products (no
+    // register copies: a copy would wait for the load).  Chunks c >= nc lie
+    // beyond the buffer resources' range: those loads return zeros, so the
+    // loop runs whole chunk pairs with no condition for the compiler to sink
+    // a load into.
+    ix4 a0[S], b0[S], a1[S], b1[S];
+    auto load = [&](int c, ix4 (&a)[S], ix4 (&b)[S]) {
+      const int o = c * S * 1024 + lo;
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        a[i] = __builtin_amdgcn_raw_buffer_load_b128(rA, o + i * 1024, 0, 2);  // streamed once: non-temporal
+        b[i] = __builtin_amdgcn_raw_buffer_load_b128(rB, o + i * 1024, 0, 0);
+      }
+    };
+    auto mma = [&](const ix4 (&a)[S], const ix4 (&b)[S]) {
+#pragma unroll
+      for (int i = 0; i < S; ++i)
+#pragma unroll
+        for (int j = 0; i + j < S; ++j)
+          acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i + j], 0, 0, 0);
+    };
+    load(0, a0, b0);
+    for (int c = 0; c < nc; c += 2) {
+      load(c + 1, a1, b1);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(a0, b0);
+      __builtin_amdgcn_sched_barrier(0);
+      load(c + 2, a0, b0);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(a1, b1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  double v[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    double x = (double)acc[S - 1][e];
+#pragma unroll
+    for (int k = S - 2; k >= 0; --k) x = fma(x, 0x1p-8, (double)acc[k][e]);
+    v[e] = ldexp(x, td.z);
+    xch[p][e][lane] = v[e];
+  }
+  __syncthreads();
+  // wave 0: ring t = E + O, wave 1: ring L - t = E - O, scaled per column
+  const double sc = ldexp(1.0, eB[(int64_t)m * 32 + lc] - 16);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const double o = xch[p ^ 1][e][lane];
+    v[e] = (p == 0 ? v[e] + o : o - v[e]) * sc;
+  }
+  __syncthreads();
+  // transpose through this wave's half of xch: tile[row][col], row = ring
+  double* tile = &xch[p][0][0];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) tile[(8 * (e >> 2) + 4 * hi + (e & 3)) * 33 + lc] = v[e];
+  // G[f][m][t] complex, rings fastest: 32 consecutive rings of one field
+  // are 512 contiguous bytes (lane = ring, lanes 32..63 the next field)
+  const int R = L / 2;
+  double2* G = reinterpret_cast<double2*>(F);
+  for (int i = lane; i < 32 * nfp; i += 64) {
+    const int row = i & 31, f = i >> 5;
+    const int t = 1 + kI8Rings * rt + row;
+    if (t > R || (p == 1 && L - t == t)) continue;
+    const int tt = p == 0 ? t : L - t;
+    G[((int64_t)(f0 + f) * L + m) * (L + 1) + tt] = double2{tile[row * 33 + 2 * f], tile[row * 33 + 2 * f + 1]};
+  }
+}
+
+// Builds the digit table of S digits for plan p (once per mode change).
+static int build_i8_table(cmm_sht_plan* p, int S) {
+  if (p->i8S == S && p->i8_tab) return CMM_OK;
+  (void)hipFree(p->i8_tab);
+  (void)hipFree(p->i8_tdesc);
+  p->i8_tab = nullptr;
+  p->i8_tdesc = nullptr;
+  p->i8S = 0;
+  const int L = p->L, nrt = p->nrt;
+  const int nkmax = ((L + 1) / 2 + kI8Deg - 1) / kI8Deg;
+  const size_t ncm = (size_t)L * 2 * nrt * nkmax;
+  double* cmax = nullptr;
+  CMM_HIP(hipMalloc(&cmax, ncm * sizeof(double)));
+  const dim3 grid((unsigned)((nrt + 1) / 2), (unsigned)L);
+  hipError_t e = hipMemset(cmax, 0, ncm * sizeof(double));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_i8_chunk_max, grid, dim3(64), 0, 0, L, nrt, nkmax, p->x_t, p->dP_t, p->dS_t, p->rec, cmax);
+    e = hipGetLastError();
+  }
+  std::vector<double> hc(ncm);
+  if (e == hipSuccess) e = hipMemcpy(hc.data(), cmax, ncm * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(cmax);
+  if (e != hipSuccess) return hip_check(e, "cmm_sht_set_legendre: int8 table maxima");
+  std::vector<int4> td((size_t)L * 2 * nrt);
+  int64_t blocks = 0;
+  for (int m = 0; m < L; ++m)
+    for (int par = 0; par < 2; ++par) {
+      const int nk = ((L - m - par + 1) / 2 + kI8Deg - 1) / kI8Deg;
+      for (int rt = 0; rt < nrt; ++rt) {
+        const double* c = &hc[(((size_t)m * 2 + par) * nrt + rt) * nkmax];
+        double tmax = 0.0;
+        for (int kc = 0; kc < nk; ++kc) tmax = std::max(tmax, c[kc]);
+        const int eT = i8_exponent(tmax);
+        int k0 = nk;
+        for (int kc = 0; kc < nk; ++kc)
+          if (tmax > 0.0 && std::ldexp(c[kc], 8 * S - eT) >= 0.25) {
+            k0 = kc;
+            break;
+          }
+        td[((size_t)m * 2 + par) * nrt + rt] = int4{k0, (int)blocks, eT, 0};
+        blocks += nk - k0;
+      }
+    }
+  if (blocks >= ((int64_t)1 << 31) / S) return fail(CMM_E_RANGE, "cmm_sht_set_legendre: int8 table too large");
+  int rc = upload(&p->i8_tdesc, td.data(), td.size());
+  if (rc) return rc;
+  CMM_HIP(hipMalloc(&p->i8_tab, std::max<int64_t>(1, blocks) * S * 1024));
+  switch (S) {
+    case 4: hipLaunchKernelGGL(k_i8_table<4>, grid, dim3(64), 0, 0, L, nrt, p->x_t, p->dP_t, p->dS_t, p->rec, p->i8_tdesc, p->i8_tab); break;
+    case 5: hipLaunchKernelGGL(k_i8_table<5>, grid, dim3(64), 0, 0, L, nrt, p->x_t, p->dP_t, p->dS_t, p->rec, p->i8_tdesc, p->i8_tab); break;
+    case 6: hipLaunchKernelGGL(k_i8_table<6>, grid, dim3(64), 0, 0, L, nrt, p->x_t, p->dP_t, p->dS_t, p->rec, p->i8_tdesc, p->i8_tab); break;
+    case 7: hipLaunchKernelGGL(k_i8_table<7>, grid, dim3(64), 0, 0, L, nrt, p->x_t, p->dP_t, p->dS_t, p->rec, p->i8_tdesc, p->i8_tab); break;
+    default: hipLaunchKernelGGL(k_i8_table<8>, grid, dim3(64), 0, 0, L, nrt, p->x_t, p->dP_t, p->dS_t, p->rec, p->i8_tdesc, p->i8_tab); break;
+  }
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_check(e, "cmm_sht_set_legendre: int8 table digits");
+  p->i8S = S;
+  p->i8_blocks = blocks;
+  return CMM_OK;
+}
+
+template <int S>
+static int i8_inverse_s(const cmm_sht_plan* p, const double* flm, int nf, int f0, int nfp, double2* F, char* wk,
+                        hipStream_t st) {
+  const int L = p->L;
+  unsigned char* Bd = reinterpret_cast<unsigned char*>(wk);
+  int* eB = reinterpret_cast<int*>(wk + sht_i8_dbytes(p));
+  hipLaunchKernelGGL(k_i8_coeffs<S>, dim3(L), dim3(512), 0, st, L, nf, f0, 2 * nfp, p->i8_boff, flm, eB, Bd,
+                     reinterpret_cast<double*>(F));
+  CMM_LAUNCH_CHECK("cmm_sht_inverse: int8 coefficient digits");
+  hipLaunchKernelGGL(k_leg_inverse_i8<S>, dim3((unsigned)(64 * p->nrt * ((L + 63) / 64))), dim3(128), 0, st, L, nf, f0,
+                     nfp, p->nrt,
+                     p->i8_tdesc, p->i8_boff, reinterpret_cast<const ix4*>(p->i8_tab),
+                     reinterpret_cast<const ix4*>(Bd), eB, reinterpret_cast<double*>(F));
+  CMM_LAUNCH_CHECK("cmm_sht_inverse: int8 legendre");
+  return CMM_OK;
+}
+
+static int i8_inverse(const cmm_sht_plan* p, const double* flm, int nf, int f0, int nfp, double2* F, char* wk,
+                      hipStream_t st) {
+  switch (p->i8S) {
+    case 4: return i8_inverse_s<4>(p, flm, nf, f0, nfp, F, wk, st);
+    case 5: return i8_inverse_s<5>(p, flm, nf, f0, nfp, F, wk, st);
+    case 6: return i8_inverse_s<6>(p, flm, nf, f0, nfp, F, wk, st);
+    case 7: return i8_inverse_s<7>(p, flm, nf, f0, nfp, F, wk, st);
+    default: return i8_inverse_s<8>(p, flm, nf, f0, nfp, F, wk, st);
   }
 }
 
@@ -1719,7 +2234,15 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   const dim3 grid((unsigned)p->inv_waves), block(64);
   // staged recurrence pairs: at most ceil(L / 8) + 2 chunks of 8 degrees
   const size_t lds_rec = (size_t)((L + kInvDeg - 1) / kInvDeg + 2) * kInvDeg * sizeof(double2);
-  if (p->legendre >= CMM_LEGENDRE_BF16X3) {
+  if (p->legendre >= CMM_LEGENDRE_I8X4) {
+    char* wk = reinterpret_cast<char*>(work) + sht_f_bytes(p, nf);
+    for (int f0 = 0; f0 < nf;) {
+      const int nfp = std::min(16, nf - f0);
+      const int rc = i8_inverse(p, flm, nf, f0, nfp, F, wk, st);
+      if (rc) return rc;
+      f0 += nfp;
+    }
+  } else if (p->legendre >= CMM_LEGENDRE_BF16X3) {
     char* oz = reinterpret_cast<char*>(work) + sht_f_bytes(p, nf);
     for (int f0 = 0; f0 < nf;) {
       const int nfp = std::min(16, nf - f0);
@@ -1740,7 +2263,11 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   }
   int pairs = (L + 2) / 2;
   size_t lds = (size_t)2 * p->N * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_c2r, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, F, L, f);
+  if (p->legendre >= CMM_LEGENDRE_I8X4)  // rings-fastest coefficients
+    hipLaunchKernelGGL(k_fft_c2r<true>, dim3((unsigned)(32 * ((pairs + 31) / 32)), (unsigned)nf), dim3(256), lds, st,
+                       fft_plan(p), p->tw, F, L, f);
+  else
+    hipLaunchKernelGGL(k_fft_c2r<false>, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, F, L, f);
   CMM_LAUNCH_CHECK("cmm_sht_inverse: fft");
   return CMM_OK;
 }
@@ -2003,7 +2530,7 @@ extern "C" int cmm_sht_forward_adjoint(const cmm_sht_plan* p, const double* flm_
   CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: resample");
   const int pairs = (L + 2) / 2;
   const size_t lds = (size_t)2 * p->N * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_c2r, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, F, L, fbar);
+  hipLaunchKernelGGL(k_fft_c2r<false>, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, F, L, fbar);
   CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: fft");
   return CMM_OK;
 }

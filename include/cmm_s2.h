@@ -286,6 +286,22 @@ int cmm_sht_plan_destroy(cmm_sht_plan* plan);
 #define CMM_LEGENDRE_BF16X5 4
 #define CMM_LEGENDRE_BF16X6 5
 #define CMM_LEGENDRE_BF16X7 6
+/* I8XS (S = 4..8): the Ozaki scheme on int8 matrix cores.  Pbar at the
+ * northern MWSS rings is held as S balanced base-256 int8 digits per value
+ * (one exponent per order, parity of l - m and tile of 32 rings), built once
+ * by this call (device memory ~S x 30 MB at L = 512, ~S x 240 MB at
+ * L = 1024; chunks of 32 degrees whose digits are all zero are not stored);
+ * the coefficients are split the same way per call (one exponent per order
+ * and column) and the S (S + 1) / 2 leading digit products are summed
+ * exactly in int32 accumulators by weight, converted to FP64 once per
+ * parity.  Error ~2^(-8 S) of the operands' scales (S = 5: ~1e-12,
+ * S = 7: ~1e-16, S = 8: FP64 rounding level).  The pole rings are summed
+ * in FP64. */
+#define CMM_LEGENDRE_I8X4 7
+#define CMM_LEGENDRE_I8X5 8
+#define CMM_LEGENDRE_I8X6 9
+#define CMM_LEGENDRE_I8X7 10
+#define CMM_LEGENDRE_I8X8 11
 int cmm_sht_set_legendre(cmm_sht_plan* plan, int mode);
 /* host copies of the plan's Gauss-Legendre nodes / weights (L each) */
 int cmm_sht_plan_gl(const cmm_sht_plan* plan, double* x, double* w);

@@ -365,6 +365,58 @@ def test_split_bf16_legendre_synthesis_error(L, nf):
         prev = max(errs)
 
 
+_I8_BOUND = {"i8x4": 1e-6, "i8x5": 5e-9, "i8x6": 2e-11, "i8x7": 1e-13, "i8x8": 1e-13}
+
+
+@pytest.mark.parametrize("L,nf", [(16, 1), (27, 5), (64, 12), (255, 3), (512, 12), (1024, 20)])
+def test_int8_table_legendre_synthesis_error(L, nf):
+    """Int8 digit-table synthesis (S balanced int8 digits per operand, the
+    Legendre digits tabulated once per plan, exact int32 sums by weight on
+    the int8 matrix cores) against the FP64 synthesis on the same spectrum:
+    relative max-norm error per field below the bound of each S, falling
+    with S down to the FP64 rounding level (S = 7, 8).  Odd L = 27 and 255
+    exercise partial ring tiles and the equator-free ring sets, nf = 20 two
+    field groups."""
+    from cmm_s2.core import spherical_harmonic_tools as st
+    p64 = st.SHT.get(L, "fp64")
+    g = torch.Generator(device="cuda").manual_seed(2000 + L)
+    ls = torch.from_numpy(p64.l_of.astype(np.float64)).cuda()
+    flm = torch.randn(st.nlm(L), nf, 2, dtype=torch.float64, device="cuda", generator=g)
+    flm *= (1.0 + ls)[:, None, None] ** -1.5
+    f64 = p64.inverse(flm)
+    prev = None
+    for mode, bound in _I8_BOUND.items():
+        fs = st.SHT.get(L, mode).inverse(flm)
+        errs = [float((fs[k] - f64[k]).abs().max() / f64[k].abs().max()) for k in range(nf)]
+        print(L, nf, mode, "max rel err", max(errs), flush=True)
+        assert max(errs) < bound, (mode, max(errs))
+        if prev is not None and mode != "i8x8":
+            assert max(errs) < prev, (mode, max(errs), prev)
+        prev = max(errs)
+
+
+@pytest.mark.parametrize("L,nf", [(8, 2), (33, 3), (64, 12)])
+def test_int8_table_synthesis_against_oracle(L, nf):
+    """The int8 synthesis with 7 and 8 digits against the oracle's FP64
+    synthesis (the FP64 tolerance), 5 digits within its bound; the pole
+    rings (summed in FP64) included."""
+    from cmm_s2.core import spherical_harmonic_tools as st
+    from oracle import sht as osh
+    rng = np.random.default_rng(L)
+    p64 = st.SHT.get(L, "fp64")
+    a = np.zeros((nf, L, L), complex)
+    for f in range(nf):
+        a[f] = rng.standard_normal((L, L)) + 1j * rng.standard_normal((L, L))
+    a *= np.tril(np.ones((L, L)))[None]  # m <= l
+    a[:, :, 0] = a[:, :, 0].real
+    ref = osh.inverse_lm(a, L)
+    flm = np.stack([a[:, p64.l_of, p64.m_of].real.T, a[:, p64.l_of, p64.m_of].imag.T], -1)
+    for mode, bound in [("i8x5", 5e-9), ("i8x7", T_POS), ("i8x8", T_POS)]:
+        got = st.SHT.get(L, mode).inverse(torch.from_numpy(np.ascontiguousarray(flm)).cuda()).cpu().numpy()
+        for f in range(nf):
+            assert rel_err(got[f], ref[f]) < bound, (mode, f, rel_err(got[f], ref[f]))
+
+
 def test_enstrophy_energy_against_oracle():
     """Device enstrophy / energy (forward transform + weighted sums) against
     the oracle's restatement of figures/data_gen_convergence.py:58-95 on a
