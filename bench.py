@@ -219,8 +219,10 @@ def main():
         f"{cons['enstrophy_rel_err']:.3e}, energy rel. change {cons['energy_rel_err']:.3e} over the run")
     mixed = None
     if not a.compose_only and not a.no_mixed:
-        mixed = {mode: mixed_run(a, mesh, w0, wf, grid, s_step, sim, mode) for mode in _MIXED}
-        mixed["fp64_initial_field_perturbed_2^-50"] = sensitivity_run(a, mesh, w0, wf, grid)
+        floor = sensitivity_run(a, mesh, w0, wf, grid)
+        mixed = {mode: mixed_run(a, mesh, w0, wf, grid, s_step, sim, mode, floor["vorticity_max_rel_err"])
+                 for mode in _MIXED}
+        mixed["fp64_initial_field_perturbed_2^-50"] = floor
     cells = int(mesh.stats()["cells_per_face"])
     traffic = _profile_value(TRAFFIC_PROFILE, f"traffic_bytes[ico={a.ico},Lf={a.Lf},depth={depth},cells={cells},"
                                               f"tile_rows={tile_rows}]")
@@ -476,7 +478,13 @@ _MIXED = {"fp32": "fp32 matrix cores, fp64 recurrence, fp64-compensated 32-degre
           "bf16x6": "split-bf16, 6 fixed-point digits per operand (21 digit products) on bf16 matrix cores, "
                     "exact fp32 sums of the four leading weights, fp64 flush every 192 degrees per parity",
           "bf16x7": "split-bf16, 7 fixed-point digits per operand (28 digit products) on bf16 matrix cores, "
-                    "exact fp32 sums of the four leading weights, fp64 flush every 192 degrees per parity"}
+                    "exact fp32 sums of the four leading weights, fp64 flush every 192 degrees per parity",
+          "i8x5": "int8 Ozaki scheme, 5 int8 digits per operand (15 digit products) on int8 matrix cores, Legendre "
+                  "digits tabulated once per plan, exact int32 sums per 32-degree chunk, fp64 across chunks",
+          "i8x6": "int8 Ozaki scheme, 6 int8 digits per operand (21 digit products) on int8 matrix cores, Legendre "
+                  "digits tabulated once per plan, exact int32 sums per 32-degree chunk, fp64 across chunks",
+          "i8x7": "int8 Ozaki scheme, 7 int8 digits per operand (28 digit products) on int8 matrix cores, Legendre "
+                  "digits tabulated once per plan, exact int32 sums per 32-degree chunk, fp64 across chunks"}
 
 
 def sensitivity_run(a, mesh, w0, wf64, grid):
@@ -496,14 +504,16 @@ def sensitivity_run(a, mesh, w0, wf64, grid):
     return {"perturbation": "initial vorticity jets x (1 + 2^-50)", "vorticity_max_rel_err": err}
 
 
-def mixed_run(a, mesh, w0, wf64, grid, s64, sim64, mode):
+def mixed_run(a, mesh, w0, wf64, grid, s64, sim64, mode, floor):
     """The same CMM run with the Legendre synthesis in a lower-precision form
-    (``mode``: fp32 or split-bf16 with 3 / 5 / 6 / 7 digits; FP64 recurrence):
-    seconds per steady step, the relative max-norm vorticity difference on the
-    fine grid, and the relative max-norm difference of every intermediate
-    output against the FP64 run: psi_lm, u and grad u on the grid and the
-    velocity coefficients of the last step, the map positions (jet values at
-    the vertices) of every submap."""
+    (``mode``: fp32, split-bf16 with 3 / 5 / 6 / 7 digits or int8 digit
+    tables with 5 / 6 / 7 digits): seconds per steady step, the relative
+    max-norm vorticity difference on the fine grid, and the relative max-norm
+    difference of every intermediate output against the FP64 run: psi_lm, u
+    and grad u on the grid and the velocity coefficients of the last step,
+    the map positions (jet values at the vertices) of every submap.
+    ``floor``: the vorticity difference of the FP64 run from a 2^-50-perturbed
+    initial field (below it two arithmetics cannot be told apart)."""
     import numpy as np
     import torch
     from cmm_s2.core.spherical_simulations import BarotropicCMM
@@ -540,7 +550,8 @@ def mixed_run(a, mesh, w0, wf64, grid, s64, sim64, mode):
             "s_per_step": round(s32, 6), "fp64_s_per_step": round(s64, 6),
             "step_speedup_vs_fp64": round(s64 / s32, 4), "vorticity_max_rel_err": err,
             "per_output_rel_err_vs_fp64_gpu": per,
-            "target_rel_err": 1e-10, "meets_target": bool(err <= 1e-10 and s32 < s64)}
+            "target_rel_err": 1e-10, "meets_target": bool(err <= 1e-10 and s32 < s64),
+            "rounding_floor": floor, "faster_than_fp64_at_floor": bool(err <= max(1e-10, 1.2 * floor) and s32 < s64)}
 
 
 def _cube_cells(x, R):

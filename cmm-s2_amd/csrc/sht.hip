@@ -95,6 +95,7 @@ struct cmm_sht_plan {
   int nrt = 0;
   int i8_nb = 0;
   int* i8_boff = nullptr;
+  int* i8_cdesc = nullptr;   // (i8_nb): chunk block -> (m << 16) | (p << 15) | kc
   int i8S = 0;
   int64_t i8_blocks = 0;
   int4* i8_tdesc = nullptr;
@@ -181,7 +182,7 @@ extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
   (void)hipFree(p->units); (void)hipFree(p->unit0); (void)hipFree(p->st_p); (void)hipFree(p->st_s);
   (void)hipFree(p->st_pr); (void)hipFree(p->st_sr); (void)hipFree(p->w_r); (void)hipFree(p->msc_r);
   (void)hipFree(p->K2); (void)hipFree(p->inv_tasks); (void)hipFree(p->inv_off);
-  (void)hipFree(p->i8_boff); (void)hipFree(p->i8_tdesc); (void)hipFree(p->i8_tab);
+  (void)hipFree(p->i8_boff); (void)hipFree(p->i8_cdesc); (void)hipFree(p->i8_tdesc); (void)hipFree(p->i8_tab);
   delete p;
   return CMM_OK;
 }
@@ -273,7 +274,12 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
       boff[2 * m + par + 1] = boff[2 * m + par] + (nl + kI8Deg - 1) / kI8Deg;
     }
   p->i8_nb = boff[2 * L];
-  if ((rc = upload(&p->i8_boff, boff.data(), boff.size()))) {
+  std::vector<int> cdesc(p->i8_nb);
+  for (int m = 0; m < L; ++m)
+    for (int par = 0; par < 2; ++par)
+      for (int kc = 0; kc < boff[2 * m + par + 1] - boff[2 * m + par]; ++kc)
+        cdesc[boff[2 * m + par] + kc] = (m << 16) | (par << 15) | kc;
+  if ((rc = upload(&p->i8_boff, boff.data(), boff.size())) || (rc = upload(&p->i8_cdesc, cdesc.data(), cdesc.size()))) {
     cmm_sht_plan_destroy(p);
     return rc;
   }
@@ -338,13 +344,13 @@ extern "C" int cmm_sht_plan_gl(const cmm_sht_plan* p, double* x, double* w) {
 static int64_t sht_f_bytes(const cmm_sht_plan* p, int nf) { return (int64_t)nf * (p->L + 1) * p->L * 16; }
 static int64_t sht_oz_dstride(const cmm_sht_plan* p, int nfp) { return p->nJtot * 2 * (2 * nfp) * kOzJ; }
 // int8 synthesis: the coefficient digits of one group of fields (room for
-// kI8MaxS digits), then their exponents (L, 32)
+// kI8MaxS digits), then their exponents (i8_nb chunks, 32 columns)
 static int64_t sht_i8_dbytes(const cmm_sht_plan* p) { return ((int64_t)kI8MaxS * p->i8_nb * 1024 + 255) / 256 * 256; }
 static int64_t sht_work_bytes(const cmm_sht_plan* p, int nf) {
   const int64_t L = p->L;
   const int64_t g = (int64_t)nf * L * L * 16;
   const int64_t oz = kOzMaxS * sht_oz_dstride(p, std::min(nf, 16)) * 2 + 256 + L * 32 * 4;
-  const int64_t i8 = sht_i8_dbytes(p) + L * 32 * 4;
+  const int64_t i8 = sht_i8_dbytes(p) + (int64_t)p->i8_nb * 32 * 4;
   return sht_f_bytes(p, nf) + std::max(g, std::max(oz, i8));
 }
 
@@ -523,6 +529,24 @@ __global__ __launch_bounds__(256) void k_fft_c2r(FFTPlanDev P, const double2* __
   const double2* Fa = RF ? F + (int64_t)fi * L * (L + 1) + ta : F + ((int64_t)fi * (L + 1) + ta) * L;
   const double2* Fb = RF ? F + (int64_t)fi * L * (L + 1) + (hasb ? tb : ta)
                          : F + ((int64_t)fi * (L + 1) + (hasb ? tb : ta)) * L;
+  if constexpr (RF) {
+    // each coefficient read once (strided by the ring count): element m fills
+    // n = m and its conjugate n = N - m
+    for (int n = threadIdx.x; n <= L; n += blockDim.x) {
+      if (n == L) {
+        b0[L] = double2{0.0, 0.0};
+        continue;
+      }
+      const double2 a = Fa[n * st];
+      const double2 b = hasb ? Fb[n * st] : double2{0.0, 0.0};
+      if (n == 0) {
+        b0[0] = double2{a.x, b.x};
+      } else {
+        b0[n] = double2{a.x - b.y, a.y + b.x};      // a + i b
+        b0[N - n] = double2{a.x + b.y, b.x - a.y};  // conj(a) + i conj(b)
+      }
+    }
+  } else
   for (int n = threadIdx.x; n < N; n += blockDim.x) {
     double2 xa, xb;
     if (n == 0) {
@@ -1136,12 +1160,12 @@ __global__ __launch_bounds__(128, S >= 6 ? 1 : 2) void k_leg_inverse_oz(int L, i
 // with one carried into the digit above; the exponents leave |value| 2^-e
 // <= 126/256, so the leading digit stays in range after a carry).  eT is one
 // exponent per (order m, parity p of l - m, tile of 32 northern rings), eB
-// one per (order, column).  Products d_i c_j are exact integers below 2^14
-// and are summed over the degrees in exact int32 accumulators, one per
-// weight k = i + j < S (at most (k + 1) 2^14 L/2 < 2^31 for L <= 2048), so
-// the only errors are the digit truncation of both operands and the left-out
-// products with i + j >= S, each ~2^(-8 S) of the operands' scales; the
-// accumulators become FP64 once per parity.  The table holds the digits of
+// one per chunk of 32 degrees and column.  Products d_i c_j are exact
+// integers below 2^14 and are summed over a chunk's degrees in exact int32
+// accumulators, one per weight k = i + j < S (at most (k + 1) 2^14 32 < 2^31),
+// so the only errors are the digit truncation of both operands and the
+// left-out products with i + j >= S, each ~2^(-8 S) of the operands' scales;
+// the accumulators enter FP64 sums once per chunk.  The table holds the digits of
 // every (m, p, tile, chunk of 32 degrees of parity p) in the instruction's A
 // layout, chunks before the first one with a nonzero digit left out (their
 // rings lie beyond the turning point of the order, where Pbar_lm is below
@@ -1288,113 +1312,118 @@ __global__ __launch_bounds__(64) void k_i8_table(int L, int nrt, const double* _
   }
 }
 
-// Coefficient digits of order m = blockIdx.x for the columns col < ncols
-// (field f0 + col / 2, real / imaginary part) and the pole rings.  Phase 1
-// (thread = column x one of 16 degree groups): eB[m][col] from max_l |flm|;
-// for m = 0 the FP64 pole sums, written to F[f][0][0] and F[f][L][0] (zeros
-// for m > 0).  Phase 2 (wave = chunk): the B operand of chunk (m, p, kc),
-// byte 16 lane + j of digit i at chunk block boff[2 m + p] + kc = degree
-// l = m + p + 2 (32 kc + 16 (lane / 32) + j), column lane % 32 (zero for
-// l >= L and col >= ncols).
+// Coefficient digits, one wave per chunk block (the chunk (m, p, kc) of
+// cdesc) for the columns col < ncols (field f0 + col / 2, real / imaginary
+// part): lane = (column lane % 32, degrees 16 (lane / 32) .. + 15 of the
+// chunk), so that byte 16 lane + j of digit i of the chunk's block is degree
+// l = m + p + 2 (32 kc + 16 (lane / 32) + j) (zero for l >= L and col >=
+// ncols): the B operand.  Each chunk and column has its own exponent
+// eB[chunk][col] (max over its 32 degrees, |flm| 2^-eB <= 126/256).  The
+// last nfp blocks sum the pole rings of field f0 + (block - nblk): order 0
+// (Pbar_l0(+-1) = (+-1)^l sqrt((2 l + 1) / 4 pi), FP64) into G[f][0][0] and
+// G[f][0][L]; the first chunk of each order m > 0 zeroes G[f][m][0] and
+// G[f][m][L].
 template <int S>
-__global__ __launch_bounds__(512) void k_i8_coeffs(int L, int nf, int f0, int ncols, const int* __restrict__ boff,
-                                                   const double* __restrict__ flm, int* __restrict__ eB,
-                                                   unsigned char* __restrict__ Bd, double* __restrict__ F) {
-  __shared__ double red[3][16][33];
-  __shared__ int esh[32];
-  const int m = blockIdx.x;
-  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+__global__ __launch_bounds__(256) void k_i8_coeffs(int L, int nf, int f0, int ncols, int nb,
+                                                   const int* __restrict__ cdesc, const double* __restrict__ flm,
+                                                   int* __restrict__ eB, unsigned char* __restrict__ Bd,
+                                                   double* __restrict__ F) {
   const int64_t st = (int64_t)nf * 2;
-  {
-    const int col = threadIdx.x & 31, g = threadIdx.x >> 5;
-    const int cc = col < ncols ? col : 0;
-    const double* base = flm + (lm0 * nf + f0 + (cc >> 1)) * 2 + (cc & 1);
-    double amax = 0.0, pn = 0.0, ps = 0.0;
-    for (int l0 = m + g; l0 < L; l0 += 16 * 8) {
-      double v[8];
+  double2* G = reinterpret_cast<double2*>(F);
+  const int nblk = (nb + 3) / 4;
+  if ((int)blockIdx.x >= nblk) {
+    // pole rings of one field: thread = degree group, both columns
+    __shared__ double2 red[2][256];
+    const int f = (int)blockIdx.x - nblk;
+    double2 sn = {0.0, 0.0}, ss = {0.0, 0.0};
+    for (int l0 = threadIdx.x; l0 < L; l0 += 256 * 4) {
+      double2 x[4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {  // clamped addresses: loads without branches, all in flight
-        const double x = base[(min(l0 + 16 * u, L - 1) - m) * st];
-        v[u] = l0 + 16 * u < L ? x : 0.0;
+      for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const double2*>(flm + ((int64_t)min(l0 + 256 * u, L - 1) * nf + f0 + f) * 2);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int l = l0 + 256 * u;
+        const double c = l < L ? sqrt((2.0 * l + 1.0) / (4.0 * M_PI)) : 0.0;
+        const double sg = (l & 1) ? -c : c;
+        sn.x += c * x[u].x; sn.y += c * x[u].y;
+        ss.x += sg * x[u].x; ss.y += sg * x[u].y;
       }
+    }
+    red[0][threadIdx.x] = sn;
+    red[1][threadIdx.x] = ss;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w)
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        amax = fmax(amax, fabs(v[u]));
-        if (m == 0) {
-          const int l = l0 + 16 * u;
-          const double c = sqrt((2.0 * l + 1.0) / (4.0 * M_PI)) * v[u];
-          pn += c;
-          ps += (l & 1) ? -c : c;
+        for (int q = 0; q < 2; ++q) {
+          red[q][threadIdx.x].x += red[q][threadIdx.x + w].x;
+          red[q][threadIdx.x].y += red[q][threadIdx.x + w].y;
         }
-      }
+      __syncthreads();
     }
-    red[0][g][col] = amax;
-    red[1][g][col] = pn;
-    red[2][g][col] = ps;
-    __syncthreads();
-    if (g == 0) {
-      double a = 0.0, sn = 0.0, ss = 0.0;
-      for (int i = 0; i < 16; ++i) {
-        a = fmax(a, red[0][i][col]);
-        sn += red[1][i][col];
-        ss += red[2][i][col];
-      }
-      const int e = i8_exponent(a);
-      esh[col] = e;
-      if (col < ncols) {
-        eB[(int64_t)m * 32 + col] = e;
-        const int64_t gb = ((int64_t)(f0 + (col >> 1)) * L + m) * (L + 1);
-        F[(gb + 0) * 2 + (col & 1)] = sn;
-        F[(gb + L) * 2 + (col & 1)] = ss;
-      }
+    if (threadIdx.x == 0) {
+      const int64_t gb = (int64_t)(f0 + f) * L * (L + 1);
+      G[gb + 0] = red[0][0];
+      G[gb + L] = red[1][0];
     }
-    __syncthreads();
+    return;
   }
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int bc = lane & 31, h = lane >> 5;
-  const int nk0 = ((L - m + 1) / 2 + kI8Deg - 1) / kI8Deg, nk1 = ((L - m) / 2 + kI8Deg - 1) / kI8Deg;
-  const double sc = ldexp(1.0, -esh[bc]);
+  const int ci = 4 * (int)blockIdx.x + ((int)threadIdx.x >> 6);
+  if (ci >= nb) return;
+  const int lane = threadIdx.x & 63, bc = lane & 31, h = lane >> 5;
+  const int cd = cdesc[ci];
+  const int m = cd >> 16, p = (cd >> 15) & 1, kc = cd & 0x7fff;
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  if (m > 0 && p == 0 && kc == 0 && lane < 2 * (ncols >> 1)) {
+    const int64_t gb = ((int64_t)(f0 + (lane >> 1)) * L + m) * (L + 1);
+    F[(gb + ((lane & 1) ? L : 0)) * 2 + 0] = 0.0;
+    F[(gb + ((lane & 1) ? L : 0)) * 2 + 1] = 0.0;
+  }
   const double* bb = flm + (lm0 * nf + f0 + (min(bc, ncols - 1) >> 1)) * 2 + (bc & 1);
-  const int bo0 = boff[2 * m], bo1 = boff[2 * m + 1];
-  for (int ci = w; ci < nk0 + nk1; ci += 8) {
-    const int p = ci < nk0 ? 0 : 1, kc = ci < nk0 ? ci : ci - nk0;
-    uint32_t word[S][4];
+  const int l0 = m + p + 2 * (kI8Deg * kc + 16 * h);
+  double x[16];
 #pragma unroll
-    for (int i = 0; i < S; ++i)
+  for (int j = 0; j < 16; ++j) x[j] = bb[(min(l0 + 2 * j, L - 1) - m) * st];
+  __builtin_amdgcn_sched_barrier(0);  // the 16 loads in flight together
+  double mx = 0.0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) word[i][q] = 0u;
-    const int l0 = m + p + 2 * (kI8Deg * kc + 16 * h);
-    double x[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) x[j] = bb[(min(l0 + 2 * j, L - 1) - m) * st];
-    __builtin_amdgcn_sched_barrier(0);  // the 16 loads in flight together
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const double v = (l0 + 2 * j < L && bc < ncols) ? x[j] * sc : 0.0;
-      int d[S];
-      i8_digits<S>(v, d);
-#pragma unroll
-      for (int i = 0; i < S; ++i) word[i][j >> 2] |= (uint32_t)(d[i] & 0xff) << (8 * (j & 3));
-    }
-    unsigned char* dst = Bd + ((int64_t)((p ? bo1 : bo0) + kc) * S) * 1024 + lane * 16;
-#pragma unroll
-    for (int i = 0; i < S; ++i)
-      *reinterpret_cast<uint4*>(dst + i * 1024) = uint4{word[i][0], word[i][1], word[i][2], word[i][3]};
+  for (int j = 0; j < 16; ++j) {
+    x[j] = (l0 + 2 * j < L && bc < ncols) ? x[j] : 0.0;
+    mx = fmax(mx, fabs(x[j]));
   }
+  mx = fmax(mx, __shfl_xor(mx, 32));
+  const int e = i8_exponent(mx);
+  if (h == 0) eB[(int64_t)ci * 32 + bc] = e;
+  uint32_t word[S][4];
+#pragma unroll
+  for (int i = 0; i < S; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) word[i][q] = 0u;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    int d[S];
+    i8_digits<S>(ldexp(x[j], -e), d);
+#pragma unroll
+    for (int i = 0; i < S; ++i) word[i][j >> 2] |= (uint32_t)(d[i] & 0xff) << (8 * (j & 3));
+  }
+  unsigned char* dst = Bd + (int64_t)ci * S * 1024 + lane * 16;
+#pragma unroll
+  for (int i = 0; i < S; ++i)
+    *reinterpret_cast<uint4*>(dst + i * 1024) = uint4{word[i][0], word[i][1], word[i][2], word[i][3]};
 }
 
 // Synthesis: one block of two waves per (order m, tile rt); wave w sums the
 // degrees with l - m = w (mod 2).  Blocks are mapped so that groups of eight
-// consecutive orders run on one XCD, the eight orders of a tile back to back:
-// their outputs F[f][t][m .. m + 7] share cache lines, which that XCD's L2
-// then completes before writing them back (with the orders spread over the
-// XCDs the 8-byte stores went out as partial lines: 69 against 30 us at
-// L = 512), and its L2 holds the group's coefficient digits for all tiles;
-// groups ascending (longest first).  Per chunk: the operands of the next chunk are
-// issued before the S (S + 1) / 2 products of this one (buffer loads, fenced
-// from the products for the scheduler) into the S int32 accumulators.  The
-// parity sums meet in LDS: E + O and E - O scaled by 2^(eT + eB - 16) give
-// ring t (wave 0) and L - t (wave 1).
+// consecutive orders run on one XCD (its L2 holds the group's coefficient
+// digits for all tiles), groups ascending (longest first).  Per chunk: the
+// operands of the next chunk are issued before the S (S + 1) / 2 products of
+// this one (buffer loads, fenced from the products for the scheduler) into S
+// int32 sums, which then enter FP64 sums with the chunk's column scale.  The
+// parity sums meet in LDS: E + O and E - O (times 2^eT) give ring t (wave 0)
+// and L - t (wave 1), written rings fastest through a wave-local LDS
+// transpose (whole 512-byte runs: written as F[f][t][m], one order per wave,
+// the 8-byte stores reached DRAM as partial lines, 132 MB for 50 MB of
+// output at L = 512, and the kernel took 67 instead of 38 us).
 template <int S>
 __global__ __launch_bounds__(128, S >= 7 ? 1 : 2) void k_leg_inverse_i8(int L, int nf, int f0, int nfp, int nrt,
                                                                        const int4* __restrict__ tdesc,
@@ -1415,14 +1444,17 @@ __global__ __launch_bounds__(128, S >= 7 ? 1 : 2) void k_leg_inverse_i8(int L, i
   const int nk = ((L - m - p + 1) / 2 + kI8Deg - 1) / kI8Deg;
   const int4 td = tdesc[((int64_t)m * 2 + p) * nrt + rt];
   const int nc = nk - td.x;
-  ix16 acc[S];
+  double v[16];
 #pragma unroll
-  for (int k = 0; k < S; ++k) acc[k] = ix16{};
+  for (int e = 0; e < 16; ++e) v[e] = 0.0;
   if (nc > 0) {
+    const int64_t cb = (int64_t)boff[2 * m + p] + td.x;  // first coefficient chunk block of the tile
     const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(tab + (int64_t)td.y * S * 64), (short)0, nc * S * 1024, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(Bd + ((int64_t)boff[2 * m + p] + td.x) * S * 64), (short)0, nc * S * 1024, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc((void*)(Bd + cb * S * 64), (short)0,
+                                                                        nc * S * 1024, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rE = __builtin_amdgcn_make_buffer_rsrc((void*)(eB + cb * 32), (short)0, nc * 128,
+                                                                        0x00020000);
     const int lo = lane * 16;
     // two operand sets, each loaded one chunk ahead of its products (no
     // register copies: a copy would wait for the load).  Chunks c >= nc lie
@@ -1430,49 +1462,60 @@ __global__ __launch_bounds__(128, S >= 7 ? 1 : 2) void k_leg_inverse_i8(int L, i
     // loop runs whole chunk pairs with no condition for the compiler to sink
     // a load into.
     ix4 a0[S], b0[S], a1[S], b1[S];
-    auto load = [&](int c, ix4 (&a)[S], ix4 (&b)[S]) {
+    int e0, e1;
+    auto load = [&](int c, ix4 (&a)[S], ix4 (&b)[S], int& ex) {
       const int o = c * S * 1024 + lo;
 #pragma unroll
       for (int i = 0; i < S; ++i) {
         a[i] = __builtin_amdgcn_raw_buffer_load_b128(rA, o + i * 1024, 0, 2);  // streamed once: non-temporal
         b[i] = __builtin_amdgcn_raw_buffer_load_b128(rB, o + i * 1024, 0, 0);
       }
+      ex = __builtin_amdgcn_raw_buffer_load_b32(rE, c * 128 + 4 * lc, 0, 0);
     };
-    auto mma = [&](const ix4 (&a)[S], const ix4 (&b)[S]) {
+    // one chunk: the S (S + 1) / 2 leading products into S exact int32 sums
+    // by weight (the first product of each weight starts from zero), then
+    // into the FP64 sums with the chunk's column scale 2^(eB - 16)
+    auto mma = [&](const ix4 (&a)[S], const ix4 (&b)[S], int ex) {
+      ix16 acc[S];
 #pragma unroll
-      for (int i = 0; i < S; ++i)
+      for (int j = 0; j < S; ++j) acc[j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[0], b[j], ix16{}, 0, 0, 0);
+#pragma unroll
+      for (int i = 1; i < S; ++i)
 #pragma unroll
         for (int j = 0; i + j < S; ++j)
           acc[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i + j], 0, 0, 0);
+      const double sc = ldexp(1.0, ex - 16);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        double x = (double)acc[S - 1][e];
+#pragma unroll
+        for (int k = S - 2; k >= 0; --k) x = fma(x, 0x1p-8, (double)acc[k][e]);
+        v[e] = fma(x, sc, v[e]);
+      }
     };
-    load(0, a0, b0);
+    load(0, a0, b0, e0);
     for (int c = 0; c < nc; c += 2) {
-      load(c + 1, a1, b1);
+      load(c + 1, a1, b1, e1);
       __builtin_amdgcn_sched_barrier(0);
-      mma(a0, b0);
+      mma(a0, b0, e0);
       __builtin_amdgcn_sched_barrier(0);
-      load(c + 2, a0, b0);
+      load(c + 2, a0, b0, e0);
       __builtin_amdgcn_sched_barrier(0);
-      mma(a1, b1);
+      mma(a1, b1, e1);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  double v[16];
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
-    double x = (double)acc[S - 1][e];
-#pragma unroll
-    for (int k = S - 2; k >= 0; --k) x = fma(x, 0x1p-8, (double)acc[k][e]);
-    v[e] = ldexp(x, td.z);
+    v[e] = ldexp(v[e], td.z);
     xch[p][e][lane] = v[e];
   }
   __syncthreads();
-  // wave 0: ring t = E + O, wave 1: ring L - t = E - O, scaled per column
-  const double sc = ldexp(1.0, eB[(int64_t)m * 32 + lc] - 16);
+  // wave 0: ring t = E + O, wave 1: ring L - t = E - O
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
     const double o = xch[p ^ 1][e][lane];
-    v[e] = (p == 0 ? v[e] + o : o - v[e]) * sc;
+    v[e] = p == 0 ? v[e] + o : o - v[e];
   }
   __syncthreads();
   // transpose through this wave's half of xch: tile[row][col], row = ring
@@ -1560,8 +1603,8 @@ static int i8_inverse_s(const cmm_sht_plan* p, const double* flm, int nf, int f0
   const int L = p->L;
   unsigned char* Bd = reinterpret_cast<unsigned char*>(wk);
   int* eB = reinterpret_cast<int*>(wk + sht_i8_dbytes(p));
-  hipLaunchKernelGGL(k_i8_coeffs<S>, dim3(L), dim3(512), 0, st, L, nf, f0, 2 * nfp, p->i8_boff, flm, eB, Bd,
-                     reinterpret_cast<double*>(F));
+  hipLaunchKernelGGL(k_i8_coeffs<S>, dim3((unsigned)((p->i8_nb + 3) / 4 + nfp)), dim3(256), 0, st, L, nf, f0, 2 * nfp,
+                     p->i8_nb, p->i8_cdesc, flm, eB, Bd, reinterpret_cast<double*>(F));
   CMM_LAUNCH_CHECK("cmm_sht_inverse: int8 coefficient digits");
   hipLaunchKernelGGL(k_leg_inverse_i8<S>, dim3((unsigned)(64 * p->nrt * ((L + 63) / 64))), dim3(128), 0, st, L, nf, f0,
                      nfp, p->nrt,
