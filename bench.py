@@ -327,7 +327,7 @@ def synthesis_bench(a):
     import torch
     from cmm_s2.core import spherical_harmonic_tools as st
     torch.cuda.set_device(0)
-    forms = ("fp64", "fp32", "bf16x3", "bf16x4", "bf16x5", "bf16x6", "bf16x7")
+    forms = ("fp64", "fp32", "bf16x3", "bf16x4", "bf16x5", "bf16x6", "bf16x7", "i8x4", "i8x5", "i8x6", "i8x7", "i8x8")
     res = {}
     for L in (512, 1024):
         p64 = st.SHT.get(L, "fp64")

@@ -1,7 +1,7 @@
 """GPU checks of the mixed-precision configuration: the barotropic run at
 L = 1024 (1025 x 2048 grid) on the ico-7 map mesh (seed 3, 20 RK4 steps of
-dt = 1/1024, remap every 10) with the Legendre synthesis in FP64, FP32 and
-split-BF16 with 5 and 6 digits.  Per output (psi_lm, u and grad u on the
+dt = 1/1024, remap every 10) with the Legendre synthesis in FP64, FP32,
+split-BF16 with 5 and 6 digits and int8 digit tables with 5 and 6 digits.  Per output (psi_lm, u and grad u on the
 grid, velocity coefficients, map positions at each remap, final vorticity)
 the relative max-norm difference from the FP64 run, and the synthesis of
 the same spectra against the FP64 oracle."""
@@ -16,14 +16,15 @@ pytestmark = pytest.mark.gpu
 L, NT, NMAPS = 1024, 20, 10
 # relative max-norm bounds per arithmetic: (against the FP64 run, synthesis
 # against the FP64 oracle)
-BOUNDS = {"fp64": (0.0, 1e-11), "fp32": (2e-6, 1e-6), "bf16x5": (2e-9, 2e-9), "bf16x6": (2e-10, 1e-11)}
+BOUNDS = {"fp64": (0.0, 1e-11), "fp32": (2e-6, 1e-6), "bf16x5": (2e-9, 2e-9), "bf16x6": (2e-10, 1e-11),
+          "i8x5": (2e-9, 2e-9), "i8x6": (2e-10, 1e-11)}
 
 
 @pytest.fixture(scope="module")
 def runs():
     from cmm_s2.core import mesh_functions as mf, spherical_harmonic_tools as st, dynamical_fields as df
     from cmm_s2.core.spherical_simulations import BarotropicCMM
-    print("config 3: L=1024 ico-7, fp64 / fp32 / bf16x5 / bf16x6", flush=True)
+    print("config 3: L=1024 ico-7, fp64 / fp32 / bf16x5 / bf16x6 / i8x5 / i8x6", flush=True)
     ico = mf.icosahedral_mesh(7)
     mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
     coeffs = df.random_sh_coefficients(3, 64, L)
@@ -48,7 +49,7 @@ def _outputs(sim, w):
     return out
 
 
-@pytest.mark.parametrize("mode", ["fp32", "bf16x5", "bf16x6"])
+@pytest.mark.parametrize("mode", ["fp32", "bf16x5", "bf16x6", "i8x5", "i8x6"])
 def test_mixed_run_against_fp64_run(runs, mode):
     ref = _outputs(*runs["fp64"])
     got = _outputs(*runs[mode])
