@@ -296,6 +296,20 @@ def ensemble_bench(a):
         torch.cuda.empty_cache()
     nf, secf = ens.timed_full_run()
     log(f"ensemble eager full run (bootstrap included): {nf} member steps in {secf:.3f} s ({nf / secf:.1f}/s)")
+    # every step as one launch per kernel for all members (bootstrap not timed)
+    for m in ens.members:
+        m.reset()
+    torch.cuda.synchronize()
+    batched = {}
+    for parts in (1, 2, 4):
+        nb, secb, per_step = ens.timed_batched_run(parts)
+        batched[parts] = (nb, secb, per_step)
+        log(f"ensemble batched steps, {parts} partition(s): {nb} member steps in {secb:.3f} s "
+            f"({nb / secb:.1f} member-steps/s), {per_step} launches per step for {len(ens.members)} members")
+        for m in ens.members:
+            m.reset()
+        torch.cuda.synchronize()
+    nb, secb, per_step = batched[1]
     t0 = time.perf_counter()
     ens.capture()
     cap = time.perf_counter() - t0
@@ -314,6 +328,11 @@ def ensemble_bench(a):
            "value_kind": "HIP-graph replay of whole runs (bootstrap included), member streams",
            "eager_full_run_value": round(nf / secf, 2),
            "eager_streams_value": round(n / sec, 2), "eager_one_stream_value": round(n1 / sec1, 2),
+           "batched_value": round(nb / secb, 2), "batched_launches_per_step": per_step,
+           "batched_partitions_value": {str(k): round(v[0] / v[1], 2) for k, v in batched.items()},
+           "batched_partitions_launches_per_step": {str(k): v[2] for k, v in batched.items()},
+           "batched_note": "every step after the bootstrap issued as one launch per kernel for all members "
+                           "(member-batched grids, blocks mapped to members through a prefix table)",
            "graph_speedup_vs_eager_full_run": round(secf / (secg / 3), 4),
            "member_steps": ng, "seconds": round(secg, 4), "capture_seconds": round(cap, 2)}
     print(json.dumps(rec), flush=True)

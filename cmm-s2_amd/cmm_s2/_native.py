@@ -73,6 +73,13 @@ SIGNATURES = {
     "cmm_sht_inverse": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
     "cmm_sht_forward": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
     "cmm_sht_forward_noalloc": (C.c_int, [vp, vp, C.c_int, vp, vp]),
+    "cmm_batch_create": (C.c_int, [i64, P_vp]),
+    "cmm_batch_destroy": (C.c_int, [vp]),
+    "cmm_batch_begin": (C.c_int, [vp]),
+    "cmm_batch_member": (C.c_int, [vp, C.c_int]),
+    "cmm_batch_cancel": (C.c_int, [vp]),
+    "cmm_batch_launch": (C.c_int, [vp, vp, P_i64]),
+    "cmm_copy_f64": (C.c_int, [vp, vp, i64, vp]),
     "cmm_sht_stream_operators": (C.c_int, [vp, vp, C.c_int, vp, vp]),
     "cmm_sht_scalar_operators": (C.c_int, [vp, vp, vp, vp]),
     "cmm_mwss_velocity_jets": (C.c_int, [vp, vp, vp, vp]),

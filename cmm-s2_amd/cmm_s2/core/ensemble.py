@@ -16,6 +16,15 @@ re-runs it from the members' initial fields with one graph launch, without
 the host's per-step work.  Every kernel argument is fixed at capture (buffer
 addresses, times, depths), so a replay writes the bits of the eager run; the
 initial vorticity buffers may be rewritten in place between replays.
+
+``batched_step()`` advances every member by one step with ONE launch per
+kernel of the step for all of them: the library records each member's
+launches (cmm_batch_begin / cmm_batch_member) and issues, per kernel, one
+grid whose blocks map to the members through a prefix table of their block
+counts (cmm_batch_launch), on the ensemble's batch stream.  Each block runs
+its member's code and arguments, so members keep the bits of their own runs.
+Members whose steps differ in kind (a remap, the identity map after one, the
+Legendre arithmetic) are batched in separate groups.
 """
 import ctypes
 import time
@@ -68,6 +77,10 @@ class Ensemble(object):
         self._cap = _OwnStream()
         self.streams = [o.stream for o in self._own] if streams else [self._own[0].stream] * n
         self.graph = None
+        self._batches = []      # one batch object and stream per partition
+        self._bstreams = []
+        self.batched_launches = 0
+        self.partitions = None
         self.members = []
         for b, s in zip(builders, self.streams):
             with torch.cuda.stream(s):
@@ -79,9 +92,13 @@ class Ensemble(object):
         torch.cuda.synchronize()
         self.graph = None
         self.members = []
-        for o in self._own + [self._cap]:
+        for b in self._batches:
+            N.call("cmm_batch_destroy", b)
+        self._batches = []
+        for o in self._own + [self._cap] + self._bstreams:
             o.destroy()
         self._own = []
+        self._bstreams = []
 
     def __del__(self):
         try:
@@ -105,6 +122,92 @@ class Ensemble(object):
                     m.step()
                 k += 1
         return k
+
+    # bytes of device memory for one batched step's per-member arguments
+    BATCH_BYTES = 8 << 20
+
+    def set_partitions(self, n):
+        """Split the members into ``n`` partitions of about equal work (cost
+        of a step estimated as 2 L^2 + 5 Nv: the velocity grid and the map's
+        vertex feet), each batched on its own stream, so that the hardware
+        queues run the partitions' kernels concurrently."""
+        if not self.own_streams:
+            raise N.CmmError("batched steps need a stream per member (their work buffers are per stream)")
+        n = max(1, min(int(n), len(self.members)))
+        cost = [2.0 * m.L * m.L + 5.0 * m.mesh.nv for m in self.members]
+        order = sorted(range(len(self.members)), key=lambda i: -cost[i])
+        load = [0.0] * n
+        parts = [[] for _ in range(n)]
+        for i in order:  # longest first onto the least loaded partition
+            p = min(range(n), key=lambda k: load[k])
+            parts[p].append(i)
+            load[p] += cost[i]
+        self.partitions = [sorted(p) for p in parts if p]
+        while len(self._batches) < len(self.partitions):
+            h = N.vp()
+            N.call("cmm_batch_create", int(self.BATCH_BYTES), ctypes.byref(h))
+            self._batches.append(h.value)
+            self._bstreams.append(_OwnStream())
+        return self.partitions
+
+    def batched_step(self):
+        """One step of every member that has steps left: per partition (one
+        unless ``set_partitions``), one launch per kernel for each group of
+        its members whose steps are of the same kind, on the partition's batch
+        stream.  Returns the number of member steps issued."""
+        if self.partitions is None:
+            self.set_partitions(1)
+        n = ctypes.c_int64()
+        launches, steps = 0, 0
+        for part, b, bs in zip(self.partitions, self._batches, self._bstreams):
+            groups = {}
+            for i in part:
+                m = self.members[i]
+                if m.count < m.Nt:
+                    key = ((m.count + 1) in m.ns, m.identity, m.legendre, bool(m.U.sparse))
+                    groups.setdefault(key, []).append(i)
+            for idx in groups.values():
+                N.call("cmm_batch_begin", b)
+                try:
+                    for k, i in enumerate(idx):
+                        N.call("cmm_batch_member", b, k)
+                        with torch.cuda.stream(self.streams[i]):
+                            self.members[i].step()
+                except BaseException:
+                    N.call("cmm_batch_cancel", b)
+                    raise
+                N.call("cmm_batch_launch", b, bs.handle, ctypes.byref(n))
+                launches += n.value
+                steps += len(idx)
+        self.batched_launches = launches
+        return steps
+
+    def batched_run(self, partitions=1):
+        """Bootstrap on the member streams, then every step batched."""
+        self.set_partitions(partitions)
+        self.bootstrap()
+        torch.cuda.synchronize()
+        while self.batched_step():
+            pass
+        torch.cuda.synchronize()
+        return [m.submaps for m in self.members]
+
+    def timed_batched_run(self, partitions=1):
+        """Bootstrap, then time the remaining lockstep steps issued batched:
+        returns (member steps, seconds, launches of the last step)."""
+        self.set_partitions(partitions)
+        self.bootstrap()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n, last = 0, 0
+        while True:
+            k = self.batched_step()
+            if not k:
+                break
+            n += k
+            last = self.batched_launches
+        torch.cuda.synchronize()
+        return n, time.perf_counter() - t0, last
 
     def run(self):
         self.bootstrap()

@@ -178,7 +178,13 @@ class composite_sphere_diffeomorphism(object):
 
     def assemble_coefficients(self, i, vals):
         """spherical_spline.py:471-508."""
-        self.vals[i] = B.to_dev(vals)
+        dst = self.vals[i]
+        if isinstance(vals, torch.Tensor) and vals.is_cuda and vals.dtype == torch.float64 and vals.is_contiguous() \
+                and tuple(vals.shape) == tuple(dst.shape) and dst.is_contiguous():
+            # a library kernel (it joins a member-batched step in order)
+            N.call("cmm_copy_f64", dst.data_ptr(), vals.data_ptr(), dst.numel(), B.stream())
+        else:
+            self.vals[i] = B.to_dev(vals)
         dev = None
         if self._coeffs is not None:
             dev, host = _canonical_out(self._coeffs[i], (3, 19, self.mesh.ntri))

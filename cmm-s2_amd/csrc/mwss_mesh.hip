@@ -11,6 +11,7 @@
 #include <cmath>
 #include <vector>
 
+#include "batch.h"
 #include "common.h"
 #include "scatter.h"
 
@@ -31,6 +32,7 @@ struct cmm_mwss_mesh {
 extern "C" int cmm_mwss_mesh_create(int L, const double* thetas, const double* phis,
                                     const double* sin_t, const double* cos_t, const double* sin_p,
                                     const double* cos_p, cmm_mwss_mesh** out) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_mesh_create: no batched form");
   if (L < 2 || !thetas || !phis || !sin_t || !cos_t || !sin_p || !cos_p || !out)
     return fail(CMM_E_ARG, "cmm_mwss_mesh_create: bad arguments");
   if (L > 8192) return fail(CMM_E_RANGE, "cmm_mwss_mesh_create: L > 8192");
@@ -620,11 +622,35 @@ __global__ __launch_bounds__(128) void k_advect(VelDev V, int integ, double dt, 
 // the other slots its four stencil points, so the feet of one vertex sit in
 // neighbouring lanes and their velocity gathers share cache lines
 template <int P, int VK>
-__global__ __launch_bounds__(128) void k_advect_stencil(VelDev V, int integ, double dt, const double* __restrict__ xv,
-                                                        int64_t xv_row, const double* __restrict__ xs, int64_t xs_row,
-                                                        int64_t nv, double* __restrict__ yv, int64_t yv_row,
-                                                        double* __restrict__ ys, int64_t ys_row) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+struct KAdvectStencil {
+  struct Args {
+    VelDev V;
+    int integ;
+    double dt;
+    const double* xv;
+    int64_t xv_row;
+    const double* xs;
+    int64_t xs_row, nv;
+    double* yv;
+    int64_t yv_row;
+    double* ys;
+    int64_t ys_row;
+  };
+  static constexpr int kThreads = 128;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+template <int P, int VK>
+__device__ __forceinline__ void KAdvectStencil<P, VK>::run(const Args& a, int bid) {
+  const VelDev& V = a.V;
+  const int integ = a.integ;
+  const double dt = a.dt;
+  const double* __restrict__ xv = a.xv;
+  const double* __restrict__ xs = a.xs;
+  double* __restrict__ yv = a.yv;
+  double* __restrict__ ys = a.ys;
+  const int64_t xv_row = a.xv_row, xs_row = a.xs_row, nv = a.nv, yv_row = a.yv_row, ys_row = a.ys_row;
+  const int64_t i = (int64_t)bid * blockDim.x + threadIdx.x;
   if (i >= P * nv) return;
   const int64_t v = i / P;
   const int k = (int)(i - v * P) - (P - 4);  // stencil index, -1 = vertex
@@ -636,6 +662,11 @@ __global__ __launch_bounds__(128) void k_advect_stencil(VelDev V, int integ, dou
   dst[0] = out.x;
   dst[orow] = out.y;
   dst[2 * orow] = out.z;
+}
+
+template <int P, int VK>
+__global__ __launch_bounds__(128) void k_advect_stencil(typename KAdvectStencil<P, VK>::Args a) {
+  KAdvectStencil<P, VK>::run(a, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_mwss_locate(cmm_mwss_mesh M, const double* __restrict__ q, int64_t n, int64_t sp,
@@ -765,16 +796,34 @@ constexpr int kPolarRows = 4;
 // assembly are about as many as the selected triangles fill.  The polar rows
 // (always selected) come in chunks of 64, the rest in chunks of 256, so no
 // wave carries more than one round of the assembly on average.
-__global__ __launch_bounds__(64) void k_mwss_assemble_sparse(cmm_mwss_mesh M, const double* __restrict__ vals,
-                                                             double* __restrict__ packed, uint8_t* __restrict__ vbits,
-                                                             int slot, int inherit, uint8_t* __restrict__ mark,
-                                                             int64_t npolar, int64_t polar_chunks) {
+struct KMwssAssembleSparse {
+  struct Args {
+    cmm_mwss_mesh M;
+    const double* vals;
+    double* packed;
+    uint8_t* vbits;
+    int slot, inherit;
+    uint8_t* mark;
+    int64_t npolar, polar_chunks;
+  };
+  static constexpr int kThreads = 64;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+__device__ __forceinline__ void KMwssAssembleSparse::run(const Args& a, int bid) {
+  const cmm_mwss_mesh& M = a.M;
+  const double* __restrict__ vals = a.vals;
+  double* __restrict__ packed = a.packed;
+  uint8_t* __restrict__ vbits = a.vbits;
+  uint8_t* __restrict__ mark = a.mark;
+  const int slot = a.slot, inherit = a.inherit;
+  const int64_t npolar = a.npolar, polar_chunks = a.polar_chunks;
   __shared__ int32_t ids[256];
   __shared__ __align__(16) double stage[16 * 108];
   const int lane = threadIdx.x;
   const unsigned bit = 1u << slot;
   const int64_t Tu = M.Tu;
-  const int64_t b = blockIdx.x;
+  const int64_t b = bid;
   int64_t c0, c1;
   if (b < polar_chunks) {  // [0, npolar) then [Tu - npolar, Tu), 64 at a time
     const int64_t half = (polar_chunks + 1) / 2;
@@ -838,6 +887,10 @@ __global__ __launch_bounds__(64) void k_mwss_assemble_sparse(cmm_mwss_mesh M, co
   }
 }
 
+__global__ __launch_bounds__(64) void k_mwss_assemble_sparse(KMwssAssembleSparse::Args a) {
+  KMwssAssembleSparse::run(a, blockIdx.x);
+}
+
 // grid index (ring, column) of mesh vertex v: poles take the phi = 0 sample
 __device__ inline void vertex_grid(const cmm_mwss_mesh& M, int64_t v, int& t, int& p) {
   if (v == 0) { t = 0; p = 0; }
@@ -852,8 +905,21 @@ __device__ inline d3 rot_grad(d3 g, d3 s) {
   return d3{a.x - scl_ * s.x, a.y - scl_ * s.y, a.z - scl_ * s.z};
 }
 
-__global__ __launch_bounds__(256) void k_velocity_jets(cmm_mwss_mesh M, const double* __restrict__ F, double* __restrict__ vals) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+struct KVelocityJets {
+  struct Args {
+    cmm_mwss_mesh M;
+    const double* F;
+    double* vals;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+__device__ __forceinline__ void KVelocityJets::run(const Args& a, int bid) {
+  const cmm_mwss_mesh& M = a.M;
+  const double* __restrict__ F = a.F;
+  double* __restrict__ vals = a.vals;
+  int64_t v = (int64_t)bid * blockDim.x + threadIdx.x;
   if (v >= M.Nu) return;
   int t, p;
   vertex_grid(M, v, t, p);
@@ -870,6 +936,8 @@ __global__ __launch_bounds__(256) void k_velocity_jets(cmm_mwss_mesh M, const do
     vals[(2 * 4 + 1 + i) * Nu + v] = r.z;
   }
 }
+
+__global__ __launch_bounds__(256) void k_velocity_jets(KVelocityJets::Args a) { KVelocityJets::run(a, blockIdx.x); }
 
 __global__ __launch_bounds__(256) void k_scalar_jets(cmm_mwss_mesh M, const double* __restrict__ F, double* __restrict__ vals) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -991,10 +1059,29 @@ __device__ inline double sh_series(const double* __restrict__ a, int Ls, double 
   return sum;
 }
 
-__global__ __launch_bounds__(256) void k_sample_vorticity(VortArgs A, cmm_mwss_mesh W, const double* __restrict__ wpacked,
-                                   const double* __restrict__ X, int64_t n, int64_t sp, int64_t sc,
-                                   const double* __restrict__ cosg, double* __restrict__ out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+struct KSampleVorticity {
+  struct Args {
+    VortArgs A;
+    cmm_mwss_mesh W;
+    const double* wpacked;
+    const double* X;
+    int64_t n, sp, sc;
+    const double* cosg;
+    double* out;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+__device__ __forceinline__ void KSampleVorticity::run(const Args& a, int bid) {
+  const VortArgs& A = a.A;
+  const cmm_mwss_mesh& W = a.W;
+  const double* __restrict__ wpacked = a.wpacked;
+  const double* __restrict__ X = a.X;
+  const double* __restrict__ cosg = a.cosg;
+  double* __restrict__ out = a.out;
+  const int64_t n = a.n, sp = a.sp, sc = a.sc;
+  int64_t i = (int64_t)bid * blockDim.x + threadIdx.x;
   if (i >= n) return;
   d3 x = load_pt(X, i, sp, sc);
   double w0, cth;
@@ -1023,6 +1110,8 @@ __global__ __launch_bounds__(256) void k_sample_vorticity(VortArgs A, cmm_mwss_m
   int ring = (int)(i / A.ring_len);
   out[i] = w0 + 2 * A.rot * cth - 2 * A.rot * cosg[ring];
 }
+
+__global__ __launch_bounds__(256) void k_sample_vorticity(KSampleVorticity::Args a) { KSampleVorticity::run(a, blockIdx.x); }
 
 // ---------------------------------------------------------------------------
 // adjoints (reverse mode).  Kernels marked "accumulate" add into their
@@ -1835,6 +1924,7 @@ __global__ __launch_bounds__(256) void k_sample_vorticity_adj(cmm_mwss_mesh W, c
 // ---------------------------------------------------------------------------
 extern "C" int cmm_mwss_locate(const cmm_mwss_mesh* m, const double* q, int64_t n, int64_t sp,
                                int64_t sc, int32_t* tri, double* bcc, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_locate: no batched form");
   if (!m || n < 0 || (n > 0 && (!q || !tri))) return fail(CMM_E_ARG, "cmm_mwss_locate: bad arguments");
   if (n == 0) return CMM_OK;
   hipLaunchKernelGGL(k_mwss_locate, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, q, n,
@@ -1846,6 +1936,7 @@ extern "C" int cmm_mwss_locate(const cmm_mwss_mesh* m, const double* q, int64_t 
 extern "C" int cmm_mwss_split_locate(const cmm_mwss_mesh* m, const double* q, int64_t n, int64_t sp, int64_t sc,
                                      const int32_t* known, int32_t* tri, int32_t* sub, double* cb, double* w,
                                      void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_split_locate: no batched form");
   if (!m || n < 0 || (n > 0 && (!q || !tri))) return fail(CMM_E_ARG, "cmm_mwss_split_locate: bad arguments");
   if (n == 0) return CMM_OK;
   hipLaunchKernelGGL(k_mwss_split_locate, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, q, n, sp,
@@ -1856,6 +1947,7 @@ extern "C" int cmm_mwss_split_locate(const cmm_mwss_mesh* m, const double* q, in
 
 extern "C" int cmm_mwss_assemble(const cmm_mwss_mesh* m, const double* vals, int ncomp,
                                  double* coeffs, double* packed, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_assemble: no batched form");
   if (!m || !vals || (ncomp != 1 && ncomp != 3)) return fail(CMM_E_ARG, "cmm_mwss_assemble: bad arguments");
   if (!coeffs && !packed) return CMM_OK;
   if (ncomp == 3)
@@ -1876,15 +1968,15 @@ extern "C" int cmm_mwss_assemble_sparse(const cmm_mwss_mesh* m, const double* va
   const int64_t npolar = std::min<int64_t>(m->N + (int64_t)2 * m->N * (kPolarRows - 1), m->Tu / 2);
   const int64_t pc = 2 * grid_for(npolar, 64);
   const int64_t inner = grid_for(m->Tu - 2 * npolar, 256);
-  hipLaunchKernelGGL(k_mwss_assemble_sparse, dim3((unsigned)(pc + inner)), dim3(64), 0, as_stream(stream), *m, vals,
-                     packed, vbits, slot, inherit, mark, npolar, pc);
-  CMM_LAUNCH_CHECK("cmm_mwss_assemble_sparse");
-  return CMM_OK;
+  return launch<KMwssAssembleSparse>(k_mwss_assemble_sparse, (unsigned)(pc + inner), 64, 0, as_stream(stream),
+                                     KMwssAssembleSparse::Args{*m, vals, packed, vbits, slot, inherit, mark, npolar, pc},
+                                     "cmm_mwss_assemble_sparse");
 }
 
 extern "C" int cmm_mwss_eval(const cmm_mwss_mesh* m, const double* const* levels, const double* w,
                              int nlev, int ncomp, const double* q, int64_t n, int64_t sp, int64_t sc,
                              double* out, int64_t osp, int64_t osc, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_eval: no batched form");
   if (!m || !levels || !w || nlev < 1 || nlev > 3 || (ncomp != 1 && ncomp != 3) || n < 0 ||
       (n > 0 && (!q || !out)))
     return fail(CMM_E_ARG, "cmm_mwss_eval: bad arguments");
@@ -1942,6 +2034,7 @@ static int velocity_device(const cmm_velocity* vel, const char* who, VelDev& V) 
 
 extern "C" int cmm_advect_points(const cmm_velocity* vel, int integ, double dt, const double* x,
                                  int64_t m, double* y, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_advect_points: no batched form");
   if (!vel || m < 0 || (m > 0 && (!x || !y)) || integ < 0 || integ > 4)
     return fail(CMM_E_ARG, "cmm_advect_points: bad arguments");
   VelDev V;
@@ -1966,19 +2059,29 @@ extern "C" int cmm_advect_stencil(const cmm_velocity* vel, int integ, double dt,
   if (rc) return rc;
   if (nv == 0) return CMM_OK;
   const int vk = V.kind != CMM_VEL_SPLINE ? kVelAnalytic : (V.sv.vbits ? kVelSparse : kVelSpline);
-  auto kern = verts ? (vk == kVelAnalytic ? k_advect_stencil<5, kVelAnalytic>
-                       : vk == kVelSpline ? k_advect_stencil<5, kVelSpline> : k_advect_stencil<5, kVelSparse>)
-                    : (vk == kVelAnalytic ? k_advect_stencil<4, kVelAnalytic>
-                       : vk == kVelSpline ? k_advect_stencil<4, kVelSpline> : k_advect_stencil<4, kVelSparse>);
   const int P = verts ? 5 : 4;
-  hipLaunchKernelGGL(kern, dim3(grid_for(P * nv, 128)), dim3(128), 0, as_stream(stream), V, integ, dt, verts,
-                     verts_row, st, st_row, nv, yv, yv_row, yst, yst_row);
-  CMM_LAUNCH_CHECK("cmm_advect_stencil");
-  return CMM_OK;
+  const unsigned blocks = (unsigned)grid_for(P * nv, 128);
+  hipStream_t st_ = as_stream(stream);
+  const char* what = "cmm_advect_stencil";
+#define CMM_ADVECT_LAUNCH(PP, VKK)                                                                              \
+  return launch<KAdvectStencil<PP, VKK>>(k_advect_stencil<PP, VKK>, blocks, 128, 0, st_,                        \
+                                         KAdvectStencil<PP, VKK>::Args{V, integ, dt, verts, verts_row, st, st_row, \
+                                                                       nv, yv, yv_row, yst, yst_row},             \
+                                         what)
+  if (verts) {
+    if (vk == kVelAnalytic) CMM_ADVECT_LAUNCH(5, kVelAnalytic);
+    if (vk == kVelSpline) CMM_ADVECT_LAUNCH(5, kVelSpline);
+    CMM_ADVECT_LAUNCH(5, kVelSparse);
+  }
+  if (vk == kVelAnalytic) CMM_ADVECT_LAUNCH(4, kVelAnalytic);
+  if (vk == kVelSpline) CMM_ADVECT_LAUNCH(4, kVelSpline);
+  CMM_ADVECT_LAUNCH(4, kVelSparse);
+#undef CMM_ADVECT_LAUNCH
 }
 
 extern "C" int cmm_mwss_eval_grad(const cmm_mwss_mesh* m, const double* packed, const double* q, int64_t n,
                                   int64_t sp, int64_t sc, double* out, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_eval_grad: no batched form");
   if (!m || !packed || n < 0 || (n > 0 && (!q || !out))) return fail(CMM_E_ARG, "cmm_mwss_eval_grad: bad arguments");
   if (n == 0) return CMM_OK;
   hipLaunchKernelGGL(k_mwss_eval_grad, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, packed, q, n,
@@ -1990,14 +2093,13 @@ extern "C" int cmm_mwss_eval_grad(const cmm_mwss_mesh* m, const double* packed, 
 extern "C" int cmm_mwss_velocity_jets(const cmm_mwss_mesh* m, const double* fields, double* vals,
                                       void* stream) {
   if (!m || !fields || !vals) return fail(CMM_E_ARG, "cmm_mwss_velocity_jets: bad arguments");
-  hipLaunchKernelGGL(k_velocity_jets, dim3(grid_for(m->Nu, 256)), dim3(256), 0, as_stream(stream), *m,
-                     fields, vals);
-  CMM_LAUNCH_CHECK("cmm_mwss_velocity_jets");
-  return CMM_OK;
+  return launch<KVelocityJets>(k_velocity_jets, (unsigned)grid_for(m->Nu, 256), 256, 0, as_stream(stream),
+                               KVelocityJets::Args{*m, fields, vals}, "cmm_mwss_velocity_jets");
 }
 
 extern "C" int cmm_mwss_scalar_jets(const cmm_mwss_mesh* m, const double* fields, double* vals,
                                     void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_scalar_jets: no batched form");
   if (!m || !fields || !vals) return fail(CMM_E_ARG, "cmm_mwss_scalar_jets: bad arguments");
   hipLaunchKernelGGL(k_scalar_jets, dim3(grid_for(m->Nu, 256)), dim3(256), 0, as_stream(stream), *m,
                      fields, vals);
@@ -2024,15 +2126,15 @@ extern "C" int cmm_sample_vorticity(int kind, const cmm_mwss_mesh* wmesh, const 
   cmm_mwss_mesh W;
   if (wmesh) W = *wmesh;
   if (n == 0) return CMM_OK;
-  hipLaunchKernelGGL(k_sample_vorticity, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), A, W,
-                     wpacked, X, n, sp, sc, cosg, out);
-  CMM_LAUNCH_CHECK("cmm_sample_vorticity");
-  return CMM_OK;
+  return launch<KSampleVorticity>(k_sample_vorticity, (unsigned)grid_for(n, 256), 256, 0, as_stream(stream),
+                                  KSampleVorticity::Args{A, W, wpacked, X, n, sp, sc, cosg, out},
+                                  "cmm_sample_vorticity");
 }
 
 // ---- adjoints -------------------------------------------------------------
 extern "C" int cmm_mwss_velocity_jets_adjoint(const cmm_mwss_mesh* m, const double* vals_bar, double* fields_bar,
                                               void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_velocity_jets_adjoint: no batched form");
   if (!m || !vals_bar || !fields_bar) return fail(CMM_E_ARG, "cmm_mwss_velocity_jets_adjoint: bad arguments");
   const int64_t G = (int64_t)(m->L + 1) * m->N;
   hipLaunchKernelGGL(k_grid_jets_adj<12>, dim3(grid_for(G, 256)), dim3(256), 0, as_stream(stream), *m, vals_bar,
@@ -2043,6 +2145,7 @@ extern "C" int cmm_mwss_velocity_jets_adjoint(const cmm_mwss_mesh* m, const doub
 
 extern "C" int cmm_mwss_scalar_jets_adjoint(const cmm_mwss_mesh* m, const double* vals_bar, double* fields_bar,
                                             void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_scalar_jets_adjoint: no batched form");
   if (!m || !vals_bar || !fields_bar) return fail(CMM_E_ARG, "cmm_mwss_scalar_jets_adjoint: bad arguments");
   const int64_t G = (int64_t)(m->L + 1) * m->N;
   hipLaunchKernelGGL(k_grid_jets_adj<4>, dim3(grid_for(G, 256)), dim3(256), 0, as_stream(stream), *m, vals_bar,
@@ -2053,6 +2156,7 @@ extern "C" int cmm_mwss_scalar_jets_adjoint(const cmm_mwss_mesh* m, const double
 
 extern "C" int cmm_mwss_assemble_adjoint(const cmm_mwss_mesh* m, const double* packed_bar, int ncomp,
                                          double* vals_bar, void* work, int64_t work_bytes, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_assemble_adjoint: no batched form");
   if (!m || !packed_bar || !vals_bar || (ncomp != 1 && ncomp != 3))
     return fail(CMM_E_ARG, "cmm_mwss_assemble_adjoint: bad arguments");
   if (work) {
@@ -2086,6 +2190,7 @@ extern "C" int cmm_advect_stencil_adjoint(const cmm_velocity* vel, int integ, do
                                           int64_t st_row, int64_t nv, const double* yst_bar, int64_t yb_row,
                                           double* const* level_bars, double* st_bar, int64_t sb_row, void* work,
                                           int64_t work_bytes, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_advect_stencil_adjoint: no batched form");
   if (!vel || !st || !yst_bar || !level_bars || nv < 1 || st_row < 4 * nv || yb_row < 4 * nv ||
       (st_bar && sb_row < 4 * nv) ||
       (integ != CMM_INT_EULER && integ != CMM_INT_IMPROVED_EULER && integ != CMM_INT_RK4))
@@ -2134,6 +2239,7 @@ extern "C" int cmm_advect_stencil_adjoint_corners(const cmm_velocity* vel, int i
                                                   double* const* level_corners, uint8_t* const* level_touched,
                                                   double* st_bar, int64_t sb_row, void* work, int64_t work_bytes,
                                                   void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_advect_stencil_adjoint_corners: no batched form");
   if (!vel || !st || !yst_bar || !level_corners || !level_touched || !work || nv < 1 || st_row < 4 * nv ||
       yb_row < 4 * nv || (st_bar && sb_row < 4 * nv) ||
       (integ != CMM_INT_EULER && integ != CMM_INT_IMPROVED_EULER && integ != CMM_INT_RK4))
@@ -2187,6 +2293,7 @@ extern "C" int cmm_advect_stencil_adjoint_corners(const cmm_velocity* vel, int i
 
 extern "C" int cmm_mwss_corner_jets_adjoint(const cmm_mwss_mesh* m, const double* corners, const uint8_t* touched,
                                             double* vals_bar, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_mwss_corner_jets_adjoint: no batched form");
   if (!m || !corners || !touched || !vals_bar) return fail(CMM_E_ARG, "cmm_mwss_corner_jets_adjoint: bad arguments");
   const int64_t ni = m->Nu - 2;
   hipLaunchKernelGGL(k_mwss_assemble_adj_gather<3>, dim3(grid_for(ni, 256) + 2), dim3(256), 0, as_stream(stream), *m,
@@ -2199,6 +2306,7 @@ extern "C" int cmm_sample_vorticity_adjoint(const cmm_mwss_mesh* wmesh, const do
                                             int64_t n, int64_t sp, int64_t sc, const double* wbar, double rot_rate,
                                             double* xbar, int64_t bsp, int64_t bsc, double* wpacked_bar, void* work,
                                             int64_t work_bytes, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_sample_vorticity_adjoint: no batched form");
   if (!wmesh || !wpacked || n < 0 || (n > 0 && (!X || !wbar)))
     return fail(CMM_E_ARG, "cmm_sample_vorticity_adjoint: bad arguments");
   if (n == 0 || (!xbar && !wpacked_bar)) return CMM_OK;

@@ -20,6 +20,7 @@
 #include <queue>
 #include <vector>
 
+#include "batch.h"
 #include "common.h"
 
 using namespace cmm;
@@ -188,6 +189,7 @@ extern "C" int cmm_sht_plan_destroy(cmm_sht_plan* p) {
 }
 
 extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin_t, cmm_sht_plan** out) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_sht_plan_create: no batched form");
   if (L < 2 || !cos_t || !sin_t || !out) return fail(CMM_E_ARG, "cmm_sht_plan_create: bad arguments");
   if (2 * L > kMaxFFT) return fail(CMM_E_RANGE, "cmm_sht_plan_create: L > 2048");
   cmm_sht_plan* p = new cmm_sht_plan();
@@ -321,6 +323,7 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
 static int build_i8_table(cmm_sht_plan* p, int S);
 
 extern "C" int cmm_sht_set_legendre(cmm_sht_plan* p, int mode) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_sht_set_legendre: no batched form");
   if (!p || mode < CMM_LEGENDRE_FP64 || mode > CMM_LEGENDRE_I8X8)
     return fail(CMM_E_ARG, "cmm_sht_set_legendre: bad arguments");
   if (mode >= CMM_LEGENDRE_I8X4) {
@@ -472,14 +475,30 @@ __device__ double2* fft_lds(const FFTPlanDev& P, const double2* __restrict__ tw,
 }
 
 // real rings -> half spectrum F[f][t][m] (m < L), scaled by pi / L
-__global__ __launch_bounds__(256) void k_fft_r2c(FFTPlanDev P, const double2* __restrict__ tw, const double* __restrict__ f,
-                          int L, double2* __restrict__ F) {
+struct KFftR2c {
+  struct Args {
+    FFTPlanDev P;
+    const double2* tw;
+    const double* f;
+    int L;
+    double2* F;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+__device__ __forceinline__ void KFftR2c::run(const Args& a, int bid) {
+  const FFTPlanDev& P = a.P;
+  const double2* __restrict__ tw = a.tw;
+  const double* __restrict__ f = a.f;
+  const int L = a.L;
+  double2* __restrict__ F = a.F;
   extern __shared__ double2 lds[];
   double2* b0 = lds;
   double2* b1 = lds + P.N;
   const int N = P.N;
   int pairs = (L + 2) / 2;
-  int fi = blockIdx.x / pairs, pr = blockIdx.x % pairs;
+  int fi = bid / pairs, pr = bid % pairs;
   int ta = 2 * pr, tb = 2 * pr + 1;
   bool hasb = tb <= L;
   const double* ra = f + ((int64_t)fi * (L + 1) + ta) * N;
@@ -497,6 +516,8 @@ __global__ __launch_bounds__(256) void k_fft_r2c(FFTPlanDev P, const double2* __
   }
 }
 
+__global__ __launch_bounds__(256) void k_fft_r2c(KFftR2c::Args a) { KFftR2c::run(a, blockIdx.x); }
+
 // half spectrum F[f][t][m] -> real rings (irfft * 2L)
 // RF (rings fastest): the coefficients come as G[f][m][t] (the int8
 // synthesis writes whole rings of one order contiguously) instead of
@@ -505,8 +526,26 @@ __global__ __launch_bounds__(256) void k_fft_r2c(FFTPlanDev P, const double2* __
 // 128-byte line of G) run on one XCD: block b runs on XCD b % 8 and its k-th
 // block there (k = b / 8) is pair 4 (8 (k / 4) + b % 8) + k % 4.
 template <bool RF>
-__global__ __launch_bounds__(256) void k_fft_c2r(FFTPlanDev P, const double2* __restrict__ tw, const double2* __restrict__ F,
-                          int L, double* __restrict__ f) {
+struct KFftC2r {
+  struct Args {
+    FFTPlanDev P;
+    const double2* tw;
+    const double2* F;
+    int L;
+    double* f;
+    int gx;  // RF: blocks per field (the grid's x extent)
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+template <bool RF>
+__device__ __forceinline__ void KFftC2r<RF>::run(const Args& a, int bid) {
+  const FFTPlanDev& P = a.P;
+  const double2* __restrict__ tw = a.tw;
+  const double2* __restrict__ F = a.F;
+  const int L = a.L;
+  double* __restrict__ f = a.f;
   extern __shared__ double2 lds[];
   double2* b0 = lds;
   double2* b1 = lds + P.N;
@@ -514,13 +553,13 @@ __global__ __launch_bounds__(256) void k_fft_c2r(FFTPlanDev P, const double2* __
   int pairs = (L + 2) / 2;
   int fi, pr;
   if constexpr (RF) {
-    const int b = (int)blockIdx.x, k = b >> 3;
-    fi = blockIdx.y;
+    const int b = bid % a.gx, k = b >> 3;
+    fi = bid / a.gx;
     pr = 4 * (8 * (k >> 2) + (b & 7)) + (k & 3);
     if (pr >= pairs) return;
   } else {
-    fi = blockIdx.x / pairs;
-    pr = blockIdx.x % pairs;
+    fi = bid / pairs;
+    pr = bid % pairs;
   }
   int ta = 2 * pr, tb = 2 * pr + 1;
   bool hasb = tb <= L;
@@ -578,6 +617,11 @@ __global__ __launch_bounds__(256) void k_fft_c2r(FFTPlanDev P, const double2* __
     ra[n] = Z[n].x;
     if (hasb) rb[n] = Z[n].y;
   }
+}
+
+template <bool RF>
+__global__ __launch_bounds__(256) void k_fft_c2r(typename KFftC2r<RF>::Args a) {
+  KFftC2r<RF>::run(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -647,15 +691,38 @@ typedef float fx4 __attribute__((ext_vector_type(4)));
 constexpr int kMixedFlush = 8;
 
 template <int NT, bool MIXED>
-__global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, int nfp, const int2* __restrict__ tasks,
-                                                       const int* __restrict__ toff, const double* __restrict__ x_t,
-                                                       const double* __restrict__ dP, const int* __restrict__ dS,
-                                                       const double2* __restrict__ rec,
-                                                       const double* __restrict__ flm, double* __restrict__ F) {
+struct KLegInverse {
+  struct Args {
+    int L, nf, f0, nfp;
+    const int2* tasks;
+    const int* toff;
+    const double* x_t;
+    const double* dP;
+    const int* dS;
+    const double2* rec;
+    const double* flm;
+    double* F;
+  };
+  static constexpr int kThreads = 64;
+  static constexpr int kMinEU = 2;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+template <int NT, bool MIXED>
+__device__ __forceinline__ void KLegInverse<NT, MIXED>::run(const Args& a, int bid) {
+  const int L = a.L, nf = a.nf, f0 = a.f0, nfp = a.nfp;
+  const int2* __restrict__ tasks = a.tasks;
+  const int* __restrict__ toff = a.toff;
+  const double* __restrict__ x_t = a.x_t;
+  const double* __restrict__ dP = a.dP;
+  const int* __restrict__ dS = a.dS;
+  const double2* __restrict__ rec = a.rec;
+  const double* __restrict__ flm = a.flm;
+  double* __restrict__ F = a.F;
   __shared__ double Pt[kInvDeg][kInvRings + 2];
   extern __shared__ double2 rsh[];  // recurrence pairs of this order, degrees m .. L-1 (+ padding)
-  const int k1 = toff[blockIdx.x + 1];
-  for (int k = toff[blockIdx.x]; k < k1; ++k) {
+  const int k1 = toff[bid + 1];
+  for (int k = toff[bid]; k < k1; ++k) {
   __syncthreads();  // the previous task's reads of rsh and Pt are done
   const int m = tasks[k].x, rb = tasks[k].y;
   const int lane = threadIdx.x;
@@ -803,6 +870,11 @@ __global__ __launch_bounds__(64, 2) void k_leg_inverse(int L, int nf, int f0, in
       }
     }
   }  // task
+}
+
+template <int NT, bool MIXED>
+__global__ __launch_bounds__(64, 2) void k_leg_inverse(typename KLegInverse<NT, MIXED>::Args a) {
+  KLegInverse<NT, MIXED>::run(a, blockIdx.x);
 }
 
 // Static schedule of k_leg_inverse: the (order, 64-ring block) tasks,
@@ -1324,17 +1396,34 @@ __global__ __launch_bounds__(64) void k_i8_table(int L, int nrt, const double* _
 // G[f][0][L]; the first chunk of each order m > 0 zeroes G[f][m][0] and
 // G[f][m][L].
 template <int S>
-__global__ __launch_bounds__(256) void k_i8_coeffs(int L, int nf, int f0, int ncols, int nb,
-                                                   const int* __restrict__ cdesc, const double* __restrict__ flm,
-                                                   int* __restrict__ eB, unsigned char* __restrict__ Bd,
-                                                   double* __restrict__ F) {
+struct KI8Coeffs {
+  struct Args {
+    int L, nf, f0, ncols, nb;
+    const int* cdesc;
+    const double* flm;
+    int* eB;
+    unsigned char* Bd;
+    double* F;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+template <int S>
+__device__ __forceinline__ void KI8Coeffs<S>::run(const Args& a, int bid) {
+  const int L = a.L, nf = a.nf, f0 = a.f0, ncols = a.ncols, nb = a.nb;
+  const int* __restrict__ cdesc = a.cdesc;
+  const double* __restrict__ flm = a.flm;
+  int* __restrict__ eB = a.eB;
+  unsigned char* __restrict__ Bd = a.Bd;
+  double* __restrict__ F = a.F;
   const int64_t st = (int64_t)nf * 2;
   double2* G = reinterpret_cast<double2*>(F);
   const int nblk = (nb + 3) / 4;
-  if ((int)blockIdx.x >= nblk) {
+  if (bid >= nblk) {
     // pole rings of one field: thread = degree group, both columns
     __shared__ double2 red[2][256];
-    const int f = (int)blockIdx.x - nblk;
+    const int f = bid - nblk;
     double2 sn = {0.0, 0.0}, ss = {0.0, 0.0};
     for (int l0 = threadIdx.x; l0 < L; l0 += 256 * 4) {
       double2 x[4];
@@ -1368,7 +1457,7 @@ __global__ __launch_bounds__(256) void k_i8_coeffs(int L, int nf, int f0, int nc
     }
     return;
   }
-  const int ci = 4 * (int)blockIdx.x + ((int)threadIdx.x >> 6);
+  const int ci = 4 * bid + ((int)threadIdx.x >> 6);
   if (ci >= nb) return;
   const int lane = threadIdx.x & 63, bc = lane & 31, h = lane >> 5;
   const int cd = cdesc[ci];
@@ -1412,6 +1501,9 @@ __global__ __launch_bounds__(256) void k_i8_coeffs(int L, int nf, int f0, int nc
     *reinterpret_cast<uint4*>(dst + i * 1024) = uint4{word[i][0], word[i][1], word[i][2], word[i][3]};
 }
 
+template <int S>
+__global__ __launch_bounds__(256) void k_i8_coeffs(typename KI8Coeffs<S>::Args a) { KI8Coeffs<S>::run(a, blockIdx.x); }
+
 // Synthesis: one block of two waves per (order m, tile rt); wave w sums the
 // degrees with l - m = w (mod 2).  Blocks are mapped so that groups of eight
 // consecutive orders run on one XCD (its L2 holds the group's coefficient
@@ -1425,17 +1517,34 @@ __global__ __launch_bounds__(256) void k_i8_coeffs(int L, int nf, int f0, int nc
 // the 8-byte stores reached DRAM as partial lines, 132 MB for 50 MB of
 // output at L = 512, and the kernel took 67 instead of 38 us).
 template <int S>
-__global__ __launch_bounds__(128, S >= 7 ? 1 : 2) void k_leg_inverse_i8(int L, int nf, int f0, int nfp, int nrt,
-                                                                       const int4* __restrict__ tdesc,
-                                                                       const int* __restrict__ boff,
-                                                                       const ix4* __restrict__ tab,
-                                                                       const ix4* __restrict__ Bd,
-                                                                       const int* __restrict__ eB,
-                                                                       double* __restrict__ F) {
+struct KLegInverseI8 {
+  struct Args {
+    int L, nf, f0, nfp, nrt;
+    const int4* tdesc;
+    const int* boff;
+    const ix4* tab;
+    const ix4* Bd;
+    const int* eB;
+    double* F;
+  };
+  static constexpr int kThreads = 128;
+  static constexpr int kMinEU = S >= 7 ? 1 : 2;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+template <int S>
+__device__ __forceinline__ void KLegInverseI8<S>::run(const Args& a, int bid) {
+  const int L = a.L, nf = a.nf, f0 = a.f0, nfp = a.nfp, nrt = a.nrt;
+  const int4* __restrict__ tdesc = a.tdesc;
+  const int* __restrict__ boff = a.boff;
+  const ix4* __restrict__ tab = a.tab;
+  const ix4* __restrict__ Bd = a.Bd;
+  const int* __restrict__ eB = a.eB;
+  double* __restrict__ F = a.F;
   __shared__ double xch[2][16][66];  // (64 lanes; or a 32 x 33 tile)
   // block b runs on XCD b % 8; its k-th block there (k = b / 8) is order
   // 8 (8 (k / (8 nrt)) + b % 8) + k % 8 of tile (k / 8) % nrt
-  const int b = (int)blockIdx.x;
+  const int b = bid;
   const int k = b >> 3;
   const int m = 8 * (8 * (k / (8 * nrt)) + (b & 7)) + (k & 7), rt = (k >> 3) % nrt;
   if (m >= L) return;
@@ -1535,6 +1644,11 @@ __global__ __launch_bounds__(128, S >= 7 ? 1 : 2) void k_leg_inverse_i8(int L, i
   }
 }
 
+template <int S>
+__global__ __launch_bounds__(128, S >= 7 ? 1 : 2) void k_leg_inverse_i8(typename KLegInverseI8<S>::Args a) {
+  KLegInverseI8<S>::run(a, blockIdx.x);
+}
+
 // Builds the digit table of S digits for plan p (once per mode change).
 static int build_i8_table(cmm_sht_plan* p, int S) {
   if (p->i8S == S && p->i8_tab) return CMM_OK;
@@ -1603,15 +1717,17 @@ static int i8_inverse_s(const cmm_sht_plan* p, const double* flm, int nf, int f0
   const int L = p->L;
   unsigned char* Bd = reinterpret_cast<unsigned char*>(wk);
   int* eB = reinterpret_cast<int*>(wk + sht_i8_dbytes(p));
-  hipLaunchKernelGGL(k_i8_coeffs<S>, dim3((unsigned)((p->i8_nb + 3) / 4 + nfp)), dim3(256), 0, st, L, nf, f0, 2 * nfp,
-                     p->i8_nb, p->i8_cdesc, flm, eB, Bd, reinterpret_cast<double*>(F));
-  CMM_LAUNCH_CHECK("cmm_sht_inverse: int8 coefficient digits");
-  hipLaunchKernelGGL(k_leg_inverse_i8<S>, dim3((unsigned)(64 * p->nrt * ((L + 63) / 64))), dim3(128), 0, st, L, nf, f0,
-                     nfp, p->nrt,
-                     p->i8_tdesc, p->i8_boff, reinterpret_cast<const ix4*>(p->i8_tab),
-                     reinterpret_cast<const ix4*>(Bd), eB, reinterpret_cast<double*>(F));
-  CMM_LAUNCH_CHECK("cmm_sht_inverse: int8 legendre");
-  return CMM_OK;
+  int rc = launch<KI8Coeffs<S>>(k_i8_coeffs<S>, (unsigned)((p->i8_nb + 3) / 4 + nfp), 256, 0, st,
+                                typename KI8Coeffs<S>::Args{L, nf, f0, 2 * nfp, p->i8_nb, p->i8_cdesc, flm, eB, Bd,
+                                                            reinterpret_cast<double*>(F)},
+                                "cmm_sht_inverse: int8 coefficient digits");
+  if (rc) return rc;
+  return launch<KLegInverseI8<S>>(k_leg_inverse_i8<S>, (unsigned)(64 * p->nrt * ((L + 63) / 64)), 128, 0, st,
+                                  typename KLegInverseI8<S>::Args{L, nf, f0, nfp, p->nrt, p->i8_tdesc, p->i8_boff,
+                                                                  reinterpret_cast<const ix4*>(p->i8_tab),
+                                                                  reinterpret_cast<const ix4*>(Bd), eB,
+                                                                  reinterpret_cast<double*>(F)},
+                                  "cmm_sht_inverse: int8 legendre");
 }
 
 static int i8_inverse(const cmm_sht_plan* p, const double* flm, int nf, int f0, int nfp, double2* F, char* wk,
@@ -1635,13 +1751,29 @@ static int i8_inverse(const cmm_sht_plan* p, const double* flm, int nf, int f0, 
 // ---------------------------------------------------------------------------
 constexpr int kResSteps = 8;
 
-__global__ __launch_bounds__(64) void k_resample(int L, int nmg, int tpad, const double* __restrict__ Kt,
-                                                 const double* __restrict__ F, double* __restrict__ G) {
+struct KResample {
+  struct Args {
+    int L, nmg, tpad;
+    const double* Kt;
+    const double* F;
+    double* G;
+    int gx, gy;  // grid extents: (orders x parities, node tiles, fields) flattened
+  };
+  static constexpr int kThreads = 64;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+__device__ __forceinline__ void KResample::run(const Args& a, int bid) {
+  const int L = a.L, nmg = a.nmg, tpad = a.tpad;
+  const double* __restrict__ Kt = a.Kt;
+  const double* __restrict__ F = a.F;
+  double* __restrict__ G = a.G;
+  const int bx = bid % a.gx, by = (bid / a.gx) % a.gy, bz = bid / (a.gx * a.gy);
   const int lane = threadIdx.x;
   const int bk = lane >> 4, bj = lane & 15;
-  const int fi = blockIdx.z;
-  const int j0 = blockIdx.y * 16;
-  const int p = (int)blockIdx.x / nmg, qb = (int)blockIdx.x % nmg;
+  const int fi = bz;
+  const int j0 = by * 16;
+  const int p = bx / nmg, qb = bx % nmg;
   const double* K = Kt + (int64_t)p * tpad * L;
   const double* Ff = F + (int64_t)fi * (L + 1) * L * 2;
   // B column of this lane in N-tile nt: order m = p + 2 (16 qb + 8 nt + bj / 2), part bj & 1
@@ -1696,6 +1828,8 @@ __global__ __launch_bounds__(64) void k_resample(int L, int nmg, int tpad, const
     }
   }
 }
+
+__global__ __launch_bounds__(64) void k_resample(KResample::Args a) { KResample::run(a, blockIdx.x); }
 
 // ---------------------------------------------------------------------------
 // forward Legendre analysis: flm[lm][f] = sum_j w_j Pbar_lm(x_j) G[f][j][m].
@@ -1753,18 +1887,43 @@ __global__ __launch_bounds__(256) void k_leg_restart(int L, int nh, int dstride,
 // the transposed synthesis over the L+1 rings (unit weights, order scale
 // msc, no pole terms for m > 0, as the synthesis skips them).
 template <int C>
-__global__ __launch_bounds__(512) void k_leg_forward(int L, int nn, int nf, const int2* __restrict__ units,
-                                                     const double* __restrict__ xg, const double* __restrict__ wg,
-                                                     const double2* __restrict__ st_p, const int* __restrict__ st_s,
-                                                     const double2* __restrict__ rec, const double2* __restrict__ G,
-                                                     double2* __restrict__ flm, const double* __restrict__ msc,
-                                                     int skip_poles) {
+struct KLegForward {
+  struct Args {
+    int L, nn, nf;
+    const int2* units;
+    const double* xg;
+    const double* wg;
+    const double2* st_p;
+    const int* st_s;
+    const double2* rec;
+    const double2* G;
+    double2* flm;
+    const double* msc;
+    int skip_poles;
+    int gx;  // work units (the grid's x extent; y = field)
+  };
+  static constexpr int kThreads = 512;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+template <int C>
+__device__ __forceinline__ void KLegForward<C>::run(const Args& a, int bid) {
+  const int L = a.L, nn = a.nn, nf = a.nf, skip_poles = a.skip_poles;
+  const int2* __restrict__ units = a.units;
+  const double* __restrict__ xg = a.xg;
+  const double* __restrict__ wg = a.wg;
+  const double2* __restrict__ st_p = a.st_p;
+  const int* __restrict__ st_s = a.st_s;
+  const double2* __restrict__ rec = a.rec;
+  const double2* __restrict__ G = a.G;
+  double2* __restrict__ flm = a.flm;
+  const double* __restrict__ msc = a.msc;
   constexpr int RQ = 64 / C;
   extern __shared__ double2 tile[];  // [C][T + 4]
   __shared__ double2 part[2][8][C];
   __shared__ double2 rs[kFwdSeg];  // recurrence pairs of the segment
-  const int fi = blockIdx.y;
-  const int u = blockIdx.x;
+  const int fi = bid / a.gx;
+  const int u = bid % a.gx;
   const int m = units[u].x, ls = units[u].y;
   const int le = min(L, ls + kFwdSeg);
   const int nh = (nn + 1) / 2;  // northern nodes (incl. the middle one when nn is odd)
@@ -1852,6 +2011,11 @@ __global__ __launch_bounds__(512) void k_leg_forward(int L, int nn, int nf, cons
   }
 }
 
+template <int C>
+__global__ __launch_bounds__(512) void k_leg_forward(typename KLegForward<C>::Args a) {
+  KLegForward<C>::run(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // spectral operators (spherical_harmonic_tools.py:14-19, 49-62)
 // ---------------------------------------------------------------------------
@@ -1906,9 +2070,24 @@ __device__ inline void angmom(int l, int m, F get, Cplx out[3]) {
   angmom_from(l, m, coef_any(l, m - 1, get), coef_any(l, m + 1, get), coef_any(l, m, get), out);
 }
 
-__global__ __launch_bounds__(128) void k_stream_ops(int L, const double2* __restrict__ w, const double* __restrict__ dinv,
-                             int apply_dinv, double2* __restrict__ out) {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+struct KStreamOps {
+  struct Args {
+    int L;
+    const double2* w;
+    const double* dinv;
+    int apply_dinv;
+    double2* out;
+  };
+  static constexpr int kThreads = 128;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+__device__ __forceinline__ void KStreamOps::run(const Args& a, int bid) {
+  const int L = a.L, apply_dinv = a.apply_dinv;
+  const double2* __restrict__ w = a.w;
+  const double* __restrict__ dinv = a.dinv;
+  double2* __restrict__ out = a.out;
+  int64_t idx = (int64_t)bid * blockDim.x + threadIdx.x;
   int64_t total = (int64_t)L * (L + 1) / 2;
   if (idx >= total) return;
   // decode (l, m) from the packed index
@@ -1950,6 +2129,8 @@ __global__ __launch_bounds__(128) void k_stream_ops(int L, const double2* __rest
     for (int k = 0; k < 3; ++k) out[idx * 12 + 3 + 3 * i + k] = double2{g[k].re, g[k].im};
   }
 }
+
+__global__ __launch_bounds__(128) void k_stream_ops(KStreamOps::Args a) { KStreamOps::run(a, blockIdx.x); }
 
 __global__ __launch_bounds__(128) void k_scalar_ops(int L, const double2* __restrict__ w, double2* __restrict__ out) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2274,7 +2455,6 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   const int L = p->L;
   double2* F = reinterpret_cast<double2*>(work);
   hipStream_t st = as_stream(stream);
-  const dim3 grid((unsigned)p->inv_waves), block(64);
   // staged recurrence pairs: at most ceil(L / 8) + 2 chunks of 8 degrees
   const size_t lds_rec = (size_t)((L + kInvDeg - 1) / kInvDeg + 2) * kInvDeg * sizeof(double2);
   if (p->legendre >= CMM_LEGENDRE_I8X4) {
@@ -2286,6 +2466,7 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
       f0 += nfp;
     }
   } else if (p->legendre >= CMM_LEGENDRE_BF16X3) {
+    if (batch_recording()) return fail(CMM_E_STATE, "cmm_sht_inverse: the split-BF16 forms have no batched form");
     char* oz = reinterpret_cast<char*>(work) + sht_f_bytes(p, nf);
     for (int f0 = 0; f0 < nf;) {
       const int nfp = std::min(16, nf - f0);
@@ -2296,23 +2477,30 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
   } else {
     for (int f0 = 0; f0 < nf;) {
       const int nfp = std::min(16, nf - f0);
-      auto kern = p->legendre == CMM_LEGENDRE_FP32 ? (nfp > 8 ? k_leg_inverse<2, true> : k_leg_inverse<1, true>)
-                                                   : (nfp > 8 ? k_leg_inverse<2, false> : k_leg_inverse<1, false>);
-      hipLaunchKernelGGL(kern, grid, block, lds_rec, st, L, nf, f0, nfp, p->inv_tasks, p->inv_off, p->x_t, p->dP_t,
-                         p->dS_t, p->rec, flm, reinterpret_cast<double*>(F));
-      CMM_LAUNCH_CHECK("cmm_sht_inverse: legendre");
+      const unsigned nb = (unsigned)p->inv_waves;
+      const char* what = "cmm_sht_inverse: legendre";
+      double* Fd = reinterpret_cast<double*>(F);
+#define CMM_LEG_INV(NT, MX)                                                                                   \
+  launch<KLegInverse<NT, MX>>(k_leg_inverse<NT, MX>, nb, 64, lds_rec, st,                                    \
+                              typename KLegInverse<NT, MX>::Args{L, nf, f0, nfp, p->inv_tasks, p->inv_off, p->x_t, \
+                                                                 p->dP_t, p->dS_t, p->rec, flm, Fd},           \
+                              what)
+      const int rc = p->legendre == CMM_LEGENDRE_FP32 ? (nfp > 8 ? CMM_LEG_INV(2, true) : CMM_LEG_INV(1, true))
+                                                      : (nfp > 8 ? CMM_LEG_INV(2, false) : CMM_LEG_INV(1, false));
+#undef CMM_LEG_INV
+      if (rc) return rc;
       f0 += nfp;
     }
   }
   int pairs = (L + 2) / 2;
   size_t lds = (size_t)2 * p->N * sizeof(double2);
-  if (p->legendre >= CMM_LEGENDRE_I8X4)  // rings-fastest coefficients
-    hipLaunchKernelGGL(k_fft_c2r<true>, dim3((unsigned)(32 * ((pairs + 31) / 32)), (unsigned)nf), dim3(256), lds, st,
-                       fft_plan(p), p->tw, F, L, f);
-  else
-    hipLaunchKernelGGL(k_fft_c2r<false>, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, F, L, f);
-  CMM_LAUNCH_CHECK("cmm_sht_inverse: fft");
-  return CMM_OK;
+  if (p->legendre >= CMM_LEGENDRE_I8X4) {  // rings-fastest coefficients
+    const int gx = 32 * ((pairs + 31) / 32);
+    return launch<KFftC2r<true>>(k_fft_c2r<true>, (unsigned)(gx * nf), 256, lds, st,
+                                 KFftC2r<true>::Args{fft_plan(p), p->tw, F, L, f, gx}, "cmm_sht_inverse: fft");
+  }
+  return launch<KFftC2r<false>>(k_fft_c2r<false>, (unsigned)(nf * pairs), 256, lds, st,
+                                KFftC2r<false>::Args{fft_plan(p), p->tw, F, L, f, 0}, "cmm_sht_inverse: fft");
 }
 
 // analysis workspace: F (nf, L+1, L) then G (nf, L, L), complex
@@ -2329,6 +2517,7 @@ extern "C" int cmm_sht_forward(const cmm_sht_plan* p, const double* f, int nf, d
 }
 
 extern "C" int cmm_sht_forward_noalloc(const cmm_sht_plan* p, const double* f, int nf, double* flm, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_sht_forward_noalloc: no batched form (members would share its scratch)");
   if (!p || !flm || !f || nf < 1) return fail(CMM_E_ARG, "cmm_sht_forward_noalloc: bad arguments");
   if (!p->scratch) return fail(CMM_E_ARG, "cmm_sht_forward_noalloc: plan has no scratch address");
   const int64_t need = sht_forward_bytes(p, nf);
@@ -2348,37 +2537,47 @@ static int sht_forward(const cmm_sht_plan* p, const double* f, int nf, double* f
   hipStream_t st = as_stream(stream);
   int pairs = (L + 2) / 2;
   size_t lds = (size_t)2 * p->N * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_r2c, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, f, L, F);
-  CMM_LAUNCH_CHECK("cmm_sht_forward: fft");
+  int rc = launch<KFftR2c>(k_fft_r2c, (unsigned)(nf * pairs), 256, lds, st, KFftR2c::Args{fft_plan(p), p->tw, f, L, F},
+                           "cmm_sht_forward: fft");
+  if (rc) return rc;
   const int nmg = ((L + 1) / 2 + 15) / 16;  // groups of 16 orders of one parity (at most ceil(L / 2) orders)
-  hipLaunchKernelGGL(k_resample, dim3(2 * nmg, (L + 15) / 16, nf), dim3(64), 0, st, L, nmg, p->tpad, p->K,
-                     reinterpret_cast<const double*>(F), reinterpret_cast<double*>(G));
-  CMM_LAUNCH_CHECK("cmm_sht_forward: resample");
+  const int gy = (L + 15) / 16;
+  rc = launch<KResample>(k_resample, (unsigned)(2 * nmg * gy * nf), 64, 0, st,
+                         KResample::Args{L, nmg, p->tpad, p->K, reinterpret_cast<const double*>(F),
+                                         reinterpret_cast<double*>(G), 2 * nmg, gy},
+                         "cmm_sht_forward: resample");
+  if (rc) return rc;
   const int nh = (L + 1) / 2;
-  const int T = std::max(64, ((nh + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
+  // at least 128 threads: bandlimits up to 512 share one block size (a
+  // batched launch has one), so their sums run in the same order in a batch
+  // as alone
+  const int T = std::max(128, ((nh + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
   if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_forward: L > 2048");
   // chunks of 8 degrees: half the LDS tile of the 16-degree chunks the
   // restart table is built on (values and scale states agree: rescaling is
   // exact and the restart points are whole chunks of either), so more
   // blocks stay resident on this latency-bound kernel
   size_t lds2 = (size_t)kFwdChunkA * (T + 4) * sizeof(double2);
-  hipLaunchKernelGGL(k_leg_forward<kFwdChunkA>, dim3(p->nunits, nf), dim3(T), lds2, st, L, L, nf, p->units, p->x_g,
-                     p->w_g, p->st_p, p->st_s, p->rec, G, reinterpret_cast<double2*>(flm), nullptr, 0);
-  CMM_LAUNCH_CHECK("cmm_sht_forward: legendre");
-  return CMM_OK;
+  return launch<KLegForward<kFwdChunkA>>(k_leg_forward<kFwdChunkA>, (unsigned)(p->nunits * nf), T, lds2, st,
+                                         KLegForward<kFwdChunkA>::Args{L, L, nf, p->units, p->x_g, p->w_g, p->st_p,
+                                                                       p->st_s, p->rec, G,
+                                                                       reinterpret_cast<double2*>(flm), nullptr, 0,
+                                                                       p->nunits},
+                                         "cmm_sht_forward: legendre");
 }
 
 extern "C" int cmm_sht_stream_operators(const cmm_sht_plan* p, const double* w, int apply_dinv,
                                         double* out, void* stream) {
   if (!p || !w || !out) return fail(CMM_E_ARG, "cmm_sht_stream_operators: bad arguments");
   int64_t total = (int64_t)p->L * (p->L + 1) / 2;
-  hipLaunchKernelGGL(k_stream_ops, dim3(grid_for(total, 128)), dim3(128), 0, as_stream(stream), p->L,
-                     reinterpret_cast<const double2*>(w), p->dinv, apply_dinv, reinterpret_cast<double2*>(out));
-  CMM_LAUNCH_CHECK("cmm_sht_stream_operators");
-  return CMM_OK;
+  return launch<KStreamOps>(k_stream_ops, (unsigned)grid_for(total, 128), 128, 0, as_stream(stream),
+                            KStreamOps::Args{p->L, reinterpret_cast<const double2*>(w), p->dinv, apply_dinv,
+                                             reinterpret_cast<double2*>(out)},
+                            "cmm_sht_stream_operators");
 }
 
 extern "C" int cmm_sht_scalar_operators(const cmm_sht_plan* p, const double* w, double* out, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_sht_scalar_operators: no batched form");
   if (!p || !w || !out) return fail(CMM_E_ARG, "cmm_sht_scalar_operators: bad arguments");
   int64_t total = (int64_t)p->L * (p->L + 1) / 2;
   hipLaunchKernelGGL(k_scalar_ops, dim3(grid_for(total, 128)), dim3(128), 0, as_stream(stream), p->L,
@@ -2525,13 +2724,14 @@ __global__ __launch_bounds__(320) void k_leg_adj_mfma(int L, int nf, int f0, int
 
 extern "C" int cmm_sht_inverse_adjoint(const cmm_sht_plan* p, const double* fbar, int nf, double* flm_bar,
                                        void* work, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_sht_inverse_adjoint: no batched form");
   if (!p || !fbar || !flm_bar || !work || nf < 1) return fail(CMM_E_ARG, "cmm_sht_inverse_adjoint: bad arguments");
   const int L = p->L;
   double2* F = reinterpret_cast<double2*>(work);
   hipStream_t st = as_stream(stream);
   const int pairs = (L + 2) / 2;
   const size_t lds = (size_t)2 * p->N * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_r2c, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, fbar, L, F);
+  hipLaunchKernelGGL(k_fft_r2c, dim3(nf * pairs), dim3(256), lds, st, KFftR2c::Args{fft_plan(p), p->tw, fbar, L, F});
   CMM_LAUNCH_CHECK("cmm_sht_inverse_adjoint: fft");
   const int nhr = L / 2 + 1;
   if (nhr <= 5 * 64) {  // matrix-core form: every field of a group shares the recurrence
@@ -2548,14 +2748,16 @@ extern "C" int cmm_sht_inverse_adjoint(const cmm_sht_plan* p, const double* fbar
   const int T = std::max(64, ((nhr + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
   if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_inverse_adjoint: L > 2046");
   const size_t lds2 = (size_t)kFwdChunk * (T + 4) * sizeof(double2);
-  hipLaunchKernelGGL(k_leg_forward<kFwdChunk>, dim3(p->nunits, nf), dim3(T), lds2, st, L, L + 1, nf, p->units,
-                     p->x_t, p->w_r, p->st_pr, p->st_sr, p->rec, F, reinterpret_cast<double2*>(flm_bar), p->msc_r, 1);
+  hipLaunchKernelGGL(k_leg_forward<kFwdChunk>, dim3(p->nunits * nf), dim3(T), lds2, st,
+                     KLegForward<kFwdChunk>::Args{L, L + 1, nf, p->units, p->x_t, p->w_r, p->st_pr, p->st_sr, p->rec, F,
+                                                  reinterpret_cast<double2*>(flm_bar), p->msc_r, 1, p->nunits});
   CMM_LAUNCH_CHECK("cmm_sht_inverse_adjoint: legendre");
   return CMM_OK;
 }
 
 extern "C" int cmm_sht_forward_adjoint(const cmm_sht_plan* p, const double* flm_bar, int nf, double* fbar,
                                        void* work, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_sht_forward_adjoint: no batched form");
   if (!p || !fbar || !flm_bar || !work || nf < 1) return fail(CMM_E_ARG, "cmm_sht_forward_adjoint: bad arguments");
   const int L = p->L;
   double2* F = reinterpret_cast<double2*>(work);
@@ -2573,13 +2775,15 @@ extern "C" int cmm_sht_forward_adjoint(const cmm_sht_plan* p, const double* flm_
   CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: resample");
   const int pairs = (L + 2) / 2;
   const size_t lds = (size_t)2 * p->N * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_c2r<false>, dim3(nf * pairs), dim3(256), lds, st, fft_plan(p), p->tw, F, L, fbar);
+  hipLaunchKernelGGL(k_fft_c2r<false>, dim3(nf * pairs), dim3(256), lds, st,
+                     KFftC2r<false>::Args{fft_plan(p), p->tw, F, L, fbar, 0});
   CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: fft");
   return CMM_OK;
 }
 
 extern "C" int cmm_sht_stream_operators_adjoint(const cmm_sht_plan* p, const double* ops_bar, int apply_dinv,
                                                 double* w_bar, double* scratch, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_sht_stream_operators_adjoint: no batched form");
   if (!p || !ops_bar || !w_bar || !scratch) return fail(CMM_E_ARG, "cmm_sht_stream_operators_adjoint: bad arguments");
   const int64_t total = (int64_t)p->L * (p->L + 1) / 2;
   hipStream_t st = as_stream(stream);
@@ -2595,6 +2799,7 @@ extern "C" int cmm_sht_stream_operators_adjoint(const cmm_sht_plan* p, const dou
 
 extern "C" int cmm_sht_scalar_operators_adjoint(const cmm_sht_plan* p, const double* ops_bar, double* w_bar,
                                                 void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_sht_scalar_operators_adjoint: no batched form");
   if (!p || !ops_bar || !w_bar) return fail(CMM_E_ARG, "cmm_sht_scalar_operators_adjoint: bad arguments");
   const int64_t total = (int64_t)p->L * (p->L + 1) / 2;
   hipLaunchKernelGGL(k_scalar_ops_adj, dim3(grid_for(total, 128)), dim3(128), 0, as_stream(stream), p->L,

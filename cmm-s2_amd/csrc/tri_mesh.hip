@@ -16,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 
+#include "batch.h"
 #include "common.h"
 #include "scatter.h"
 
@@ -156,6 +157,7 @@ void raster_tri(d3 a, d3 b, d3 c, int32_t tri, int R, std::vector<std::pair<int3
 
 extern "C" int cmm_tri_mesh_create(const double* vertices, int64_t nv, const int64_t* simplices,
                                    int64_t ntri, int cells_per_face, cmm_tri_mesh** out) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_tri_mesh_create: no batched form");
   if (!vertices || !simplices || !out || nv < 3 || ntri < 1)
     return fail(CMM_E_ARG, "cmm_tri_mesh_create: bad arguments");
   if (ntri > (int64_t)1 << 30 || nv > (int64_t)1 << 30)
@@ -485,10 +487,34 @@ __device__ inline int64_t grid_point(int64_t t, int64_t nring, int64_t ncol, int
   return (r < nring && c < ncol) ? r * ncol + c : -1;
 }
 
-__global__ __launch_bounds__(256) void k_tri_compose(cmm_tri_mesh M, LevelPtrs lv, int depth, const double* __restrict__ q,
-                              int64_t n, int64_t sp, int64_t sc, double* __restrict__ out, int64_t osp,
-                              int64_t osc, double* __restrict__ inter, int64_t nring, int64_t ncol, int tr) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+struct KTriCompose {
+  struct Args {
+    cmm_tri_mesh M;
+    LevelPtrs lv;
+    int depth;
+    const double* q;
+    int64_t n, sp, sc;
+    double* out;
+    int64_t osp, osc;
+    double* inter;
+    int64_t nring, ncol;
+    int tr;
+  };
+  static constexpr int kThreads = 256;
+  static constexpr bool kArgsInPlace = true;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+__device__ __forceinline__ void KTriCompose::run(const Args& a, int bid) {
+  const cmm_tri_mesh& M = a.M;
+  const LevelPtrs& lv = a.lv;
+  const int depth = a.depth;
+  const double* __restrict__ q = a.q;
+  const int64_t n = a.n, sp = a.sp, sc = a.sc, osp = a.osp, osc = a.osc, nring = a.nring, ncol = a.ncol;
+  double* __restrict__ out = a.out;
+  double* __restrict__ inter = a.inter;
+  const int tr = a.tr;
+  int64_t i = (int64_t)bid * blockDim.x + threadIdx.x;
   if (ncol > 0) {
     i = grid_point(i, nring, ncol, tr);
     if (i < 0) return;
@@ -511,6 +537,8 @@ __global__ __launch_bounds__(256) void k_tri_compose(cmm_tri_mesh M, LevelPtrs l
   }
   store_pt(out, i, osp, osc, x);
 }
+
+__global__ __launch_bounds__(256) void k_tri_compose(KTriCompose::Args a) { KTriCompose::run(a, blockIdx.x); }
 
 // map Jacobian (ncomp 3, out (3,3,n) [c][d]) or tangential gradient of a
 // scalar spline (ncomp 1, out (3,n)); NaN where the point is not located
@@ -583,10 +611,25 @@ __global__ __launch_bounds__(256) void k_tri_compose_grad(cmm_tri_mesh M, LevelP
 }
 
 template <int NC>
-__global__ __launch_bounds__(256) void k_tri_assemble(cmm_tri_mesh M, const double* __restrict__ vals,
-                                                      double* __restrict__ coeffs, double* __restrict__ packed) {
+struct KTriAssemble {
+  struct Args {
+    cmm_tri_mesh M;
+    const double* vals;
+    double* coeffs;
+    double* packed;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+template <int NC>
+__device__ __forceinline__ void KTriAssemble<NC>::run(const Args& a, int bid) {
+  const cmm_tri_mesh& M = a.M;
+  const double* __restrict__ vals = a.vals;
+  double* __restrict__ coeffs = a.coeffs;
+  double* __restrict__ packed = a.packed;
   __shared__ __align__(16) double lds[64 * 6 * NC * 6];
-  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t t0 = (int64_t)bid * blockDim.x;
   const int64_t t = t0 + threadIdx.x;
   double cf[NC][19];
   if (t < M.ntri) {
@@ -619,12 +662,39 @@ __global__ __launch_bounds__(256) void k_tri_assemble(cmm_tri_mesh M, const doub
   if (packed) store_packed_block<NC>(cf, t0, M.ntri, packed, lds);
 }
 
-__global__ __launch_bounds__(128) void k_tri_stencil_jets(cmm_tri_mesh M, const double* __restrict__ cur,
-                                   const double* __restrict__ yv, int64_t yrow,
-                                   const double* __restrict__ sf, int64_t srow,
-                                   const double* __restrict__ tanv, double eps,
-                                   double* __restrict__ jets, double* __restrict__ st) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+template <int NC>
+__global__ __launch_bounds__(256) void k_tri_assemble(typename KTriAssemble<NC>::Args a) {
+  KTriAssemble<NC>::run(a, blockIdx.x);
+}
+
+struct KTriStencilJets {
+  struct Args {
+    cmm_tri_mesh M;
+    const double* cur;
+    const double* yv;
+    int64_t yrow;
+    const double* sf;
+    int64_t srow;
+    const double* tanv;
+    double eps;
+    double* jets;
+    double* st;
+  };
+  static constexpr int kThreads = 128;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+__device__ __forceinline__ void KTriStencilJets::run(const Args& a, int bid) {
+  const cmm_tri_mesh& M = a.M;
+  const double* __restrict__ cur = a.cur;
+  const double* __restrict__ yv = a.yv;
+  const double* __restrict__ sf = a.sf;
+  const double* __restrict__ tanv = a.tanv;
+  double* __restrict__ jets = a.jets;
+  double* __restrict__ st = a.st;
+  const int64_t yrow = a.yrow, srow = a.srow;
+  const double eps = a.eps;
+  int64_t v = (int64_t)bid * blockDim.x + threadIdx.x;
   const int64_t Nv = M.nv;
   if (v >= Nv) return;
   d3 S[4];
@@ -671,6 +741,8 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets(cmm_tri_mesh M, const 
   for (int c = 0; c < 3; ++c)
     for (int d = 0; d < 3; ++d) jets[(d * 4 + 1 + c) * Nv + v] = dc1[c] * A1[d] + dc2[c] * A2[d];
 }
+
+__global__ __launch_bounds__(128) void k_tri_stencil_jets(KTriStencilJets::Args a) { KTriStencilJets::run(a, blockIdx.x); }
 
 // ---------------------------------------------------------------------------
 // adjoints (reverse mode).  Coefficient and jet adjoints accumulate with
@@ -1016,6 +1088,7 @@ __global__ __launch_bounds__(128) void k_tri_stencil_jets_adj(cmm_tri_mesh M, co
 // ---------------------------------------------------------------------------
 extern "C" int cmm_tri_locate(const cmm_tri_mesh* m, const double* q, int64_t n, int64_t sp,
                               int64_t sc, int32_t* tri, double* bcc, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_tri_locate: no batched form");
   if (!m || (!q && n > 0) || (!tri && n > 0) || n < 0) return fail(CMM_E_ARG, "cmm_tri_locate: bad arguments");
   if (n == 0) return CMM_OK;
   hipLaunchKernelGGL(k_tri_locate, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, q, n,
@@ -1028,19 +1101,18 @@ extern "C" int cmm_tri_assemble(const cmm_tri_mesh* m, const double* vals, int n
                                 double* packed, void* stream) {
   if (!m || !vals || (ncomp != 1 && ncomp != 3)) return fail(CMM_E_ARG, "cmm_tri_assemble: bad arguments");
   if (!coeffs && !packed) return CMM_OK;
+  const unsigned blocks = (unsigned)grid_for(m->ntri, 256);
   if (ncomp == 3)
-    hipLaunchKernelGGL(k_tri_assemble<3>, dim3(grid_for(m->ntri, 256)), dim3(256), 0, as_stream(stream), *m,
-                       vals, coeffs, packed);
-  else
-    hipLaunchKernelGGL(k_tri_assemble<1>, dim3(grid_for(m->ntri, 256)), dim3(256), 0, as_stream(stream), *m,
-                       vals, coeffs, packed);
-  CMM_LAUNCH_CHECK("cmm_tri_assemble");
-  return CMM_OK;
+    return launch<KTriAssemble<3>>(k_tri_assemble<3>, blocks, 256, 0, as_stream(stream),
+                                   KTriAssemble<3>::Args{*m, vals, coeffs, packed}, "cmm_tri_assemble");
+  return launch<KTriAssemble<1>>(k_tri_assemble<1>, blocks, 256, 0, as_stream(stream),
+                                 KTriAssemble<1>::Args{*m, vals, coeffs, packed}, "cmm_tri_assemble");
 }
 
 extern "C" int cmm_tri_eval(const cmm_tri_mesh* m, const double* packed, int ncomp, int normalize,
                             const double* q, int64_t n, int64_t sp, int64_t sc, double* out,
                             int64_t osp, int64_t osc, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_tri_eval: no batched form");
   if (!m || !packed || (ncomp != 1 && ncomp != 3) || n < 0 || (n > 0 && (!q || !out)))
     return fail(CMM_E_ARG, "cmm_tri_eval: bad arguments");
   if (n == 0) return CMM_OK;
@@ -1062,10 +1134,9 @@ extern "C" int cmm_tri_compose(const cmm_tri_mesh* m, const double* const* level
     lv.p[i] = levels[i];
   }
   if (n == 0) return CMM_OK;
-  hipLaunchKernelGGL(k_tri_compose, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, lv,
-                     depth, q, n, sp, sc, out, osp, osc, inter, (int64_t)0, (int64_t)0, 1);
-  CMM_LAUNCH_CHECK("cmm_tri_compose");
-  return CMM_OK;
+  return launch<KTriCompose>(k_tri_compose, (unsigned)grid_for(n, 256), 256, 0, as_stream(stream),
+                             KTriCompose::Args{*m, lv, depth, q, n, sp, sc, out, osp, osc, inter, 0, 0, 1},
+                             "cmm_tri_compose");
 }
 
 extern "C" int cmm_tri_compose_grid(const cmm_tri_mesh* m, const double* const* levels, int depth,
@@ -1086,14 +1157,15 @@ extern "C" int cmm_tri_compose_grid(const cmm_tri_mesh* m, const double* const* 
   if (n == 0) return CMM_OK;
   const int tc = 64 / tile_rows;
   const int64_t threads = (nring + tile_rows - 1) / tile_rows * ((ncol + tc - 1) / tc) * 64;
-  hipLaunchKernelGGL(k_tri_compose, dim3(grid_for(threads, 256)), dim3(256), 0, as_stream(stream), *m, lv, depth, q,
-                     n, q_stride_pt, q_stride_comp, out, osp, osc, inter, nring, ncol, tile_rows);
-  CMM_LAUNCH_CHECK("cmm_tri_compose_grid");
-  return CMM_OK;
+  return launch<KTriCompose>(k_tri_compose, (unsigned)grid_for(threads, 256), 256, 0, as_stream(stream),
+                             KTriCompose::Args{*m, lv, depth, q, n, q_stride_pt, q_stride_comp, out, osp, osc,
+                                               inter, nring, ncol, tile_rows},
+                             "cmm_tri_compose_grid");
 }
 
 extern "C" int cmm_tri_eval_grad(const cmm_tri_mesh* m, const double* packed, int ncomp, const double* q,
                                  int64_t n, int64_t sp, int64_t sc, double* out, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_tri_eval_grad: no batched form");
   if (!m || !packed || (ncomp != 1 && ncomp != 3) || n < 0 || (n > 0 && (!q || !out)))
     return fail(CMM_E_ARG, "cmm_tri_eval_grad: bad arguments");
   if (n == 0) return CMM_OK;
@@ -1106,6 +1178,7 @@ extern "C" int cmm_tri_eval_grad(const cmm_tri_mesh* m, const double* packed, in
 extern "C" int cmm_tri_compose_grad(const cmm_tri_mesh* m, const double* const* levels, int depth,
                                     const double* q, int64_t n, int64_t sp, int64_t sc, double* out,
                                     int64_t osp, int64_t osc, double* jac, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_tri_compose_grad: no batched form");
   if (!m || depth < 0 || depth > kMaxDepth || (depth > 0 && !levels) || n < 0 ||
       (n > 0 && (!q || !out || !jac)))
     return fail(CMM_E_ARG, "cmm_tri_compose_grad: bad arguments");
@@ -1126,10 +1199,9 @@ extern "C" int cmm_tri_stencil_jets(const cmm_tri_mesh* m, const double* cur, co
                                     double eps, double* jets, double* st, void* stream) {
   if (!m || !sf || !tanv || !jets || !(eps > 0) || (cur && !yv))
     return fail(CMM_E_ARG, "cmm_tri_stencil_jets: bad arguments");
-  hipLaunchKernelGGL(k_tri_stencil_jets, dim3(grid_for(m->nv, 128)), dim3(128), 0, as_stream(stream), *m,
-                     cur, yv, yrow, sf, srow, tanv, eps, jets, st);
-  CMM_LAUNCH_CHECK("cmm_tri_stencil_jets");
-  return CMM_OK;
+  return launch<KTriStencilJets>(k_tri_stencil_jets, (unsigned)grid_for(m->nv, 128), 128, 0, as_stream(stream),
+                                 KTriStencilJets::Args{*m, cur, yv, yrow, sf, srow, tanv, eps, jets, st},
+                                 "cmm_tri_stencil_jets");
 }
 
 // ---- adjoints -------------------------------------------------------------
@@ -1137,6 +1209,7 @@ extern "C" int cmm_tri_compose_adjoint(const cmm_tri_mesh* m, const double* cons
                                        double* const* level_bars, int depth, const double* q, int64_t n,
                                        int64_t sp, int64_t sc, const double* out_bar, double* q_bar, void* work,
                                        int64_t work_bytes, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_tri_compose_adjoint: no batched form");
   if (!m || depth < 1 || n < 0 || !levels || !level_bars || (n > 0 && (!q || !out_bar)))
     return fail(CMM_E_ARG, "cmm_tri_compose_adjoint: bad arguments");
   if (depth > kAdjMaxDepth) return fail(CMM_E_RANGE, "cmm_tri_compose_adjoint: depth > 8");
@@ -1176,6 +1249,7 @@ extern "C" int cmm_tri_compose_adjoint(const cmm_tri_mesh* m, const double* cons
 
 extern "C" int cmm_tri_assemble_adjoint(const cmm_tri_mesh* m, const double* packed_bar, double* vals_bar,
                                         void* work, int64_t work_bytes, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_tri_assemble_adjoint: no batched form");
   if (!m || !packed_bar || !vals_bar) return fail(CMM_E_ARG, "cmm_tri_assemble_adjoint: bad arguments");
   hipStream_t st = as_stream(stream);
   if (!work) {
@@ -1196,6 +1270,7 @@ extern "C" int cmm_tri_stencil_jets_adjoint(const cmm_tri_mesh* m, const double*
                                             int64_t yrow, const double* sf, int64_t srow, const double* tanv,
                                             double eps, const double* jets_bar, double* sf_bar, int64_t sbrow,
                                             double* cur_bar, void* work, int64_t work_bytes, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_tri_stencil_jets_adjoint: no batched form");
   if (!m || !sf || !tanv || !jets_bar || !sf_bar || !(eps > 0) || (cur && !yv) || srow < 4 * m->nv ||
       sbrow < 4 * m->nv)
     return fail(CMM_E_ARG, "cmm_tri_stencil_jets_adjoint: bad arguments");

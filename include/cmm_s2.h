@@ -36,6 +36,7 @@ extern "C" {
 #define CMM_E_ARG (-1)      /* bad argument / shape */
 #define CMM_E_HIP (-2)      /* HIP runtime error (launch, alloc, copy) */
 #define CMM_E_RANGE (-3)    /* size outside the supported range */
+#define CMM_E_STATE (-4)    /* call out of order (batch recording) */
 
 #define CMM_ABI_VERSION 1
 
@@ -469,6 +470,35 @@ int cmm_sample_vorticity_adjoint(const cmm_mwss_mesh* wmesh, const double* wpack
                                  const double* wbar, double rot_rate, double* x_bar,
                                  int64_t xb_stride_pt, int64_t xb_stride_comp, double* wpacked_bar,
                                  void* work, int64_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Member-batched launches (ensembles).  No counterpart in the reference,    */
+/* which runs one simulation per process; this is the one-launch-per-kernel  */
+/* step of an ensemble of simulations with different bandlimits and meshes. */
+/* Between cmm_batch_begin and cmm_batch_launch the calling host thread      */
+/* records the launches of the step entry points (advect, stencil jets, map  */
+/* assembly, composition, vorticity sampling, the transforms, spectral       */
+/* operators, velocity jets and assembly) instead of issuing them, member by */
+/* member (cmm_batch_member(b, 0), the member's calls, cmm_batch_member(b,   */
+/* 1), ...); cmm_batch_launch then issues one grid per recorded position     */
+/* for all members on `stream` (the j-th launches of every member must be    */
+/* the same kernel with the same block size) and reports the grids issued.   */
+/* Each block runs its member's code and arguments, so every member gets the */
+/* bits of its own run.  The batch owns `bytes` of device memory for the     */
+/* per-member arguments of one step and two pinned staging buffers.  An      */
+/* entry point without a batched form returns CMM_E_STATE while recording.   */
+/* ------------------------------------------------------------------------ */
+typedef struct cmm_batch cmm_batch;
+int cmm_batch_create(int64_t bytes, cmm_batch** out);
+int cmm_batch_destroy(cmm_batch* batch);
+int cmm_batch_begin(cmm_batch* batch);
+int cmm_batch_member(cmm_batch* batch, int member);
+int cmm_batch_cancel(cmm_batch* batch);
+int cmm_batch_launch(cmm_batch* batch, void* stream, int64_t* launches);
+/* dst[i] = src[i], i < n, on `stream` (batchable): the remap's copy of the
+ * current map's jets into the submap stack (spherical_spline.py:471-473
+ * assigns them). */
+int cmm_copy_f64(double* dst, const double* src, int64_t n, void* stream);
 
 #ifdef __cplusplus
 }

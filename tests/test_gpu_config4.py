@@ -48,3 +48,25 @@ def test_64_member_ensemble_matches_solo_runs():
         assert torch.equal(m.submaps(q), solo.submaps(q)), k
         del solo
     ens.close()
+
+
+def test_64_member_batched_steps_match_solo_runs():
+    """The same 64 members with every step issued as one launch per kernel
+    for all of them: members keep the bits of their solo runs."""
+    from cmm_s2.core.ensemble import Ensemble, random_members
+    print("config 4: 64-member ensemble, batched steps", flush=True)
+    builders, params = random_members(64)
+    ens = Ensemble(builders)
+    ens.batched_run()
+    assert all(m.count == 50 and m.submaps.Nc == 5 for m in ens.members)
+    assert ens.batched_launches <= 16  # one partition: one launch per kernel
+    for k in (0, 17, 63):
+        m = ens.members[k]
+        solo = builders[k]()
+        solo.run()
+        torch.cuda.synchronize()
+        nc = m.submaps.Nc
+        assert torch.equal(m.submaps.packed[:nc], solo.submaps.packed[:nc]), k
+        assert torch.equal(m.X, solo.X), k
+        del solo
+    ens.close()
