@@ -24,7 +24,8 @@ grid whose blocks map to the members through a prefix table of their block
 counts (cmm_batch_launch), on the ensemble's batch stream.  Each block runs
 its member's code and arguments, so members keep the bits of their own runs.
 Members whose steps differ in kind (a remap, the identity map after one, the
-Legendre arithmetic) are batched in separate groups.
+Legendre arithmetic, a bandlimit above 512 whose kernels run other block
+sizes) are batched in separate groups.
 """
 import ctypes
 import time
@@ -164,7 +165,10 @@ class Ensemble(object):
             for i in part:
                 m = self.members[i]
                 if m.count < m.Nt:
-                    key = ((m.count + 1) in m.ns, m.identity, m.legendre, bool(m.U.sparse))
+                    # bandlimits up to 512 share every kernel's block size;
+                    # larger ones batch only with their own bandlimit
+                    shape = 0 if m.L <= 512 else m.L
+                    key = ((m.count + 1) in m.ns, m.identity, m.legendre, bool(m.U.sparse), shape)
                     groups.setdefault(key, []).append(i)
             for idx in groups.values():
                 N.call("cmm_batch_begin", b)

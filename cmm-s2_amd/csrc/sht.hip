@@ -220,8 +220,9 @@ extern "C" int cmm_sht_plan_create(int L, const double* cos_t, const double* sin
   legendre_diag(L, sg, Pg, Sg);
   // resampling matrices K_par[j][t] = D(th_j - th_t) + (-1)^par D(th_j + th_t)
   // (second term for 0 < t < L only), stored transposed K[par][t][j] with
-  // the rows t > L up to a multiple of 4 set to zero
-  const int tpad = (L + 1 + 3) / 4 * 4;
+  // zero rows t > L up to a multiple of 4, at least one (the resampling
+  // kernel's loads past the last step are clamped onto the last row)
+  const int tpad = ((L + 1) / 4 + 1) * 4;
   p->tpad = tpad;
   std::vector<double> K((size_t)2 * L * tpad, 0.0);
   for (int j = 0; j < L; ++j)
@@ -1751,6 +1752,7 @@ static int i8_inverse(const cmm_sht_plan* p, const double* flm, int nf, int f0, 
 // ---------------------------------------------------------------------------
 constexpr int kResSteps = 8;
 
+template <bool SPLIT>
 struct KResample {
   struct Args {
     int L, nmg, tpad;
@@ -1759,17 +1761,29 @@ struct KResample {
     double* G;
     int gx, gy;  // grid extents: (orders x parities, node tiles, fields) flattened
   };
-  static constexpr int kThreads = 64;
+  static constexpr int kThreads = SPLIT ? 256 : 64;
   static __device__ __forceinline__ void run(const Args& a, int bid);
 };
 
-__device__ __forceinline__ void KResample::run(const Args& a, int bid) {
+// SPLIT: the K dimension (rings) of every output tile is split into
+// kResWaves contiguous ranges, one per wave of the block; the partial tiles
+// meet in LDS and wave 0 adds them in wave order (deterministic).  Small
+// bandlimits have too few tiles to give every SIMD more than one wave
+// (L = 512: 1,024 tiles), so they split; from L = 1024 on one wave runs a
+// whole tile.
+constexpr int kResWaves = 4;
+__host__ __device__ constexpr bool res_split(int L) { return L <= 512; }
+
+template <bool SPLIT>
+__device__ __forceinline__ void KResample<SPLIT>::run(const Args& a, int bid) {
   const int L = a.L, nmg = a.nmg, tpad = a.tpad;
   const double* __restrict__ Kt = a.Kt;
   const double* __restrict__ F = a.F;
   double* __restrict__ G = a.G;
+  __shared__ dx4 part[SPLIT ? kResWaves - 1 : 1][2][64];
   const int bx = bid % a.gx, by = (bid / a.gx) % a.gy, bz = bid / (a.gx * a.gy);
-  const int lane = threadIdx.x;
+  constexpr int W = SPLIT ? kResWaves : 1;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int bk = lane >> 4, bj = lane & 15;
   const int fi = bz;
   const int j0 = by * 16;
@@ -1785,38 +1799,62 @@ __device__ __forceinline__ void KResample::run(const Args& a, int bid) {
   int mcc[2];
 #pragma unroll
   for (int nt = 0; nt < 2; ++nt) mcc[nt] = min(mcol[nt], L - 1);
-  // unconditional loads from clamped addresses: rows t > L of K are zero
-  // (padding to tpad rows), rows j >= L and columns m >= L are never stored
+  // this wave's steps of 4 rings: [sb, se)
+  const int nsteps = tpad / 4;
+  const int per = (nsteps + W - 1) / W;
+  const int sb = w * per, se = min(nsteps, sb + per);
+  // unconditional loads from clamped addresses (rows t > L of K are zero
+  // padding to tpad rows, so steps past the last read zeros; rows j >= L and
+  // columns m >= L are never stored); split: steps outside the wave's range
+  // take a zero A operand
   auto load = [&](int s0, double (&a)[kResSteps], double (&b)[kResSteps][2]) {
 #pragma unroll
     for (int u = 0; u < kResSteps; ++u) {
       const int tw = 4 * (s0 + u) + bk;
-      a[u] = K[(int64_t)min(tw, tpad - 1) * L + jac];
+      const double av = K[(int64_t)min(tw, tpad - 1) * L + jac];
+      a[u] = (!SPLIT || s0 + u < se) ? av : 0.0;
       const int t = min(tw, L);
 #pragma unroll
       for (int nt = 0; nt < 2; ++nt) b[u][nt] = Ff[((int64_t)t * L + mcc[nt]) * 2 + (bj & 1)];
     }
   };
   dx4 acc[2] = {dx4{0.0, 0.0, 0.0, 0.0}, dx4{0.0, 0.0, 0.0, 0.0}};
-  const int nsteps = tpad / 4;
   auto mma = [&](const double (&a)[kResSteps], const double (&b)[kResSteps][2]) {
 #pragma unroll
     for (int u = 0; u < kResSteps; ++u)
 #pragma unroll
       for (int nt = 0; nt < 2; ++nt) acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][nt], acc[nt], 0, 0, 0);
   };
-  double aA[kResSteps], bA[kResSteps][2], aB[kResSteps], bB[kResSteps][2];
-  load(0, aA, bA);
-  for (int s0 = 0;;) {
-    load(s0 + kResSteps, aB, bB);
-    mma(aA, bA);
-    s0 += kResSteps;
-    if (s0 >= nsteps) break;
-    load(s0 + kResSteps, aA, bA);
-    mma(aB, bB);
-    s0 += kResSteps;
-    if (s0 >= nsteps) break;
+  if (sb < se) {
+    double aA[kResSteps], bA[kResSteps][2], aB[kResSteps], bB[kResSteps][2];
+    load(sb, aA, bA);
+    for (int s0 = sb;;) {
+      load(s0 + kResSteps, aB, bB);
+      mma(aA, bA);
+      s0 += kResSteps;
+      if (s0 >= se) break;
+      load(s0 + kResSteps, aA, bA);
+      mma(aB, bB);
+      s0 += kResSteps;
+      if (s0 >= se) break;
+    }
   }
+  if (SPLIT) {
+    if (w > 0) {
+      part[w - 1][0][lane] = acc[0];
+      part[w - 1][1][lane] = acc[1];
+    }
+    __syncthreads();
+    if (w > 0) return;
+  }
+#pragma unroll
+  for (int ww = 0; ww < W - 1; ++ww)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+      const dx4 v = part[ww][nt][lane];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[nt][e] += v[e];
+    }
   double* Gf = G + (int64_t)fi * L * L * 2;
 #pragma unroll
   for (int nt = 0; nt < 2; ++nt) {
@@ -1829,7 +1867,10 @@ __device__ __forceinline__ void KResample::run(const Args& a, int bid) {
   }
 }
 
-__global__ __launch_bounds__(64) void k_resample(KResample::Args a) { KResample::run(a, blockIdx.x); }
+template <bool SPLIT>
+__global__ __launch_bounds__(KResample<SPLIT>::kThreads) void k_resample(typename KResample<SPLIT>::Args a) {
+  KResample<SPLIT>::run(a, blockIdx.x);
+}
 
 // ---------------------------------------------------------------------------
 // forward Legendre analysis: flm[lm][f] = sum_j w_j Pbar_lm(x_j) G[f][j][m].
@@ -1848,7 +1889,6 @@ __global__ __launch_bounds__(64) void k_resample(KResample::Args a) { KResample:
 // ---------------------------------------------------------------------------
 constexpr int kFwdSeg = 64;
 constexpr int kFwdChunk = 16;
-constexpr int kFwdChunkA = 8;  // degrees per LDS chunk of the analysis kernel
 constexpr int kFwdNpt = 2;
 
 // state (P_{l-2}, P_{l-1}, s) before the chunk at every segment start
@@ -2014,6 +2054,232 @@ __device__ __forceinline__ void KLegForward<C>::run(const Args& a, int bid) {
 template <int C>
 __global__ __launch_bounds__(512) void k_leg_forward(typename KLegForward<C>::Args a) {
   KLegForward<C>::run(a, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Forward analysis with wave-level sums: a block of W waves per (unit,
+// field); lane t of wave w owns the northern nodes j = 64 (NP w + q) + t,
+// q < NP.  Per chunk of 16 degrees (the restart table's chunk) each lane
+// adds its nodes' products into 32 partial sums (degree, re / im), and the
+// wave sums them over its 64 lanes by a butterfly that halves the values a
+// lane holds at every step (permlane32 / permlane16 swaps, then row
+// rotations, swizzles and quad permutations): no LDS tile, no barrier, 63
+// exchanges for 32 sums.  Lane t ends with sum number t >> 1 (degree t >> 2,
+// re / im by bit 1).  With W > 1 waves the sums of all of a unit's chunks
+// meet in LDS once, after the unit.  Fixed summation order: deterministic.
+// ---------------------------------------------------------------------------
+constexpr int kFwdWDeg = 16;  // degrees per chunk (== kFwdChunk, the restart schedule)
+#ifndef CMM_FWD_NP
+#define CMM_FWD_NP 4
+#endif
+#ifndef CMM_FWD_MINW
+#define CMM_FWD_MINW 1
+#endif
+constexpr int kFwdWNp = CMM_FWD_NP;          // nodes per lane
+constexpr int kFwdWMinWaves = CMM_FWD_MINW;  // waves per block at least
+static_assert(kFwdWDeg == kFwdChunk, "the restart table's chunk");
+
+__device__ __forceinline__ double dpp_ror8(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), 0x128, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x128, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_quad(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double swz_xor4(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffll), 0x1F | (4 << 10));
+  const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x1F | (4 << 10));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// a + b after exchanging the upper 32 lanes of a with the lower 32 lanes of
+// b: lanes 0-31 get a's sum over lanes t and t + 32, lanes 32-63 b's
+__device__ __forceinline__ double swap32_add(double a, double b) {
+  const long long A = __double_as_longlong(a), B = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)(A & 0xffffffffll), (unsigned)(B & 0xffffffffll),
+                                                   false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(A >> 32), (unsigned)(B >> 32), false, false);
+  const double na = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+  const double nb = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+  return na + nb;
+}
+// the same between the odd 16-lane rows of a and the even rows of b: even
+// rows get a's sum over lanes t and t ^ 16, odd rows b's
+__device__ __forceinline__ double swap16_add(double a, double b) {
+  const long long A = __double_as_longlong(a), B = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)(A & 0xffffffffll), (unsigned)(B & 0xffffffffll),
+                                                   false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(A >> 32), (unsigned)(B >> 32), false, false);
+  const double na = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+  const double nb = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+  return na + nb;
+}
+
+// r[0..31] of every lane -> lane t holds sum over the wave of r[t >> 1]
+__device__ __forceinline__ double wave_sum32(double (&r)[32], int lane) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) r[k] = swap32_add(r[k], r[k + 16]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = swap16_add(r[k], r[k + 8]);
+  {
+    const bool h = (lane >> 3) & 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double keep = h ? r[k + 4] : r[k], send = h ? r[k] : r[k + 4];
+      r[k] = keep + dpp_ror8(send);
+    }
+  }
+  {
+    const bool h = (lane >> 2) & 1;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const double keep = h ? r[k + 2] : r[k], send = h ? r[k] : r[k + 2];
+      r[k] = keep + swz_xor4(send);
+    }
+  }
+  {
+    const bool h = (lane >> 1) & 1;
+    const double keep = h ? r[1] : r[0], send = h ? r[0] : r[1];
+    r[0] = keep + dpp_quad<0x4E>(send);  // lane ^ 2
+  }
+  return r[0] + dpp_quad<0xB1>(r[0]);  // lane ^ 1
+}
+
+template <int NP>
+struct KLegForwardW {
+  using Args = typename KLegForward<kFwdChunk>::Args;
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a, int bid);
+};
+
+template <int NP>
+__device__ __forceinline__ void KLegForwardW<NP>::run(const Args& a, int bid) {
+  const int L = a.L, nn = a.nn, nf = a.nf, skip_poles = a.skip_poles;
+  const double* __restrict__ xg = a.xg;
+  const double* __restrict__ wg = a.wg;
+  const double2* __restrict__ st_p = a.st_p;
+  const int* __restrict__ st_s = a.st_s;
+  const double2* __restrict__ rec = a.rec;
+  const double2* __restrict__ G = a.G;
+  double* __restrict__ flm = reinterpret_cast<double*>(a.flm);
+  __shared__ double part[3][kFwdSeg / kFwdWDeg][32];
+  const int fi = bid / a.gx;
+  const int u = bid % a.gx;
+  const int2 un = a.units[u];
+  const int m = un.x, ls = un.y;
+  const int le = min(L, ls + kFwdSeg);
+  const int nh = (nn + 1) / 2;
+  const double cm = a.msc ? a.msc[m] : 1.0;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, W = blockDim.x >> 6;
+  double x[NP], pp[NP], pc[NP];
+  int s[NP];
+  double2 S[NP], D[NP];
+  bool act[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int jw = 64 * (NP * w + q) + lane;
+    act[q] = jw < nh;
+    const int j = act[q] ? jw : 0;
+    const int jm = nn - 1 - j;
+    x[q] = xg[j];
+    const double2 ps = st_p[(int64_t)u * nh + j];
+    pp[q] = ps.x;
+    pc[q] = ps.y;
+    s[q] = st_s[(int64_t)u * nh + j];
+    const double2 ga = G[((int64_t)fi * nn + j) * L + m];
+    const double wj = (act[q] && !(skip_poles && m > 0 && j == 0)) ? cm * wg[j] : 0.0;
+    const double2 gb = G[((int64_t)fi * nn + jm) * L + m];
+    if (jm != j) {
+      S[q] = double2{wj * (ga.x + gb.x), wj * (ga.y + gb.y)};
+      D[q] = double2{wj * (ga.x - gb.x), wj * (ga.y - gb.y)};
+    } else {
+      S[q] = double2{wj * ga.x, wj * ga.y};
+      D[q] = S[q];
+    }
+  }
+  const int64_t lm0 = (int64_t)m * L - (int64_t)m * (m - 1) / 2;
+  // the recurrence pairs through the constant address space: wave-uniform
+  // scalar loads (the table is read-only here)
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) double2 CD2;
+  const CD2* rc = (const CD2*)(rec + lm0 - m);
+#else
+  const double2* rc = rec + lm0 - m;  // host pass: the body is not compiled for the host
+#endif
+  double tk[kFwdSeg / kFwdWDeg];  // this wave's sums of the unit's chunks
+#pragma unroll
+  for (int c = 0; c < kFwdSeg / kFwdWDeg; ++c) {
+    const int lc = ls + kFwdWDeg * c;
+    if (lc >= le) break;
+    // values below the scale floor (s < 0) count as zero: their weights are
+    // zeroed for the chunk instead of the values at every degree
+    double2 Sl[NP], Dl[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      leg_rescale(pp[q], pc[q], s[q]);
+      const bool live = (s[q] == 0) & act[q];
+      Sl[q] = live ? S[q] : double2{0.0, 0.0};
+      Dl[q] = live ? D[q] : double2{0.0, 0.0};
+    }
+    double r[32];
+#pragma unroll
+    for (int i = 0; i < kFwdWDeg; ++i) {
+      // lc - m is a multiple of the chunk, so l - m has the parity of i;
+      // only the first degree of an order (l == m) takes no step
+      const int l = lc + i;
+      const double2 ab = rc[min(l, L - 1)];
+      if (i > 0 || lc > m) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) leg_step(ab.x, ab.y, x[q], pp[q], pc[q]);
+      }
+      double vx = 0.0, vy = 0.0;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const double2 src = (i & 1) ? Dl[q] : Sl[q];
+        vx = fma(pc[q], src.x, vx);
+        vy = fma(pc[q], src.y, vy);
+      }
+      r[2 * i] = vx;
+      r[2 * i + 1] = vy;
+    }
+    tk[c] = wave_sum32(r, lane);
+  }
+  // waves > 0 hand their sums to wave 0 through LDS, once per unit
+  const int nc = (le - ls + kFwdWDeg - 1) / kFwdWDeg;
+  constexpr int NC = kFwdSeg / kFwdWDeg;
+  if (W > 1) {
+    if (w > 0 && !(lane & 1)) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < nc) part[w - 1][c][lane >> 1] = tk[c];
+    }
+    __syncthreads();
+    if (w > 0) return;
+    for (int ww = 0; ww < W - 1; ++ww) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < nc) tk[c] += part[ww][c][lane >> 1];
+    }
+  }
+  const int d = lane >> 2, comp = (lane >> 1) & 1;
+  if (!(lane & 1)) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int l = ls + kFwdWDeg * c + d;
+      if (c < nc && l < le) flm[2 * ((lm0 + (l - m)) * nf + fi) + comp] = tk[c];
+    }
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void k_leg_forward_w(typename KLegForwardW<NP>::Args a) {
+  KLegForwardW<NP>::run(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2542,28 +2808,28 @@ static int sht_forward(const cmm_sht_plan* p, const double* f, int nf, double* f
   if (rc) return rc;
   const int nmg = ((L + 1) / 2 + 15) / 16;  // groups of 16 orders of one parity (at most ceil(L / 2) orders)
   const int gy = (L + 15) / 16;
-  rc = launch<KResample>(k_resample, (unsigned)(2 * nmg * gy * nf), 64, 0, st,
-                         KResample::Args{L, nmg, p->tpad, p->K, reinterpret_cast<const double*>(F),
-                                         reinterpret_cast<double*>(G), 2 * nmg, gy},
-                         "cmm_sht_forward: resample");
+  const KResample<true>::Args ra{L, nmg, p->tpad, p->K, reinterpret_cast<const double*>(F),
+                                 reinterpret_cast<double*>(G), 2 * nmg, gy};
+  const unsigned rb = (unsigned)(2 * nmg * gy * nf);
+  rc = res_split(L) ? launch<KResample<true>>(k_resample<true>, rb, 64 * kResWaves, 0, st, ra,
+                                              "cmm_sht_forward: resample")
+                    : launch<KResample<false>>(k_resample<false>, rb, 64, 0, st,
+                                               KResample<false>::Args{ra.L, ra.nmg, ra.tpad, ra.Kt, ra.F, ra.G,
+                                                                      ra.gx, ra.gy},
+                                               "cmm_sht_forward: resample");
   if (rc) return rc;
   const int nh = (L + 1) / 2;
-  // at least 128 threads: bandlimits up to 512 share one block size (a
-  // batched launch has one), so their sums run in the same order in a batch
-  // as alone
-  const int T = std::max(128, ((nh + kFwdNpt - 1) / kFwdNpt + 63) / 64 * 64);
-  if (T > 512) return fail(CMM_E_RANGE, "cmm_sht_forward: L > 2048");
-  // chunks of 8 degrees: half the LDS tile of the 16-degree chunks the
-  // restart table is built on (values and scale states agree: rescaling is
-  // exact and the restart points are whole chunks of either), so more
-  // blocks stay resident on this latency-bound kernel
-  size_t lds2 = (size_t)kFwdChunkA * (T + 4) * sizeof(double2);
-  return launch<KLegForward<kFwdChunkA>>(k_leg_forward<kFwdChunkA>, (unsigned)(p->nunits * nf), T, lds2, st,
-                                         KLegForward<kFwdChunkA>::Args{L, L, nf, p->units, p->x_g, p->w_g, p->st_p,
-                                                                       p->st_s, p->rec, G,
-                                                                       reinterpret_cast<double2*>(flm), nullptr, 0,
-                                                                       p->nunits},
-                                         "cmm_sht_forward: legendre");
+  // waves of kFwdWNp x 64 nodes; bandlimits up to 512 share one block size
+  // (a batched launch has one), so their sums run in the same order in a
+  // batch as alone
+  const int W = std::max(kFwdWMinWaves, (nh + 64 * kFwdWNp - 1) / (64 * kFwdWNp));
+  if (W > 4) return fail(CMM_E_RANGE, "cmm_sht_forward: L > 2048");
+  return launch<KLegForwardW<kFwdWNp>>(k_leg_forward_w<kFwdWNp>, (unsigned)(p->nunits * nf), 64 * W, 0, st,
+                                       KLegForwardW<kFwdWNp>::Args{L, L, nf, p->units, p->x_g, p->w_g, p->st_p,
+                                                                   p->st_s, p->rec, G,
+                                                                   reinterpret_cast<double2*>(flm), nullptr, 0,
+                                                                   p->nunits},
+                                       "cmm_sht_forward: legendre");
 }
 
 extern "C" int cmm_sht_stream_operators(const cmm_sht_plan* p, const double* w, int apply_dinv,
