@@ -52,6 +52,9 @@ SIGNATURES = {
     "cmm_tri_eval_grad": (C.c_int, [vp, vp, C.c_int, vp, i64, i64, i64, vp, vp]),
     "cmm_tri_compose_grid": (C.c_int, [vp, P_vp, C.c_int, vp, i64, i64, C.c_int, i64, i64, vp, i64, i64, vp,
                                        vp]),
+    "cmm_tri_compose_grid_located": (C.c_int, [vp, P_vp, C.c_int, vp, i64, i64, C.c_int, i64, i64, vp, i64, i64,
+                                               vp, vp, vp, vp]),
+    "cmm_tri_split_locate": (C.c_int, [vp, vp, i64, i64, i64, vp, vp, vp]),
     "cmm_tri_compose_grad": (C.c_int, [vp, P_vp, C.c_int, vp, i64, i64, i64, vp, i64, i64, vp, vp]),
     "cmm_mwss_eval_grad": (C.c_int, [vp, vp, vp, i64, i64, i64, vp, vp]),
     "cmm_tri_stencil_jets": (C.c_int, [vp, vp, vp, i64, vp, i64, vp, dbl, vp, vp, vp]),
@@ -135,9 +138,16 @@ def check(rc, what=""):
         raise CmmError(f"{what or 'cmm'} failed ({rc}): {msg}")
 
 
+_fns = {}
+
+
 def call(name, *args):
-    rc = getattr(lib(), name)(*args)
-    check(rc, name)
+    f = _fns.get(name)
+    if f is None:
+        f = _fns[name] = getattr(lib(), name)
+    rc = f(*args)
+    if rc != 0:
+        check(rc, name)
     return rc
 
 

@@ -14,7 +14,17 @@ def device():
     return torch.device("cuda", torch.cuda.current_device())
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_cur_dev = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def stream():
+    """Handle of torch's current HIP stream on the current device (the
+    capture stream during graph capture).  Called once per launch, so it
+    takes torch's direct accessors when they exist (~10x cheaper than
+    torch.cuda.current_stream())."""
+    if _raw_stream is not None and _cur_dev is not None:
+        return _raw_stream(_cur_dev())
     return torch.cuda.current_stream().cuda_stream
 
 

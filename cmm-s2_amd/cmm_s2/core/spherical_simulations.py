@@ -112,6 +112,11 @@ class BarotropicCMM(object):
         self.sampler = VorticitySampler(vorticity, self.mesh_u, rot_rate)
         self.samples = self.mesh_u.sample_points()
         n = self.samples.shape[0]
+        # the sample grid's location in the map mesh, for the first level of
+        # every step's composition (the current map; mesh and grid are
+        # fixed): computed here, outside any capture or batch recording
+        from .spherical_spline import split_locate
+        self._loc0 = split_locate(mesh, self.samples)
         self.X = B.empty(n, 3)
         self.w = B.empty(1, L + 1, 2 * L)
         self.w_lm = B.empty(nlm(L), 1, 2)
@@ -173,7 +178,13 @@ class BarotropicCMM(object):
         from .spherical_spline import tune_grid_tiles
         grid = (self.L + 1, 2 * self.L)
         for depth in range(1, len(self.ns) + 2):
-            tune_grid_tiles(self.mesh, [self.curr.packed] * depth, self.samples, grid, out=self.X, reps=2)
+            tune_grid_tiles(self.mesh, [self.curr.packed] * depth, self.samples, grid, out=self.X, reps=2,
+                            located=self.grid_location())
+
+    def grid_location(self):
+        """Location of the sample grid in the map mesh (the evaluation's own
+        block indices and sub-triangle barycentrics, stored at construction)."""
+        return self._loc0
 
     def step(self):
         """One loop iteration (spherical_simulations.py:263-294).  Returns True
@@ -185,7 +196,8 @@ class BarotropicCMM(object):
         self.U.mark_every = self.count % self.RESEED == 0
         self.curr = evol.advect_project_sphere(self.curr, evol.RK4_proj, t, self.dt, self.U,
                                                identity=self.identity)
-        self.submaps.compose(self.samples, "N3", out=self.X, pre=[self.curr.packed], grid=(self.L + 1, 2 * self.L))
+        self.submaps.compose(self.samples, "N3", out=self.X, pre=[self.curr.packed], grid=(self.L + 1, 2 * self.L),
+                             located=self.grid_location())
         self._velocity_from(self.X, 2)
         self.U.ts = self.U.ts + self.dt
         self.count += 1

@@ -198,11 +198,12 @@ class composite_sphere_diffeomorphism(object):
             N_c = self.Nc
         return [self.packed[i] for i in reversed(range(N_c))]
 
-    def compose(self, q, layout="N3", out=None, pre=(), N_c=None, inter=None, grid=None):
+    def compose(self, q, layout="N3", out=None, pre=(), N_c=None, inter=None, grid=None, located=None):
         """Fused device composition: apply maps ``pre`` (packed buffers), then
-        submaps N_c-1 .. 0."""
+        submaps N_c-1 .. 0.  ``located``: the first level's location of the
+        grid points (split_locate(mesh, q)), used instead of a search."""
         lv = list(pre) + self.levels(N_c)
-        return compose_packed(self.mesh, lv, q, layout, out, inter, grid)
+        return compose_packed(self.mesh, lv, q, layout, out, inter, grid, located)
 
     def __call__(self, q_pts, N_c=None):
         """spherical_spline.py:365-398: q (N,3) -> (N,3)."""
@@ -239,7 +240,20 @@ class composite_sphere_diffeomorphism(object):
 _tiles = {}
 
 
-def tune_grid_tiles(mesh, levels, q, grid, out=None, reps=3, candidates=(1, 2, 4, 8, 16)):
+def split_locate(mesh, q):
+    """Location of the points q (N,3) for evaluation on ``mesh``: (block
+    int32 (N,) = 6 triangle + split sub-triangle, -1 if not located;
+    sub-triangle barycentrics (N,3)), as the evaluation computes them."""
+    q = B.to_dev(q)
+    n = q.shape[0]
+    sp, sc = B.point_strides(q, "N3")
+    blk = torch.empty(n, dtype=torch.int32, device=q.device)
+    cb = B.empty(n, 3)
+    N.call("cmm_tri_split_locate", mesh._h, q.data_ptr(), n, sp, sc, blk.data_ptr(), cb.data_ptr(), B.stream())
+    return blk, cb
+
+
+def tune_grid_tiles(mesh, levels, q, grid, out=None, reps=3, candidates=(1, 2, 4, 8, 16), located=None):
     """Time the grid composition with each tile shape (rows x 64/rows) on
     these inputs and keep the fastest for (mesh, grid shape, depth): a
     measured choice of the launch shape, results are identical for every
@@ -248,25 +262,26 @@ def tune_grid_tiles(mesh, levels, q, grid, out=None, reps=3, candidates=(1, 2, 4
     n = q.shape[0]
     if out is None:
         out = B.empty(n, 3)
-    key = (mesh._h, int(grid[0]), int(grid[1]), len(levels))
+    key = (mesh._h, int(grid[0]), int(grid[1]), len(levels), located is not None)
     times = {}
     for rows in candidates:
         _tiles[key] = rows
-        compose_packed(mesh, levels, q, "N3", out, grid=grid)
+        compose_packed(mesh, levels, q, "N3", out, grid=grid, located=located)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            compose_packed(mesh, levels, q, "N3", out, grid=grid)
+            compose_packed(mesh, levels, q, "N3", out, grid=grid, located=located)
         torch.cuda.synchronize()
         times[rows] = (time.perf_counter() - t0) / reps
     _tiles[key] = min(times, key=times.get)
     return times
 
 
-def compose_packed(mesh, levels, q, layout="N3", out=None, inter=None, grid=None):
+def compose_packed(mesh, levels, q, layout="N3", out=None, inter=None, grid=None, located=None):
     """Fused composition through packed levels (application order).  ``grid``
     = (nring, ncol) declares q a ring-major grid: same result, tiled
-    traversal."""
+    traversal.  ``located`` (grid only): split_locate(mesh, q), the first
+    level's location of the points: same result without its search."""
     n = q.shape[0] if layout == "N3" else q.shape[1]
     if out is None:
         out = B.empty(n, 3) if layout == "N3" else B.empty(3, n)
@@ -277,10 +292,19 @@ def compose_packed(mesh, levels, q, layout="N3", out=None, inter=None, grid=None
         nring, ncol = int(grid[0]), int(grid[1])
         if nring * ncol != n:
             raise N.CmmError(f"grid {grid} does not hold {n} points")
-        rows = _tiles.get((mesh._h, nring, ncol, len(levels)), 4)
-        N.call("cmm_tri_compose_grid", mesh._h, arr, len(levels), q.data_ptr(), nring, ncol, rows, sp, sc,
-               out.data_ptr(), osp, osc, B.ptr(inter), B.stream())
+        rows = _tiles.get((mesh._h, nring, ncol, len(levels), located is not None), 4)
+        if located is not None:
+            blk, cb = located
+            if blk.shape[0] != n or cb.shape != (n, 3) or not cb.is_contiguous():
+                raise N.CmmError("located does not hold the grid's points")
+            N.call("cmm_tri_compose_grid_located", mesh._h, arr, len(levels), q.data_ptr(), nring, ncol, rows, sp,
+                   sc, out.data_ptr(), osp, osc, B.ptr(inter), blk.data_ptr(), cb.data_ptr(), B.stream())
+        else:
+            N.call("cmm_tri_compose_grid", mesh._h, arr, len(levels), q.data_ptr(), nring, ncol, rows, sp, sc,
+                   out.data_ptr(), osp, osc, B.ptr(inter), B.stream())
     else:
+        if located is not None:
+            raise N.CmmError("located needs grid=")
         N.call("cmm_tri_compose", mesh._h, arr, len(levels), q.data_ptr(), n, sp, sc, out.data_ptr(), osp,
                osc, B.ptr(inter), B.stream())
     return out

@@ -398,11 +398,10 @@ __device__ inline bool tri_locate(const cmm_tri_mesh& M, const double* __restric
   return true;
 }
 
-// evaluation of one packed level at a located point (raw polynomial)
-__device__ inline d3 eval_level3(const double* __restrict__ packed, const Located& L, d3 bq,
-                                 int s) {
-  d3 c = sub_bary(s, bq.x, bq.y, bq.z, L.s12, L.s23, L.s31, L.s4);
-  const double2* p = reinterpret_cast<const double2*>(packed + ((int64_t)L.tri * 6 + s) * 18);
+// evaluation of coefficient block `blk` (= 6 triangle + sub-triangle) at
+// sub-triangle barycentrics c (raw polynomial)
+__device__ inline d3 eval_block3(const double* __restrict__ packed, int64_t blk, d3 c) {
+  const double2* p = reinterpret_cast<const double2*>(packed + blk * 18);
   double buf[18];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -411,6 +410,36 @@ __device__ inline d3 eval_level3(const double* __restrict__ packed, const Locate
     buf[2 * i + 1] = v.y;
   }
   return d3{bb6(c, buf), bb6(c, buf + 6), bb6(c, buf + 12)};
+}
+
+// evaluation of one packed level at a located point (raw polynomial)
+__device__ inline d3 eval_level3(const double* __restrict__ packed, const Located& L, d3 bq,
+                                 int s) {
+  const d3 c = sub_bary(s, bq.x, bq.y, bq.z, L.s12, L.s23, L.s31, L.s4);
+  return eval_block3(packed, (int64_t)L.tri * 6 + s, c);
+}
+
+// location of points for evaluation: coefficient block 6 t + s of the
+// containing triangle t and split sub-triangle s, and the sub-triangle
+// barycentrics (the values eval_level3 works with); -1 / NaN if not located
+__global__ __launch_bounds__(256) void k_tri_split_locate(cmm_tri_mesh M, const double* __restrict__ q, int64_t n,
+                                                          int64_t sp, int64_t sc, int32_t* __restrict__ blk,
+                                                          double* __restrict__ cb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Located L;
+  if (!tri_locate(M, M.geom, M.cells, M.over, load_pt(q, i, sp, sc), L)) {
+    const double nan = __builtin_nan("");
+    blk[i] = -1;
+    cb[3 * i] = cb[3 * i + 1] = cb[3 * i + 2] = nan;
+    return;
+  }
+  const int s = split_index(L.b0, L.b1, L.b2);
+  const d3 c = sub_bary(s, L.b0, L.b1, L.b2, L.s12, L.s23, L.s31, L.s4);
+  blk[i] = L.tri * 6 + s;
+  cb[3 * i] = c.x;
+  cb[3 * i + 1] = c.y;
+  cb[3 * i + 2] = c.z;
 }
 
 // value and ambient gradients of the raw polynomial components at a located
@@ -499,6 +528,11 @@ struct KTriCompose {
     double* inter;
     int64_t nring, ncol;
     int tr;
+    // optional: the first level's location of every point, computed once
+    // (cmm_tri_split_locate): coefficient block 6 t + s (-1: not located)
+    // and sub-triangle barycentrics (n, 3)
+    const int32_t* pre_blk;
+    const double* pre_cb;
   };
   static constexpr int kThreads = 256;
   static constexpr bool kArgsInPlace = true;
@@ -520,8 +554,28 @@ __device__ __forceinline__ void KTriCompose::run(const Args& a, int bid) {
     if (i < 0) return;
   }
   if (i >= n) return;
-  d3 x = load_pt(q, i, sp, sc);
-  for (int l = 0; l < depth; ++l) {
+  d3 x;
+  int l0 = 0;
+  if (a.pre_blk && depth > 0) {
+    // first level at the stored location: the values the search below
+    // would produce (same block, same sub-triangle barycentrics)
+    const int blk = a.pre_blk[i];
+    if (blk < 0) {
+      const double nan = __builtin_nan("");
+      x = d3{nan, nan, nan};
+    } else {
+      const d3 c = d3{a.pre_cb[3 * i], a.pre_cb[3 * i + 1], a.pre_cb[3 * i + 2]};
+      const d3 r = eval_block3(lv.p[0], blk, c);
+      double nn = sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+      double inv = 1 / nn;
+      x = d3{inv * r.x, inv * r.y, inv * r.z};
+    }
+    if (inter) store_pt(inter, i, 3, 1, x);
+    l0 = 1;
+  } else {
+    x = load_pt(q, i, sp, sc);
+  }
+  for (int l = l0; l < depth; ++l) {
     Located L;
     if (!tri_locate(M, M.geom, M.cells, M.over, x, L)) {
       double nan = __builtin_nan("");
@@ -1135,17 +1189,18 @@ extern "C" int cmm_tri_compose(const cmm_tri_mesh* m, const double* const* level
   }
   if (n == 0) return CMM_OK;
   return launch<KTriCompose>(k_tri_compose, (unsigned)grid_for(n, 256), 256, 0, as_stream(stream),
-                             KTriCompose::Args{*m, lv, depth, q, n, sp, sc, out, osp, osc, inter, 0, 0, 1},
+                             KTriCompose::Args{*m, lv, depth, q, n, sp, sc, out, osp, osc, inter, 0, 0, 1,
+                                               nullptr, nullptr},
                              "cmm_tri_compose");
 }
 
-extern "C" int cmm_tri_compose_grid(const cmm_tri_mesh* m, const double* const* levels, int depth,
-                                    const double* q, int64_t nring, int64_t ncol, int tile_rows,
-                                    int64_t q_stride_pt, int64_t q_stride_comp, double* out, int64_t osp,
-                                    int64_t osc, double* inter, void* stream) {
+static int compose_grid(const cmm_tri_mesh* m, const double* const* levels, int depth, const double* q,
+                        int64_t nring, int64_t ncol, int tile_rows, int64_t q_stride_pt, int64_t q_stride_comp,
+                        double* out, int64_t osp, int64_t osc, double* inter, const int32_t* pre_blk,
+                        const double* pre_cb, void* stream) {
   if (tile_rows == 0) tile_rows = 4;
   if (!m || depth < 0 || depth > kMaxDepth || (depth > 0 && !levels) || nring < 0 || ncol < 1 ||
-      tile_rows < 1 || tile_rows > 64 || (tile_rows & (tile_rows - 1)))
+      tile_rows < 1 || tile_rows > 64 || (tile_rows & (tile_rows - 1)) || (!pre_blk != !pre_cb))
     return fail(CMM_E_ARG, "cmm_tri_compose_grid: bad arguments");
   const int64_t n = nring * ncol;
   if (n > 0 && (!q || !out)) return fail(CMM_E_ARG, "cmm_tri_compose_grid: bad arguments");
@@ -1159,8 +1214,38 @@ extern "C" int cmm_tri_compose_grid(const cmm_tri_mesh* m, const double* const* 
   const int64_t threads = (nring + tile_rows - 1) / tile_rows * ((ncol + tc - 1) / tc) * 64;
   return launch<KTriCompose>(k_tri_compose, (unsigned)grid_for(threads, 256), 256, 0, as_stream(stream),
                              KTriCompose::Args{*m, lv, depth, q, n, q_stride_pt, q_stride_comp, out, osp, osc,
-                                               inter, nring, ncol, tile_rows},
+                                               inter, nring, ncol, tile_rows, pre_blk, pre_cb},
                              "cmm_tri_compose_grid");
+}
+
+extern "C" int cmm_tri_compose_grid(const cmm_tri_mesh* m, const double* const* levels, int depth,
+                                    const double* q, int64_t nring, int64_t ncol, int tile_rows,
+                                    int64_t q_stride_pt, int64_t q_stride_comp, double* out, int64_t osp,
+                                    int64_t osc, double* inter, void* stream) {
+  return compose_grid(m, levels, depth, q, nring, ncol, tile_rows, q_stride_pt, q_stride_comp, out, osp, osc, inter,
+                      nullptr, nullptr, stream);
+}
+
+extern "C" int cmm_tri_compose_grid_located(const cmm_tri_mesh* m, const double* const* levels, int depth,
+                                            const double* q, int64_t nring, int64_t ncol, int tile_rows,
+                                            int64_t q_stride_pt, int64_t q_stride_comp, double* out, int64_t osp,
+                                            int64_t osc, double* inter, const int32_t* first_block,
+                                            const double* first_sub_bary, void* stream) {
+  if (!first_block || !first_sub_bary) return fail(CMM_E_ARG, "cmm_tri_compose_grid_located: bad arguments");
+  return compose_grid(m, levels, depth, q, nring, ncol, tile_rows, q_stride_pt, q_stride_comp, out, osp, osc, inter,
+                      first_block, first_sub_bary, stream);
+}
+
+extern "C" int cmm_tri_split_locate(const cmm_tri_mesh* m, const double* q, int64_t n, int64_t sp, int64_t sc,
+                                    int32_t* block, double* sub_bary, void* stream) {
+  if (batch_recording()) return fail(CMM_E_STATE, "cmm_tri_split_locate: no batched form");
+  if (!m || n < 0 || (n > 0 && (!q || !block || !sub_bary))) return fail(CMM_E_ARG, "cmm_tri_split_locate: bad arguments");
+  if (n > 0x7fffffffll) return fail(CMM_E_RANGE, "cmm_tri_split_locate: n > 2^31 - 1");
+  if (n == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_tri_split_locate, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), *m, q, n, sp, sc,
+                     block, sub_bary);
+  CMM_LAUNCH_CHECK("cmm_tri_split_locate");
+  return CMM_OK;
 }
 
 extern "C" int cmm_tri_eval_grad(const cmm_tri_mesh* m, const double* packed, int ncomp, const double* q,

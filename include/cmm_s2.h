@@ -100,6 +100,28 @@ int cmm_tri_compose_grid(const cmm_tri_mesh* mesh, const double* const* levels, 
                          int64_t q_stride_comp, double* out, int64_t out_stride_pt,
                          int64_t out_stride_comp, double* inter, void* stream);
 
+/* Location of points for spline evaluation on the mesh: block[i] = 6 t + s
+ * for the containing triangle t (the search of cmm_tri_locate) and its
+ * Powell-Sabin split sub-triangle s (spherical_spline.py:660-666 picks the
+ * same one), sub_bary (n,3) the barycentrics in that sub-triangle; -1 / NaN
+ * where the point is not located.  No counterpart in the reference (it
+ * locates the evaluation grid anew in every step, spherical_spline.py:
+ * 365-398); these are the values the evaluation computes, stored. */
+int cmm_tri_split_locate(const cmm_tri_mesh* mesh, const double* q, int64_t n, int64_t q_stride_pt,
+                         int64_t q_stride_comp, int32_t* block, double* sub_bary, void* stream);
+
+/* cmm_tri_compose_grid whose first level (levels[0]) is evaluated at the
+ * stored location of every point (first_block, first_sub_bary from
+ * cmm_tri_split_locate on the same points and mesh) instead of a search:
+ * identical results.  For a grid composed at every step, as the CMM loop's
+ * sample grid (spherical_simulations.py:263-294).  No counterpart in the
+ * reference. */
+int cmm_tri_compose_grid_located(const cmm_tri_mesh* mesh, const double* const* levels, int depth,
+                                 const double* q, int64_t nring, int64_t ncol, int tile_rows,
+                                 int64_t q_stride_pt, int64_t q_stride_comp, double* out,
+                                 int64_t out_stride_pt, int64_t out_stride_comp, double* inter,
+                                 const int32_t* first_block, const double* first_sub_bary, void* stream);
+
 /* Differential of an evaluated spline.  ncomp 3: Jacobian of the normalised
  * map x -> p(x)/|p(x)| restricted to the tangent plane at x, out (3,3,n)
  * row-major [c][d] at out[(3c + d) n + i] (sphere_diffeomorphism.eval_grad,

@@ -194,7 +194,7 @@ def test_tiled_grid_composition_is_identical():
             inter_a = torch.empty(3, n, 3, dtype=torch.float64, device="cuda")
             a = ss.compose_packed(pm, levels, q, "N3", inter=inter_a)
             assert torch.isnan(a[5]).all() and torch.isfinite(a[6]).all()
-            key = (pm._h, L + 1, 2 * L, len(levels))
+            key = (pm._h, L + 1, 2 * L, len(levels), False)
             for rows in (1, 2, 4, 8, 16):
                 ss._tiles[key] = rows
                 inter_b = torch.full_like(inter_a, 7.0)
@@ -202,3 +202,16 @@ def test_tiled_grid_composition_is_identical():
                 assert torch.equal(a.nan_to_num(9.0), b.nan_to_num(9.0)), (ico_level, L, rows)
                 assert torch.equal(inter_a.nan_to_num(9.0), inter_b.nan_to_num(9.0)), (ico_level, L, rows)
             ss._tiles.pop(key)
+            # the first level at stored locations: the same bits
+            blk, cb = ss.split_locate(pm, q)
+            assert int(blk[5]) == -1 and int(blk[2 * L + 7]) == -1 and bool((blk[6:2 * L] >= 0).all())
+            tri = pm.locate(q)[0]
+            assert torch.equal((blk // 6).long(), torch.where(tri >= 0, tri, -1).long().where(blk >= 0, -1))
+            keyl = key[:-1] + (True,)
+            for rows in (1, 4, 16):
+                ss._tiles[keyl] = rows
+                inter_c = torch.full_like(inter_a, 7.0)
+                c = ss.compose_packed(pm, levels, q, "N3", inter=inter_c, grid=(L + 1, 2 * L), located=(blk, cb))
+                assert torch.equal(a.nan_to_num(9.0), c.nan_to_num(9.0)), (ico_level, L, rows, "located")
+                assert torch.equal(inter_a.nan_to_num(9.0), inter_c.nan_to_num(9.0)), (ico_level, L, rows)
+            ss._tiles.pop(keyl)
