@@ -79,6 +79,44 @@ def test_stencil_advection_matches_pointwise():
                 assert bool((yv == -7.0).all())
 
 
+@pytest.mark.parametrize("ico_level,L", [(5, 64), (6, 128)])
+def test_stencil_advection_sparse_velocity_matches_pointwise(ico_level, L):
+    """The RK4 feet of a run's map on its sparse velocity levels (blocks
+    missing where no foot fell in the previous step, rebuilt from the jets
+    during the evaluation): the vertex and stencil feet of one launch give
+    the bits of the per-point kernel (cmm_advect_points) and of the
+    stencil-only launch."""
+    from cmm_s2 import _native as N
+    from cmm_s2.core import mesh_functions as mf, evolution_functions as ev, dynamical_fields as df
+    from cmm_s2.core import _backend as B
+    from cmm_s2.core.spherical_simulations import BarotropicCMM
+    ico = mf.icosahedral_mesh(ico_level)
+    mesh = mf.spherical_triangulation(ico.points, simplices=ico.simplices)
+    sim = BarotropicCMM(L, 8, 8 / L, mesh, df.rossby_wave.at(t=0.2), 2 * np.pi, n_maps=4)
+    sim.bootstrap()
+    for _ in range(3):
+        sim.step()
+    torch.cuda.synchronize()
+    U = sim.U
+    assert U.sparse
+    xv, xs = mesh.d_verts3, mesh.d_spts
+    t, dt = float(sim.tspan[sim.count]), sim.dt
+    ref_v = ev.integrate(N.INT_RK4, t, dt, xv, U)
+    outs = {}
+    for with_verts in (True, False):
+        yv = torch.full_like(xv, -7.0)
+        ys = torch.empty_like(xs)
+        desc = ev.velocity_descriptor(U, ev._STAGES[N.INT_RK4](t, dt), dt)
+        N.call("cmm_advect_stencil", desc, N.INT_RK4, dt, xv.data_ptr() if with_verts else None, xv.stride(0),
+               xs.data_ptr(), xs.stride(0), mesh.nv, yv.data_ptr(), yv.stride(0), ys.data_ptr(), ys.stride(0),
+               B.stream())
+        torch.cuda.synchronize()
+        outs[with_verts] = (yv, ys)
+    assert torch.isfinite(outs[True][1]).all()
+    assert torch.equal(outs[True][1], outs[False][1])
+    assert torch.equal(outs[True][0], ref_v)
+
+
 def test_fused_map_update_matches_oracle():
     """advect_project_sphere with identity=True then False, analytic and
     spline velocities, against evolution_functions.py restated."""
