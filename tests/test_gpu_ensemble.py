@@ -192,6 +192,36 @@ def test_partitioned_batched_steps_match_solo_runs():
     ens.close()
 
 
+def test_batched_steps_across_block_size_classes():
+    """Members above bandlimit 512 run some kernels with other block sizes
+    (forward Legendre waves, resampling split): batched_step groups them
+    apart from the small ones, and every member keeps its solo bits."""
+    from cmm_s2.core import dynamical_fields as df
+    from cmm_s2.core.ensemble import Ensemble
+    from cmm_s2.core.spherical_simulations import BarotropicCMM
+    meshes = _meshes()
+    specs = [(24, 2, 5), (520, 2, 4), (32, 2, 5)]
+    mk = lambda L, i, Nt: (lambda: BarotropicCMM(L, Nt, Nt / L, meshes[i], df.rossby_wave, 2 * np.pi, n_maps=3))
+    ens = Ensemble([mk(*s) for s in specs])
+    ens.bootstrap()
+    torch.cuda.synchronize()
+    groups = []
+    while True:
+        k = ens.batched_step()
+        if not k:
+            break
+        groups.append((k, ens.batched_launches))
+    torch.cuda.synchronize()
+    for spec, m in zip(specs, ens.members):
+        solo = mk(*spec)()
+        solo.run()
+        torch.cuda.synchronize()
+        assert m.count == solo.count == spec[2]
+        for j, (a, b) in enumerate(zip(_state(m), _state(solo))):
+            assert torch.equal(a, b), (spec, j)
+    ens.close()
+
+
 def test_batch_recording_refuses_unbatched_entry_points():
     """While a batch records, an entry point without a batched form returns
     CMM_E_STATE instead of launching out of order; members that diverge are
