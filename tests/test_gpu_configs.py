@@ -51,26 +51,19 @@ def test_plain_cmm_rh4_ico5(frame):
     err = np.abs(got[:, sub] - ref).max(axis=0)
     # Tolerance: every step rebuilds the map from jets that are eps = 1e-5
     # finite differences of stencil values, so round-off differences of the
-    # two evaluations (Cramer vs dual-basis barycentrics) reach the jets
-    # multiplied by 1/(4 eps) (the 1e-9 jet tolerance) and the map values
-    # multiplied by the mesh spacing h ~ 0.04: 1e-10 after 10 steps (typical
-    # differences are ~1e-12).
+    # two evaluations (dual-basis vs Cramer barycentrics, FMA contraction vs
+    # NumPy's separate roundings) reach the jets multiplied by 1/(4 eps) (the
+    # 1e-9 jet tolerance) and the map values multiplied by the mesh spacing
+    # h ~ 0.04: measured after 10 steps, max 6.2e-11 and median 2.8e-12 in
+    # both frames (99.9th percentile 4e-11).  With the poles on the flow's
+    # fixed axis the pole stencils are location ties (the flow fixes the pole
+    # vertices); the measured differences near the poles are no larger.
     tol = 1e-10
-    if frame == "rotated":
-        # no mesh vertex on the flow's fixed axis: no location ties
-        assert err.max() < tol
-    else:
-        # The flow fixes both poles exactly (u = 0 there) and the poles are
-        # mesh vertices, so the stencil evaluation at a pole is a location
-        # tie: the oracle (Qhull's walk) and the GPU (first candidate) may
-        # take different sub-triangles, whose polynomials differ at
-        # O(eps^2) at the stencil points.  Each step carries that difference
-        # one vertex ring further (~0.04 rad at ico-5), so after 10 steps it
-        # stays inside caps of ~0.4 rad; outside them the maps agree to the
-        # tolerance above.
-        away = np.abs(q[sub][:, 2]) < np.cos(0.5)
-        assert err[away].max() < tol
-        assert err.max() < 1e-7
+    away = np.abs(q[sub][:, 2]) < np.cos(0.5)
+    print(f"C1 {frame}: max |gpu - oracle| {err.max():.3e}, outside the polar caps {err[away].max():.3e}, "
+          f"median {np.median(err):.3e}, 99.9th percentile {np.quantile(err, 0.999):.3e}", flush=True)
+    assert err.max() < tol
+    assert np.median(err) < 1e-11
 
 
 def test_zoom_vorticity_matches_oracle():
