@@ -1744,7 +1744,8 @@ static int i8_inverse(const cmm_sht_plan* p, const double* flm, int nf, int f0, 
 
 // ---------------------------------------------------------------------------
 // forward: resampling GEMM on FP64 matrix cores,
-//   G[f][j][m] = sum_t K_{m%2}[j][t] F[f][t][m],  j < L, t <= L.
+//   G[f][m][j] = sum_t K_{m%2}[j][t] F[f][t][m],  j < L, t <= L
+// (orders-major, so the analysis reads every order's nodes contiguously).
 // One wave per (16 nodes j) x (16 orders m of one parity, re/im = 32 real
 // columns) x field; K is stored transposed (Kt[p][t][j]) so the A operand
 // is a coalesced row read.  Degrees of freedom t advance 4 per MFMA; the
@@ -1862,7 +1863,7 @@ __device__ __forceinline__ void KResample<SPLIT>::run(const Args& a, int bid) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int j = j0 + 4 * e + bk;
-      if (j < L) Gf[((int64_t)j * L + mcol[nt]) * 2 + (bj & 1)] = acc[nt][e];
+      if (j < L) Gf[((int64_t)mcol[nt] * L + j) * 2 + (bj & 1)] = acc[nt][e];
     }
   }
 }
@@ -2057,8 +2058,9 @@ __global__ __launch_bounds__(512) void k_leg_forward(typename KLegForward<C>::Ar
 }
 
 // ---------------------------------------------------------------------------
-// Forward analysis with wave-level sums: a block of W waves per (unit,
-// field); lane t of wave w owns the northern nodes j = 64 (NP w + q) + t,
+// Forward analysis with wave-level sums (G orders-major, G[f][m][j], as the
+// resampling writes it): a block of W waves per (unit, field); lane t of
+// wave w owns the northern nodes j = 64 (NP w + q) + t,
 // q < NP.  Per chunk of 16 degrees (the restart table's chunk) each lane
 // adds its nodes' products into 32 partial sums (degree, re / im), and the
 // wave sums them over its 64 lanes by a butterfly that halves the values a
@@ -2192,9 +2194,9 @@ __device__ __forceinline__ void KLegForwardW<NP>::run(const Args& a, int bid) {
     pp[q] = ps.x;
     pc[q] = ps.y;
     s[q] = st_s[(int64_t)u * nh + j];
-    const double2 ga = G[((int64_t)fi * nn + j) * L + m];
+    const double2 ga = G[((int64_t)fi * L + m) * nn + j];  // orders-major: node j of order m
     const double wj = (act[q] && !(skip_poles && m > 0 && j == 0)) ? cm * wg[j] : 0.0;
-    const double2 gb = G[((int64_t)fi * nn + jm) * L + m];
+    const double2 gb = G[((int64_t)fi * L + m) * nn + jm];
     if (jm != j) {
       S[q] = double2{wj * (ga.x + gb.x), wj * (ga.y + gb.y)};
       D[q] = double2{wj * (ga.x - gb.x), wj * (ga.y - gb.y)};
