@@ -475,7 +475,9 @@ __device__ double2* fft_lds(const FFTPlanDev& P, const double2* __restrict__ tw,
   return in;
 }
 
-// real rings -> half spectrum F[f][t][m] (m < L), scaled by pi / L
+// real rings -> half spectrum F[f][t][m] (m < L), scaled by pi / L; with
+// psplit each ring's orders are stored even m first, then odd m (the
+// resampling reads one parity at a time)
 struct KFftR2c {
   struct Args {
     FFTPlanDev P;
@@ -483,6 +485,7 @@ struct KFftR2c {
     const double* f;
     int L;
     double2* F;
+    int psplit;  // 1: each ring's orders parity-split (even m, then odd m)
   };
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(const Args& a, int bid);
@@ -508,12 +511,14 @@ __device__ __forceinline__ void KFftR2c::run(const Args& a, int bid) {
   __syncthreads();
   double2* Z = fft_lds(P, tw, b0, b1, -1.0);
   const double sc = M_PI / L;
+  const int half = (L + 1) / 2;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
     double2 zk = Z[m], zn = Z[(N - m) % N];
     double2 fa = double2{0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y)};
     double2 fb = double2{0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x)};
-    F[((int64_t)fi * (L + 1) + ta) * L + m] = double2{sc * fa.x, sc * fa.y};
-    if (hasb) F[((int64_t)fi * (L + 1) + tb) * L + m] = double2{sc * fb.x, sc * fb.y};
+    const int mi = a.psplit ? (m & 1) * half + (m >> 1) : m;
+    F[((int64_t)fi * (L + 1) + ta) * L + mi] = double2{sc * fa.x, sc * fa.y};
+    if (hasb) F[((int64_t)fi * (L + 1) + tb) * L + mi] = double2{sc * fb.x, sc * fb.y};
   }
 }
 
@@ -1797,9 +1802,14 @@ __device__ __forceinline__ void KResample<SPLIT>::run(const Args& a, int bid) {
   for (int nt = 0; nt < 2; ++nt) mcol[nt] = p + 2 * (16 * qb + 8 * nt + (bj >> 1));
   const int ja = j0 + bj;  // A row of this lane
   const int jac = min(ja, L - 1);
+  // F rows parity-split (even orders, then odd): order m at column mi(m)
+  const int half = (L + 1) / 2;
   int mcc[2];
 #pragma unroll
-  for (int nt = 0; nt < 2; ++nt) mcc[nt] = min(mcol[nt], L - 1);
+  for (int nt = 0; nt < 2; ++nt) {
+    const int mm = min(mcol[nt], L - 1);
+    mcc[nt] = (mm & 1) * half + (mm >> 1);
+  }
   // this wave's steps of 4 rings: [sb, se)
   const int nsteps = tpad / 4;
   const int per = (nsteps + W - 1) / W;
@@ -2805,7 +2815,7 @@ static int sht_forward(const cmm_sht_plan* p, const double* f, int nf, double* f
   hipStream_t st = as_stream(stream);
   int pairs = (L + 2) / 2;
   size_t lds = (size_t)2 * p->N * sizeof(double2);
-  int rc = launch<KFftR2c>(k_fft_r2c, (unsigned)(nf * pairs), 256, lds, st, KFftR2c::Args{fft_plan(p), p->tw, f, L, F},
+  int rc = launch<KFftR2c>(k_fft_r2c, (unsigned)(nf * pairs), 256, lds, st, KFftR2c::Args{fft_plan(p), p->tw, f, L, F, 1},
                            "cmm_sht_forward: fft");
   if (rc) return rc;
   const int nmg = ((L + 1) / 2 + 15) / 16;  // groups of 16 orders of one parity (at most ceil(L / 2) orders)
@@ -2999,7 +3009,7 @@ extern "C" int cmm_sht_inverse_adjoint(const cmm_sht_plan* p, const double* fbar
   hipStream_t st = as_stream(stream);
   const int pairs = (L + 2) / 2;
   const size_t lds = (size_t)2 * p->N * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_r2c, dim3(nf * pairs), dim3(256), lds, st, KFftR2c::Args{fft_plan(p), p->tw, fbar, L, F});
+  hipLaunchKernelGGL(k_fft_r2c, dim3(nf * pairs), dim3(256), lds, st, KFftR2c::Args{fft_plan(p), p->tw, fbar, L, F, 0});
   CMM_LAUNCH_CHECK("cmm_sht_inverse_adjoint: fft");
   const int nhr = L / 2 + 1;
   if (nhr <= 5 * 64) {  // matrix-core form: every field of a group shares the recurrence
