@@ -535,6 +535,9 @@ struct KTriCompose {
     const double* pre_cb;
   };
   static constexpr int kThreads = 256;
+  // 7 waves per SIMD: the kernel fits 72 VGPRs without spills (78 unbounded,
+  // 6 waves) and the composition is 0.8 % faster
+  static constexpr int kMinEU = 7;
   static constexpr bool kArgsInPlace = true;
   static __device__ __forceinline__ void run(const Args& a, int bid);
 };
@@ -592,7 +595,7 @@ __device__ __forceinline__ void KTriCompose::run(const Args& a, int bid) {
   store_pt(out, i, osp, osc, x);
 }
 
-__global__ __launch_bounds__(256) void k_tri_compose(KTriCompose::Args a) { KTriCompose::run(a, blockIdx.x); }
+__global__ __launch_bounds__(256, 7) void k_tri_compose(KTriCompose::Args a) { KTriCompose::run(a, blockIdx.x); }
 
 // map Jacobian (ncomp 3, out (3,3,n) [c][d]) or tangential gradient of a
 // scalar spline (ncomp 1, out (3,n)); NaN where the point is not located
