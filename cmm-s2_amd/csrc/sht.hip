@@ -381,6 +381,16 @@ __device__ inline double2 csub(double2 a, double2 b) { return double2{a.x - b.x,
 // s * i * a
 __device__ inline double2 cmuli(double2 a, double s) { return double2{-s * a.y, s * a.x}; }
 
+// radix-4 butterfly (4-point DFT, root exp(sign 2 pi i / 4))
+__device__ inline void bfly4(const double2 (&v)[4], double2 (&y)[4], double sign) {
+  double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+  double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+  y[0] = cadd(a0, b0);
+  y[2] = csub(a0, b0);
+  y[1] = cadd(a1, cmuli(b1, sign));
+  y[3] = csub(a1, cmuli(b1, sign));
+}
+
 template <int R>
 __device__ inline void fft_pass(const double2* __restrict__ tw, const double2* in, double2* out, int N,
                                 int Ns, double sign) {
@@ -404,12 +414,7 @@ __device__ inline void fft_pass(const double2* __restrict__ tw, const double2* i
       y[0] = cadd(v[0], v[1]);
       y[1] = csub(v[0], v[1]);
     } else if constexpr (R == 4) {
-      double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-      double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
-      y[0] = cadd(a0, b0);
-      y[2] = csub(a0, b0);
-      y[1] = cadd(a1, cmuli(b1, sign));
-      y[3] = csub(a1, cmuli(b1, sign));
+      bfly4(v, y, sign);
     } else {
       const double h = 0.86602540378443864676;  // sqrt(3)/2
       double2 t1 = cadd(v[1], v[2]);
@@ -455,12 +460,12 @@ __device__ inline void fft_pass_dft(const double2* __restrict__ tw, const double
 // FFT of length N held in buf0 (ping-pong with buf1); sign -1 forward,
 // +1 inverse (unnormalised).  Returns the buffer holding the result.
 __device__ double2* fft_lds(const FFTPlanDev& P, const double2* __restrict__ tw, double2* buf0,
-                            double2* buf1, double sign) {
+                            double2* buf1, double sign, int pass0 = 0, int Ns0 = 1) {
   const int N = P.N;
-  int Ns = 1;
+  int Ns = Ns0;
   double2* in = buf0;
   double2* out = buf1;
-  for (int pass = 0; pass < P.nrad; ++pass) {
+  for (int pass = pass0; pass < P.nrad; ++pass) {
     const int R = P.radix[pass];
     if (R == 4) fft_pass<4>(tw, in, out, N, Ns, sign);
     else if (R == 2) fft_pass<2>(tw, in, out, N, Ns, sign);
@@ -591,8 +596,9 @@ __device__ __forceinline__ void KFftC2r<RF>::run(const Args& a, int bid) {
         b0[N - n] = double2{a.x + b.y, b.x - a.y};  // conj(a) + i conj(b)
       }
     }
-  } else
-  for (int n = threadIdx.x; n < N; n += blockDim.x) {
+  }
+  // element n of the complex sequence xa + i xb (conjugate-symmetric fill)
+  auto xval = [&](int n) -> double2 {
     double2 xa, xb;
     if (n == 0) {
       xa = double2{Fa[0].x, 0.0};
@@ -613,10 +619,30 @@ __device__ __forceinline__ void KFftC2r<RF>::run(const Args& a, int bid) {
         xb = double2{0.0, 0.0};
       }
     }
-    b0[n] = double2{xa.x - xb.y, xa.y + xb.x};  // xa + i xb
+    return double2{xa.x - xb.y, xa.y + xb.x};  // xa + i xb
+  };
+  double2* Z;
+  if (!RF && P.radix[0] == 4) {
+    // the first radix-4 pass (span 1: no twiddles) straight from the
+    // spectrum into LDS: fft_pass<4>'s arithmetic without staging the input
+    const int NR = N / 4;
+    for (int j = threadIdx.x; j < NR; j += blockDim.x) {
+      double2 v[4], y[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[r] = xval(j + r * NR);
+      bfly4(v, y, +1.0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) b1[4 * j + r] = y[r];
+    }
+    __syncthreads();
+    Z = fft_lds(P, tw, b1, b0, +1.0, 1, 4);
+  } else {
+    if constexpr (!RF) {
+      for (int n = threadIdx.x; n < N; n += blockDim.x) b0[n] = xval(n);
+    }
+    __syncthreads();
+    Z = fft_lds(P, tw, b0, b1, +1.0);
   }
-  __syncthreads();
-  double2* Z = fft_lds(P, tw, b0, b1, +1.0);
   double* ra = f + ((int64_t)fi * (L + 1) + ta) * N;
   double* rb = f + ((int64_t)fi * (L + 1) + (hasb ? tb : ta)) * N;
   for (int n = threadIdx.x; n < N; n += blockDim.x) {
