@@ -512,9 +512,25 @@ __device__ __forceinline__ void KFftR2c::run(const Args& a, int bid) {
   bool hasb = tb <= L;
   const double* ra = f + ((int64_t)fi * (L + 1) + ta) * N;
   const double* rb = f + ((int64_t)fi * (L + 1) + (hasb ? tb : ta)) * N;
-  for (int n = threadIdx.x; n < N; n += blockDim.x) b0[n] = double2{ra[n], hasb ? rb[n] : 0.0};
-  __syncthreads();
-  double2* Z = fft_lds(P, tw, b0, b1, -1.0);
+  double2* Z;
+  if (P.radix[0] == 4) {
+    // the first radix-4 pass (span 1: no twiddles) straight from the rings
+    const int NR = N / 4;
+    for (int j = threadIdx.x; j < NR; j += blockDim.x) {
+      double2 v[4], y[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[r] = double2{ra[j + r * NR], hasb ? rb[j + r * NR] : 0.0};
+      bfly4(v, y, -1.0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) b1[4 * j + r] = y[r];
+    }
+    __syncthreads();
+    Z = fft_lds(P, tw, b1, b0, -1.0, 1, 4);
+  } else {
+    for (int n = threadIdx.x; n < N; n += blockDim.x) b0[n] = double2{ra[n], hasb ? rb[n] : 0.0};
+    __syncthreads();
+    Z = fft_lds(P, tw, b0, b1, -1.0);
+  }
   const double sc = M_PI / L;
   const int half = (L + 1) / 2;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
