@@ -391,9 +391,10 @@ __device__ inline void bfly4(const double2 (&v)[4], double2 (&y)[4], double sign
   y[3] = csub(a1, cmuli(b1, sign));
 }
 
-template <int R>
-__device__ inline void fft_pass(const double2* __restrict__ tw, const double2* in, double2* out, int N,
-                                int Ns, double sign) {
+// one Stockham pass of radix R; output element i goes to store(i, value)
+template <int R, class Store>
+__device__ inline void fft_pass_to(const double2* __restrict__ tw, const double2* in, int N, int Ns, double sign,
+                                   Store&& store) {
   const int NR = N / R;
   const int stride = N / (Ns * R);
   for (int j = threadIdx.x; j < NR; j += blockDim.x) {
@@ -426,16 +427,23 @@ __device__ inline void fft_pass(const double2* __restrict__ tw, const double2* i
     }
     const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) out[base + r * Ns] = y[r];
+    for (int r = 0; r < R; ++r) store(base + r * Ns, y[r]);
   }
+}
+
+template <int R>
+__device__ inline void fft_pass(const double2* __restrict__ tw, const double2* in, double2* out, int N,
+                                int Ns, double sign) {
+  fft_pass_to<R>(tw, in, N, Ns, sign, [out](int i, double2 v) { out[i] = v; });
 }
 
 // Stockham pass of any radix R as a direct DFT: output q of butterfly group
 // j (k = j mod Ns) is sum_r in[j + r N/R] W^(r (k + q Ns) N / (Ns R)), W =
 // exp(sign 2 pi i / N) -- the group twiddle and the R-point DFT root in one
 // table lookup; one thread per output, R complex products each.
-__device__ inline void fft_pass_dft(const double2* __restrict__ tw, const double2* in, double2* out, int N, int Ns,
-                                    int R, double sign) {
+template <class Store>
+__device__ inline void fft_pass_dft_to(const double2* __restrict__ tw, const double2* in, int N, int Ns, int R,
+                                       double sign, Store&& store) {
   const int NR = N / R;
   const int unit = N / (Ns * R);
   for (int o = threadIdx.x; o < N; o += blockDim.x) {
@@ -453,19 +461,40 @@ __device__ inline void fft_pass_dft(const double2* __restrict__ tw, const double
       e += e1;
       if (e >= N) e -= N;
     }
-    out[(j / Ns) * Ns * R + k + q * Ns] = acc;
+    store((j / Ns) * Ns * R + k + q * Ns, acc);
   }
 }
 
+__device__ inline void fft_pass_dft(const double2* __restrict__ tw, const double2* in, double2* out, int N, int Ns,
+                                    int R, double sign) {
+  fft_pass_dft_to(tw, in, N, Ns, R, sign, [out](int i, double2 v) { out[i] = v; });
+}
+
+// the last pass of an FFT (span N / R), its outputs handed to store(i, value)
+// instead of written to LDS
+template <class Store>
+__device__ inline void fft_last_pass_to(const FFTPlanDev& P, const double2* __restrict__ tw, const double2* in,
+                                        double sign, Store&& store) {
+  const int R = P.radix[P.nrad - 1];
+  const int Ns = P.N / R;
+  if (R == 4) fft_pass_to<4>(tw, in, P.N, Ns, sign, store);
+  else if (R == 2) fft_pass_to<2>(tw, in, P.N, Ns, sign, store);
+  else if (R == 3) fft_pass_to<3>(tw, in, P.N, Ns, sign, store);
+  else fft_pass_dft_to(tw, in, P.N, Ns, R, sign, store);
+}
+
 // FFT of length N held in buf0 (ping-pong with buf1); sign -1 forward,
-// +1 inverse (unnormalised).  Returns the buffer holding the result.
+// +1 inverse (unnormalised).  Runs passes pass0 .. pass_end - 1 (all by
+// default; pass0 > 0 takes the span Ns0 the earlier passes reached).
+// Returns the buffer holding the result.
 __device__ double2* fft_lds(const FFTPlanDev& P, const double2* __restrict__ tw, double2* buf0,
-                            double2* buf1, double sign, int pass0 = 0, int Ns0 = 1) {
+                            double2* buf1, double sign, int pass0 = 0, int Ns0 = 1, int pass_end = -1) {
   const int N = P.N;
   int Ns = Ns0;
   double2* in = buf0;
   double2* out = buf1;
-  for (int pass = pass0; pass < P.nrad; ++pass) {
+  const int pe = pass_end < 0 ? P.nrad : pass_end;
+  for (int pass = pass0; pass < pe; ++pass) {
     const int R = P.radix[pass];
     if (R == 4) fft_pass<4>(tw, in, out, N, Ns, sign);
     else if (R == 2) fft_pass<2>(tw, in, out, N, Ns, sign);
@@ -637,8 +666,12 @@ __device__ __forceinline__ void KFftC2r<RF>::run(const Args& a, int bid) {
     }
     return double2{xa.x - xb.y, xa.y + xb.x};  // xa + i xb
   };
+  // the last pass stores the real rings itself (element n of the pass's
+  // output: ring ta gets its real part, ring tb its imaginary part)
+  const bool direct0 = !RF && P.radix[0] == 4;
+  const int pend = P.nrad - 1 >= (direct0 ? 1 : 0) ? P.nrad - 1 : P.nrad;
   double2* Z;
-  if (!RF && P.radix[0] == 4) {
+  if (direct0) {
     // the first radix-4 pass (span 1: no twiddles) straight from the
     // spectrum into LDS: fft_pass<4>'s arithmetic without staging the input
     const int NR = N / 4;
@@ -651,19 +684,26 @@ __device__ __forceinline__ void KFftC2r<RF>::run(const Args& a, int bid) {
       for (int r = 0; r < 4; ++r) b1[4 * j + r] = y[r];
     }
     __syncthreads();
-    Z = fft_lds(P, tw, b1, b0, +1.0, 1, 4);
+    Z = fft_lds(P, tw, b1, b0, +1.0, 1, 4, pend);
   } else {
     if constexpr (!RF) {
       for (int n = threadIdx.x; n < N; n += blockDim.x) b0[n] = xval(n);
     }
     __syncthreads();
-    Z = fft_lds(P, tw, b0, b1, +1.0);
+    Z = fft_lds(P, tw, b0, b1, +1.0, 0, 1, pend);
   }
   double* ra = f + ((int64_t)fi * (L + 1) + ta) * N;
   double* rb = f + ((int64_t)fi * (L + 1) + (hasb ? tb : ta)) * N;
-  for (int n = threadIdx.x; n < N; n += blockDim.x) {
-    ra[n] = Z[n].x;
-    if (hasb) rb[n] = Z[n].y;
+  if (pend < P.nrad) {
+    fft_last_pass_to(P, tw, Z, +1.0, [=](int n, double2 v) {
+      ra[n] = v.x;
+      if (hasb) rb[n] = v.y;
+    });
+  } else {
+    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+      ra[n] = Z[n].x;
+      if (hasb) rb[n] = Z[n].y;
+    }
   }
 }
 
