@@ -2440,16 +2440,30 @@ struct KStreamOps {
   };
   static constexpr int kThreads = 128;
   static __device__ __forceinline__ void run(const Args& a, int bid);
+  // the 12 outputs of packed coefficient idx, into res[0..11]
+  static __device__ __forceinline__ void compute(const Args& a, int64_t idx, double2* res);
 };
 
 __device__ __forceinline__ void KStreamOps::run(const Args& a, int bid) {
+  const int L = a.L;
+  double2* __restrict__ out = a.out;
+  // the block's 12 outputs per coefficient are staged in LDS (row of 13 per
+  // thread against bank conflicts) and stored as one contiguous run
+  __shared__ double2 stage[kThreads * 13];
+  const int64_t base = (int64_t)bid * kThreads;
+  const int64_t idx = base + threadIdx.x;
+  const int64_t total = (int64_t)L * (L + 1) / 2;
+  if (idx < total) compute(a, idx, stage + threadIdx.x * 13);
+  __syncthreads();
+  const int nv = (int)(total - base < kThreads ? total - base : kThreads);
+  double2* o = out + base * 12;
+  for (int e = threadIdx.x; e < nv * 12; e += kThreads) o[e] = stage[(e / 12) * 13 + e % 12];
+}
+
+__device__ __forceinline__ void KStreamOps::compute(const Args& a, int64_t idx, double2* res) {
   const int L = a.L, apply_dinv = a.apply_dinv;
   const double2* __restrict__ w = a.w;
   const double* __restrict__ dinv = a.dinv;
-  double2* __restrict__ out = a.out;
-  int64_t idx = (int64_t)bid * blockDim.x + threadIdx.x;
-  int64_t total = (int64_t)L * (L + 1) / 2;
-  if (idx >= total) return;
   // decode (l, m) from the packed index
   int m = 0;
   {
@@ -2482,11 +2496,11 @@ __device__ __forceinline__ void KStreamOps::run(const Args& a, int bid) {
   } else {
     um[0] = um[1] = um[2] = z;
   }
-  for (int i = 0; i < 3; ++i) out[idx * 12 + i] = double2{u[i].re, u[i].im};
+  for (int i = 0; i < 3; ++i) res[i] = double2{u[i].re, u[i].im};
   for (int i = 0; i < 3; ++i) {
     Cplx g[3];
     angmom_from(l, m, um[i], up[i], u[i], g);
-    for (int k = 0; k < 3; ++k) out[idx * 12 + 3 + 3 * i + k] = double2{g[k].re, g[k].im};
+    for (int k = 0; k < 3; ++k) res[3 + 3 * i + k] = double2{g[k].re, g[k].im};
   }
 }
 
