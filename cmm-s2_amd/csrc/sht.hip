@@ -391,44 +391,93 @@ __device__ inline void bfly4(const double2 (&v)[4], double2 (&y)[4], double sign
   y[3] = csub(a1, cmuli(b1, sign));
 }
 
+// butterfly j of a Stockham pass of radix R and span Ns: its R inputs
+// in[j + r N / R] (twiddled) into the R-point DFT y
+template <int R>
+__device__ inline void fft_bfly(const double2* __restrict__ tw, const double2* in, int N, int Ns, int j, double sign,
+                                double2 (&y)[R]) {
+  const int NR = N / R;
+  const int stride = N / (Ns * R);
+  const int k = j % Ns;
+  double2 v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double2 a = in[j + r * NR];
+    if (r > 0 && k > 0) {
+      double2 w = tw[k * r * stride];
+      if (sign > 0) w.y = -w.y;
+      a = cmul(a, w);
+    }
+    v[r] = a;
+  }
+  if constexpr (R == 2) {
+    y[0] = cadd(v[0], v[1]);
+    y[1] = csub(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    bfly4(v, y, sign);
+  } else {
+    const double h = 0.86602540378443864676;  // sqrt(3)/2
+    double2 t1 = cadd(v[1], v[2]);
+    double2 t2 = double2{v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y};
+    double2 t3 = cmuli(csub(v[1], v[2]), sign * h);
+    y[0] = cadd(v[0], t1);
+    y[1] = cadd(t2, t3);
+    y[2] = csub(t2, t3);
+  }
+}
+
 // one Stockham pass of radix R; output element i goes to store(i, value)
 template <int R, class Store>
 __device__ inline void fft_pass_to(const double2* __restrict__ tw, const double2* in, int N, int Ns, double sign,
                                    Store&& store) {
   const int NR = N / R;
-  const int stride = N / (Ns * R);
   for (int j = threadIdx.x; j < NR; j += blockDim.x) {
-    const int k = j % Ns;
-    double2 v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      double2 a = in[j + r * NR];
-      if (r > 0 && k > 0) {
-        double2 w = tw[k * r * stride];
-        if (sign > 0) w.y = -w.y;
-        a = cmul(a, w);
-      }
-      v[r] = a;
-    }
     double2 y[R];
-    if constexpr (R == 2) {
-      y[0] = cadd(v[0], v[1]);
-      y[1] = csub(v[0], v[1]);
-    } else if constexpr (R == 4) {
-      bfly4(v, y, sign);
-    } else {
-      const double h = 0.86602540378443864676;  // sqrt(3)/2
-      double2 t1 = cadd(v[1], v[2]);
-      double2 t2 = double2{v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y};
-      double2 t3 = cmuli(csub(v[1], v[2]), sign * h);
-      y[0] = cadd(v[0], t1);
-      y[1] = cadd(t2, t3);
-      y[2] = csub(t2, t3);
-    }
-    const int base = (j / Ns) * Ns * R + k;
+    fft_bfly<R>(tw, in, N, Ns, j, sign, y);
+    const int base = (j / Ns) * Ns * R + j % Ns;
 #pragma unroll
     for (int r = 0; r < R; ++r) store(base + r * Ns, y[r]);
   }
+}
+
+// the same pass in place on one buffer: every thread reads the inputs of its
+// (at most KB) butterflies into registers, the block synchronises, then
+// writes the outputs; ends synchronised
+template <int R, int KB>
+__device__ inline void fft_pass_inplace(const double2* __restrict__ tw, double2* buf, int N, int Ns, double sign) {
+  const int NR = N / R;
+  double2 y[KB][R];
+#pragma unroll
+  for (int b = 0; b < KB; ++b) {
+    const int j = threadIdx.x + b * blockDim.x;
+    if (j < NR) fft_bfly<R>(tw, buf, N, Ns, j, sign, y[b]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < KB; ++b) {
+    const int j = threadIdx.x + b * blockDim.x;
+    if (j < NR) {
+      const int base = (j / Ns) * Ns * R + j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[base + r * Ns] = y[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// FFTs that run in place on one LDS buffer of N elements (256 threads):
+// radices 2, 3 and 4 only, N <= 2048, so a thread holds at most 2 radix-4,
+// 3 radix-3 or 4 radix-2 butterflies of a pass
+constexpr int kFftThreads = 256;
+__host__ __device__ inline bool fft_one_buffer(const FFTPlanDev& P) {
+  if (P.N > 2048) return false;
+  for (int i = 0; i < P.nrad; ++i)
+    if (P.radix[i] < 2 || P.radix[i] > 4) return false;
+  return true;
+}
+// dynamic LDS bytes of the ring FFT kernels
+__host__ __device__ inline size_t fft_lds_bytes(const FFTPlanDev& P) {
+  return (size_t)(fft_one_buffer(P) ? 1 : 2) * P.N * 16;
 }
 
 template <int R>
@@ -509,6 +558,42 @@ __device__ double2* fft_lds(const FFTPlanDev& P, const double2* __restrict__ tw,
   return in;
 }
 
+// passes pass0 .. pass_end - 1 in place on buf (fft_one_buffer plans)
+__device__ void fft_lds_inplace(const FFTPlanDev& P, const double2* __restrict__ tw, double2* buf, double sign,
+                                int pass0, int Ns0, int pass_end) {
+  const int N = P.N;
+  int Ns = Ns0;
+  for (int pass = pass0; pass < pass_end; ++pass) {
+    const int R = P.radix[pass];
+    const int kb = (N / R + kFftThreads - 1) / kFftThreads;
+    if (R == 4) {
+      if (kb <= 1) fft_pass_inplace<4, 1>(tw, buf, N, Ns, sign);
+      else fft_pass_inplace<4, 2>(tw, buf, N, Ns, sign);
+    } else if (R == 2) {
+      if (kb <= 1) fft_pass_inplace<2, 1>(tw, buf, N, Ns, sign);
+      else if (kb <= 2) fft_pass_inplace<2, 2>(tw, buf, N, Ns, sign);
+      else fft_pass_inplace<2, 4>(tw, buf, N, Ns, sign);
+    } else {
+      if (kb <= 1) fft_pass_inplace<3, 1>(tw, buf, N, Ns, sign);
+      else if (kb <= 2) fft_pass_inplace<3, 2>(tw, buf, N, Ns, sign);
+      else fft_pass_inplace<3, 3>(tw, buf, N, Ns, sign);
+    }
+    Ns *= R;
+  }
+}
+
+// passes pass0 .. pass_end - 1 of the data in cur (lds or lds + N): in place
+// for fft_one_buffer plans, else ping-pong with the other half of lds.
+// Returns the buffer holding the result.
+__device__ double2* fft_run(const FFTPlanDev& P, const double2* __restrict__ tw, double2* lds, double2* cur,
+                            double sign, int pass0, int Ns0, int pass_end) {
+  if (fft_one_buffer(P)) {
+    fft_lds_inplace(P, tw, cur, sign, pass0, Ns0, pass_end);
+    return cur;
+  }
+  return fft_lds(P, tw, cur, cur == lds ? lds + P.N : lds, sign, pass0, Ns0, pass_end);
+}
+
 // real rings -> half spectrum F[f][t][m] (m < L), scaled by pi / L; with
 // psplit each ring's orders are stored even m first, then odd m (the
 // resampling reads one parity at a time)
@@ -544,6 +629,7 @@ __device__ __forceinline__ void KFftR2c::run(const Args& a, int bid) {
   double2* Z;
   if (P.radix[0] == 4) {
     // the first radix-4 pass (span 1: no twiddles) straight from the rings
+    double2* dst = fft_one_buffer(P) ? b0 : b1;
     const int NR = N / 4;
     for (int j = threadIdx.x; j < NR; j += blockDim.x) {
       double2 v[4], y[4];
@@ -551,14 +637,14 @@ __device__ __forceinline__ void KFftR2c::run(const Args& a, int bid) {
       for (int r = 0; r < 4; ++r) v[r] = double2{ra[j + r * NR], hasb ? rb[j + r * NR] : 0.0};
       bfly4(v, y, -1.0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) b1[4 * j + r] = y[r];
+      for (int r = 0; r < 4; ++r) dst[4 * j + r] = y[r];
     }
     __syncthreads();
-    Z = fft_lds(P, tw, b1, b0, -1.0, 1, 4);
+    Z = fft_run(P, tw, lds, dst, -1.0, 1, 4, P.nrad);
   } else {
     for (int n = threadIdx.x; n < N; n += blockDim.x) b0[n] = double2{ra[n], hasb ? rb[n] : 0.0};
     __syncthreads();
-    Z = fft_lds(P, tw, b0, b1, -1.0);
+    Z = fft_run(P, tw, lds, b0, -1.0, 0, 1, P.nrad);
   }
   const double sc = M_PI / L;
   const int half = (L + 1) / 2;
@@ -674,6 +760,7 @@ __device__ __forceinline__ void KFftC2r<RF>::run(const Args& a, int bid) {
   if (direct0) {
     // the first radix-4 pass (span 1: no twiddles) straight from the
     // spectrum into LDS: fft_pass<4>'s arithmetic without staging the input
+    double2* dst = fft_one_buffer(P) ? b0 : b1;
     const int NR = N / 4;
     for (int j = threadIdx.x; j < NR; j += blockDim.x) {
       double2 v[4], y[4];
@@ -681,16 +768,16 @@ __device__ __forceinline__ void KFftC2r<RF>::run(const Args& a, int bid) {
       for (int r = 0; r < 4; ++r) v[r] = xval(j + r * NR);
       bfly4(v, y, +1.0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) b1[4 * j + r] = y[r];
+      for (int r = 0; r < 4; ++r) dst[4 * j + r] = y[r];
     }
     __syncthreads();
-    Z = fft_lds(P, tw, b1, b0, +1.0, 1, 4, pend);
+    Z = fft_run(P, tw, lds, dst, +1.0, 1, 4, pend);
   } else {
     if constexpr (!RF) {
       for (int n = threadIdx.x; n < N; n += blockDim.x) b0[n] = xval(n);
     }
     __syncthreads();
-    Z = fft_lds(P, tw, b0, b1, +1.0, 0, 1, pend);
+    Z = fft_run(P, tw, lds, b0, +1.0, 0, 1, pend);
   }
   double* ra = f + ((int64_t)fi * (L + 1) + ta) * N;
   double* rb = f + ((int64_t)fi * (L + 1) + (hasb ? tb : ta)) * N;
@@ -2867,7 +2954,7 @@ extern "C" int cmm_sht_inverse(const cmm_sht_plan* p, const double* flm, int nf,
     }
   }
   int pairs = (L + 2) / 2;
-  size_t lds = (size_t)2 * p->N * sizeof(double2);
+  size_t lds = fft_lds_bytes(fft_plan(p));
   if (p->legendre >= CMM_LEGENDRE_I8X4) {  // rings-fastest coefficients
     const int gx = 32 * ((pairs + 31) / 32);
     return launch<KFftC2r<true>>(k_fft_c2r<true>, (unsigned)(gx * nf), 256, lds, st,
@@ -2910,7 +2997,7 @@ static int sht_forward(const cmm_sht_plan* p, const double* f, int nf, double* f
   double2* G = F + (int64_t)nf * (L + 1) * L;
   hipStream_t st = as_stream(stream);
   int pairs = (L + 2) / 2;
-  size_t lds = (size_t)2 * p->N * sizeof(double2);
+  size_t lds = fft_lds_bytes(fft_plan(p));
   int rc = launch<KFftR2c>(k_fft_r2c, (unsigned)(nf * pairs), 256, lds, st, KFftR2c::Args{fft_plan(p), p->tw, f, L, F, 1},
                            "cmm_sht_forward: fft");
   if (rc) return rc;
@@ -3104,7 +3191,7 @@ extern "C" int cmm_sht_inverse_adjoint(const cmm_sht_plan* p, const double* fbar
   double2* F = reinterpret_cast<double2*>(work);
   hipStream_t st = as_stream(stream);
   const int pairs = (L + 2) / 2;
-  const size_t lds = (size_t)2 * p->N * sizeof(double2);
+  const size_t lds = fft_lds_bytes(fft_plan(p));
   hipLaunchKernelGGL(k_fft_r2c, dim3(nf * pairs), dim3(256), lds, st, KFftR2c::Args{fft_plan(p), p->tw, fbar, L, F, 0});
   CMM_LAUNCH_CHECK("cmm_sht_inverse_adjoint: fft");
   const int nhr = L / 2 + 1;
@@ -3148,7 +3235,7 @@ extern "C" int cmm_sht_forward_adjoint(const cmm_sht_plan* p, const double* flm_
                      p->K2, reinterpret_cast<const double*>(G), reinterpret_cast<double*>(F));
   CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: resample");
   const int pairs = (L + 2) / 2;
-  const size_t lds = (size_t)2 * p->N * sizeof(double2);
+  const size_t lds = fft_lds_bytes(fft_plan(p));
   hipLaunchKernelGGL(k_fft_c2r<false>, dim3(nf * pairs), dim3(256), lds, st,
                      KFftC2r<false>::Args{fft_plan(p), p->tw, F, L, fbar, 0});
   CMM_LAUNCH_CHECK("cmm_sht_forward_adjoint: fft");
