@@ -207,12 +207,15 @@ def test_sht_forward_non_band_limited(L):
         assert np.abs(got[:, fi] - ref_p).max() < 1e-12 * np.abs(ref_p).max(), fi
 
 
-@pytest.mark.parametrize("L", [37, 43, 1000, 2048])
+@pytest.mark.parametrize("L", [27, 37, 43, 384, 768, 1000, 1024, 2048])
 def test_sht_any_bandlimit(L):
     """Bandlimits whose ring length 2L has prime factors other than 2 and 3
     (2 x 37; 2 x 43, whose L + 1 rings fill whole steps of 4 with no zero
-    padding ring; 2^4 x 5^3, the reference's upsampling grids Lf = 1000) and the
-    largest plan (L = 2048, the reference's Poisson test up to 2048):
+    padding ring; 2^4 x 5^3, the reference's upsampling grids Lf = 1000), the
+    largest plan (L = 2048, the reference's Poisson test up to 2048), and the
+    ring FFTs that run in place on one LDS buffer with radix-2 and radix-3
+    passes and up to 4 butterflies per thread (2L = 54 = 2 x 3^3, starting
+    with a radix-2 pass; 768 = 4^4 x 3; 1536 = 4^4 x 2 x 3; 2048 = 4^5 x 2):
     synthesis of single Y_lm against scipy's sph_harm_y on the grid, the
     Rossby-Haurwitz (5,4) Poisson closed form of the reference's test
     (tests/test_spherical_harmonic_tools.py:48-56), and analysis of random
